@@ -1,0 +1,126 @@
+/*
+ * naja_amd — C ABI of the MI355X (gfx950) coordinate hit-and-run sampler.
+ *
+ * Every entry point takes plain pointers and sizes. Matrix and vector
+ * arguments are DEVICE pointers (hipMalloc'd or torch CUDA tensors) unless a
+ * comment says "host". `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream). Calls are asynchronous on that stream; they return after the
+ * launches are queued. Return value: 0 on success or a negative NAJA_ERR_*
+ * code; naja_last_error() then holds a thread-local message. No C++ exception
+ * crosses this boundary.
+ *
+ * The reference boundary these replace is the C++ family in naja::gpu,
+ * src/gpu/gpusamplers.h:30-86, which takes DMatrix<double>/DVector<double>
+ * device containers and throws std::invalid_argument / std::runtime_error.
+ */
+#ifndef NAJA_AMD_H
+#define NAJA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NAJA_OK 0
+#define NAJA_ERR_INVALID_ARGUMENT (-1) /* reference: std::invalid_argument (backmap.cu:27-35) */
+#define NAJA_ERR_RUNTIME (-2)          /* reference: std::runtime_error (chr.cu:549) */
+#define NAJA_ERR_HIP (-3)              /* HIP API failure (the reference only prints: helper.h:12-27) */
+#define NAJA_ERR_UNSUPPORTED (-4)      /* shape outside the compiled kernel variants */
+
+/* Sample layout selectors for naja_diag. */
+#define NAJA_LAYOUT_INTERLEAVED 0 /* row s*nchains + c is sample s of chain c (samples.npy as written, npy.h:96-141) */
+#define NAJA_LAYOUT_BLOCKS 1      /* rows [c*spc, (c+1)*spc) are chain c (naja validate's grouping, mcmc.cpp:103-110) */
+
+/* Thread-local message describing the last failed call on this thread. */
+const char* naja_last_error(void);
+
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int naja_version(void);
+
+/*
+ * Sampler parameters. Field meanings follow the arguments of
+ * naja::gpu::CoordinateHitAndRun (gpusamplers.h:30-48) and the
+ * `naja sample run` flags (cmd_run.cpp:76-103).
+ */
+typedef struct naja_chr_params {
+    int nspc;             /* samples per chain (reference `nspc`, --n-samples) */
+    int thinning;         /* steps between saved samples (--thinning; > 0) */
+    int nchains;          /* chains per polytope / per variant (--n-chains) */
+    int tpb;              /* reference --tpb: workgroup size request; <= 0 lets the library choose */
+    uint64_t seed;        /* Philox key (the reference's int seed, widened) */
+    double pair_prob;     /* --pair-prob */
+    int resync_interval;  /* --resync-interval: FP64 s = b - A y every R steps; 0 = automatic
+                             max(1024, 16 d) window; < 0 = never after t = 0 */
+    double ksparse_prob;  /* --ksparse-prob */
+    int ksparse_k;        /* --ksparse-k (clamped to [1, min(32, d)] as chr.cu:300-304) */
+    int pair_mode;        /* 1 = (e_i - e_j)/sqrt2, 2 = schedule pairs (plan.h:13-19) */
+    const int* pair_i;    /* device, n_pairs (pair mode 2; schedule_io.cpp:12-75 columns i, j, c, s) */
+    const int* pair_j;
+    const float* pair_c;
+    const float* pair_s;
+    int n_pairs;
+    int chains_per_group; /* G chains share one move draw (1 = independent moves, as naja) */
+    int32_t* move_log;    /* device, optional: (type, i, j) of chain 0's moves for t < nlog */
+    int64_t nlog;
+} naja_chr_params;
+
+/*
+ * naja::gpu::CoordinateHitAndRun (gpusamplers.h:30-48, chr.cu:13-70).
+ *   A       : m x d, column-major (lda = m), FP32
+ *   b       : m, FP64 (used by the FP64 slack computation s = b - A y)
+ *   y0      : d x nchains column-major start points (the reference's X_d)
+ *   samples : d x (nchains*nspc) column-major; column s*nchains + c holds
+ *             sample s of chain c (chr.cu:284, chr.cu:460). Read as a C-order
+ *             (nchains*nspc, d) array it is samples.npy (npy.h:96-141).
+ *   y_final : optional d x nchains final states (the reference's
+ *             persist_state write-back, chr.cu:485-489).
+ */
+int naja_chr_sample(const float* A, const double* b, int m, int d,
+                    const float* y0, const naja_chr_params* p,
+                    float* samples, float* y_final, void* stream);
+
+/*
+ * Batched conditioned variants in one persistent launch: variant v samples
+ * {y : [A; E_v] y <= [b; f_v]} with E_v its own k_v extra rows (the
+ * reference stacks one extra block per model: extra_constraints.cpp:17-58,
+ * and runs models one job at a time: job_bulk.cpp:115-189).
+ *   extra_rows : HOST, n_variants counts k_v >= 0
+ *   A_extra    : concatenated column-major k_v x d blocks, variant order
+ *   b_extra    : concatenated k_v right-hand sides (FP64)
+ *   y0         : d x (n_variants*nchains), variant-major
+ *   samples    : n_variants consecutive d x (nchains*nspc) blocks
+ * Variant v uses Philox key (seed, variant v).
+ */
+int naja_chr_sample_batched(const float* A, const double* b, int m0, int d,
+                            int n_variants, const int* extra_rows,
+                            const float* A_extra, const double* b_extra,
+                            const float* y0, const naja_chr_params* p,
+                            float* samples, float* y_final, void* stream);
+
+/*
+ * backmap_into (backmap.cu:21-48): V = T Y + shift, FP32 MFMA.
+ *   T     : n x d row-major (reaction rows, as the CSV stores T)
+ *   shift : n
+ *   Y     : ns x d row-major (sample rows)
+ *   V     : ns x n row-major (backmapped sample rows)
+ */
+int naja_backmap(const float* T, const float* shift, const float* Y, float* V,
+                 int n, int d, int64_t ns, void* stream);
+
+/*
+ * compute_ess + compute_split_rhat (mcmc.cpp:91-194) on the GPU.
+ *   samples   : n_total x dim row-major (samples.npy), n_total % nchains == 0
+ *   layout    : NAJA_LAYOUT_INTERLEAVED or NAJA_LAYOUT_BLOCKS
+ *   ess       : dim, per-dim minimum over chains
+ *   rhat      : dim, classic split-R-hat
+ *   ess_chain : optional dim x nchains per-chain ESS
+ */
+int naja_diag(const float* samples, int dim, int n_total, int nchains, int layout,
+              float* ess, float* rhat, float* ess_chain, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NAJA_AMD_H */

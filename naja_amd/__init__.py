@@ -1,0 +1,291 @@
+"""naja_amd: MI355X (gfx950) coordinate hit-and-run sampler.
+
+Python host mirror of the reference's GPU sampler API, `naja::gpu` in
+src/gpu/gpusamplers.h:30-86, over the C ABI of ``lib/libnaja_amd.so``
+(include/naja_amd.h). Device memory and streams come from PyTorch tensors;
+every computation runs in the HIP library. There is no CPU fallback: if the
+shared library is missing or fails to load, importing the compute entry points
+raises.
+
+Tensor conventions (natural torch shapes; the library sees the reference's
+column-major buffers):
+  A        (m, d)  polytope rows, any float dtype; stored FP32 column-major
+  b        (m,)    right-hand side, kept FP64 for the FP64 slack computation
+  X0       (nchains, d) start points (the reference's X_d, d x nchains)
+  samples  (nchains * nspc, d) float32, row s*nchains + c = sample s of chain c
+           (exactly samples.npy, npy.h:96-141)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libnaja_amd.so")
+
+NAJA_OK = 0
+NAJA_ERR_INVALID_ARGUMENT = -1
+NAJA_ERR_RUNTIME = -2
+NAJA_ERR_HIP = -3
+NAJA_ERR_UNSUPPORTED = -4
+
+LAYOUT_INTERLEAVED = 0
+LAYOUT_BLOCKS = 1
+
+EXPORTED_SYMBOLS = (
+    "naja_last_error",
+    "naja_version",
+    "naja_chr_sample",
+    "naja_chr_sample_batched",
+    "naja_backmap",
+    "naja_diag",
+)
+
+
+class NajaError(RuntimeError):
+    """A failed library call (the reference throws std::runtime_error)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"naja error {code}: {msg}")
+        self.code = code
+
+
+class NajaInvalidArgument(NajaError, ValueError):
+    """Shape / argument error (the reference throws std::invalid_argument)."""
+
+
+class ChrParams(ctypes.Structure):
+    _fields_ = [
+        ("nspc", ctypes.c_int),
+        ("thinning", ctypes.c_int),
+        ("nchains", ctypes.c_int),
+        ("tpb", ctypes.c_int),
+        ("seed", ctypes.c_uint64),
+        ("pair_prob", ctypes.c_double),
+        ("resync_interval", ctypes.c_int),
+        ("ksparse_prob", ctypes.c_double),
+        ("ksparse_k", ctypes.c_int),
+        ("pair_mode", ctypes.c_int),
+        ("pair_i", ctypes.c_void_p),
+        ("pair_j", ctypes.c_void_p),
+        ("pair_c", ctypes.c_void_p),
+        ("pair_s", ctypes.c_void_p),
+        ("n_pairs", ctypes.c_int),
+        ("chains_per_group", ctypes.c_int),
+        ("move_log", ctypes.c_void_p),
+        ("nlog", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libnaja_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"naja_amd native library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    lib.naja_last_error.restype = ctypes.c_char_p
+    lib.naja_version.restype = i32
+    lib.naja_chr_sample.argtypes = [vp, vp, i32, i32, vp, ctypes.POINTER(ChrParams), vp, vp, vp]
+    lib.naja_chr_sample.restype = i32
+    lib.naja_chr_sample_batched.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp,
+                                            ctypes.POINTER(ChrParams), vp, vp, vp]
+    lib.naja_chr_sample_batched.restype = i32
+    lib.naja_backmap.argtypes = [vp, vp, vp, vp, i32, i32, i64, vp]
+    lib.naja_backmap.restype = i32
+    lib.naja_diag.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
+    lib.naja_diag.restype = i32
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc == NAJA_OK:
+        return
+    msg = _lib.naja_last_error().decode(errors="replace")
+    if rc == NAJA_ERR_INVALID_ARGUMENT:
+        raise NajaInvalidArgument(rc, msg)
+    raise NajaError(rc, msg)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream_ptr(stream) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _dev(t, dtype):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(t)
+    if not t.is_cuda:
+        t = t.cuda()
+    return t.to(dtype).contiguous()
+
+
+def _colmajor_f32(A):
+    """(m, d) tensor -> device FP32 buffer laid out column-major (lda = m)."""
+    torch = _torch()
+    A = _dev(A, torch.float32)
+    return A.t().contiguous()
+
+
+def chr_params(nspc, thinning, nchains, tpb_ss=-1, seed=0, pair_prob=0.0, resync_interval=0,
+               ksparse_prob=0.0, ksparse_k=8, pair_mode=1, pair=None, chains_per_group=1,
+               move_log=None, nlog=0) -> ChrParams:
+    p = ChrParams()
+    p.nspc, p.thinning, p.nchains, p.tpb = int(nspc), int(thinning), int(nchains), int(tpb_ss)
+    p.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    p.pair_prob, p.resync_interval = float(pair_prob), int(resync_interval)
+    p.ksparse_prob, p.ksparse_k, p.pair_mode = float(ksparse_prob), int(ksparse_k), int(pair_mode)
+    if pair is not None:
+        pi, pj, pc, ps = pair
+        p.pair_i, p.pair_j = pi.data_ptr(), pj.data_ptr()
+        p.pair_c, p.pair_s = pc.data_ptr(), ps.data_ptr()
+        p.n_pairs = int(pi.numel())
+    p.chains_per_group = int(chains_per_group)
+    if move_log is not None:
+        p.move_log = move_log.data_ptr()
+        p.nlog = int(nlog)
+    return p
+
+
+def _pair_tensors(pair_i, pair_j, pair_c, pair_s):
+    if pair_i is None:
+        return None
+    torch = _torch()
+    return (_dev(pair_i, torch.int32), _dev(pair_j, torch.int32),
+            _dev(pair_c, torch.float32), _dev(pair_s, torch.float32))
+
+
+def coordinate_hit_and_run(A, b, X0, nspc, thinning, nchains, tpb_ss=-1, seed=0, pair_prob=0.0,
+                           resync_interval=0, ksparse_prob=0.0, ksparse_k=8, pair_mode=1,
+                           pair_i=None, pair_j=None, pair_c=None, pair_s=None,
+                           chains_per_group=1, return_final=False, move_log_steps=0, stream=None):
+    """naja::gpu::CoordinateHitAndRun (gpusamplers.h:30-48, chr.cu:13-70).
+
+    Returns samples (nchains*nspc, d) float32 on the device; with
+    return_final also the final states (nchains, d); with move_log_steps > 0
+    also the (type, i, j) move log of chain 0 as an int32 (steps, 3) tensor.
+    """
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    X = _dev(X0, torch.float32)
+    if X.dim() == 1:
+        X = X.unsqueeze(0).expand(nchains, d).contiguous()
+    if tuple(X.shape) != (nchains, d):
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"X0 shape {tuple(X.shape)} != ({nchains}, {d})")
+    if b_d.numel() != m:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"b has {b_d.numel()} entries, A has {m} rows")
+    pair = _pair_tensors(pair_i, pair_j, pair_c, pair_s)
+    samples = torch.empty((nchains * nspc, d), dtype=torch.float32, device=A_cm.device)
+    final = torch.empty((nchains, d), dtype=torch.float32, device=A_cm.device) if return_final else None
+    mlog = None
+    if move_log_steps > 0:
+        mlog = torch.full((move_log_steps, 3), -7, dtype=torch.int32, device=A_cm.device)
+    p = chr_params(nspc, thinning, nchains, tpb_ss, seed, pair_prob, resync_interval, ksparse_prob,
+                   ksparse_k, pair_mode, pair, chains_per_group, mlog, move_log_steps)
+    rc = lib.naja_chr_sample(A_cm.data_ptr(), b_d.data_ptr(), m, d, X.data_ptr(), ctypes.byref(p),
+                             samples.data_ptr(), final.data_ptr() if final is not None else None,
+                             _stream_ptr(stream))
+    _check(rc)
+    out = [samples]
+    if return_final:
+        out.append(final)
+    if mlog is not None:
+        out.append(mlog)
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, nchains, tpb_ss=-1,
+                                   seed=0, pair_prob=0.0, resync_interval=0, ksparse_prob=0.0,
+                                   ksparse_k=8, pair_mode=1, chains_per_group=1, return_final=False,
+                                   stream=None):
+    """All conditioned variants [A; E_v] in one launch.
+
+    extra_A / extra_b: lists (one entry per variant) of (k_v, d) and (k_v,)
+    tensors, k_v >= 0. X0: (n_variants, nchains, d). Returns samples
+    (n_variants, nchains*nspc, d).
+    """
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m0 = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    nv = len(extra_A)
+    ks = [int(e.shape[0]) for e in extra_A]
+    blocks = [_colmajor_f32(e).reshape(-1) for e in extra_A if e.shape[0] > 0]
+    Ae = torch.cat(blocks) if blocks else None
+    be = torch.cat([_dev(f, torch.float64).reshape(-1) for f in extra_b if f.numel() > 0]) if blocks else None
+    X = _dev(X0, torch.float32)
+    if tuple(X.shape) != (nv, nchains, d):
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"X0 shape {tuple(X.shape)} != ({nv}, {nchains}, {d})")
+    samples = torch.empty((nv, nchains * nspc, d), dtype=torch.float32, device=A_cm.device)
+    final = torch.empty((nv, nchains, d), dtype=torch.float32, device=A_cm.device) if return_final else None
+    karr = (ctypes.c_int * nv)(*ks)
+    p = chr_params(nspc, thinning, nchains, tpb_ss, seed, pair_prob, resync_interval, ksparse_prob,
+                   ksparse_k, pair_mode, None, chains_per_group)
+    rc = lib.naja_chr_sample_batched(A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr,
+                                     Ae.data_ptr() if Ae is not None else None,
+                                     be.data_ptr() if be is not None else None,
+                                     X.data_ptr(), ctypes.byref(p), samples.data_ptr(),
+                                     final.data_ptr() if final is not None else None, _stream_ptr(stream))
+    _check(rc)
+    return (samples, final) if return_final else samples
+
+
+def backmap(T, shift, Y, stream=None):
+    """backmap_into (backmap.cu:21-48): V = Y T^T + shift, rows are samples."""
+    torch = _torch()
+    lib = load_library()
+    T_d = _dev(T, torch.float32)
+    sh = _dev(shift, torch.float32).reshape(-1)
+    Y_d = _dev(Y, torch.float32)
+    n, d = T_d.shape
+    if Y_d.shape[1] != d:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT,
+                                  "backmap_into: transformation shape mismatch with samples")
+    if sh.numel() != n:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT,
+                                  "backmap_into: shift length mismatch with transformation rows")
+    V = torch.empty((Y_d.shape[0], n), dtype=torch.float32, device=T_d.device)
+    _check(lib.naja_backmap(T_d.data_ptr(), sh.data_ptr(), Y_d.data_ptr(), V.data_ptr(), n, d,
+                            Y_d.shape[0], _stream_ptr(stream)))
+    return V
+
+
+def coordinate_hit_and_run_backmap(A, b, X0, T, shift, nspc, thinning, nchains, **kw):
+    """CoordinateHitAndRunBackmap (chr.cu:72-99): sample, then V = T y + shift."""
+    Y = coordinate_hit_and_run(A, b, X0, nspc, thinning, nchains, **kw)
+    return backmap(T, shift, Y, stream=kw.get("stream"))
+
+
+def diagnostics(samples, nchains, layout=LAYOUT_INTERLEAVED, stream=None):
+    """compute_ess + compute_split_rhat (mcmc.cpp:91-194) on the GPU.
+
+    samples: (n_total, dim). Returns (ess[dim], rhat[dim], ess_chain[dim, nchains]).
+    """
+    torch = _torch()
+    lib = load_library()
+    S = _dev(samples, torch.float32)
+    n_total, dim = S.shape
+    ess = torch.empty(dim, dtype=torch.float32, device=S.device)
+    rhat = torch.empty(dim, dtype=torch.float32, device=S.device)
+    ess_chain = torch.empty((dim, nchains), dtype=torch.float32, device=S.device)
+    _check(lib.naja_diag(S.data_ptr(), dim, n_total, nchains, layout, ess.data_ptr(), rhat.data_ptr(),
+                         ess_chain.data_ptr(), _stream_ptr(stream)))
+    return ess, rhat, ess_chain

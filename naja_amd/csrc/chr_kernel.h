@@ -1,0 +1,79 @@
+// Internal interface between the C ABI (capi.cpp) and the CHR step kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace naja {
+
+// One conditioned variant, packed by chr_pack_kernel: rows [A; E_v] in a
+// column-major block with lda = m rounded up to 64; padding rows hold A = 0
+// and b = +inf, so their inverse distance is 0/inf = 0 and they never move
+// the chord of a bounded polytope.
+struct ChrVariant {
+    int m;               // real rows m0 + k
+    int lda;             // padded rows (multiple of 64)
+    int64_t a_off;       // float offset of the packed block
+    int64_t b_off;       // offset of the packed FP64 right-hand side
+    int64_t y0_col;      // first column of this variant's chains in y0 / y_final
+    int64_t sample_off;  // float offset of this variant's sample block
+};
+
+// Work item: up to G chains of one variant that share the move stream.
+struct ChrItem {
+    int variant;
+    int c0;  // first chain (a multiple of G)
+};
+
+struct ChrArgs {
+    const float* A;   // packed variant blocks
+    const double* b;  // packed right-hand sides
+    int d;
+    const ChrVariant* variants;
+    const ChrItem* items;
+    const float* y0;
+    float* samples;
+    float* y_final;
+    int nchains;
+    int thin;
+    int64_t nsteps;
+    uint32_t seed_lo;
+    uint32_t seed_hi;
+    uint64_t thr_k;   // k-sparse move: w0 < thr_k
+    uint64_t thr_kp;  // pair move: w0 < thr_kp (and not k-sparse)
+    int has_pair;
+    int pair_mode;
+    const int* pair_i;
+    const int* pair_j;
+    const float* pair_c;
+    const float* pair_s;
+    int n_pairs;
+    int ks_k;
+    float ks_mag;
+    int resync;
+    int32_t* move_log;
+    int64_t nlog;
+};
+
+// FP64 resync window actually used: R > 0 as given; 0 = automatic
+// max(1024, 16 d) steps (FP32 slack needs a bounded drift window, where the
+// reference's FP64 slack does not, chr.cu:386); R < 0 = never after t = 0.
+inline int chr_effective_resync(int requested, int d)
+{
+    if (requested > 0) return requested;
+    if (requested < 0) return 0;
+    return d * 16 > 1024 ? d * 16 : 1024;
+}
+
+// Wavefronts per workgroup for padded rows lda_max (tpb > 0 forces it).
+int chr_pick_waves(int lda_max, int tpb);
+// Smallest compiled rows-per-lane R with 64*W*R >= lda_max; false if none.
+bool chr_shape_available(int W, int G, int lda_max, int* R_out);
+void chr_launch(const ChrArgs& args, int n_items, int W, int G, int R, hipStream_t stream);
+
+// Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks.
+void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
+              const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
+              float* A_pack, double* b_pack, int lda_max, hipStream_t stream);
+
+}  // namespace naja

@@ -1,0 +1,61 @@
+// Shared host-side helpers of the naja_amd library: thread-local error state
+// and HIP error checks that turn failures into NAJA_ERR_* codes (the
+// reference prints CUDA errors and carries on, helper.h:12-27; this library
+// reports them to the caller instead).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/naja_amd.h"
+
+namespace naja {
+
+void set_error(const char* fmt, ...);
+void clear_error();
+
+struct Error {
+    int code;
+    explicit Error(int c) : code(c) {}
+};
+
+#define NAJA_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t err_ = (call);                                                       \
+        if (err_ != hipSuccess) {                                                       \
+            ::naja::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(err_),  \
+                              __FILE__, __LINE__);                                      \
+            throw ::naja::Error(NAJA_ERR_HIP);                                          \
+        }                                                                               \
+    } while (0)
+
+#define NAJA_REQUIRE(cond, code, ...)          \
+    do {                                       \
+        if (!(cond)) {                         \
+            ::naja::set_error(__VA_ARGS__);    \
+            throw ::naja::Error(code);         \
+        }                                      \
+    } while (0)
+
+// Runs body, mapping naja::Error and std exceptions to return codes.
+template <typename F>
+int guarded(F&& body)
+{
+    clear_error();
+    try {
+        body();
+        return NAJA_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error("%s", e.what());
+        return NAJA_ERR_RUNTIME;
+    } catch (...) {
+        set_error("unknown C++ exception");
+        return NAJA_ERR_RUNTIME;
+    }
+}
+
+}  // namespace naja

@@ -1,0 +1,315 @@
+// GPU ESS and split-R-hat, replacing naja validate's host loops
+// compute_ess / compute_split_rhat (mcmc.cpp:91-194) and their vendored
+// kissfft dependency.
+//
+// ESS (mcmc.cpp:17-76): one workgroup per (dim, chain) series of length n.
+//   * mean summed in FP64, cast to float; series centred in float;
+//   * autocovariance by a hand-written FFT: the real series of length
+//     N = 2^ceil(log2(2n)) (zero padded; linear, not circular, correlation)
+//     is packed as N/2 complex points z_j = x_2j + i x_2j+1 and transformed
+//     in LDS by an in-place radix-2 decimation-in-frequency FFT (natural in,
+//     bit-reversed out); the real spectrum X_k is split out, |X_k|^2 is
+//     re-packed in place and inverted by a decimation-in-time FFT
+//     (bit-reversed in, natural out), so no reordering pass is needed; the
+//     interleaved result is the unnormalised autocorrelation, scaled by
+//     1/(N n) as mcmc.cpp:49-53 scales kissfft's by 1/(2n n);
+//   * truncated pair-sum tau in float, sequential as mcmc.cpp:66-72.
+// Split-R-hat (mcmc.cpp:126-194): FP64 sums per (dim, half chain), then the
+// reference's B / W / var_hat formula per dim.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "diag.h"
+
+namespace naja {
+
+namespace {
+
+constexpr int kDiagThreads = 256;
+constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (128 KB)
+
+__device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
+{
+    return layout == 0 ? (int64_t)s * nchains + c : (int64_t)c * spc + s;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// In-place radix-2 FFTs over H points; tw[k] = exp(-2 pi i k / N), N = 2H,
+// so the twiddle of a length-L butterfly block is W_L^p = tw[p * N / L].
+// Decimation in frequency: natural-order input, bit-reversed output.
+__device__ void fft_dif(float2* z, int H, int logH, const float2* __restrict__ tw)
+{
+    for (int st = logH; st >= 1; --st) {
+        const int half = 1 << (st - 1);
+        const int tstride = (H >> st) * 2;
+        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
+            const int pos = b & (half - 1);
+            const int i0 = ((b >> (st - 1)) << st) + pos;
+            const int i1 = i0 + half;
+            const float2 u = z[i0];
+            const float2 v = z[i1];
+            z[i0] = make_float2(u.x + v.x, u.y + v.y);
+            z[i1] = cmul(make_float2(u.x - v.x, u.y - v.y), tw[pos * tstride]);
+        }
+        __syncthreads();
+    }
+}
+
+// Decimation in time with conjugate twiddles (unnormalised inverse):
+// bit-reversed input, natural-order output.
+__device__ void ifft_dit(float2* z, int H, int logH, const float2* __restrict__ tw)
+{
+    for (int st = 1; st <= logH; ++st) {
+        const int half = 1 << (st - 1);
+        const int tstride = (H >> st) * 2;
+        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
+            const int pos = b & (half - 1);
+            const int i0 = ((b >> (st - 1)) << st) + pos;
+            const int i1 = i0 + half;
+            float2 w = tw[pos * tstride];
+            w.y = -w.y;
+            const float2 u = z[i0];
+            const float2 v = cmul(w, z[i1]);
+            z[i0] = make_float2(u.x + v.x, u.y + v.y);
+            z[i1] = make_float2(u.x - v.x, u.y - v.y);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int bitrev(int x, int bits)
+{
+    return (int)(__builtin_bitreverse32((uint32_t)x) >> (32 - bits));
+}
+
+__global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ samples, int dim,
+                                                                  int nchains, int spc, int layout, int logH,
+                                                                  const float2* __restrict__ tw,
+                                                                  float* __restrict__ ess_chain)
+{
+    extern __shared__ float2 z[];
+    __shared__ double red[kDiagThreads / 64];
+    __shared__ float meanf_s;
+
+    const int series = blockIdx.x;  // series = c * dim + d
+    const int d = series % dim;
+    const int c = series / dim;
+    const int n = spc;
+    const int H = 1 << logH;
+    const int N = 2 * H;
+    const int tid = threadIdx.x;
+
+    if (n < 4) {
+        if (tid == 0) ess_chain[(int64_t)d * nchains + c] = (float)n;
+        return;
+    }
+
+    // Mean in FP64 (mcmc.cpp:25-27).
+    double part = 0.0;
+    for (int s = tid; s < n; s += blockDim.x) part += (double)samples[sample_row(layout, s, c, nchains, spc) * dim + d];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if ((tid & 63) == 0) red[tid >> 6] = part;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kDiagThreads / 64; ++w) sum += red[w];
+        meanf_s = (float)(sum / n);
+    }
+    __syncthreads();
+    const float meanf = meanf_s;
+
+    // Pack z_j = x_2j + i x_2j+1 (centred, zero padded), natural order.
+    for (int j = tid; j < H; j += blockDim.x) {
+        const int s0 = 2 * j, s1 = 2 * j + 1;
+        const float x0 = s0 < n ? samples[sample_row(layout, s0, c, nchains, spc) * dim + d] - meanf : 0.0f;
+        const float x1 = s1 < n ? samples[sample_row(layout, s1, c, nchains, spc) * dim + d] - meanf : 0.0f;
+        z[j] = make_float2(x0, x1);
+    }
+    __syncthreads();
+    fft_dif(z, H, logH, tw);  // Z_k now sits at z[bitrev(k)]
+
+    // Split the real spectrum, form P = |X|^2 and re-pack it for the
+    // half-size inverse: A_k = P_k + P_{H-k}, B_k = (P_k - P_{H-k}) e^{+2 pi i k/N},
+    // Z'_k = A_k + i B_k, written back to z[bitrev(k)] (the slot it came
+    // from, which is where the DIT inverse wants it). Each thread owns the
+    // pair {k, H-k}, so no other thread touches those two slots.
+    for (int k = tid; k <= H / 2; k += blockDim.x) {
+        const int kb = (H - k) & (H - 1);
+        const int pk = bitrev(k, logH);
+        const int pb = bitrev(kb, logH);
+        const float2 zk = z[pk];
+        const float2 zb = z[pb];
+        // E = (Z_k + conj Z_{H-k})/2, O = (Z_k - conj Z_{H-k})/(2i)
+        const float2 Ek = make_float2(0.5f * (zk.x + zb.x), 0.5f * (zk.y - zb.y));
+        const float2 Ok = make_float2(0.5f * (zk.y + zb.y), -0.5f * (zk.x - zb.x));
+        const float2 Eb = make_float2(0.5f * (zb.x + zk.x), 0.5f * (zb.y - zk.y));
+        const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
+        // X_k = E_k + W^k O_k, W = e^{-2 pi i/N}; the partner of k = 0 is
+        // X_H = E_0 + W^H O_0 with W^H = -1.
+        const float2 wk = tw[k];
+        const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw[H - k];
+        const float2 Xk = make_float2(Ek.x + (wk.x * Ok.x - wk.y * Ok.y), Ek.y + (wk.x * Ok.y + wk.y * Ok.x));
+        const float2 Xb = make_float2(Eb.x + (wb.x * Ob.x - wb.y * Ob.y), Eb.y + (wb.x * Ob.y + wb.y * Ob.x));
+        const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
+        const float Pb = Xb.x * Xb.x + Xb.y * Xb.y;
+        {
+            const float Ak = Pk + Pb;
+            const float Dk = Pk - Pb;
+            // e^{+2 pi i k/N} = conj(tw[k])
+            z[pk] = make_float2(Ak + Dk * wk.y, Dk * wk.x);
+        }
+        if (k != 0 && kb != k) {
+            const float Ab = Pb + Pk;
+            const float Db = Pb - Pk;
+            z[pb] = make_float2(Ab + Db * wb.y, Db * wb.x);
+        }
+    }
+    __syncthreads();
+    ifft_dit(z, H, logH, tw);
+
+    // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
+    if (tid == 0) {
+        const float* raw = reinterpret_cast<const float*>(z);
+        const float norm = 1.0f / ((float)N * (float)n);
+        const float var = raw[0] * norm;
+        float e;
+        if (var < 1e-30f) {
+            e = 1.0f;
+        } else {
+            float tau = 1.0f;
+            for (int k = 1; k < n / 2; ++k) {
+                const float a1 = raw[2 * k - 1] * norm;
+                const float a2 = raw[2 * k] * norm;
+                const float pair_sum = a1 / var + a2 / var;
+                if (pair_sum < 0.0f) break;
+                tau += 2.0f * pair_sum;
+            }
+            tau = fmaxf(tau, 1.0f);
+            e = (float)n / tau;
+        }
+        ess_chain[(int64_t)d * nchains + c] = e;
+    }
+}
+
+__global__ void ess_min_kernel(const float* __restrict__ ess_chain, int dim, int nchains, float* __restrict__ ess)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= dim) return;
+    float m = 3.402823466e+38f;
+    for (int c = 0; c < nchains; ++c) m = fminf(m, ess_chain[(int64_t)d * nchains + c]);
+    ess[d] = m;
+}
+
+// Sum and sum of squares (FP64) of 64 dims x one half chain per workgroup.
+__global__ __launch_bounds__(256) void rhat_moments_kernel(const float* __restrict__ samples, int dim, int nchains,
+                                                          int spc, int half, int layout, double* __restrict__ mom)
+{
+    __shared__ double sh[2][4][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int d = blockIdx.x * 64 + lane;
+    const int hc = blockIdx.y;  // half chain = 2*c + h
+    const int c = hc >> 1;
+    const int h = hc & 1;
+    double s = 0.0, s2 = 0.0;
+    if (d < dim) {
+        for (int i = wave; i < half; i += 4) {
+            const double v = (double)samples[sample_row(layout, h * half + i, c, nchains, spc) * dim + d];
+            s += v;
+            s2 += v * v;
+        }
+    }
+    sh[0][wave][lane] = s;
+    sh[1][wave][lane] = s2;
+    __syncthreads();
+    if (wave == 0 && d < dim) {
+        const double S = sh[0][0][lane] + sh[0][1][lane] + sh[0][2][lane] + sh[0][3][lane];
+        const double S2 = sh[1][0][lane] + sh[1][1][lane] + sh[1][2][lane] + sh[1][3][lane];
+        mom[((int64_t)hc * dim + d) * 2 + 0] = S;
+        mom[((int64_t)hc * dim + d) * 2 + 1] = S2;
+    }
+}
+
+// mcmc.cpp:150-186 per dim.
+__global__ void rhat_final_kernel(const double* __restrict__ mom, int dim, int nchains, int half,
+                                  float* __restrict__ rhat)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= dim) return;
+    const int m = 2 * nchains;
+    double grand = 0.0;
+    for (int j = 0; j < m; ++j) grand += mom[((int64_t)j * dim + d) * 2] / half;
+    const double gm = grand / m;
+    double B = 0.0, W = 0.0;
+    for (int j = 0; j < m; ++j) {
+        const double S = mom[((int64_t)j * dim + d) * 2];
+        const double S2 = mom[((int64_t)j * dim + d) * 2 + 1];
+        const double mean = S / half;
+        double var = (S2 / half) - mean * mean;
+        var = var * half / (half - 1);
+        const double df = mean - gm;
+        B += df * df;
+        W += var;
+    }
+    B *= (double)half / (m - 1);
+    W /= m;
+    const double var_hat = ((half - 1.0) / half) * W + B / half;
+    rhat[d] = (float)((W > 1e-30) ? sqrt(var_hat / W) : 1.0);
+}
+
+}  // namespace
+
+int diag_max_chain_length() { return kMaxHalf; }
+
+void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
+                 float* ess_chain, hipStream_t stream)
+{
+    const int spc = n_total / nchains;
+    int logH = 1;
+    while ((1 << (logH + 1)) < 2 * spc) ++logH;  // N = 2H >= 2 spc
+    const int H = 1 << logH;
+    NAJA_REQUIRE(H <= kMaxHalf, NAJA_ERR_UNSUPPORTED, "chains of %d samples exceed the LDS FFT limit (%d)", spc,
+                 kMaxHalf);
+    const int N = 2 * H;
+
+    std::vector<float2> tw_h((size_t)H);
+    for (int k = 0; k < H; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)N;
+        tw_h[(size_t)k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    float2* tw = nullptr;
+    double* mom = nullptr;
+    float* ec = ess_chain;
+    const int half = spc / 2;
+    NAJA_HIP(hipMallocAsync((void**)&tw, sizeof(float2) * (size_t)H, stream));
+    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * (size_t)H, hipMemcpyHostToDevice, stream));
+    if (!ec) NAJA_HIP(hipMallocAsync((void**)&ec, sizeof(float) * (size_t)dim * nchains, stream));
+    NAJA_HIP(hipMallocAsync((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains, stream));
+
+    const size_t lds = sizeof(float2) * (size_t)H;
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)(dim * nchains)), dim3(kDiagThreads), lds, stream, samples,
+                       dim, nchains, spc, layout, logH, tw, ec);
+    NAJA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256), dim3(256), 0, stream, ec, dim, nchains, ess);
+    NAJA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(rhat_moments_kernel, dim3((dim + 63) / 64, 2 * nchains), dim3(256), 0, stream, samples, dim,
+                       nchains, spc, half, layout, mom);
+    NAJA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(rhat_final_kernel, dim3((dim + 255) / 256), dim3(256), 0, stream, mom, dim, nchains, half,
+                       rhat);
+    NAJA_HIP(hipGetLastError());
+
+    NAJA_HIP(hipFreeAsync(tw, stream));
+    NAJA_HIP(hipFreeAsync(mom, stream));
+    if (ec != ess_chain) NAJA_HIP(hipFreeAsync(ec, stream));
+}
+
+}  // namespace naja

@@ -1,0 +1,181 @@
+"""GPU CHR kernel vs the FP32 oracle on the same Philox stream (through the C ABI)."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.polytopes import cube, genome_like, random_polytope, simplex, thin_slab
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(torch, A, b, y0, nspc, thin, tpb=-1, **kw):
+    import naja_amd
+
+    out = naja_amd.coordinate_hit_and_run(torch.from_numpy(np.asarray(A, np.float32)),
+                                          torch.from_numpy(np.asarray(b, np.float64)),
+                                          torch.from_numpy(np.asarray(y0, np.float32)),
+                                          nspc, thin, y0.shape[0], tpb_ss=tpb, return_final=True, **kw)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in out]
+
+
+def report(tag, gpu, ref):
+    diff = np.abs(gpu.astype(np.float64) - ref.astype(np.float64))
+    rel = diff / np.maximum(np.abs(ref.astype(np.float64)), 1e-3)
+    exact = float((gpu == ref).mean())
+    print(f"[{tag}] exact={exact:.6f} max_abs={diff.max():.3e} max_rel={rel.max():.3e}", flush=True)
+    return rel.max(), exact
+
+
+MOVE_CASES = {
+    "coord": dict(),
+    "pair1": dict(pair_prob=0.3),
+    "ksparse": dict(pair_prob=0.2, ksparse_prob=0.1, ksparse_k=5),
+}
+
+
+@pytest.mark.parametrize("case", list(MOVE_CASES))
+@pytest.mark.parametrize("tpb", [64, 256, 1024])
+def test_move_log_bit_exact(gpu, case, tpb):
+    A, b, y0 = random_polytope(9, 40, seed=3)
+    n = 4096
+    kw = MOVE_CASES[case]
+    _, _, mlog = run_gpu(gpu, A, b, np.zeros((2, 9)), n, 1, tpb=tpb, move_log_steps=n, **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((2, 9), np.float32), n, 1, nlog=n, **kw)
+    np.testing.assert_array_equal(mlog, ref["move_log"])
+
+
+def test_move_log_pair_mode2(gpu):
+    import naja_amd  # noqa: F401
+
+    d = 8
+    A, b, y0 = random_polytope(d, 30, seed=4)
+    pi = np.array([0, 2, 4, 6], np.int32)
+    pj = np.array([1, 3, 5, 7], np.int32)
+    th = np.array([0.3, 0.9, 1.7, 2.5])
+    pc, ps = np.cos(th).astype(np.float32), np.sin(th).astype(np.float32)
+    n = 3000
+    t = gpu
+    samples, fin, mlog = run_gpu(t, A, b, np.zeros((2, d)), n, 1, move_log_steps=n, pair_prob=0.5, pair_mode=2,
+                                 pair_i=t.from_numpy(pi), pair_j=t.from_numpy(pj), pair_c=t.from_numpy(pc),
+                                 pair_s=t.from_numpy(ps))
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((2, d), np.float32), n, 1, nlog=n, pair_prob=0.5,
+                             pair_mode=2, pair=(pi, pj, pc, ps))
+    np.testing.assert_array_equal(mlog, ref["move_log"])
+    rel, _ = report("pair-mode-2", samples, ref["samples"])
+    assert rel <= 1e-5
+
+
+TRAJ_CASES = {
+    # name: (polytope builder, tpb, extra kwargs)
+    "rand-w1": (lambda: random_polytope(10, 30, seed=21), 64, dict(pair_prob=0.3)),
+    "rand-w4": (lambda: random_polytope(24, 300, seed=22), 256, dict(pair_prob=0.3)),
+    "rand-w16": (lambda: random_polytope(40, 2100, seed=23), 1024, dict(pair_prob=0.3)),
+    "rand-auto-resync": (lambda: random_polytope(16, 200, seed=24), -1, dict(pair_prob=0.3, resync_interval=50)),
+    "ksparse": (lambda: random_polytope(12, 60, seed=25), -1, dict(ksparse_prob=0.2, ksparse_k=6)),
+    "slab": (lambda: thin_slab(10, 1e-4), -1, dict(pair_prob=0.3)),
+    "simplex": (lambda: simplex(6), -1, dict()),
+}
+
+
+@pytest.mark.parametrize("case", list(TRAJ_CASES))
+def test_trajectory_matches_oracle(gpu, case):
+    """Indices are bit-exact by construction; states agree within 1e-5 relative
+    over 1e4 steps (the kernel repeats the oracle's FP32 operations, so in
+    practice they agree bit for bit)."""
+    build, tpb, kw = TRAJ_CASES[case]
+    A, b, y0 = build()
+    d = A.shape[1]
+    nch = 3
+    Y0 = np.tile(y0, (nch, 1))
+    n = 10000
+    samples, fin = run_gpu(gpu, A, b, Y0, n, 1, tpb=tpb, **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), n, 1, **kw)
+    rel, exact = report(case, samples, ref["samples"])
+    assert rel <= 1e-5
+    np.testing.assert_array_equal(fin, ref["y_final"]) if exact == 1.0 else None
+
+
+def test_chains_per_group_matches_oracle(gpu):
+    A, b, y0 = random_polytope(12, 70, seed=31)
+    nch = 6  # one full group of 4 and a partial group of 2
+    Y0 = np.zeros((nch, 12))
+    samples, _ = run_gpu(gpu, A, b, Y0, 3000, 2, pair_prob=0.3, chains_per_group=4)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), 3000, 2, pair_prob=0.3,
+                             chains_per_group=4)
+    rel, _ = report("G4", samples, ref["samples"])
+    assert rel <= 1e-5
+
+
+def test_batched_variants_match_oracle(gpu):
+    import naja_amd
+
+    t = gpu
+    d = 10
+    A, b, _ = cube(d)
+    rng = np.random.default_rng(8)
+    ks = [0, 3, 17, 40, 64, 1]
+    EA, Eb = [], []
+    for k in ks:
+        rows = rng.standard_normal((k, d))
+        rows /= np.maximum(np.linalg.norm(rows, axis=1, keepdims=True), 1e-12)
+        EA.append(rows)
+        Eb.append(10.0 ** rng.uniform(-2, 0, k))
+    nch, nspc, thin = 4, 800, 3
+    X0 = np.zeros((len(ks), nch, d), np.float32)
+    out = naja_amd.coordinate_hit_and_run_batched(
+        t.from_numpy(A.astype(np.float32)), t.from_numpy(b), [t.from_numpy(e.astype(np.float32)) for e in EA],
+        [t.from_numpy(f) for f in Eb], t.from_numpy(X0), nspc, thin, nch, seed=1234, pair_prob=0.3,
+        resync_interval=100)
+    t.cuda.synchronize()
+    out = out.cpu().numpy()
+    for v, k in enumerate(ks):
+        Av = np.vstack([A, EA[v]]).astype(np.float32)
+        bv = np.concatenate([b, Eb[v]])
+        ref = oracle.chr_run_f32(Av, bv, X0[v], nspc, thin, seed=1234, variant=v, pair_prob=0.3,
+                                 resync_interval=100)
+        rel, _ = report(f"variant{v}-k{k}", out[v], ref["samples"])
+        assert rel <= 1e-5
+
+
+def test_cube_d100_moments(gpu):
+    """[-1,1]^100, 4 chains x 12500, thin 16, seed 0: mean 0 and variance 1/3
+    per coordinate within 3 Monte Carlo standard errors (rho = 0.99^16)."""
+    d, nch, nspc, thin = 100, 4, 12500, 16
+    A, b, _ = cube(d)
+    samples, _ = run_gpu(gpu, A, b, np.zeros((nch, d)), nspc, thin, seed=0)
+    x = samples.astype(np.float64)
+    N = x.shape[0]
+    rho = (1 - 1 / d) ** thin
+    infl = (1 + rho) / (1 - rho)
+    zm = x.mean(0) / np.sqrt((1 / 3) / N * infl)
+    zv = (x.var(0) - 1 / 3) / np.sqrt((4 / 45) / N * infl)
+    print(f"cube d=100: max|z_mean|={np.abs(zm).max():.2f} max|z_var|={np.abs(zv).max():.2f} "
+          f"frac(|z|<3)={(np.abs(zm) < 3).mean():.3f},{(np.abs(zv) < 3).mean():.3f}", flush=True)
+    # 100 coordinates: expect ~0.3 outside 3 SE by chance; none beyond 4.5 SE
+    assert (np.abs(zm) < 3).mean() >= 0.97 and (np.abs(zv) < 3).mean() >= 0.97
+    assert np.abs(zm).max() < 4.5 and np.abs(zv).max() < 4.5
+    assert np.abs(x).max() <= 1 + 1e-5
+
+
+def test_genome_like_feasible(gpu):
+    g = genome_like(300, 64, seed=5)
+    A, b = g["A"], g["b"]
+    nch = 4
+    samples, _ = run_gpu(gpu, A, b, np.zeros((nch, 64)), 2000, 20, pair_prob=0.3)
+    x = samples.astype(np.float64)
+    viol = (A.astype(np.float32).astype(np.float64) @ x.T - b[:, None]) / np.maximum(1.0, np.abs(b))[:, None]
+    assert viol.max() <= 1e-5, viol.max()
+
+
+def test_invalid_arguments_raise(gpu):
+    import naja_amd
+
+    t = gpu
+    A, b, _ = cube(4)
+    with pytest.raises(naja_amd.NajaError):
+        naja_amd.coordinate_hit_and_run(t.from_numpy(A.astype(np.float32)), t.from_numpy(b),
+                                        t.zeros((2, 4)), 10, 0, 2)
+    with pytest.raises(naja_amd.NajaInvalidArgument):
+        naja_amd.coordinate_hit_and_run(t.from_numpy(A.astype(np.float32)), t.from_numpy(b[:-1]),
+                                        t.zeros((2, 4)), 10, 1, 2)

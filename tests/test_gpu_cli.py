@@ -1,0 +1,105 @@
+"""End-to-end `naja sample run` / `naja validate` on the GPU against the oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.test_cli import NAJA, npy_header_rule, write_csv
+
+pytestmark = pytest.mark.gpu
+
+
+def make_model(tmp_path, name="GEM"):
+    rng = np.random.default_rng(12)
+    d, n = 6, 9
+    T = rng.standard_normal((n, d))
+    shift = rng.standard_normal(n)
+    A = np.vstack([T, -T])
+    m_hi = 10 ** rng.uniform(-1, 1, n)
+    m_lo = 10 ** rng.uniform(-1, 1, n)
+    b = np.concatenate([m_hi, m_lo])
+    md = tmp_path / "models" / name
+    (md / "rounding").mkdir(parents=True)
+    (md / "gem").mkdir()
+    pre = md / "rounding" / f"{name}_rounding_"
+    write_csv(str(pre) + "A.csv", A)
+    write_csv(str(pre) + "b.csv", b[:, None])
+    write_csv(str(pre) + "start.csv", np.zeros((d, 1)))
+    write_csv(str(pre) + "T.csv", T)
+    write_csv(str(pre) + "shift.csv", shift[:, None])
+    write_csv(md / "gem" / "l_bounds.csv", (shift - m_lo)[:, None])
+    write_csv(md / "gem" / "u_bounds.csv", (shift + m_hi)[:, None])
+    (md / "gem" / "reaction_ids.txt").write_text("".join(f"R{i}\n" for i in range(n)))
+    return md, A, b, T, shift
+
+
+def run_dir(out):
+    runs = sorted(os.listdir(out))
+    return out / runs[-1]
+
+
+def test_sample_run_reduced_matches_oracle(gpu, tmp_path):
+    md, A, b, T, shift = make_model(tmp_path)
+    out = tmp_path / "out"
+    r = subprocess.run([NAJA, "sample", "run", "--model-dir", str(md), "--out-root", str(out), "--gpu", "0",
+                        "--n-chains", "3", "--n-samples", "400", "--thinning", "4", "--pair-prob", "0.3",
+                        "--seed", "77", "--write-npy"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    rd = run_dir(out)
+    raw = (rd / "samples.npy").read_bytes()
+    hdr = npy_header_rule("<f4", "1200, 6")
+    assert raw[:len(hdr)] == hdr
+    S = np.frombuffer(raw[len(hdr):], dtype=np.float32).reshape(1200, 6)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((3, 6), np.float32), 400, 4, seed=77,
+                             pair_prob=0.3)
+    assert np.abs(S.astype(np.float64) - ref["samples"]).max() <= 1e-5
+    prof = json.loads((rd / "profile.json").read_text())
+    assert prof["config"]["n_chains"] == 3 and prof["config"]["thinning"] == 4
+    assert prof["performance"]["total_samples"] == 1200
+    assert "THINNING=4" in (rd / "config_used.txt").read_text()
+
+
+def test_sample_run_backmap_filter_and_validate(gpu, tmp_path):
+    md, A, b, T, shift = make_model(tmp_path)
+    out = tmp_path / "out"
+    r = subprocess.run([NAJA, "sample", "run", "--model-dir", str(md), "--out-root", str(out), "--n-chains", "4",
+                        "--n-samples", "500", "--seed", "3", "--write-npy", "--backmap", "--bounds-policy", "filter",
+                        "--write-samples-valid", "--quiet"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rd = run_dir(out)
+    V = np.load(rd / "samples.npy")
+    assert V.shape == (2000, 9) and V.dtype == np.float32
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((4, 6), np.float32), 500, 1, seed=3)
+    Vref = oracle.backmap(T.astype(np.float32).astype(np.float64), shift.astype(np.float32).astype(np.float64),
+                          ref["samples"])
+    assert np.abs(V - Vref).max() <= 1e-4
+    mask = np.load(rd / "valid_mask.npy")
+    assert mask.dtype == np.uint8 and mask.shape == (2000,)
+    rep = json.loads((rd / "bounds_report.json").read_text())
+    assert rep["n_total"] == 2000 and rep["n_valid"] == int(mask.sum())
+    sv = np.load(rd / "samples_valid.npy")
+    assert sv.shape == (9, int(mask.sum())) and sv.dtype == np.float64
+
+    # naja validate on the reduced samples of the same run
+    r2 = subprocess.run([NAJA, "sample", "run", "--model-dir", str(md), "--out-root", str(out), "--n-chains", "4",
+                         "--n-samples", "500", "--seed", "3", "--write-npy", "--quiet"], capture_output=True,
+                        text=True, timeout=120)
+    assert r2.returncode == 0
+    red = run_dir(out) / "samples.npy"
+    v = subprocess.run([NAJA, "validate", "--samples", str(red), "--n-chains", "4", "--model-dir", str(md)],
+                       capture_output=True, text=True, timeout=120)
+    assert v.returncode == 0, v.stderr
+    js = json.loads(v.stdout)
+    S = np.load(red)
+    e_ref, _ = oracle.ess(S, 4)
+    r_ref, a11, a12 = oracle.split_rhat(S, 4)
+    np.testing.assert_allclose(js["per_dim_ess"], np.round(e_ref, 1), rtol=2e-3, atol=0.6)
+    assert js["rhat"]["above_1_1"] == a11 and js["chain_layout"] == "interleaved"
+    assert js["bounds"]["violations"] == 0 and js["chords"]["positive_dims"] == 6
+    vb = subprocess.run([NAJA, "validate", "--samples", str(red), "--n-chains", "4", "--chain-layout",
+                         "naja-blocks"], capture_output=True, text=True, timeout=120)
+    assert vb.returncode == 0 and json.loads(vb.stdout)["chain_layout"] == "naja-blocks"

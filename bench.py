@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nspc", type=int, default=12500)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after timing, copy a buffer of this size (a known-size kernel for rocprof counter scaling)")
     return ap.parse_args()
 
 
@@ -109,6 +111,13 @@ def main():
                 "kernel": "chr_step_kernel", "kernel_ms": round(chr_ms, 3),
                 "bytes_per_chain_step": round(b_cs, 1), "chain_steps": chain_steps,
                 "us_per_step": round(chr_ms * 1e3 / (nspc * thin), 4)}
+
+    if args.copy_probe_mb > 0:
+        nel = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None
     if not args.no_cpu:

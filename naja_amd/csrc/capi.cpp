@@ -8,6 +8,7 @@
 // (helper.h:12-27).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -39,37 +40,6 @@ void clear_error() { g_error[0] = 0; }
 
 namespace {
 
-int round_up(int x, int a) { return (x + a - 1) / a * a; }
-
-// Launch shape: W wavefronts per workgroup, G chains per group, R row slots per lane.
-struct Shape {
-    int W, G, R;
-};
-
-Shape choose_shape(int lda_max, int G, int tpb)
-{
-    int W = chr_pick_waves(lda_max, tpb);
-    int R = -1;
-    if (!chr_shape_available(W, G, lda_max, &R)) {
-        NAJA_REQUIRE(tpb <= 0, NAJA_ERR_RUNTIME,
-                     "Invalid threads per block: tpb=%d cannot hold %d rows with %d chains per group", tpb, lda_max,
-                     G);
-        const int order[] = {1, 4, 8, 16};
-        bool found = false;
-        for (int w : order) {
-            if (w < W) continue;
-            if (chr_shape_available(w, G, lda_max, &R)) {
-                W = w;
-                found = true;
-                break;
-            }
-        }
-        NAJA_REQUIRE(found, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel holds %d rows with %d chains per group",
-                     lda_max, G);
-    }
-    return Shape{W, G, R};
-}
-
 void check_params(const naja_chr_params* p, int d)
 {
     NAJA_REQUIRE(p != nullptr, NAJA_ERR_INVALID_ARGUMENT, "params is NULL");
@@ -79,6 +49,11 @@ void check_params(const naja_chr_params* p, int d)
     NAJA_REQUIRE(p->pair_prob >= 0.0 && p->pair_prob <= 1.0, NAJA_ERR_INVALID_ARGUMENT, "PAIR_PROB must be in [0,1]");
     NAJA_REQUIRE(p->ksparse_prob >= 0.0 && p->ksparse_prob <= 1.0, NAJA_ERR_INVALID_ARGUMENT,
                  "invalid ksparse_prob (must be in [0,1])");
+    // chr.cu:529-551 accepts threads-per-block 32..1024 (powers of two) and
+    // throws runtime_error otherwise; <= 0 lets the library choose.
+    NAJA_REQUIRE(p->tpb <= 0 || p->tpb == 32 || p->tpb == 64 || p->tpb == 128 || p->tpb == 256 || p->tpb == 512 ||
+                     p->tpb == 1024,
+                 NAJA_ERR_RUNTIME, "Invalid threads per block (%d)", p->tpb);
     NAJA_REQUIRE(p->chains_per_group == 0 || p->chains_per_group == 1 || p->chains_per_group == 4,
                  NAJA_ERR_UNSUPPORTED, "chains_per_group must be 1 or 4 (got %d)", p->chains_per_group);
     // Pair mode 2 without a schedule falls back to coordinate moves, as the
@@ -118,32 +93,37 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int 
         check_params(p, d);
         const int G = p->chains_per_group > 0 ? p->chains_per_group : 1;
 
-        std::vector<ChrVariant> var((size_t)n_variants);
-        std::vector<int64_t> eoff((size_t)n_variants);
-        std::vector<int> ek((size_t)n_variants);
-        int64_t a_off = 0, b_off = 0, e_off = 0;
-        int lda_max = 0;
-        const int64_t per_variant_samples = (int64_t)p->nchains * p->nspc * d;
+        int m_max = 0;
         for (int v = 0; v < n_variants; ++v) {
             const int k = extra_rows ? extra_rows[v] : 0;
             NAJA_REQUIRE(k >= 0, NAJA_ERR_INVALID_ARGUMENT, "variant %d has a negative extra row count", v);
             NAJA_REQUIRE(k == 0 || (A_extra && b_extra), NAJA_ERR_INVALID_ARGUMENT,
                          "extra rows given without A_extra / b_extra");
+            m_max = std::max(m_max, m0 + k);
+        }
+        const ChrShape sh = chr_choose_shape(m_max, G, p->tpb);
+        const int lda = sh.rows();
+
+        std::vector<ChrVariant> var((size_t)n_variants);
+        std::vector<int64_t> eoff((size_t)n_variants);
+        std::vector<int> ek((size_t)n_variants);
+        int64_t a_off = 0, b_off = 0, e_off = 0;
+        const int64_t per_variant_samples = (int64_t)p->nchains * p->nspc * d;
+        for (int v = 0; v < n_variants; ++v) {
+            const int k = extra_rows ? extra_rows[v] : 0;
             ChrVariant& V = var[(size_t)v];
             V.m = m0 + k;
-            V.lda = round_up(V.m, 64);
+            V.lda = lda;
             V.a_off = a_off;
             V.b_off = b_off;
             V.y0_col = (int64_t)v * p->nchains;
             V.sample_off = (int64_t)v * per_variant_samples;
             eoff[(size_t)v] = e_off;
             ek[(size_t)v] = k;
-            a_off += (int64_t)V.lda * d;
-            b_off += V.lda;
+            a_off += (int64_t)lda * d;
+            b_off += lda;
             e_off += k;
-            if (V.lda > lda_max) lda_max = V.lda;
         }
-        const Shape sh = choose_shape(lda_max, G, p->tpb);
 
         std::vector<ChrItem> items;
         for (int v = 0; v < n_variants; ++v)
@@ -166,7 +146,7 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int 
         NAJA_HIP(hipMemcpyAsync(eoff_d, eoff.data(), sizeof(int64_t) * eoff.size(), hipMemcpyHostToDevice, stream));
         NAJA_HIP(hipMemcpyAsync(ek_d, ek.data(), sizeof(int) * ek.size(), hipMemcpyHostToDevice, stream));
 
-        chr_pack(A, b, m0, d, A_extra, b_extra, var_d, eoff_d, ek_d, n_variants, A_pack, b_pack, lda_max, stream);
+        chr_pack(A, b, m0, d, A_extra, b_extra, var_d, eoff_d, ek_d, n_variants, A_pack, b_pack, lda, stream);
 
         ChrArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -204,7 +184,7 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int 
         a.nlog = p->move_log ? p->nlog : 0;
         if (a.pair_mode == 2 && a.n_pairs == 0) a.has_pair = 0;
 
-        chr_launch(a, (int)items.size(), sh.W, sh.G, sh.R, stream);
+        chr_launch(a, (int)items.size(), sh, stream);
 
         NAJA_HIP(hipFreeAsync(var_d, stream));
         NAJA_HIP(hipFreeAsync(items_d, stream));

@@ -1,0 +1,23 @@
+// Instantiates CHR step kernel shapes (W wavefronts, G chains, R rows per
+// lane) for the launch table; see chr_kernel.hip.
+#include "chr_kernel_impl.h"
+
+namespace naja {
+
+namespace {
+using chr_detail::chr_step_kernel;
+const ChrKernelEntry kTable[] = {
+    {1, 1, 1, chr_step_kernel<1, 1, 1>},
+    {1, 1, 2, chr_step_kernel<1, 1, 2>},
+    {1, 1, 3, chr_step_kernel<1, 1, 3>},
+    {1, 1, 4, chr_step_kernel<1, 1, 4>},
+    {1, 1, 5, chr_step_kernel<1, 1, 5>},
+    {1, 1, 6, chr_step_kernel<1, 1, 6>},
+    {1, 1, 7, chr_step_kernel<1, 1, 7>},
+    {1, 1, 8, chr_step_kernel<1, 1, 8>},
+};
+}  // namespace
+
+ChrKernelTable chr_table_w1() { return ChrKernelTable{kTable, (int)(sizeof(kTable) / sizeof(kTable[0]))}; }
+
+}  // namespace naja

@@ -7,12 +7,12 @@
 namespace naja {
 
 // One conditioned variant, packed by chr_pack_kernel: rows [A; E_v] in a
-// column-major block with lda = m rounded up to 64; padding rows hold A = 0
-// and b = +inf, so their inverse distance is 0/inf = 0 and they never move
-// the chord of a bounded polytope.
+// column-major block with lda = 64 W R (the launch shape's rows); padding
+// rows hold A = 0 and b = +inf, so their inverse distance is 0/inf = 0 and
+// they never move the chord of a bounded polytope.
 struct ChrVariant {
     int m;               // real rows m0 + k
-    int lda;             // padded rows (multiple of 64)
+    int lda;             // padded rows (64 W R)
     int64_t a_off;       // float offset of the packed block
     int64_t b_off;       // offset of the packed FP64 right-hand side
     int64_t y0_col;      // first column of this variant's chains in y0 / y_final
@@ -65,11 +65,34 @@ inline int chr_effective_resync(int requested, int d)
     return d * 16 > 1024 ? d * 16 : 1024;
 }
 
-// Wavefronts per workgroup for padded rows lda_max (tpb > 0 forces it).
-int chr_pick_waves(int lda_max, int tpb);
-// Smallest compiled rows-per-lane R with 64*W*R >= lda_max; false if none.
-bool chr_shape_available(int W, int G, int lda_max, int* R_out);
-void chr_launch(const ChrArgs& args, int n_items, int W, int G, int R, hipStream_t stream);
+// Launch shape: W wavefronts per workgroup, G chains per group, R row slots
+// per lane; every packed block has exactly 64 W R rows.
+struct ChrShape {
+    int W, G, R;
+    int rows() const { return 64 * W * R; }
+};
+
+using ChrKernelFn = void (*)(ChrArgs);
+struct ChrKernelEntry {
+    int W, G, R;
+    ChrKernelFn fn;
+};
+struct ChrKernelTable {
+    const ChrKernelEntry* e;
+    int n;
+};
+// Tables of compiled shapes, one per chr_inst_*.hip translation unit.
+ChrKernelTable chr_table_w1();
+ChrKernelTable chr_table_w4a();
+ChrKernelTable chr_table_w4b();
+ChrKernelTable chr_table_w8();
+ChrKernelTable chr_table_w16();
+ChrKernelTable chr_table_g4();
+
+// Shape for a batch whose largest variant has m_max rows (tpb > 0 is the
+// reference's --tpb, taken as a preference).
+ChrShape chr_choose_shape(int m_max, int G, int tpb);
+void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream);
 
 // Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks.
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,

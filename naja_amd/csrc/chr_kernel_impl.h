@@ -1,0 +1,469 @@
+// Device code of the CHR step kernel, included by the chr_inst_*.hip
+// translation units (each instantiates a subset of launch shapes, so the
+// shapes compile in parallel). The design notes are in chr_kernel.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "chr_kernel.h"
+#include "philox.h"
+
+namespace naja {
+namespace chr_detail {
+
+constexpr int kMoveCoord = 0;
+constexpr int kMovePair = 1;
+constexpr int kMoveKsparse = 2;
+constexpr int kKmax = 32;
+
+struct Move {
+    int type;
+    int i;
+    int j;
+    float ci;
+    float cj;
+    uint32_t w3;
+};
+
+__device__ __forceinline__ Move draw_move(const ChrArgs& a, uint32_t k0, uint32_t k1, uint64_t t, uint32_t group)
+{
+    const U32x4 w = philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), group, 0u, k0, k1);
+    Move mv;
+    mv.w3 = w.w;
+    mv.type = kMoveCoord;
+    mv.i = (int)rng_index(w.y, (uint32_t)a.d);
+    mv.j = -1;
+    mv.ci = 1.0f;
+    mv.cj = 0.0f;
+    const bool ks = (uint64_t)w.x < a.thr_k;
+    const bool pr = !ks && a.has_pair && (uint64_t)w.x < a.thr_kp;
+    if (ks) {
+        mv.type = kMoveKsparse;
+    } else if (pr && a.d >= 2) {
+        if (a.pair_mode == 1) {
+            const int r = (int)rng_index(w.z, (uint32_t)(a.d - 1));
+            mv.type = kMovePair;
+            mv.j = (r >= mv.i) ? r + 1 : r;
+            mv.ci = 0.70710678118654752440f;
+            mv.cj = -0.70710678118654752440f;
+        } else if (a.pair_mode == 2 && a.n_pairs > 0) {
+            const int p = (int)rng_index(w.z, (uint32_t)a.n_pairs);
+            const float c = a.pair_c[p];
+            const float s = a.pair_s[p];
+            mv.type = kMovePair;
+            mv.i = a.pair_i[p];
+            mv.j = a.pair_j[p];
+            float ci = ((w.w & 1u) == 0u) ? c : -s;
+            float cj = ((w.w & 1u) == 0u) ? s : c;
+            if ((w.w >> 1) & 1u) {
+                ci = -ci;
+                cj = -cj;
+            }
+            mv.ci = ci;
+            mv.cj = cj;
+        }
+    }
+    return mv;
+}
+
+// k-sparse entry q: index from block (t, group, 1 + q/4), sign from bit q of w3.
+__device__ __forceinline__ int ksparse_index(const ChrArgs& a, uint32_t k0, uint32_t k1, uint64_t t,
+                                             uint32_t group, int q)
+{
+    const U32x4 v = philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), group, 1u + (uint32_t)(q >> 2), k0, k1);
+    const uint32_t word = (q & 3) == 0 ? v.x : ((q & 3) == 1 ? v.y : ((q & 3) == 2 ? v.z : v.w));
+    return (int)rng_index(word, (uint32_t)a.d);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// Max and min over the 64 lanes of a wavefront: DPP within each row of 16
+// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
+// then the four row results through readlane.
+__device__ __forceinline__ void wave_maxmin(float& vmax, float& vmin)
+{
+    vmax = fmaxf(vmax, dpp_f32<0xB1>(vmax));
+    vmin = fminf(vmin, dpp_f32<0xB1>(vmin));
+    vmax = fmaxf(vmax, dpp_f32<0x4E>(vmax));
+    vmin = fminf(vmin, dpp_f32<0x4E>(vmin));
+    vmax = fmaxf(vmax, dpp_f32<0x141>(vmax));
+    vmin = fminf(vmin, dpp_f32<0x141>(vmin));
+    vmax = fmaxf(vmax, dpp_f32<0x140>(vmax));
+    vmin = fminf(vmin, dpp_f32<0x140>(vmin));
+    const float a0 = readlane_f32(vmax, 0), a1 = readlane_f32(vmax, 16);
+    const float a2 = readlane_f32(vmax, 32), a3 = readlane_f32(vmax, 48);
+    const float b0 = readlane_f32(vmin, 0), b1 = readlane_f32(vmin, 16);
+    const float b2 = readlane_f32(vmin, 32), b3 = readlane_f32(vmin, 48);
+    vmax = fmaxf(fmaxf(a0, a1), fmaxf(a2, a3));
+    vmin = fminf(fminf(b0, b1), fminf(b2, b3));
+}
+
+// chr.cu:416-426 maps s == 0 to +-inf by the sign of ae (0 if ae == 0) and a
+// NaN quotient to 0. Here a slack that FP32 rounding left at or below zero is
+// replaced by +0 (a point just outside a facet counts as on it; the FP64
+// reference has no such clamp), so IEEE division gives the same +-inf, and
+// 0/0 = NaN is skipped by fmaxf/fminf; on a bounded chord max > 0 > min, so
+// skipping a 0 never changes either. Branch-free: compare, select, divide.
+__device__ __forceinline__ float inverse_distance(float ae, float s)
+{
+    const float sp = (s > 0.0f) ? s : 0.0f;
+    return ae / sp;
+}
+
+// Lane l of every wavefront draws the move and the alpha uniforms of step
+// tb + l (VALU Philox, about three VALU instructions per step amortised);
+// each step then reads its draw with readlane.
+template <int G>
+struct MoveBatch {
+    int tij;     // type << 30 | i
+    int j;       // partner coordinate, or the k-sparse sign word w3
+    float ci;
+    float cj;
+    float u[G];  // alpha uniform of chain c0 + g
+};
+
+template <int G>
+__device__ __forceinline__ void gen_batch(const ChrArgs& a, uint32_t k0, uint32_t k1, uint64_t tb, uint32_t group,
+                                          int c0, MoveBatch<G>& mb)
+{
+    const uint64_t t = tb + (uint64_t)(threadIdx.x & 63);
+    const Move mv = draw_move(a, k0, k1, t, group);
+    mb.tij = (mv.type << 30) | mv.i;
+    mb.j = (mv.type == kMoveKsparse) ? (int)mv.w3 : mv.j;
+    mb.ci = mv.ci;
+    mb.cj = mv.cj;
+    uint32_t ublk[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const uint32_t chain = (uint32_t)(c0 + g);
+        if (g == 0 || (chain & 3u) == 0u) {
+            const U32x4 u = philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), chain >> 2, kAlphaBlock, k0, k1);
+            ublk[0] = u.x;
+            ublk[1] = u.y;
+            ublk[2] = u.z;
+            ublk[3] = u.w;
+        }
+        const uint32_t w = ublk[chain & 3u];
+        mb.u[g] = rng_uniform(w);
+    }
+}
+
+__device__ __forceinline__ int readlane_i32(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+template <int G>
+__device__ __forceinline__ Move read_move(const MoveBatch<G>& mb, int lane)
+{
+    Move mv;
+    const int tij = readlane_i32(mb.tij, lane);
+    mv.type = (int)((uint32_t)tij >> 30);
+    mv.i = tij & 0x3FFFFFFF;
+    const int jv = readlane_i32(mb.j, lane);
+    mv.j = jv;
+    mv.w3 = (uint32_t)jv;
+    mv.ci = readlane_f32(mb.ci, lane);
+    mv.cj = readlane_f32(mb.cj, lane);
+    return mv;
+}
+
+template <int W, int G, int R>
+__global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
+{
+    constexpr int T = 64 * W;
+    extern __shared__ float xs[];  // G x d current states
+    __shared__ float2 part[2][G][W];
+    __shared__ int ks_idx[kKmax];
+    __shared__ float ks_coef[kKmax];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    // readfirstlane makes the wave index provably wave-uniform.
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const ChrItem item = a.items[blockIdx.x];
+    const ChrVariant V = a.variants[item.variant];
+    const int d = a.d;
+    const int lda = V.lda;  // == T * R: every slot of every lane is a real or a padding row
+    const float* Av = a.A + V.a_off + tid;
+    const int geff = min(G, a.nchains - item.c0);
+    const uint32_t group = (uint32_t)(item.c0 / G);
+    const uint32_t k0 = a.seed_lo;
+    const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
+
+    for (int g = 0; g < G; ++g) {
+        for (int c = tid; c < d; c += T) {
+            xs[g * d + c] = (g < geff) ? a.y0[(V.y0_col + item.c0 + g) * (int64_t)d + c] : 0.0f;
+        }
+    }
+
+    // Row r = k*T + tid lives in slot k of lane tid.
+    float s[G][R];
+    float ca[R], cb[R];
+    MoveBatch<G> mb;
+    gen_batch<G>(a, k0, k1, 0, group, item.c0, mb);
+    Move mv = read_move<G>(mb, 0);
+    {
+        const float* pi = Av + (int64_t)mv.i * lda;
+        const float* pj = Av + (int64_t)(mv.type == kMovePair ? mv.j : mv.i) * lda;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            ca[k] = pi[k * T];
+            cb[k] = pj[k * T];
+        }
+    }
+
+    float last_i[G], last_j[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        last_i[g] = 0.0f;
+        last_j[g] = 0.0f;
+    }
+    int prev_i = -1, prev_j = -1;
+    int buf = 0;
+    int until_save = 0;    // steps until the next saved sample (t % thin == 0)
+    int until_resync = 0;  // steps until the next FP64 slack computation
+    int64_t sample_idx = 0;
+    const int64_t nsteps = a.nsteps;
+
+    // One CHR step. The two register sets (ca, cb) / (na, nb) alternate
+    // between consecutive steps (the loop below is unrolled by two), so the
+    // prefetched columns are never copied and their loads stay in flight
+    // across the rest of the step.
+    Move nx = mv;
+    auto step = [&](int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+        if (until_resync == 0) {
+            // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
+            __syncthreads();
+            double acc[G][R];
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+#pragma unroll
+                for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
+            const float* pc = Av;
+            for (int c = 0; c < d; ++c) {
+                double xv[G];
+#pragma unroll
+                for (int g = 0; g < G; ++g) xv[g] = (double)xs[g * d + c];
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const double av = (double)pc[k * T];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xv[g], acc[g][k]);
+                }
+                pc += lda;
+            }
+            const double* bv = a.b + V.b_off + tid;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double bb = bv[k * T];
+#pragma unroll
+                for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
+            }
+            prev_i = -1;
+            prev_j = -1;
+            until_resync = a.resync > 0 ? a.resync : -1;
+        }
+
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t < a.nlog && tid == 0) {
+            a.move_log[t * 3 + 0] = mv.type;
+            a.move_log[t * 3 + 1] = mv.i;
+            a.move_log[t * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+        }
+
+        // Alpha uniforms of step t, then the draw of step t+1 (a new batch of
+        // 64 steps when t+1 crosses a batch boundary).
+        const int L = (int)(t & 63);
+        float U[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) U[g] = readlane_f32(mb.u[g], L);
+        nx = mv;
+        const bool has_next = t + 1 < nsteps;
+        if (has_next) {
+            if (L == 63) gen_batch<G>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
+            nx = read_move<G>(mb, (L + 1) & 63);
+        }
+
+        // Prefetch the columns of step t+1 into the other register set
+        // (moves do not depend on state).
+        if (has_next && nx.type != kMoveKsparse) {
+            const float* pi = Av + (int64_t)nx.i * lda;
+#pragma unroll
+            for (int k = 0; k < R; ++k) na[k] = pi[k * T];
+            if (nx.type == kMovePair) {
+                const float* pj = Av + (int64_t)nx.j * lda;
+#pragma unroll
+                for (int k = 0; k < R; ++k) nb[k] = pj[k * T];
+            }
+        }
+
+        // ae = (A u)_r for this lane's rows.
+        float ae[R];
+        const int kq = a.ks_k;
+        if (mv.type == kMoveCoord) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) ae[k] = ca[k];
+        } else if (mv.type == kMovePair) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) ae[k] = fmaf(ca[k], mv.ci, cb[k] * mv.cj);
+        } else {
+            // Rare escape move: entry q's index and coefficient go through LDS.
+            __syncthreads();
+            if (tid < kq) {
+                ks_idx[tid] = ksparse_index(a, k0, k1, (uint64_t)t, group, tid);
+                ks_coef[tid] = ((mv.w3 >> tid) & 1u) ? -a.ks_mag : a.ks_mag;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R; ++k) ae[k] = 0.0f;
+            for (int q = 0; q < kq; ++q) {
+                const float* pq = Av + (int64_t)ks_idx[q] * lda;
+                const float cq = ks_coef[q];
+#pragma unroll
+                for (int k = 0; k < R; ++k) ae[k] = fmaf(pq[k * T], cq, ae[k]);
+            }
+        }
+
+        // Chord: max / min of inverse distances per chain.
+        float rmax[G], rmin[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            float vmax = -3.402823466e+38f, vmin = 3.402823466e+38f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const float q = inverse_distance(ae[k], s[g][k]);
+                vmax = fmaxf(vmax, q);
+                vmin = fminf(vmin, q);
+            }
+            wave_maxmin(vmax, vmin);
+            rmax[g] = vmax;
+            rmin[g] = vmin;
+            if (W > 1 && lane == 0) part[buf][g][wave] = make_float2(vmax, vmin);
+        }
+
+        // Pre-update coordinates (registers carry the previous step's update
+        // because the owner's LDS write of it is not yet ordered).
+        float oi[G], oj[G];
+        if (mv.type != kMoveKsparse) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                oi[g] = (mv.i == prev_j) ? last_j[g] : ((mv.i == prev_i) ? last_i[g] : xs[g * d + mv.i]);
+                oj[g] = 0.0f;
+                if (mv.type == kMovePair)
+                    oj[g] = (mv.j == prev_j) ? last_j[g] : ((mv.j == prev_i) ? last_i[g] : xs[g * d + mv.j]);
+            }
+        }
+
+        if (W > 1) __syncthreads();
+
+        float alpha[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            float vmax = rmax[g], vmin = rmin[g];
+            if (W > 1) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    const float2 p = part[buf][g][w];
+                    vmax = fmaxf(vmax, p.x);
+                    vmin = fminf(vmin, p.y);
+                }
+            }
+            const float lo = 1.0f / vmin;
+            const float hi = 1.0f / vmax;
+            alpha[g] = fmaf(U[g], hi - lo, lo);
+        }
+
+        // Fused rank-1 slack update s -= alpha * ae (chr.cu:464-482).
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const float nal = -alpha[g];
+#pragma unroll
+            for (int k = 0; k < R; ++k) s[g][k] = fmaf(nal, ae[k], s[g][k]);
+        }
+
+        if (mv.type != kMoveKsparse) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const float ni = fmaf(alpha[g], mv.ci, oi[g]);
+                last_i[g] = ni;
+                if (mv.type == kMovePair) {
+                    const float base = (mv.j == mv.i) ? ni : oj[g];
+                    last_j[g] = fmaf(alpha[g], mv.cj, base);
+                }
+            }
+            prev_i = mv.i;
+            prev_j = (mv.type == kMovePair) ? mv.j : -1;
+            if (tid == 0) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    xs[g * d + mv.i] = last_i[g];
+                    if (mv.type == kMovePair) xs[g * d + mv.j] = last_j[g];
+                }
+            }
+        } else {
+            __syncthreads();
+            if (tid == 0) {
+#pragma unroll
+                for (int g = 0; g < G; ++g)
+                    for (int q = 0; q < kq; ++q)
+                        xs[g * d + ks_idx[q]] = fmaf(alpha[g], ks_coef[q], xs[g * d + ks_idx[q]]);
+            }
+            __syncthreads();
+            prev_i = -1;
+            prev_j = -1;
+        }
+
+        if (until_save == 0) {
+            const int64_t col = sample_idx * a.nchains + item.c0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (g < geff) {
+                    float* dst = a.samples + V.sample_off + (col + g) * (int64_t)d;
+                    for (int c = tid; c < d; c += T) {
+                        float v = xs[g * d + c];
+                        if (c == prev_i) v = last_i[g];
+                        if (c == prev_j) v = last_j[g];
+                        dst[c] = v;
+                    }
+                }
+            }
+            ++sample_idx;
+            until_save = a.thin;
+        }
+        --until_save;
+        if (until_resync > 0) --until_resync;
+
+    mv = nx;
+    buf ^= 1;
+    };
+    float na[R], nb[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        na[k] = 0.0f;
+        nb[k] = 0.0f;
+    }
+    int64_t t = 0;
+    for (; t + 1 < nsteps; t += 2) {
+        step(t, ca, cb, na, nb);
+        step(t + 1, na, nb, ca, cb);
+    }
+    if (t < nsteps) step(t, ca, cb, na, nb);
+
+    __syncthreads();
+    if (a.y_final) {
+        for (int g = 0; g < geff; ++g)
+            for (int c = tid; c < d; c += T) a.y_final[(V.y0_col + item.c0 + g) * (int64_t)d + c] = xs[g * d + c];
+    }
+}
+
+}  // namespace chr_detail
+}  // namespace naja

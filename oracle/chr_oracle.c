@@ -9,8 +9,8 @@
  *   1. draw the move (chr.cu:285-374): k-sparse / pair / coordinate by one
  *      uniform, then the coordinate i, then the partner j or pair index;
  *   2. optional resync s = b - A x every resync_interval steps (chr.cu:386-395);
- *   3. per row ae = (A u)_r, inverse distance ae/s with s<=0 -> +-inf/0 and
- *      NaN -> 0, reduced to max and min (chr.cu:397-434);
+ *   3. per row ae = (A u)_r, inverse distance ae/s (s == 0 -> +-inf), reduced
+ *      to max and min (chr.cu:397-434);
  *   4. alpha = 1/min + U (1/max - 1/min), x += alpha u (chr.cu:435-453);
  *   5. save x when t % thin == 0 into column (t/thin)*nchains + chain
  *      (chr.cu:283-284, chr.cu:458-461);
@@ -24,10 +24,12 @@
  *     (the reference computes the initial slack by DGEMM, chr.cu:34-40);
  *     resync_interval 0 selects an automatic window (orc_effective_resync);
  *   - alpha = fmaf(U, 1/max - 1/min, 1/min);
- *   - a slack s <= 0 takes the reference's s == 0 branch (+-inf by the sign
- *     of ae): a point that rounding left just outside a facet is treated as
- *     on it, so the chord cannot run through the violated facet. The
- *     reference (FP64, chr.cu:416-426) has no clamp.
+ *   - a slack s <= 0 is replaced by +0 before the IEEE division, which gives
+ *     the reference's s == 0 result (+-inf by the sign of ae): a point that
+ *     rounding left just outside a facet is treated as on it, so the chord
+ *     cannot run through the violated facet (the FP64 reference has no
+ *     clamp); 0/0 = NaN is skipped by fmaxf/fminf where the reference maps
+ *     it to 0, which cannot change max > 0 > min of a bounded chord.
  * The FP64 variant (orc_chr_run_f64) keeps every quantity in double on the
  * same random stream; it measures FP32 drift.
  */
@@ -207,14 +209,8 @@ static void* orc_chain_worker32(void* arg)
                     for (int q = 0; q < mv.k; ++q) a = fmaf(J->A[(size_t)mv.idx[q] * m + r], mv.coef[q], a);
                 }
                 ae[r] = a;
-                const float sr = s[r];
-                float inv;
-                if (sr <= 0.0f) {
-                    inv = (a > 0.0f) ? INFINITY : ((a < 0.0f) ? -INFINITY : 0.0f);
-                } else {
-                    inv = a / sr;
-                    if (isnan(inv)) inv = 0.0f;
-                }
+                const float sp = (s[r] > 0.0f) ? s[r] : 0.0f;
+                const float inv = a / sp;
                 vmax = fmaxf(vmax, inv);
                 vmin = fminf(vmin, inv);
             }
@@ -290,9 +286,7 @@ static void* orc_chain_worker64(void* arg)
                     for (int q = 0; q < mv.k; ++q) a = fma(J->A[(size_t)mv.idx[q] * m + r], mv.coef64[q], a);
                 }
                 ae[r] = a;
-                double inv;
-                if (s[r] <= 0.0) inv = (a > 0.0) ? INFINITY : ((a < 0.0) ? -INFINITY : 0.0);
-                else { inv = a / s[r]; if (isnan(inv)) inv = 0.0; }
+                const double inv = a / ((s[r] > 0.0) ? s[r] : 0.0);
                 vmax = fmax(vmax, inv);
                 vmin = fmin(vmin, inv);
             }

@@ -134,23 +134,26 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int 
         int64_t* eoff_d = nullptr;
         int* ek_d = nullptr;
         float* A_pack = nullptr;
+        float* Ainv_pack = nullptr;
         double* b_pack = nullptr;
         NAJA_HIP(hipMallocAsync((void**)&var_d, sizeof(ChrVariant) * var.size(), stream));
         NAJA_HIP(hipMallocAsync((void**)&items_d, sizeof(ChrItem) * items.size(), stream));
         NAJA_HIP(hipMallocAsync((void**)&eoff_d, sizeof(int64_t) * eoff.size(), stream));
         NAJA_HIP(hipMallocAsync((void**)&ek_d, sizeof(int) * ek.size(), stream));
         NAJA_HIP(hipMallocAsync((void**)&A_pack, sizeof(float) * (size_t)a_off, stream));
+        NAJA_HIP(hipMallocAsync((void**)&Ainv_pack, sizeof(float) * (size_t)a_off, stream));
         NAJA_HIP(hipMallocAsync((void**)&b_pack, sizeof(double) * (size_t)b_off, stream));
         NAJA_HIP(hipMemcpyAsync(var_d, var.data(), sizeof(ChrVariant) * var.size(), hipMemcpyHostToDevice, stream));
         NAJA_HIP(hipMemcpyAsync(items_d, items.data(), sizeof(ChrItem) * items.size(), hipMemcpyHostToDevice, stream));
         NAJA_HIP(hipMemcpyAsync(eoff_d, eoff.data(), sizeof(int64_t) * eoff.size(), hipMemcpyHostToDevice, stream));
         NAJA_HIP(hipMemcpyAsync(ek_d, ek.data(), sizeof(int) * ek.size(), hipMemcpyHostToDevice, stream));
 
-        chr_pack(A, b, m0, d, A_extra, b_extra, var_d, eoff_d, ek_d, n_variants, A_pack, b_pack, lda, stream);
+        chr_pack(A, b, m0, d, A_extra, b_extra, var_d, eoff_d, ek_d, n_variants, A_pack, Ainv_pack, b_pack, lda, stream);
 
         ChrArgs a;
         std::memset(&a, 0, sizeof(a));
         a.A = A_pack;
+        a.Ainv = Ainv_pack;
         a.b = b_pack;
         a.d = d;
         a.variants = var_d;
@@ -191,6 +194,7 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int 
         NAJA_HIP(hipFreeAsync(eoff_d, stream));
         NAJA_HIP(hipFreeAsync(ek_d, stream));
         NAJA_HIP(hipFreeAsync(A_pack, stream));
+        NAJA_HIP(hipFreeAsync(Ainv_pack, stream));
         NAJA_HIP(hipFreeAsync(b_pack, stream));
     });
 }

@@ -26,7 +26,8 @@ struct ChrItem {
 };
 
 struct ChrArgs {
-    const float* A;   // packed variant blocks
+    const float* A;     // packed variant blocks
+    const float* Ainv;  // RN(1 / A) of the packed blocks (coordinate-move chords)
     const double* b;  // packed right-hand sides
     int d;
     const ChrVariant* variants;
@@ -97,6 +98,6 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 // Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks.
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
-              float* A_pack, double* b_pack, int lda_max, hipStream_t stream);
+              float* A_pack, float* Ainv_pack, double* b_pack, int lda_max, hipStream_t stream);
 
 }  // namespace naja

@@ -15,10 +15,12 @@
 //     step per workgroup and broadcast from registers to all G chains; the
 //     move for step t+1 depends only on the Philox counter, so its columns are
 //     prefetched while step t computes;
-//   * the chord's max/min of inverse distances is a DPP wavefront reduction
-//     plus one LDS exchange and one barrier per step (double-buffered
-//     partials), instead of the reference's two CUB block reductions and
-//     three barriers;
+//   * the chord ends (direct form q = s/ae; the least non-negative and the
+//     greatest negative q are an unsigned and a signed integer minimum over
+//     the float bits) are DPP wavefront reductions plus one LDS exchange and
+//     one barrier per step (double-buffered partials), instead of the
+//     reference's two CUB block reductions, three barriers and two final
+//     reciprocals;
 //   * x lives in LDS; the lane-0 owner applies the 1-2 coordinate update, and
 //     every lane carries the last update in registers so no second barrier is
 //     needed before the next read;
@@ -41,10 +43,11 @@ namespace naja {
 
 namespace {
 
-// Packs variant v: rows [A; E_v; 0...] with lda = V.lda, b = [b; f_v; +inf...].
+// Packs variant v: rows [A; E_v; 0...] with lda = V.lda, b = [b; f_v; +inf...],
+// and the elementwise IEEE reciprocal of the packed block.
 __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, const float* A_extra,
                                 const double* b_extra, const ChrVariant* variants, const int64_t* extra_off,
-                                const int* extra_k, float* A_pack, double* b_pack)
+                                const int* extra_k, float* A_pack, float* Ainv_pack, double* b_pack)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
@@ -58,6 +61,7 @@ __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, 
         if (r < m0) val = A[(int64_t)c * m0 + r];
         else if (r < m0 + kv) val = A_extra[eoff * d + (int64_t)c * kv + (r - m0)];
         A_pack[V.a_off + e] = val;
+        Ainv_pack[V.a_off + e] = 1.0f / val;
     }
     if (blockIdx.x == 0) {
         for (int r = threadIdx.x; r < V.lda; r += blockDim.x) {
@@ -145,13 +149,13 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
-              float* A_pack, double* b_pack, int lda_max, hipStream_t stream)
+              float* A_pack, float* Ainv_pack, double* b_pack, int lda_max, hipStream_t stream)
 {
     const int64_t per = (int64_t)lda_max * d;
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
     hipLaunchKernelGGL(chr_pack_kernel, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra, b_extra,
-                       variants_dev, extra_off_dev, extra_k_dev, A_pack, b_pack);
+                       variants_dev, extra_off_dev, extra_k_dev, A_pack, Ainv_pack, b_pack);
     NAJA_HIP(hipGetLastError());
 }
 

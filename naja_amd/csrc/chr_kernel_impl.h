@@ -89,37 +89,58 @@ __device__ __forceinline__ float readlane_f32(float v, int lane)
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
-// Max and min over the 64 lanes of a wavefront: DPP within each row of 16
-// (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror),
-// then the four row results through readlane.
-__device__ __forceinline__ void wave_maxmin(float& vmax, float& vmin)
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 {
-    vmax = fmaxf(vmax, dpp_f32<0xB1>(vmax));
-    vmin = fminf(vmin, dpp_f32<0xB1>(vmin));
-    vmax = fmaxf(vmax, dpp_f32<0x4E>(vmax));
-    vmin = fminf(vmin, dpp_f32<0x4E>(vmin));
-    vmax = fmaxf(vmax, dpp_f32<0x141>(vmax));
-    vmin = fminf(vmin, dpp_f32<0x141>(vmin));
-    vmax = fmaxf(vmax, dpp_f32<0x140>(vmax));
-    vmin = fminf(vmin, dpp_f32<0x140>(vmin));
-    const float a0 = readlane_f32(vmax, 0), a1 = readlane_f32(vmax, 16);
-    const float a2 = readlane_f32(vmax, 32), a3 = readlane_f32(vmax, 48);
-    const float b0 = readlane_f32(vmin, 0), b1 = readlane_f32(vmin, 16);
-    const float b2 = readlane_f32(vmin, 32), b3 = readlane_f32(vmin, 48);
-    vmax = fmaxf(fmaxf(a0, a1), fmaxf(a2, a3));
-    vmin = fminf(fminf(b0, b1), fminf(b2, b3));
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
 }
 
-// chr.cu:416-426 maps s == 0 to +-inf by the sign of ae (0 if ae == 0) and a
-// NaN quotient to 0. Here a slack that FP32 rounding left at or below zero is
-// replaced by +0 (a point just outside a facet counts as on it; the FP64
-// reference has no such clamp), so IEEE division gives the same +-inf, and
-// 0/0 = NaN is skipped by fmaxf/fminf; on a bounded chord max > 0 > min, so
-// skipping a 0 never changes either. Branch-free: compare, select, divide.
-__device__ __forceinline__ float inverse_distance(float ae, float s)
+__device__ __forceinline__ uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int32_t smin32(int32_t a, int32_t b) { return a < b ? a : b; }
+
+// Unsigned and signed minimum over the 64 lanes of a wavefront: DPP within
+// each row of 16 (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror), then the four row results through readlane.
+__device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
+{
+    hb = umin32(hb, dpp_u32<0xB1>(hb));
+    lb = smin32(lb, (int32_t)dpp_u32<0xB1>((uint32_t)lb));
+    hb = umin32(hb, dpp_u32<0x4E>(hb));
+    lb = smin32(lb, (int32_t)dpp_u32<0x4E>((uint32_t)lb));
+    hb = umin32(hb, dpp_u32<0x141>(hb));
+    lb = smin32(lb, (int32_t)dpp_u32<0x141>((uint32_t)lb));
+    hb = umin32(hb, dpp_u32<0x140>(hb));
+    lb = smin32(lb, (int32_t)dpp_u32<0x140>((uint32_t)lb));
+    const uint32_t h0 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0), h1 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 16);
+    const uint32_t h2 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 32), h3 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 48);
+    const int32_t l0 = __builtin_amdgcn_readlane(lb, 0), l1 = __builtin_amdgcn_readlane(lb, 16);
+    const int32_t l2 = __builtin_amdgcn_readlane(lb, 32), l3 = __builtin_amdgcn_readlane(lb, 48);
+    hb = umin32(umin32(h0, h1), umin32(h2, h3));
+    lb = smin32(smin32(l0, l1), smin32(l2, l3));
+}
+
+// Chord ends in the direct form q = s / ae:
+// hi = the least q with the sign bit clear and lo = the greatest q with the
+// sign bit set. For IEEE floats these are an unsigned and a signed integer
+// minimum over the bit patterns, so no branch, select or final reciprocal is
+// needed: ae = 0 gives +-inf and 0/0 a NaN, which never win against a real
+// facet. In exact arithmetic hi = 1/max(ae/s) and lo = 1/min(ae/s), the
+// reference's ends (chr.cu:416-438), each rounded once here instead of twice.
+// A slack that FP32 rounding left at or below zero is replaced by +0, so a
+// point just outside a facet is treated as on it (the FP64 reference has no
+// such clamp).
+__device__ __forceinline__ uint32_t chord_bits(float ae, float s)
 {
     const float sp = (s > 0.0f) ? s : 0.0f;
-    return ae / sp;
+    return __builtin_bit_cast(uint32_t, sp / ae);
+}
+
+// Coordinate moves use the packed reciprocal column rinv = RN(1/A[r,i]):
+// q = RN(s * rinv), one multiply instead of an IEEE division.
+__device__ __forceinline__ uint32_t chord_bits_rcp(float rinv, float s)
+{
+    const float sp = (s > 0.0f) ? s : 0.0f;
+    return __builtin_bit_cast(uint32_t, sp * rinv);
 }
 
 // Lane l of every wavefront draws the move and the alpha uniforms of step
@@ -182,7 +203,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 {
     constexpr int T = 64 * W;
     extern __shared__ float xs[];  // G x d current states
-    __shared__ float2 part[2][G][W];
+    __shared__ uint2 part[2][G][W];
     __shared__ int ks_idx[kKmax];
     __shared__ float ks_coef[kKmax];
 
@@ -195,6 +216,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const int d = a.d;
     const int lda = V.lda;  // == T * R: every slot of every lane is a real or a padding row
     const float* Av = a.A + V.a_off + tid;
+    const float* Ainv = a.Ainv + V.a_off + tid;
     const int geff = min(G, a.nchains - item.c0);
     const uint32_t group = (uint32_t)(item.c0 / G);
     const uint32_t k0 = a.seed_lo;
@@ -213,8 +235,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     gen_batch<G>(a, k0, k1, 0, group, item.c0, mb);
     Move mv = read_move<G>(mb, 0);
     {
-        const float* pi = Av + (int64_t)mv.i * lda;
-        const float* pj = Av + (int64_t)(mv.type == kMovePair ? mv.j : mv.i) * lda;
+        // (ca, cb) = (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j])
+        // for a pair move; a k-sparse move loads a dummy pair.
+        const float* pi = Av + (int64_t)(mv.type == kMoveKsparse ? 0 : mv.i) * lda;
+        const float* pj = (mv.type == kMovePair) ? Av + (int64_t)mv.j * lda
+                                                 : Ainv + (int64_t)(mv.type == kMoveKsparse ? 0 : mv.i) * lda;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             ca[k] = pi[k * T];
@@ -295,14 +320,17 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 
         // Prefetch the columns of step t+1 into the other register set
         // (moves do not depend on state).
-        if (has_next && nx.type != kMoveKsparse) {
-            const float* pi = Av + (int64_t)nx.i * lda;
+        // Every step issues exactly 2R loads, so the in-order vmcnt wait for
+        // this step's columns never waits on the prefetch of the next.
+        {
+            const bool real = has_next && nx.type != kMoveKsparse;
+            const bool pair = real && nx.type == kMovePair;
+            const float* pi = Av + (int64_t)(real ? nx.i : 0) * lda;
+            const float* pj = pair ? Av + (int64_t)nx.j * lda : Ainv + (int64_t)(real ? nx.i : 0) * lda;
 #pragma unroll
-            for (int k = 0; k < R; ++k) na[k] = pi[k * T];
-            if (nx.type == kMovePair) {
-                const float* pj = Av + (int64_t)nx.j * lda;
-#pragma unroll
-                for (int k = 0; k < R; ++k) nb[k] = pj[k * T];
+            for (int k = 0; k < R; ++k) {
+                na[k] = pi[k * T];
+                nb[k] = pj[k * T];
             }
         }
 
@@ -333,21 +361,32 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
         }
 
-        // Chord: max / min of inverse distances per chain.
-        float rmax[G], rmin[G];
+        // Chord ends per chain.
+        uint32_t rhi[G];
+        int32_t rlo[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            float vmax = -3.402823466e+38f, vmin = 3.402823466e+38f;
+            uint32_t hb = 0xFFFFFFFFu;
+            int32_t lb = 0x7FFFFFFF;
+            if (mv.type == kMoveCoord) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const float q = inverse_distance(ae[k], s[g][k]);
-                vmax = fmaxf(vmax, q);
-                vmin = fminf(vmin, q);
+                for (int k = 0; k < R; ++k) {
+                    const uint32_t qb = chord_bits_rcp(cb[k], s[g][k]);
+                    hb = umin32(hb, qb);
+                    lb = smin32(lb, (int32_t)qb);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const uint32_t qb = chord_bits(ae[k], s[g][k]);
+                    hb = umin32(hb, qb);
+                    lb = smin32(lb, (int32_t)qb);
+                }
             }
-            wave_maxmin(vmax, vmin);
-            rmax[g] = vmax;
-            rmin[g] = vmin;
-            if (W > 1 && lane == 0) part[buf][g][wave] = make_float2(vmax, vmin);
+            wave_min2(hb, lb);
+            rhi[g] = hb;
+            rlo[g] = lb;
+            if (W > 1 && lane == 0) part[buf][g][wave] = make_uint2(hb, (uint32_t)lb);
         }
 
         // Pre-update coordinates (registers carry the previous step's update
@@ -368,17 +407,18 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         float alpha[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            float vmax = rmax[g], vmin = rmin[g];
+            uint32_t hb = rhi[g];
+            int32_t lb = rlo[g];
             if (W > 1) {
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
-                    const float2 p = part[buf][g][w];
-                    vmax = fmaxf(vmax, p.x);
-                    vmin = fminf(vmin, p.y);
+                    const uint2 p = part[buf][g][w];
+                    hb = umin32(hb, p.x);
+                    lb = smin32(lb, (int32_t)p.y);
                 }
             }
-            const float lo = 1.0f / vmin;
-            const float hi = 1.0f / vmax;
+            const float hi = __builtin_bit_cast(float, hb);
+            const float lo = __builtin_bit_cast(float, lb);
             alpha[g] = fmaf(U[g], hi - lo, lo);
         }
 

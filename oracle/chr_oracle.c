@@ -9,9 +9,9 @@
  *   1. draw the move (chr.cu:285-374): k-sparse / pair / coordinate by one
  *      uniform, then the coordinate i, then the partner j or pair index;
  *   2. optional resync s = b - A x every resync_interval steps (chr.cu:386-395);
- *   3. per row ae = (A u)_r, inverse distance ae/s (s == 0 -> +-inf), reduced
- *      to max and min (chr.cu:397-434);
- *   4. alpha = 1/min + U (1/max - 1/min), x += alpha u (chr.cu:435-453);
+ *   3. per row ae = (A u)_r and the chord ends from the rows' inverse
+ *      distances ae/s (chr.cu:397-434);
+ *   4. alpha uniform on [lo, hi], x += alpha u (chr.cu:435-453);
  *   5. save x when t % thin == 0 into column (t/thin)*nchains + chain
  *      (chr.cu:283-284, chr.cu:458-461);
  *   6. s -= alpha ae (chr.cu:464-482).
@@ -23,13 +23,20 @@
  *     slack is computed in FP64 as s = b - A x at t = 0 and on every resync
  *     (the reference computes the initial slack by DGEMM, chr.cu:34-40);
  *     resync_interval 0 selects an automatic window (orc_effective_resync);
- *   - alpha = fmaf(U, 1/max - 1/min, 1/min);
- *   - a slack s <= 0 is replaced by +0 before the IEEE division, which gives
- *     the reference's s == 0 result (+-inf by the sign of ae): a point that
- *     rounding left just outside a facet is treated as on it, so the chord
- *     cannot run through the violated facet (the FP64 reference has no
- *     clamp); 0/0 = NaN is skipped by fmaxf/fminf where the reference maps
- *     it to 0, which cannot change max > 0 > min of a bounded chord.
+ *   - the chord is computed in the direct form q_r = s_r / ae_r (IEEE
+ *     division; for a coordinate move q_r = RN(s_r * RN(1/A[r,i])), the
+ *     reciprocal being a fixed property of A): hi = the least q with the
+ *     sign bit clear, lo = the
+ *     greatest q with the sign bit set, found as an unsigned and a signed
+ *     integer minimum over the float bit patterns; alpha =
+ *     fmaf(U, hi - lo, lo). In exact arithmetic hi = 1/max(ae/s) and
+ *     lo = 1/min(ae/s), the reference's ends; here each end is one rounding
+ *     of the exact value instead of two. ae = 0 gives +-inf and 0/0 NaN,
+ *     neither of which can win a minimum against a real facet;
+ *   - a slack s <= 0 is replaced by +0 first: a point that FP32 rounding
+ *     left just outside a facet is treated as on it (q = +-0 pins that end
+ *     of the chord at the current point), so the chain cannot run through
+ *     the violated facet. The FP64 reference has no clamp.
  * The FP64 variant (orc_chr_run_f64) keeps every quantity in double on the
  * same random stream; it measures FP32 drift.
  */
@@ -195,7 +202,8 @@ static void* orc_chain_worker32(void* arg)
             }
             if (t == 0 || (P->resync_interval > 0 && t % P->resync_interval == 0))
                 orc_resync32(J->A, J->b, x, s, m, d);
-            float vmax = -FLT_MAX, vmin = FLT_MAX;
+            uint32_t hi_bits = 0xFFFFFFFFu;   /* unsigned min over the bits of q */
+            int32_t lo_bits = 0x7FFFFFFF;      /* signed min over the bits of q */
             const float* Ai = J->A + (size_t)mv.i * m;
             const float* Aj = (mv.type == ORC_MOVE_PAIR) ? J->A + (size_t)mv.j * m : NULL;
             for (int r = 0; r < m; ++r) {
@@ -210,12 +218,16 @@ static void* orc_chain_worker32(void* arg)
                 }
                 ae[r] = a;
                 const float sp = (s[r] > 0.0f) ? s[r] : 0.0f;
-                const float inv = a / sp;
-                vmax = fmaxf(vmax, inv);
-                vmin = fminf(vmin, inv);
+                /* coordinate moves: q = RN(s * RN(1/a)); other moves: RN(s / ae) */
+                const float q = (mv.type == ORC_MOVE_COORD) ? sp * (1.0f / a) : sp / a;
+                uint32_t qb;
+                memcpy(&qb, &q, 4);
+                if (qb < hi_bits) hi_bits = qb;
+                if ((int32_t)qb < lo_bits) lo_bits = (int32_t)qb;
             }
-            const float lo = 1.0f / vmin;
-            const float hi = 1.0f / vmax;
+            float lo, hi;
+            memcpy(&hi, &hi_bits, 4);
+            memcpy(&lo, &lo_bits, 4);
             const float U = orc_alpha_uniform(P, (uint64_t)t, (uint32_t)c);
             const float alpha = fmaf(U, hi - lo, lo);
             if (mv.type == ORC_MOVE_COORD) {
@@ -274,7 +286,7 @@ static void* orc_chain_worker64(void* arg)
                     s[r] = J->b[r] - acc;
                 }
             }
-            double vmax = -DBL_MAX, vmin = DBL_MAX;
+            double hi = INFINITY, lo = -INFINITY;
             for (int r = 0; r < m; ++r) {
                 double a;
                 if (mv.type == ORC_MOVE_COORD) {
@@ -286,11 +298,10 @@ static void* orc_chain_worker64(void* arg)
                     for (int q = 0; q < mv.k; ++q) a = fma(J->A[(size_t)mv.idx[q] * m + r], mv.coef64[q], a);
                 }
                 ae[r] = a;
-                const double inv = a / ((s[r] > 0.0) ? s[r] : 0.0);
-                vmax = fmax(vmax, inv);
-                vmin = fmin(vmin, inv);
+                const double q = ((s[r] > 0.0) ? s[r] : 0.0) / a;
+                if (!signbit(q)) { if (q < hi) hi = q; }
+                else if (q > lo) lo = q;
             }
-            const double lo = 1.0 / vmin, hi = 1.0 / vmax;
             const double U = (double)orc_alpha_uniform(P, (uint64_t)t, (uint32_t)c);
             const double alpha = fma(U, hi - lo, lo);
             if (mv.type == ORC_MOVE_COORD) {

@@ -99,6 +99,47 @@ int naja_chr_sample_batched(const float* A, const double* b, int m0, int d,
                             float* samples, float* y_final, void* stream);
 
 /*
+ * Batched variants with an optional rounding transform per variant: variant
+ * v samples z on {z : [A; E_v] (c_v + P_v z) <= [b; f_v]} and returns
+ * y = c_v + P_v z (a fixed linear map keeps the uniform target uniform).
+ *   P : n_variants x d x d row-major (NULL for none), c : n_variants x d
+ *   y0: z-space starts, d x (n_variants*nchains) (z = 0 is y = c_v)
+ *   y_final: final z-space states
+ * No counterpart in the reference (naja reads an already rounded polytope,
+ * runtime_config.h:64-78); this is how the Dikin and learned preconditioners
+ * are applied per conditioned variant.
+ */
+int naja_chr_sample_batched_rounded(const float* A, const double* b, int m0, int d,
+                                    int n_variants, const int* extra_rows,
+                                    const float* A_extra, const double* b_extra,
+                                    const float* P, const float* c,
+                                    const float* y0, const naja_chr_params* p,
+                                    float* samples, float* y_final, void* stream);
+
+/*
+ * Sampling plans: pack (and transform) the polytope blocks once, then launch
+ * any number of times without allocating, so a launch can be captured in a
+ * hipGraph. A plan keeps pointers to A, b, A_extra, b_extra, P and c (the
+ * caller keeps them alive) and allocates and frees its own buffers on the
+ * creation stream. naja_chr_plan_autotune times every compiled launch shape
+ * on about 1024 steps of the real run, keeps the fastest, and remembers it
+ * for later plans of the same size (results do not depend on the shape).
+ * shape_out receives {wavefronts per workgroup, chains per group, rows per lane}.
+ * The reference builds its launch on every call (chr.cu:13-70) and has no
+ * autotuner (launchChrKernel takes --tpb as given, chr.cu:499-553).
+ */
+typedef struct naja_chr_plan naja_chr_plan;
+int naja_chr_plan_create(const float* A, const double* b, int m0, int d,
+                         int n_variants, const int* extra_rows,
+                         const float* A_extra, const double* b_extra,
+                         const float* P, const float* c,
+                         const naja_chr_params* p, void* stream, naja_chr_plan** out);
+int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, int* shape_out);
+int naja_chr_plan_shape(const naja_chr_plan* plan, int* shape_out);
+int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, float* y_final, void* stream);
+int naja_chr_plan_destroy(naja_chr_plan* plan);
+
+/*
  * backmap_into (backmap.cu:21-48): V = T Y + shift, FP32 MFMA.
  *   T     : n x d row-major (reaction rows, as the CSV stores T)
  *   shift : n

@@ -37,6 +37,12 @@ EXPORTED_SYMBOLS = (
     "naja_version",
     "naja_chr_sample",
     "naja_chr_sample_batched",
+    "naja_chr_sample_batched_rounded",
+    "naja_chr_plan_create",
+    "naja_chr_plan_autotune",
+    "naja_chr_plan_shape",
+    "naja_chr_plan_launch",
+    "naja_chr_plan_destroy",
     "naja_backmap",
     "naja_diag",
 )
@@ -96,6 +102,19 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_chr_sample_batched.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp,
                                             ctypes.POINTER(ChrParams), vp, vp, vp]
     lib.naja_chr_sample_batched.restype = i32
+    P = ctypes.POINTER(ChrParams)
+    lib.naja_chr_sample_batched_rounded.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, P, vp, vp, vp]
+    lib.naja_chr_sample_batched_rounded.restype = i32
+    lib.naja_chr_plan_create.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, P, vp, ctypes.POINTER(vp)]
+    lib.naja_chr_plan_create.restype = i32
+    lib.naja_chr_plan_autotune.argtypes = [vp, vp, vp, vp]
+    lib.naja_chr_plan_autotune.restype = i32
+    lib.naja_chr_plan_shape.argtypes = [vp, vp]
+    lib.naja_chr_plan_shape.restype = i32
+    lib.naja_chr_plan_launch.argtypes = [vp, vp, vp, vp, vp]
+    lib.naja_chr_plan_launch.restype = i32
+    lib.naja_chr_plan_destroy.argtypes = [vp]
+    lib.naja_chr_plan_destroy.restype = i32
     lib.naja_backmap.argtypes = [vp, vp, vp, vp, i32, i32, i64, vp]
     lib.naja_backmap.restype = i32
     lib.naja_diag.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
@@ -211,41 +230,113 @@ def coordinate_hit_and_run(A, b, X0, nspc, thinning, nchains, tpb_ss=-1, seed=0,
     return out[0] if len(out) == 1 else tuple(out)
 
 
-def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, nchains, tpb_ss=-1,
-                                   seed=0, pair_prob=0.0, resync_interval=0, ksparse_prob=0.0,
-                                   ksparse_k=8, pair_mode=1, chains_per_group=1, return_final=False,
-                                   stream=None):
-    """All conditioned variants [A; E_v] in one launch.
-
-    extra_A / extra_b: lists (one entry per variant) of (k_v, d) and (k_v,)
-    tensors, k_v >= 0. X0: (n_variants, nchains, d). Returns samples
-    (n_variants, nchains*nspc, d).
-    """
+def _batch_inputs(A, b, extra_A, extra_b):
     torch = _torch()
-    lib = load_library()
     A_cm = _colmajor_f32(A)
     d, m0 = A_cm.shape
     b_d = _dev(b, torch.float64)
-    nv = len(extra_A)
     ks = [int(e.shape[0]) for e in extra_A]
     blocks = [_colmajor_f32(e).reshape(-1) for e in extra_A if e.shape[0] > 0]
     Ae = torch.cat(blocks) if blocks else None
     be = torch.cat([_dev(f, torch.float64).reshape(-1) for f in extra_b if f.numel() > 0]) if blocks else None
+    return A_cm, b_d, d, m0, ks, Ae, be
+
+
+def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, nchains, tpb_ss=-1,
+                                   seed=0, pair_prob=0.0, resync_interval=0, ksparse_prob=0.0,
+                                   ksparse_k=8, pair_mode=1, chains_per_group=1, P=None, c=None,
+                                   return_final=False, stream=None):
+    """All conditioned variants [A; E_v] in one launch.
+
+    extra_A / extra_b: lists (one entry per variant) of (k_v, d) and (k_v,)
+    tensors, k_v >= 0. X0: (n_variants, nchains, d) starts. With a rounding
+    transform P (n_variants, d, d) and c (n_variants, d), variant v samples z
+    with y = c_v + P_v z, X0 holds z-space starts and the samples are returned
+    in y. Returns samples (n_variants, nchains*nspc, d).
+    """
+    torch = _torch()
+    lib = load_library()
+    A_cm, b_d, d, m0, ks, Ae, be = _batch_inputs(A, b, extra_A, extra_b)
+    nv = len(ks)
     X = _dev(X0, torch.float32)
     if tuple(X.shape) != (nv, nchains, d):
         raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"X0 shape {tuple(X.shape)} != ({nv}, {nchains}, {d})")
+    Pd = _dev(P, torch.float32) if P is not None else None
+    cd = _dev(c, torch.float32) if c is not None else None
     samples = torch.empty((nv, nchains * nspc, d), dtype=torch.float32, device=A_cm.device)
     final = torch.empty((nv, nchains, d), dtype=torch.float32, device=A_cm.device) if return_final else None
     karr = (ctypes.c_int * nv)(*ks)
     p = chr_params(nspc, thinning, nchains, tpb_ss, seed, pair_prob, resync_interval, ksparse_prob,
                    ksparse_k, pair_mode, None, chains_per_group)
-    rc = lib.naja_chr_sample_batched(A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr,
-                                     Ae.data_ptr() if Ae is not None else None,
-                                     be.data_ptr() if be is not None else None,
-                                     X.data_ptr(), ctypes.byref(p), samples.data_ptr(),
-                                     final.data_ptr() if final is not None else None, _stream_ptr(stream))
+    rc = lib.naja_chr_sample_batched_rounded(
+        A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr, Ae.data_ptr() if Ae is not None else None,
+        be.data_ptr() if be is not None else None, Pd.data_ptr() if Pd is not None else None,
+        cd.data_ptr() if cd is not None else None, X.data_ptr(), ctypes.byref(p), samples.data_ptr(),
+        final.data_ptr() if final is not None else None, _stream_ptr(stream))
     _check(rc)
     return (samples, final) if return_final else samples
+
+
+class ChrPlan:
+    """A packed batch that launches without allocating (naja_chr_plan_*).
+
+    Keeps device copies of every input alive for the plan's lifetime. launch()
+    is safe to capture in a torch.cuda.CUDAGraph (plain kernel launches on the
+    current stream).
+    """
+
+    def __init__(self, A, b, nspc, thinning, nchains, extra_A=None, extra_b=None, P=None, c=None,
+                 tpb_ss=-1, seed=0, pair_prob=0.0, resync_interval=0, ksparse_prob=0.0, ksparse_k=8,
+                 pair_mode=1, chains_per_group=1, stream=None):
+        torch = _torch()
+        self._lib = load_library()
+        extra_A = extra_A if extra_A is not None else [torch.zeros((0, A.shape[1]))]
+        extra_b = extra_b if extra_b is not None else [torch.zeros(0)]
+        self._A, self._b, self.d, self.m0, ks, self._Ae, self._be = _batch_inputs(A, b, extra_A, extra_b)
+        self.n_variants = len(ks)
+        self.nchains, self.nspc = nchains, nspc
+        self._P = _dev(P, torch.float32) if P is not None else None
+        self._c = _dev(c, torch.float32) if c is not None else None
+        self._karr = (ctypes.c_int * self.n_variants)(*ks)
+        self._params = chr_params(nspc, thinning, nchains, tpb_ss, seed, pair_prob, resync_interval,
+                                  ksparse_prob, ksparse_k, pair_mode, None, chains_per_group)
+        h = ctypes.c_void_p()
+        ptr = lambda t: t.data_ptr() if t is not None else None  # noqa: E731
+        _check(self._lib.naja_chr_plan_create(ptr(self._A), ptr(self._b), self.m0, self.d, self.n_variants,
+                                              self._karr, ptr(self._Ae), ptr(self._be), ptr(self._P),
+                                              ptr(self._c), ctypes.byref(self._params), _stream_ptr(stream),
+                                              ctypes.byref(h)))
+        self._h = h
+
+    def samples_like(self):
+        torch = _torch()
+        return torch.empty((self.n_variants, self.nchains * self.nspc, self.d), dtype=torch.float32,
+                           device=self._A.device)
+
+    def autotune(self, X0, stream=None):
+        shape = (ctypes.c_int * 3)()
+        X = _dev(X0, _torch().float32)
+        _check(self._lib.naja_chr_plan_autotune(self._h, X.data_ptr(), _stream_ptr(stream), shape))
+        return tuple(shape)
+
+    def shape(self):
+        shape = (ctypes.c_int * 3)()
+        _check(self._lib.naja_chr_plan_shape(self._h, shape))
+        return tuple(shape)
+
+    def launch(self, X0, out, final=None, stream=None):
+        _check(self._lib.naja_chr_plan_launch(self._h, X0.data_ptr(), out.data_ptr(),
+                                              final.data_ptr() if final is not None else None,
+                                              _stream_ptr(stream)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.naja_chr_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def backmap(T, shift, Y, stream=None):

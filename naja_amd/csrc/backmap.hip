@@ -29,8 +29,14 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(256) void backmap_mfma_kernel(const float* __restrict__ T, const float* __restrict__ shift,
                                                           const float* __restrict__ Y, float* __restrict__ V, int n,
-                                                          int d, int64_t ns, bool vec4)
+                                                          int d, int64_t ns, bool vec4, int64_t sT, int64_t sShift,
+                                                          int64_t sY, int64_t sV)
 {
+    // Batch item blockIdx.z uses its own T, shift, Y and V (strides in floats).
+    T += sT * blockIdx.z;
+    shift += sShift * blockIdx.z;
+    Y += sY * blockIdx.z;
+    V += sV * blockIdx.z;
     __shared__ float Ys[kBK][kBM + kPad];
     __shared__ float Ts[kBK][kBN + kPad];
 
@@ -124,18 +130,26 @@ __global__ __launch_bounds__(256) void backmap_mfma_kernel(const float* __restri
 
 }  // namespace
 
+void backmap_launch_batched(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
+                            int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
+{
+    if (ns == 0 || batch == 0) return;
+    const int64_t gx = (ns + kBM - 1) / kBM;
+    NAJA_REQUIRE(gx <= 0x7FFFFFFF, NAJA_ERR_UNSUPPORTED, "too many sample rows for one backmap launch");
+    NAJA_REQUIRE(batch <= 65535, NAJA_ERR_UNSUPPORTED, "too many backmap batch items (%d)", batch);
+    dim3 grid((unsigned)gx, (unsigned)((n + kBN - 1) / kBN), (unsigned)batch);
+    // 16-byte loads need d % 4 == 0 and 16-byte aligned operand bases.
+    const bool vec4 = (d % 4 == 0) && (reinterpret_cast<uintptr_t>(T) % 16 == 0) &&
+                      (reinterpret_cast<uintptr_t>(Y) % 16 == 0) && (sT % 4 == 0) && (sY % 4 == 0);
+    hipLaunchKernelGGL(backmap_mfma_kernel, grid, dim3(256), 0, stream, T, shift, Y, V, n, d, ns, vec4, sT, sShift, sY,
+                       sV);
+    NAJA_HIP(hipGetLastError());
+}
+
 void backmap_launch(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
                     hipStream_t stream)
 {
-    if (ns == 0) return;
-    const int64_t gx = (ns + kBM - 1) / kBM;
-    NAJA_REQUIRE(gx <= 0x7FFFFFFF, NAJA_ERR_UNSUPPORTED, "too many sample rows for one backmap launch");
-    dim3 grid((unsigned)gx, (unsigned)((n + kBN - 1) / kBN));
-    // 16-byte loads need d % 4 == 0 and 16-byte aligned operand bases.
-    const bool vec4 = (d % 4 == 0) && (reinterpret_cast<uintptr_t>(T) % 16 == 0) &&
-                      (reinterpret_cast<uintptr_t>(Y) % 16 == 0);
-    hipLaunchKernelGGL(backmap_mfma_kernel, grid, dim3(256), 0, stream, T, shift, Y, V, n, d, ns, vec4);
-    NAJA_HIP(hipGetLastError());
+    backmap_launch_batched(T, shift, Y, V, n, d, ns, 1, 0, 0, 0, 0, stream);
 }
 
 }  // namespace naja

@@ -90,6 +90,10 @@ ChrKernelTable chr_table_w8();
 ChrKernelTable chr_table_w16();
 ChrKernelTable chr_table_g4();
 
+// Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
+// per group; -1 if none.
+int chr_smallest_r(int W, int G, int rows);
+
 // Shape for a batch whose largest variant has m_max rows (tpb > 0 is the
 // reference's --tpb, taken as a preference).
 ChrShape chr_choose_shape(int m_max, int G, int tpb);
@@ -99,5 +103,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
               float* A_pack, float* Ainv_pack, double* b_pack, int lda_max, hipStream_t stream);
+
+// Rounding transform: A' = A P_v, b' = b - A c_v per variant (see chr_kernel.hip).
+void chr_transform(const float* A_pack, const double* b_pack, const ChrVariant* variants_dev, int n_variants, int d,
+                   int lda, const float* P, const float* c, float* A_out, float* Ainv_out, double* b_out,
+                   hipStream_t stream);
 
 }  // namespace naja

@@ -73,7 +73,49 @@ __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, 
     }
 }
 
+// Rounding transform of the packed blocks: variant v samples z with
+// y = c_v + P_v z, so its rows become A' = A P_v (FP32, fmaf over k in
+// order) and b' = b - A c_v (FP64). Padding rows stay A' = 0, b' = +inf.
+// Also writes the IEEE reciprocal of A'.
+__global__ void chr_transform_kernel(const float* Asrc, const double* bsrc, const ChrVariant* variants, int d,
+                                     const float* P, const float* c, float* Adst, float* Ainv, double* bdst)
+{
+    const int v = blockIdx.y;
+    const ChrVariant V = variants[v];
+    const float* A = Asrc + V.a_off;
+    const float* Pv = P + (int64_t)v * d * d;
+    const int64_t total = (int64_t)V.lda * d;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % V.lda);
+        const int j = (int)(e / V.lda);
+        float acc = 0.0f;
+        for (int k = 0; k < d; ++k) acc = fmaf(A[(int64_t)k * V.lda + r], Pv[(int64_t)k * d + j], acc);
+        Adst[V.a_off + e] = acc;
+        Ainv[V.a_off + e] = 1.0f / acc;
+    }
+    if (blockIdx.x == 0) {
+        const float* cv = c + (int64_t)v * d;
+        for (int r = threadIdx.x; r < V.lda; r += blockDim.x) {
+            double acc = 0.0;
+            for (int k = 0; k < d; ++k) acc = fma((double)A[(int64_t)k * V.lda + r], (double)cv[k], acc);
+            bdst[V.b_off + r] = bsrc[V.b_off + r] - acc;
+        }
+    }
+}
+
 }  // namespace
+
+void chr_transform(const float* A_pack, const double* b_pack, const ChrVariant* variants_dev, int n_variants, int d,
+                   int lda, const float* P, const float* c, float* A_out, float* Ainv_out, double* b_out,
+                   hipStream_t stream)
+{
+    const int64_t per = (int64_t)lda * d;
+    int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(chr_transform_kernel, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack, variants_dev, d,
+                       P, c, A_out, Ainv_out, b_out);
+    NAJA_HIP(hipGetLastError());
+}
 
 namespace {
 
@@ -90,8 +132,10 @@ const ChrKernelTable& all_kernels()
     return all;
 }
 
+}  // namespace
+
 // Smallest compiled R with 64 W R >= rows; -1 if none.
-int smallest_r(int W, int G, int rows)
+int chr_smallest_r(int W, int G, int rows)
 {
     const ChrKernelTable& t = all_kernels();
     int best = -1;
@@ -101,8 +145,6 @@ int smallest_r(int W, int G, int rows)
     }
     return best;
 }
-
-}  // namespace
 
 ChrShape chr_choose_shape(int m_max, int G, int tpb)
 {
@@ -117,12 +159,12 @@ ChrShape chr_choose_shape(int m_max, int G, int tpb)
         // The reference's --tpb (threads per chain) is a preference here.
         const int w = std::max(1, tpb / 64);
         const int Wp = w >= 16 ? 16 : (w >= 8 ? 8 : (w >= 4 ? 4 : 1));
-        if (smallest_r(Wp, G, rows) > 0) W = Wp;
+        if (chr_smallest_r(Wp, G, rows) > 0) W = Wp;
     }
-    int R = smallest_r(W, G, rows);
+    int R = chr_smallest_r(W, G, rows);
     if (R < 0) {
         for (int w : {1, 4, 8, 16}) {
-            R = smallest_r(w, G, rows);
+            R = chr_smallest_r(w, G, rows);
             if (R > 0) {
                 W = w;
                 break;

@@ -1,0 +1,420 @@
+// Sampling plans: the polytope blocks of a batch are packed once (and, with a
+// rounding transform, multiplied into A' = A P_v), then launched any number of
+// times with no allocation, so a launch can be captured in a hipGraph. The
+// plan also owns the measuring autotuner over launch shapes.
+//
+// Replaces the per-call setup of naja::gpu::CoordinateHitAndRun
+// (chr.cu:13-70: slack DGEMM, cuRAND state allocation, kernel launch) and, for
+// batches, naja's one-model-per-job loop (job_bulk.cpp:115-189).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/naja_amd.h"
+#include "backmap.h"
+#include "chr_kernel.h"
+#include "common.h"
+#include "philox.h"
+
+struct naja_chr_plan {
+    int m0 = 0, d = 0, n_variants = 0, G = 1, m_max = 0;
+    naja_chr_params p{};
+    hipStream_t stream = nullptr;  // creation stream: allocations and frees are ordered on it
+    const float* A = nullptr;
+    const double* b = nullptr;
+    const float* A_extra = nullptr;
+    const double* b_extra = nullptr;
+    const float* P = nullptr;  // optional n_variants x d x d, row-major: y = c + P z
+    const float* c = nullptr;  // optional n_variants x d
+    std::vector<int> extra_rows;
+    naja::ChrShape shape{1, 1, 1};
+    bool tuned = false;
+
+    // host copies of the device tables (kept alive for async uploads)
+    std::vector<naja::ChrVariant> var;
+    std::vector<int64_t> eoff;
+    std::vector<int> ek;
+    std::vector<naja::ChrItem> items;
+
+    naja::ChrVariant* var_d = nullptr;
+    naja::ChrItem* items_d = nullptr;
+    int64_t* eoff_d = nullptr;
+    int* ek_d = nullptr;
+    float* A_pack = nullptr;
+    float* Ainv_pack = nullptr;
+    double* b_pack = nullptr;
+    float* A_raw = nullptr;   // untransformed pack (transform plans only)
+    double* b_raw = nullptr;
+    float* zbuf = nullptr;    // z-space samples (transform plans only)
+    int64_t pack_floats = 0;  // capacity of the pack buffers (floats)
+    int64_t pack_rows = 0;    // capacity of b buffers (rows)
+};
+
+namespace naja {
+
+namespace {
+
+using TuneKey = std::tuple<int, int, int, int, int>;  // rows64, d, G, n_items, tpb
+std::mutex g_tune_mu;
+std::map<TuneKey, ChrShape> g_tuned;
+
+TuneKey tune_key(const naja_chr_plan& pl)
+{
+    return TuneKey{(pl.m_max + 63) / 64 * 64, pl.d, pl.G, (int)pl.items.size(), pl.p.tpb > 0 ? pl.p.tpb : 0};
+}
+
+void free_async(void* ptr, hipStream_t s)
+{
+    if (ptr) NAJA_HIP(hipFreeAsync(ptr, s));
+}
+
+template <typename T>
+void alloc_async(T** ptr, size_t count, hipStream_t s)
+{
+    NAJA_HIP(hipMallocAsync((void**)ptr, sizeof(T) * (count ? count : 1), s));
+}
+
+void check_params(const naja_chr_params* p, int d)
+{
+    NAJA_REQUIRE(p != nullptr, NAJA_ERR_INVALID_ARGUMENT, "params is NULL");
+    NAJA_REQUIRE(p->nspc > 0, NAJA_ERR_INVALID_ARGUMENT, "invalid nspc %d", p->nspc);
+    NAJA_REQUIRE(p->thinning > 0, NAJA_ERR_INVALID_ARGUMENT, "invalid thinning %d", p->thinning);
+    NAJA_REQUIRE(p->nchains > 0, NAJA_ERR_INVALID_ARGUMENT, "invalid nchains %d", p->nchains);
+    NAJA_REQUIRE(p->pair_prob >= 0.0 && p->pair_prob <= 1.0, NAJA_ERR_INVALID_ARGUMENT, "PAIR_PROB must be in [0,1]");
+    NAJA_REQUIRE(p->ksparse_prob >= 0.0 && p->ksparse_prob <= 1.0, NAJA_ERR_INVALID_ARGUMENT,
+                 "invalid ksparse_prob (must be in [0,1])");
+    // chr.cu:529-551 accepts threads-per-block 32..1024 (powers of two) and
+    // throws runtime_error otherwise; <= 0 lets the library choose.
+    NAJA_REQUIRE(p->tpb <= 0 || p->tpb == 32 || p->tpb == 64 || p->tpb == 128 || p->tpb == 256 || p->tpb == 512 ||
+                     p->tpb == 1024,
+                 NAJA_ERR_RUNTIME, "Invalid threads per block (%d)", p->tpb);
+    NAJA_REQUIRE(p->chains_per_group == 0 || p->chains_per_group == 1 || p->chains_per_group == 4,
+                 NAJA_ERR_UNSUPPORTED, "chains_per_group must be 1 or 4 (got %d)", p->chains_per_group);
+    // Pair mode 2 without a schedule falls back to coordinate moves, as the
+    // reference kernel does (chr.cu:324-326); a given schedule must index
+    // inside the polytope.
+    if (p->pair_mode == 2 && p->n_pairs > 0 && p->pair_i && p->pair_j && p->pair_c && p->pair_s) {
+        std::vector<int> pi((size_t)p->n_pairs), pj((size_t)p->n_pairs);
+        NAJA_HIP(hipMemcpy(pi.data(), p->pair_i, sizeof(int) * pi.size(), hipMemcpyDeviceToHost));
+        NAJA_HIP(hipMemcpy(pj.data(), p->pair_j, sizeof(int) * pj.size(), hipMemcpyDeviceToHost));
+        for (int q = 0; q < p->n_pairs; ++q) {
+            NAJA_REQUIRE(pi[(size_t)q] >= 0 && pi[(size_t)q] < d && pj[(size_t)q] >= 0 && pj[(size_t)q] < d,
+                         NAJA_ERR_INVALID_ARGUMENT, "pair schedule entry %d indexes outside [0, %d)", q, d);
+        }
+    }
+}
+
+// (Re)packs the blocks for `sh` (lda = 64 W R), growing the buffers if needed.
+void repack(naja_chr_plan& pl, const ChrShape& sh)
+{
+    const hipStream_t s = pl.stream;
+    const int lda = sh.rows();
+    int64_t a_off = 0, b_off = 0;
+    const int64_t per_variant_samples = (int64_t)pl.p.nchains * pl.p.nspc * pl.d;
+    for (int v = 0; v < pl.n_variants; ++v) {
+        ChrVariant& V = pl.var[(size_t)v];
+        V.m = pl.m0 + pl.ek[(size_t)v];
+        V.lda = lda;
+        V.a_off = a_off;
+        V.b_off = b_off;
+        V.y0_col = (int64_t)v * pl.p.nchains;
+        V.sample_off = (int64_t)v * per_variant_samples;
+        a_off += (int64_t)lda * pl.d;
+        b_off += lda;
+    }
+    if (a_off > pl.pack_floats || b_off > pl.pack_rows) {
+        free_async(pl.A_pack, s);
+        free_async(pl.Ainv_pack, s);
+        free_async(pl.b_pack, s);
+        free_async(pl.A_raw, s);
+        free_async(pl.b_raw, s);
+        pl.A_pack = pl.Ainv_pack = pl.A_raw = nullptr;
+        pl.b_pack = pl.b_raw = nullptr;
+        alloc_async(&pl.A_pack, (size_t)a_off, s);
+        alloc_async(&pl.Ainv_pack, (size_t)a_off, s);
+        alloc_async(&pl.b_pack, (size_t)b_off, s);
+        if (pl.P) {
+            alloc_async(&pl.A_raw, (size_t)a_off, s);
+            alloc_async(&pl.b_raw, (size_t)b_off, s);
+        }
+        pl.pack_floats = a_off;
+        pl.pack_rows = b_off;
+    }
+    NAJA_HIP(hipMemcpyAsync(pl.var_d, pl.var.data(), sizeof(ChrVariant) * pl.var.size(), hipMemcpyHostToDevice, s));
+    if (pl.P) {
+        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants,
+                 pl.A_raw, pl.Ainv_pack, pl.b_raw, lda, s);
+        chr_transform(pl.A_raw, pl.b_raw, pl.var_d, pl.n_variants, pl.d, lda, pl.P, pl.c, pl.A_pack, pl.Ainv_pack,
+                      pl.b_pack, s);
+    } else {
+        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants,
+                 pl.A_pack, pl.Ainv_pack, pl.b_pack, lda, s);
+    }
+    pl.shape = sh;
+}
+
+ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps)
+{
+    const naja_chr_params* p = &pl.p;
+    ChrArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.A = pl.A_pack;
+    a.Ainv = pl.Ainv_pack;
+    a.b = pl.b_pack;
+    a.d = pl.d;
+    a.variants = pl.var_d;
+    a.items = pl.items_d;
+    a.y0 = y0;
+    a.samples = samples;
+    a.y_final = y_final;
+    a.nchains = p->nchains;
+    a.thin = p->thinning;
+    a.nsteps = nsteps;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.thr_k = rng_threshold(p->ksparse_prob);
+    a.thr_kp = rng_threshold(p->ksparse_prob + p->pair_prob);
+    a.has_pair = p->pair_prob > 0.0 ? 1 : 0;
+    a.pair_mode = p->pair_mode;
+    const bool sched = p->pair_mode == 2 && p->n_pairs > 0 && p->pair_i && p->pair_j && p->pair_c && p->pair_s;
+    a.pair_i = p->pair_i;
+    a.pair_j = p->pair_j;
+    a.pair_c = p->pair_c;
+    a.pair_s = p->pair_s;
+    a.n_pairs = sched ? p->n_pairs : 0;
+    if (a.pair_mode == 2 && a.n_pairs == 0) a.has_pair = 0;
+    int k = p->ksparse_k;
+    if (k < 1) k = 1;
+    if (k > 32) k = 32;
+    if (k > pl.d) k = pl.d;
+    a.ks_k = k;
+    a.ks_mag = (float)(1.0 / std::sqrt((double)k));
+    a.resync = chr_effective_resync(p->resync_interval, pl.d);
+    a.move_log = p->move_log;
+    a.nlog = p->move_log ? p->nlog : 0;
+    return a;
+}
+
+void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s)
+{
+    float* out = pl.P ? pl.zbuf : samples;
+    const ChrArgs a = make_args(pl, y0, out, y_final, nsteps);
+    chr_launch(a, (int)pl.items.size(), pl.shape, s);
+    if (pl.P) {
+        // y = c + P z for every saved sample of every variant (batched MFMA backmap).
+        const int64_t per = (int64_t)pl.p.nchains * pl.p.nspc;
+        backmap_launch_batched(pl.P, pl.c, pl.zbuf, samples, pl.d, pl.d, per, pl.n_variants, (int64_t)pl.d * pl.d,
+                               pl.d, per * pl.d, per * pl.d, s);
+    }
+}
+
+}  // namespace
+
+}  // namespace naja
+
+extern "C" {
+
+int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_variants, const int* extra_rows,
+                         const float* A_extra, const double* b_extra, const float* P, const float* c,
+                         const naja_chr_params* p, void* stream, naja_chr_plan** out)
+{
+    using namespace naja;
+    if (out) *out = nullptr;
+    naja_chr_plan* pl = nullptr;
+    const int rc = guarded([&] {
+        NAJA_REQUIRE(out != nullptr, NAJA_ERR_INVALID_ARGUMENT, "NULL plan output");
+        NAJA_REQUIRE(A && b, NAJA_ERR_INVALID_ARGUMENT, "NULL A or b");
+        NAJA_REQUIRE(m0 > 0 && d > 0, NAJA_ERR_INVALID_ARGUMENT, "invalid polytope shape %d x %d", m0, d);
+        NAJA_REQUIRE(n_variants > 0, NAJA_ERR_INVALID_ARGUMENT, "n_variants must be > 0");
+        NAJA_REQUIRE((P == nullptr) == (c == nullptr), NAJA_ERR_INVALID_ARGUMENT,
+                     "a rounding transform needs both P and c");
+        check_params(p, d);
+        pl = new naja_chr_plan();
+        pl->m0 = m0;
+        pl->d = d;
+        pl->n_variants = n_variants;
+        pl->p = *p;
+        pl->G = p->chains_per_group > 0 ? p->chains_per_group : 1;
+        pl->stream = reinterpret_cast<hipStream_t>(stream);
+        pl->A = A;
+        pl->b = b;
+        pl->A_extra = A_extra;
+        pl->b_extra = b_extra;
+        pl->P = P;
+        pl->c = c;
+        pl->var.resize((size_t)n_variants);
+        pl->eoff.resize((size_t)n_variants);
+        pl->ek.resize((size_t)n_variants);
+        int64_t e_off = 0;
+        for (int v = 0; v < n_variants; ++v) {
+            const int k = extra_rows ? extra_rows[v] : 0;
+            NAJA_REQUIRE(k >= 0, NAJA_ERR_INVALID_ARGUMENT, "variant %d has a negative extra row count", v);
+            NAJA_REQUIRE(k == 0 || (A_extra && b_extra), NAJA_ERR_INVALID_ARGUMENT,
+                         "extra rows given without A_extra / b_extra");
+            pl->ek[(size_t)v] = k;
+            pl->eoff[(size_t)v] = e_off;
+            e_off += k;
+            pl->m_max = std::max(pl->m_max, m0 + k);
+        }
+        for (int v = 0; v < n_variants; ++v)
+            for (int c0 = 0; c0 < p->nchains; c0 += pl->G) pl->items.push_back(ChrItem{v, c0});
+        const hipStream_t s = pl->stream;
+        alloc_async(&pl->var_d, pl->var.size(), s);
+        alloc_async(&pl->items_d, pl->items.size(), s);
+        alloc_async(&pl->eoff_d, pl->eoff.size(), s);
+        alloc_async(&pl->ek_d, pl->ek.size(), s);
+        NAJA_HIP(hipMemcpyAsync(pl->items_d, pl->items.data(), sizeof(ChrItem) * pl->items.size(),
+                                hipMemcpyHostToDevice, s));
+        NAJA_HIP(hipMemcpyAsync(pl->eoff_d, pl->eoff.data(), sizeof(int64_t) * pl->eoff.size(),
+                                hipMemcpyHostToDevice, s));
+        NAJA_HIP(hipMemcpyAsync(pl->ek_d, pl->ek.data(), sizeof(int) * pl->ek.size(), hipMemcpyHostToDevice, s));
+        if (P) alloc_async(&pl->zbuf, (size_t)n_variants * p->nchains * p->nspc * d, s);
+        ChrShape sh = chr_choose_shape(pl->m_max, pl->G, p->tpb);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(tune_key(*pl));
+            if (it != g_tuned.end()) {
+                sh = it->second;
+                pl->tuned = true;
+            }
+        }
+        repack(*pl, sh);
+    });
+    if (rc != NAJA_OK) {
+        if (pl) naja_chr_plan_destroy(pl);
+        return rc;
+    }
+    *out = pl;
+    return NAJA_OK;
+}
+
+int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, float* y_final, void* stream)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(plan && y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, y0 or samples");
+        launch(*plan, y0, samples, y_final, (int64_t)plan->p.nspc * plan->p.thinning,
+               reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, int* shape_out)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(plan && y0, NAJA_ERR_INVALID_ARGUMENT, "NULL plan or y0");
+        naja_chr_plan& pl = *plan;
+        const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+        const int rows = (pl.m_max + 63) / 64 * 64;
+        std::vector<ChrShape> cand;
+        for (int W : {1, 4, 8, 16}) {
+            const int R = chr_smallest_r(W, pl.G, rows);
+            if (R > 0) cand.push_back(ChrShape{W, pl.G, R});
+        }
+        NAJA_REQUIRE(!cand.empty(), NAJA_ERR_UNSUPPORTED, "no compiled shape for %d rows", pl.m_max);
+        // Probe: about 1024 steps of the real run (at least one saved sample).
+        const int nspc_probe = std::max(1, std::min(pl.p.nspc, 1024 / pl.p.thinning));
+        const int64_t nsteps = (int64_t)nspc_probe * pl.p.thinning;
+        float* probe = nullptr;
+        alloc_async(&probe, (size_t)pl.n_variants * pl.p.nchains * pl.p.nspc * pl.d, s);
+        hipEvent_t e0, e1;
+        NAJA_HIP(hipEventCreate(&e0));
+        NAJA_HIP(hipEventCreate(&e1));
+        ChrShape best = pl.shape;
+        float best_ms = 1e30f;
+        for (const ChrShape& sh : cand) {
+            repack(pl, sh);
+            const ChrArgs a = make_args(pl, y0, pl.P ? pl.zbuf : probe, nullptr, nsteps);
+            chr_launch(a, (int)pl.items.size(), sh, s);  // warm-up
+            NAJA_HIP(hipEventRecord(e0, s));
+            chr_launch(a, (int)pl.items.size(), sh, s);
+            NAJA_HIP(hipEventRecord(e1, s));
+            NAJA_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            NAJA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = sh;
+            }
+        }
+        NAJA_HIP(hipEventDestroy(e0));
+        NAJA_HIP(hipEventDestroy(e1));
+        free_async(probe, s);
+        repack(pl, best);
+        pl.tuned = true;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[tune_key(pl)] = best;
+        }
+        if (shape_out) {
+            shape_out[0] = best.W;
+            shape_out[1] = best.G;
+            shape_out[2] = best.R;
+        }
+    });
+}
+
+int naja_chr_plan_shape(const naja_chr_plan* plan, int* shape_out)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(plan && shape_out, NAJA_ERR_INVALID_ARGUMENT, "NULL plan or output");
+        shape_out[0] = plan->shape.W;
+        shape_out[1] = plan->shape.G;
+        shape_out[2] = plan->shape.R;
+    });
+}
+
+int naja_chr_plan_destroy(naja_chr_plan* plan)
+{
+    using namespace naja;
+    if (!plan) return NAJA_OK;
+    const int rc = guarded([&] {
+        const hipStream_t s = plan->stream;
+        void* bufs[] = {plan->var_d,     plan->items_d, plan->eoff_d, plan->ek_d,  plan->A_pack,
+                        plan->Ainv_pack, plan->b_pack,  plan->A_raw,  plan->b_raw, plan->zbuf};
+        for (void* ptr : bufs) free_async(ptr, s);
+    });
+    delete plan;
+    return rc;
+}
+
+int naja_chr_sample_batched_rounded(const float* A, const double* b, int m0, int d, int n_variants,
+                                    const int* extra_rows, const float* A_extra, const double* b_extra,
+                                    const float* P, const float* c, const float* y0, const naja_chr_params* p,
+                                    float* samples, float* y_final, void* stream)
+{
+    using namespace naja;
+    naja_chr_plan* pl = nullptr;
+    int rc = guarded([&] {
+        NAJA_REQUIRE(y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL y0 or samples");
+    });
+    if (rc) return rc;
+    rc = naja_chr_plan_create(A, b, m0, d, n_variants, extra_rows, A_extra, b_extra, P, c, p, stream, &pl);
+    if (rc) return rc;
+    rc = naja_chr_plan_launch(pl, y0, samples, y_final, stream);
+    const int rc2 = naja_chr_plan_destroy(pl);
+    return rc ? rc : rc2;
+}
+
+int naja_chr_sample_batched(const float* A, const double* b, int m0, int d, int n_variants, const int* extra_rows,
+                            const float* A_extra, const double* b_extra, const float* y0, const naja_chr_params* p,
+                            float* samples, float* y_final, void* stream)
+{
+    return naja_chr_sample_batched_rounded(A, b, m0, d, n_variants, extra_rows, A_extra, b_extra, nullptr, nullptr,
+                                           y0, p, samples, y_final, stream);
+}
+
+int naja_chr_sample(const float* A, const double* b, int m, int d, const float* y0, const naja_chr_params* p,
+                    float* samples, float* y_final, void* stream)
+{
+    return naja_chr_sample_batched(A, b, m, d, 1, nullptr, nullptr, nullptr, y0, p, samples, y_final, stream);
+}
+
+}  // extern "C"

@@ -59,6 +59,7 @@ def lib() -> ctypes.CDLL:
         L.orc_split_rhat.argtypes = [vp, i32, i32, i32, i32, vp, vp]
         L.orc_percentile.restype = ctypes.c_float
         L.orc_percentile.argtypes = [vp, i32, ctypes.c_float]
+        L.orc_transform.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp]
         L.orc_backmap.restype = i32
         L.orc_backmap.argtypes = [vp, vp, vp, vp, i32, i32, i64]
         _lib = L
@@ -227,3 +228,17 @@ def kiss_autocov(x):
     if K.ref_kiss_autocov(_p(x), x.size, _p(out)):
         raise RuntimeError("kissfft allocation failed")
     return out
+
+
+def transform(A, b, P, c):
+    """A' = A P and b' = b - A c exactly as the GPU packs a rounded variant."""
+    A = np.asarray(A, dtype=np.float32)
+    m, d = A.shape
+    A_cm = np.ascontiguousarray(A.T)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    Aout = np.zeros((d, m), dtype=np.float32)
+    bout = np.zeros(m, dtype=np.float64)
+    lib().orc_transform(_p(A_cm), _p(b), m, d, _p(P), _p(c), _p(Aout), _p(bout))
+    return np.ascontiguousarray(Aout.T), bout

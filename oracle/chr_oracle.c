@@ -444,3 +444,24 @@ void orc_move_stream(int d, uint64_t seed, uint32_t variant, uint32_t group,
         out[t * 3 + 0] = mv.type; out[t * 3 + 1] = mv.i; out[t * 3 + 2] = mv.j;
     }
 }
+
+/*
+ * Rounding transform restated for the tests: A' = A P (FP32, fmaf over k in
+ * order), b' = b - A c (FP64 fma chain), A column-major m x d, P row-major
+ * d x d. Matches the GPU's transform of the packed blocks element for element.
+ */
+void orc_transform(const float* A, const double* b, int m, int d, const float* P, const float* c,
+                   float* Aout, double* bout)
+{
+    for (int j = 0; j < d; ++j)
+        for (int r = 0; r < m; ++r) {
+            float acc = 0.0f;
+            for (int k = 0; k < d; ++k) acc = fmaf(A[(size_t)k * m + r], P[(size_t)k * d + j], acc);
+            Aout[(size_t)j * m + r] = acc;
+        }
+    for (int r = 0; r < m; ++r) {
+        double acc = 0.0;
+        for (int k = 0; k < d; ++k) acc = fma((double)A[(size_t)k * m + r], (double)c[k], acc);
+        bout[r] = b[r] - acc;
+    }
+}

@@ -179,3 +179,84 @@ def test_invalid_arguments_raise(gpu):
     with pytest.raises(naja_amd.NajaInvalidArgument):
         naja_amd.coordinate_hit_and_run(t.from_numpy(A.astype(np.float32)), t.from_numpy(b[:-1]),
                                         t.zeros((2, 4)), 10, 1, 2)
+
+
+def test_rounded_batched_matches_oracle(gpu):
+    """Per-variant rounding transform y = c + P z: the packed A' = A P and
+    b' = b - A c equal the oracle's element for element, so the z-chains are
+    the oracle's; y comes back through the FP32 MFMA backmap."""
+    import naja_amd
+
+    t = gpu
+    d = 8
+    A, b, _ = cube(d)
+    rng = np.random.default_rng(21)
+    ks = [0, 6, 11]
+    EA = [rng.standard_normal((k, d)) for k in ks]
+    Eb = [np.ones(k) * 0.5 for k in ks]
+    nv, nch, nspc, thin = len(ks), 3, 400, 2
+    P = np.stack([np.eye(d) + 0.2 * rng.standard_normal((d, d)) for _ in ks]).astype(np.float32)
+    c = (0.05 * rng.standard_normal((nv, d))).astype(np.float32)
+    Z0 = np.zeros((nv, nch, d), np.float32)
+    Y = naja_amd.coordinate_hit_and_run_batched(
+        t.from_numpy(A.astype(np.float32)), t.from_numpy(b), [t.from_numpy(e.astype(np.float32)) for e in EA],
+        [t.from_numpy(f) for f in Eb], t.from_numpy(Z0), nspc, thin, nch, seed=5, pair_prob=0.3,
+        P=t.from_numpy(P), c=t.from_numpy(c))
+    t.cuda.synchronize()
+    Y = Y.cpu().numpy().astype(np.float64)
+    for v in range(nv):
+        Av = np.vstack([A, EA[v]]).astype(np.float32)
+        bv = np.concatenate([b, Eb[v]])
+        At, bt = oracle.transform(Av, bv, P[v], c[v])
+        z = oracle.chr_run_f32(At, bt, Z0[v], nspc, thin, seed=5, variant=v, pair_prob=0.3)["samples"]
+        y = c[v].astype(np.float64) + z.astype(np.float64) @ P[v].astype(np.float64).T
+        scale = 1.0 + np.abs(z).astype(np.float64) @ np.abs(P[v].astype(np.float64)).T
+        err = (np.abs(Y[v] - y) / scale).max()
+        print(f"[rounded variant {v}] max scaled err {err:.3e}", flush=True)
+        assert err <= 1e-5
+        viol = (Av.astype(np.float64) @ Y[v].T - bv[:, None]).max()
+        assert viol <= 1e-4
+
+
+def test_plan_autotune_relaunch_and_graph_replay(gpu):
+    """A plan launches without allocating: relaunches and hipGraph replays
+    (torch.cuda.CUDAGraph) reproduce the eager samples exactly, and the
+    autotuned shape gives the same samples as the default one."""
+    import naja_amd
+
+    t = gpu
+    A, b, y0 = random_polytope(20, 600, seed=41)
+    nch, nspc, thin = 4, 300, 3
+    At, bt = t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda()
+    X0 = t.zeros((1, nch, 20), device="cuda")
+    plan = naja_amd.ChrPlan(At, bt, nspc, thin, nch, seed=9, pair_prob=0.3)
+    out0 = plan.launch(X0, plan.samples_like())
+    t.cuda.synchronize()
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((nch, 20), np.float32), nspc, thin, seed=9,
+                             pair_prob=0.3)["samples"]
+    np.testing.assert_array_equal(out0.cpu().numpy()[0], ref)
+    shape = plan.autotune(X0)
+    print(f"autotuned shape (W, G, R) = {shape}", flush=True)
+    assert shape == plan.shape()
+    out1 = plan.launch(X0, plan.samples_like())
+    t.cuda.synchronize()
+    np.testing.assert_array_equal(out1.cpu().numpy()[0], ref)
+
+    static_out = plan.samples_like()
+    side = t.cuda.Stream()
+    side.wait_stream(t.cuda.current_stream())
+    with t.cuda.stream(side):
+        plan.launch(X0, static_out)  # warm-up on the capture stream
+    t.cuda.current_stream().wait_stream(side)
+    t.cuda.synchronize()
+    g = t.cuda.CUDAGraph()
+    with t.cuda.graph(g):
+        plan.launch(X0, static_out)
+    for rep in range(3):
+        static_out.fill_(-1.0)
+        _ = (At * 2.0).sum()  # unrelated eager work between replays
+        t.cuda.synchronize()
+        g.replay()
+        t.cuda.synchronize()
+        np.testing.assert_array_equal(static_out.cpu().numpy()[0], ref)
+    plan.close()

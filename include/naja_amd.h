@@ -160,6 +160,63 @@ int naja_backmap(const float* T, const float* shift, const float* Y, float* V,
 int naja_diag(const float* samples, int dim, int n_total, int nchains, int layout,
               float* ess, float* rhat, float* ess_chain, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Rounding preconditioners (host, FP64, row-major host memory). Every P
+ * returned here is the sampler's rounding transform: y = x_c + P z.
+ * ------------------------------------------------------------------------- */
+
+/*
+ * Symmetric eigendecomposition (Householder tridiagonalisation + implicit QL),
+ * standing in for Eigen::SelfAdjointEigenSolver (dikin_precondition.cpp:103).
+ * a: n x n symmetric; evals ascending; evecs n x n with eigenvector j in column j.
+ */
+int naja_symmetric_eigen(const double* a, int n, double* evals, double* evecs);
+
+/*
+ * dikin_precondition (dikin_precondition.cpp:36-183, defaults in
+ * dikin_precondition.h:35-42: k_directions 64, tight_factor 100,
+ * target_chord 0 = median coordinate chord, alpha_max 1000).
+ *   A m x d, b m, x_c d (NULL = origin).
+ *   Outputs (each nullable): P (d x d, the reference's DikinResult::P),
+ *   P_inv (d x d, y = x_c + P_inv z), A_new = A P_inv (m x d),
+ *   b_new = b - A x_c (m), n_corrected.
+ * Empty polytope (m or d = 0): NAJA_ERR_INVALID_ARGUMENT (the reference throws
+ * std::invalid_argument).
+ */
+int naja_precond_dikin(const double* A, const double* b, const double* x_c, int m, int d,
+                       int k_directions, double tight_factor, double target_chord, double alpha_max,
+                       double* P, double* P_inv, double* A_new, double* b_new, int* n_corrected);
+
+/*
+ * dikin_precondition for every conditioned variant [A0; E_v] y <= [b0; f_v]
+ * (the extra-constraint stacking of extra_constraints.cpp:17-58), variants
+ * spread over host threads (nthreads <= 0: all cores).
+ *   extra_rows[v] = k_v; extra_A: sum k_v x d rows, variant after variant;
+ *   extra_b: sum k_v; x_c: n_variants x d (NULL = origin).
+ *   P_inv: n_variants x d x d; n_corrected: n_variants (nullable).
+ */
+int naja_precond_dikin_batched(const double* A0, const double* b0, int m0, int d, int n_variants,
+                               const int* extra_rows, const double* extra_A, const double* extra_b,
+                               const double* x_c, int k_directions, double tight_factor,
+                               double target_chord, double alpha_max, double* P_inv,
+                               int* n_corrected, int nthreads);
+
+/*
+ * Dikin-ellipsoid whitening per variant: H = sum over all rows of
+ * a a^T / s^2 at x_c, H = L L^T, P = L^{-T} (so P P^T = H^{-1}). No reference
+ * counterpart: the frame the learned preconditioner's pilot chains run in.
+ * x_c must be interior (every slack > 0), else NAJA_ERR_INVALID_ARGUMENT.
+ */
+int naja_precond_ellipsoid_batched(const double* A0, const double* b0, int m0, int d, int n_variants,
+                                   const int* extra_rows, const double* extra_A, const double* extra_b,
+                                   const double* x_c, double* P, int nthreads);
+
+/*
+ * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} from the learned
+ * preconditioner's factors V (n_variants x d x rank). No reference counterpart.
+ */
+int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, double* P, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,11 @@ EXPORTED_SYMBOLS = (
     "naja_chr_plan_destroy",
     "naja_backmap",
     "naja_diag",
+    "naja_symmetric_eigen",
+    "naja_precond_dikin",
+    "naja_precond_dikin_batched",
+    "naja_precond_ellipsoid_batched",
+    "naja_precond_lowrank",
 )
 
 
@@ -119,6 +124,18 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_backmap.restype = i32
     lib.naja_diag.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
     lib.naja_diag.restype = i32
+    f64 = ctypes.c_double
+    lib.naja_symmetric_eigen.argtypes = [vp, i32, vp, vp]
+    lib.naja_symmetric_eigen.restype = i32
+    lib.naja_precond_dikin.argtypes = [vp, vp, vp, i32, i32, i32, f64, f64, f64, vp, vp, vp, vp, vp]
+    lib.naja_precond_dikin.restype = i32
+    lib.naja_precond_dikin_batched.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, i32, f64, f64, f64, vp, vp,
+                                               i32]
+    lib.naja_precond_dikin_batched.restype = i32
+    lib.naja_precond_ellipsoid_batched.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+    lib.naja_precond_ellipsoid_batched.restype = i32
+    lib.naja_precond_lowrank.argtypes = [vp, i32, i32, i32, vp, i32]
+    lib.naja_precond_lowrank.restype = i32
     _lib = lib
     return lib
 

@@ -1,0 +1,89 @@
+"""ORACLE — test infrastructure only: FP64 numpy restatements of the rounding
+preconditioners, used by tests/ as the checker of naja_amd.precond.
+
+Eigenproblems and factorisations go through numpy's LAPACK routines
+(numpy.linalg.eigh / cholesky), independent of the library's own solver.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def chord_along(A, b, x_c, v):
+    """dikin_precondition.cpp:13-32."""
+    slack = b - A @ x_c
+    Av = A @ v
+    tmin, tmax = -np.inf, np.inf
+    for i in range(A.shape[0]):
+        if abs(Av[i]) < 1e-15:
+            continue
+        t = slack[i] / Av[i]
+        if Av[i] > 0:
+            tmax = min(tmax, t)
+        else:
+            tmin = max(tmin, t)
+    return 0.0 if tmax <= tmin else tmax - tmin
+
+
+def dikin(A, b, x_c=None, k_directions=64, tight_factor=100.0, target_chord=0.0, alpha_max=1000.0):
+    """dikin_precondition (dikin_precondition.cpp:36-183)."""
+    A = np.asarray(A, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    m, d = A.shape
+    if m == 0 or d == 0:
+        raise ValueError("dikin_precondition: empty polytope")
+    x_c = np.zeros(d) if x_c is None else np.asarray(x_c, dtype=np.float64)
+    slack = b - A @ x_c
+    norms = np.linalg.norm(A, axis=1)
+    deltas = np.full(m, np.inf)
+    N = np.zeros_like(A)
+    ok = norms >= 1e-15
+    deltas[ok] = slack[ok] / norms[ok]
+    N[ok] = A[ok] / norms[ok, None]
+    pos = (deltas > 0) & np.isfinite(deltas)
+    min_delta = deltas[pos].min() if pos.any() else np.inf
+    tight = np.where((deltas > 0) & (deltas <= tight_factor * min_delta))[0]
+    I = np.eye(d)
+    if tight.size == 0:
+        return dict(P=I, P_inv=I.copy(), A_new=A.copy(), b_new=slack, n_corrected=0)
+    H = (N[tight].T * (1.0 / deltas[tight] ** 2)) @ N[tight]
+    ev, V = np.linalg.eigh(H)
+    k = min(k_directions, d)
+    med = ev[d // 2]
+    kk = 0
+    for j in range(d - 1, -1, -1):
+        if kk < k and ev[j] > 10.0 * max(med, 1e-10):
+            kk += 1
+        else:
+            break
+    kk = max(kk, 1)
+    Vk = V[:, ::-1][:, :kk]
+    if target_chord <= 0:
+        cs = sorted(c for c in (chord_along(A, b, x_c, I[j]) for j in range(min(d, 50))) if c > 0)
+        target_chord = cs[len(cs) // 2] if cs else 1.0
+    al = np.empty(kk)
+    for j in range(kk):
+        c = chord_along(A, b, x_c, Vk[:, j])
+        al[j] = min(alpha_max, 10.0) if c < 1e-15 else min(max(target_chord / c, 1.0), alpha_max)
+    P = I + (Vk * (al - 1.0)) @ Vk.T
+    Pinv = I + (Vk * (1.0 / al - 1.0)) @ Vk.T
+    return dict(P=P, P_inv=Pinv, A_new=A @ Pinv, b_new=slack, n_corrected=kk)
+
+
+def ellipsoid(A, b, x_c=None):
+    """P with P P^T = H^{-1}, H = A^T diag(1/s^2) A (upper-triangular L^{-T})."""
+    A = np.asarray(A, dtype=np.float64)
+    d = A.shape[1]
+    x_c = np.zeros(d) if x_c is None else np.asarray(x_c, dtype=np.float64)
+    s = np.asarray(b, dtype=np.float64) - A @ x_c
+    H = (A.T * (1.0 / s ** 2)) @ A
+    L = np.linalg.cholesky(H)
+    return np.linalg.inv(L).T
+
+
+def lowrank(V):
+    """(I + V V^T)^{-1/2} through the full d x d eigenproblem."""
+    V = np.asarray(V, dtype=np.float64)
+    M = np.eye(V.shape[0]) + V @ V.T
+    ev, Q = np.linalg.eigh(M)
+    return (Q / np.sqrt(ev)) @ Q.T

@@ -217,6 +217,59 @@ int naja_precond_ellipsoid_batched(const double* A0, const double* b0, int m0, i
  */
 int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, double* P, int nthreads);
 
+/* ---------------------------------------------------------------------------
+ * Learned preconditioner (no reference counterpart: the reference's
+ * preconditioners are the Dikin rounding above and the Jacobi pair schedule,
+ * runtime_warmup.cpp:16-108). A 3-layer MLP maps a condition's extra-bound
+ * features (n_features) through hidden1, hidden2 ReLU units to a dim x rank
+ * factor V = out_scale * o; its whitening is P = (I + V V^T)^{-1/2}
+ * (naja_precond_lowrank). Products run on BF16 MFMA with FP32 accumulation;
+ * weights are FP32 masters with BF16 copies; loss and gradient in FP64.
+ * Parameter vector layout (FP32): W1 (hidden1 x n_features), b1, W2
+ * (hidden2 x hidden1), b2, W3 (dim*rank x hidden2), b3 (dim*rank).
+ * All data pointers below are device memory; stream may be NULL.
+ * ------------------------------------------------------------------------- */
+typedef struct naja_lp naja_lp;
+
+/* rank must be 32 and rank <= dim <= 160 (NAJA_ERR_UNSUPPORTED otherwise). */
+int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, uint64_t seed,
+                   float out_scale, naja_lp** out);
+int naja_lp_destroy(naja_lp* lp);
+int naja_lp_param_count(const naja_lp* lp, int64_t* n);
+int naja_lp_get_params(const naja_lp* lp, float* dst, void* stream);
+/* Loads parameters and resets the Adam state. */
+int naja_lp_set_params(naja_lp* lp, const float* src, void* stream);
+/* V: batch x dim x rank. */
+int naja_lp_forward(naja_lp* lp, const float* features, int batch, float* V, void* stream);
+/*
+ * Gaussian negative log likelihood of the covariances S (batch x dim x dim,
+ * in units of the base polytope's) under the precision I + V V^T:
+ * loss[b] = tr(V^T S V) - logdet(I + V^T V); grad (nullable) = gradient of
+ * the batch mean with respect to the parameter vector.
+ */
+int naja_lp_loss_grad(naja_lp* lp, const float* features, const float* S, int batch, float* loss,
+                      float* grad, void* stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch-mean loss. */
+int naja_lp_train_step(naja_lp* lp, const float* features, const float* S, int batch, float lr,
+                       float* loss, void* stream);
+
+/*
+ * Per-variant sample covariance of pilot chains: samples n_variants x rows x d
+ * (FP32); mean (n_variants x d, FP64); S = scale / (rows - 1) * centred Gram
+ * (n_variants x d x d, FP32). FP64 accumulation.
+ */
+int naja_pilot_covariance(const float* samples, int n_variants, int64_t rows, int d, double scale,
+                          float* S, double* mean, void* stream);
+
+/*
+ * C = alpha * op(A) op(B), op(A) M x K, op(B) K x N, row-major FP32 storage,
+ * operands rounded to BF16 and multiplied on MFMA with FP32 accumulation (the
+ * products of the learned preconditioner). trans_a: A stored K x M;
+ * trans_b: B stored N x K.
+ */
+int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, const float* A, int64_t lda,
+                   const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

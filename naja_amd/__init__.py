@@ -50,6 +50,16 @@ EXPORTED_SYMBOLS = (
     "naja_precond_dikin_batched",
     "naja_precond_ellipsoid_batched",
     "naja_precond_lowrank",
+    "naja_lp_create",
+    "naja_lp_destroy",
+    "naja_lp_param_count",
+    "naja_lp_get_params",
+    "naja_lp_set_params",
+    "naja_lp_forward",
+    "naja_lp_loss_grad",
+    "naja_lp_train_step",
+    "naja_pilot_covariance",
+    "naja_gemm_bf16",
 )
 
 
@@ -136,6 +146,21 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_ellipsoid_batched.restype = i32
     lib.naja_precond_lowrank.argtypes = [vp, i32, i32, i32, vp, i32]
     lib.naja_precond_lowrank.restype = i32
+    f32 = ctypes.c_float
+    lib.naja_lp_create.argtypes = [i32, i32, i32, i32, i32, ctypes.c_uint64, f32, ctypes.POINTER(vp)]
+    lib.naja_lp_destroy.argtypes = [vp]
+    lib.naja_lp_param_count.argtypes = [vp, ctypes.POINTER(i64)]
+    lib.naja_lp_get_params.argtypes = [vp, vp, vp]
+    lib.naja_lp_set_params.argtypes = [vp, vp, vp]
+    lib.naja_lp_forward.argtypes = [vp, vp, i32, vp, vp]
+    lib.naja_lp_loss_grad.argtypes = [vp, vp, vp, i32, vp, vp, vp]
+    lib.naja_lp_train_step.argtypes = [vp, vp, vp, i32, f32, vp, vp]
+    lib.naja_pilot_covariance.argtypes = [vp, i32, i64, i32, f64, vp, vp, vp]
+    lib.naja_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, i64, vp, i64, f32, vp]
+    for name in ("naja_lp_create", "naja_lp_destroy", "naja_lp_param_count", "naja_lp_get_params",
+                 "naja_lp_set_params", "naja_lp_forward", "naja_lp_loss_grad", "naja_lp_train_step",
+                 "naja_pilot_covariance", "naja_gemm_bf16"):
+        getattr(lib, name).restype = i32
     _lib = lib
     return lib
 

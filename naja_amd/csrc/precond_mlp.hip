@@ -1,0 +1,591 @@
+// Learned preconditioner: a 3-layer MLP (features -> H1 -> H2 -> dim x rank
+// factor V) trained on the GPU with hand-written BF16 MFMA kernels.
+//
+//   z1 = x W1^T + b1, h1 = relu(z1)
+//   z2 = h1 W2^T + b2, h2 = relu(z2)
+//   o  = h2 W3^T + b3, V = out_scale * reshape(o, dim, rank)
+//
+// The whitening it encodes is P = (I + V V^T)^{-1/2} (y = c + P z), i.e. the
+// precision model Pi = I + V V^T of the condition's covariance in units of
+// the base polytope's. Training minimises the Gaussian negative log
+// likelihood of the pilot covariance S under Pi,
+//   L(V) = tr(V^T S V) - logdet(I_r + V^T V)   (= tr(Pi S) - logdet Pi - tr S),
+// averaged over the batch; dL/dV = 2 S V - 2 V (I + V^T V)^{-1}.
+//
+// Every matrix product runs on v_mfma_f32_32x32x8bf16_1k: operands are
+// rounded to BF16 (round-to-nearest-even) as they are staged into LDS and
+// accumulated in FP32. Weights are kept as FP32 masters for Adam and as BF16
+// copies that the products read. The loss and its gradient are FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/naja_amd.h"
+#include "common.h"
+
+namespace naja {
+namespace mlp {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned short bf16_t;
+
+__device__ __forceinline__ bf16_t f2bf(float x)
+{
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ bf16_t to_bf(float x) { return f2bf(x); }
+__device__ __forceinline__ bf16_t to_bf(bf16_t x) { return x; }
+
+enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2, EPI_MASK = 3 };
+
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 4;
+
+// C(m, n) = alpha * (sum_k A(m, k) B(k, n) [+ bias(n)]) [relu] [* (mask(m, n) > 0)].
+// A(m, k) = A[m*lda + k] if AK (k contiguous) else A[k*lda + m];
+// B(k, n) = B[n*ldb + k] if BKC else B[k*ldb + n].
+// 64 x 64 output tile per workgroup, 4 waves of 32 x 32, K staged 32 at a time.
+template <typename TA, typename TB, bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const TA* __restrict__ A, int64_t lda,
+                                                   const TB* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                   int64_t ldc, float alpha, const float* __restrict__ bias,
+                                                   const float* __restrict__ mask, int64_t ldm)
+{
+    __shared__ __align__(16) bf16_t As[BM][LDK];
+    __shared__ __align__(16) bf16_t Bs[BN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    floatx16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        for (int i = tid; i < BM * BK; i += 256) {
+            const int r = AK ? i / BK : i % BM;
+            const int k = AK ? i % BK : i / BM;
+            const int gm = m0 + r, gk = k0 + k;
+            bf16_t v = 0;
+            if (gm < M && gk < K) v = to_bf(AK ? A[(int64_t)gm * lda + gk] : A[(int64_t)gk * lda + gm]);
+            As[r][k] = v;
+        }
+        for (int i = tid; i < BN * BK; i += 256) {
+            const int c = BKC ? i / BK : i % BN;
+            const int k = BKC ? i % BK : i / BN;
+            const int gn = n0 + c, gk = k0 + k;
+            bf16_t v = 0;
+            if (gn < N && gk < K) v = to_bf(BKC ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn]);
+            Bs[c][k] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 8) {
+            // Lanes 0-31 carry k = kk..kk+3 of rows/columns 0-31, lanes 32-63 k = kk+4..kk+7.
+            const int kb = kk + (lane >> 5) * 4;
+            const uint2 ra = *reinterpret_cast<const uint2*>(&As[wm + (lane & 31)][kb]);
+            const uint2 rb = *reinterpret_cast<const uint2*>(&Bs[wn + (lane & 31)][kb]);
+            bf16x4 a, b;
+            a[0] = (short)(ra.x & 0xffffu);
+            a[1] = (short)(ra.x >> 16);
+            a[2] = (short)(ra.y & 0xffffu);
+            a[3] = (short)(ra.y >> 16);
+            b[0] = (short)(rb.x & 0xffffu);
+            b[1] = (short)(rb.x >> 16);
+            b[2] = (short)(rb.y & 0xffffu);
+            b[3] = (short)(rb.y >> 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int r = m0 + wm + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        const int c = n0 + wn + (lane & 31);
+        if (r < M && c < N) {
+            float v = acc[e];
+            if (EPI == EPI_BIAS || EPI == EPI_BIAS_RELU) v += bias[c];
+            v *= alpha;
+            if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.0f);
+            if (EPI == EPI_MASK) v = (mask[(int64_t)r * ldm + c] > 0.0f) ? v : 0.0f;
+            C[(int64_t)r * ldc + c] = v;
+        }
+    }
+}
+
+template <typename TA, typename TB, bool AK, bool BKC, int EPI>
+void gemm(hipStream_t s, int M, int N, int K, const TA* A, int64_t lda, const TB* B, int64_t ldb, float* C,
+          int64_t ldc, float alpha = 1.0f, const float* bias = nullptr, const float* mask = nullptr, int64_t ldm = 0)
+{
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+    hipLaunchKernelGGL((gemm_kernel<TA, TB, AK, BKC, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc,
+                       alpha, bias, mask, ldm);
+    NAJA_HIP(hipGetLastError());
+}
+
+// db(n) = sum over rows of dZ(r, n).
+__global__ void colsum_kernel(const float* __restrict__ dZ, int rows, int cols, float* __restrict__ db)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; ++r) s += dZ[(int64_t)r * cols + c];
+    db[c] = s;
+}
+
+// Adam (Kingma & Ba, 2015) on the FP32 masters; refreshes the BF16 copies.
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                            float* __restrict__ m2, bf16_t* __restrict__ pbf, int64_t n, float lr, float beta1,
+                            float beta2, float eps, float c1, float c2)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = beta1 * m1[i] + (1.0f - beta1) * gi;
+    const float b = beta2 * m2[i] + (1.0f - beta2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float mhat = a * c1, vhat = b * c2;
+    const float w = p[i] - lr * mhat / (sqrtf(vhat) + eps);
+    p[i] = w;
+    pbf[i] = f2bf(w);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ p, bf16_t* __restrict__ pbf, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pbf[i] = f2bf(p[i]);
+}
+
+// One workgroup per condition b: V = out_scale * O[b] (dim x R), S = S[b]
+// (dim x dim). loss[b] = tr(V^T S V) - logdet(M), M = I + V^T V;
+// dO[b] = out_scale * grad_scale * (2 S V - 2 V M^{-1}). FP64: M = L L^T by
+// Cholesky, L^{-1} column by column, M^{-1} = L^{-T} L^{-1}.
+template <int R>
+__global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, int64_t ldo,
+                                                  const float* __restrict__ S, int dim, float out_scale,
+                                                  float grad_scale, float* __restrict__ loss, float* __restrict__ dO)
+{
+    extern __shared__ double lds[];
+    double* V = lds;                      // dim x R
+    double* SV = V + (size_t)dim * R;     // dim x R
+    double* L = SV + (size_t)dim * R;     // R x R (M, then its Cholesky factor)
+    double* Li = L + R * R;               // R x R, L^{-1}
+    double* Mi = Li + R * R;              // R x R, M^{-1}
+    __shared__ double red[256];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const float* Ob = O + (int64_t)b * ldo;
+    const float* Sb = S + (int64_t)b * dim * dim;
+    for (int i = tid; i < dim * R; i += 256) V[i] = (double)out_scale * (double)Ob[i];
+    __syncthreads();
+    double tr = 0.0;
+    for (int i = tid; i < dim * R; i += 256) {
+        const int r = i / R, c = i % R;
+        const float* Srow = Sb + (int64_t)r * dim;
+        double s = 0.0;
+        for (int k = 0; k < dim; ++k) s += (double)Srow[k] * V[k * R + c];
+        SV[i] = s;
+        tr += V[i] * s;
+    }
+    for (int i = tid; i < R * R; i += 256) {
+        const int a = i / R, c = i % R;
+        double s = (a == c) ? 1.0 : 0.0;
+        for (int k = 0; k < dim; ++k) s += V[k * R + a] * V[k * R + c];
+        L[i] = s;
+    }
+    red[tid] = tr;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            double s = L[j * R + j];
+            for (int k = 0; k < j; ++k) s -= L[j * R + k] * L[j * R + k];
+            L[j * R + j] = sqrt(s);
+        }
+        __syncthreads();
+        const int i = j + 1 + tid;
+        if (i < R) {
+            double s = L[i * R + j];
+            for (int k = 0; k < j; ++k) s -= L[i * R + k] * L[j * R + k];
+            L[i * R + j] = s / L[j * R + j];
+        }
+        __syncthreads();
+    }
+    if (tid < R) {  // column tid of L^{-1}
+        const int c = tid;
+        for (int i = 0; i < c; ++i) Li[i * R + c] = 0.0;
+        for (int i = c; i < R; ++i) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = c; k < i; ++k) s -= L[i * R + k] * Li[k * R + c];
+            Li[i * R + c] = s / L[i * R + i];
+        }
+    }
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        double ld = 0.0;
+        for (int j = 0; j < R; ++j) ld += 2.0 * log(L[j * R + j]);
+        loss[b] = (float)(red[0] - ld);
+    }
+    for (int i = tid; i < R * R; i += 256) {
+        const int a = i / R, c = i % R;
+        double s = 0.0;
+        for (int k = (a > c ? a : c); k < R; ++k) s += Li[k * R + a] * Li[k * R + c];
+        Mi[i] = s;
+    }
+    __syncthreads();
+    const double gs = (double)out_scale * (double)grad_scale;
+    float* out = dO + (int64_t)b * ldo;
+    for (int i = tid; i < dim * R; i += 256) {
+        const int r = i / R, c = i % R;
+        double x = 0.0;
+        for (int k = 0; k < R; ++k) x += V[r * R + k] * Mi[k * R + c];
+        out[i] = (float)(gs * (2.0 * SV[i] - 2.0 * x));
+    }
+}
+
+// Per-variant sample mean (FP64): samples (nv, rows, d).
+__global__ void mean_kernel(const float* __restrict__ X, int64_t rows, int d, double* __restrict__ mean)
+{
+    const int v = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    const float* Xv = X + (int64_t)v * rows * d;
+    double s = 0.0;
+    for (int64_t r = 0; r < rows; ++r) s += (double)Xv[r * d + j];
+    mean[(int64_t)v * d + j] = s / (double)rows;
+}
+
+// S[v] = scale / (rows - 1) * sum_r (x_r - mean)(x_r - mean)^T, 32 x 32
+// output tile per workgroup, FP64 accumulation, 64 centred rows staged per pass.
+__global__ __launch_bounds__(256) void cov_kernel(const float* __restrict__ X, int64_t rows, int d,
+                                                  const double* __restrict__ mean, double scale,
+                                                  float* __restrict__ S)
+{
+    __shared__ double xi[64][33];
+    __shared__ double xj[64][33];
+    const int v = blockIdx.z, ti = blockIdx.y * 32, tj = blockIdx.x * 32, tid = threadIdx.x;
+    const float* Xv = X + (int64_t)v * rows * d;
+    const double* mu = mean + (int64_t)v * d;
+    const int c = tid & 31, r0 = tid >> 5;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t s0 = 0; s0 < rows; s0 += 64) {
+        for (int i = tid; i < 64 * 32; i += 256) {
+            const int rr = i >> 5, cc = i & 31;
+            const int64_t row = s0 + rr;
+            double a = 0.0, bb = 0.0;
+            if (row < rows) {
+                if (ti + cc < d) a = (double)Xv[row * d + ti + cc] - mu[ti + cc];
+                if (tj + cc < d) bb = (double)Xv[row * d + tj + cc] - mu[tj + cc];
+            }
+            xi[rr][cc] = a;
+            xj[rr][cc] = bb;
+        }
+        __syncthreads();
+        for (int rr = 0; rr < 64; ++rr) {
+            const double bj = xj[rr][c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] += xi[rr][r0 + 8 * q] * bj;
+        }
+        __syncthreads();
+    }
+    const double f = scale / (double)(rows - 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = ti + r0 + 8 * q, j = tj + c;
+        if (i < d && j < d) S[((int64_t)v * d + i) * d + j] = (float)(acc[q] * f);
+    }
+}
+
+}  // namespace mlp
+}  // namespace naja
+
+using namespace naja;
+using namespace naja::mlp;
+
+struct naja_lp {
+    int F, dim, rank, H1, H2, O;
+    float out_scale;
+    int64_t n_params;
+    int64_t off_w1, off_b1, off_w2, off_b2, off_w3, off_b3;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+    bf16_t* pbf = nullptr;
+    int step = 0;
+    // activations for `cap` rows
+    int cap = 0;
+    float *h1 = nullptr, *h2 = nullptr, *o = nullptr, *d1 = nullptr, *d2 = nullptr, *dO = nullptr, *lossb = nullptr;
+};
+
+namespace {
+
+constexpr int kMaxNllDim = 160;  // LDS: (2 * dim + 3 * 32) * 32 doubles <= 106 KB
+
+void free_acts(naja_lp* h)
+{
+    for (float** q : {&h->h1, &h->h2, &h->o, &h->d1, &h->d2, &h->dO, &h->lossb}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    h->cap = 0;
+}
+
+void ensure_acts(naja_lp* h, int batch)
+{
+    if (batch <= h->cap) return;
+    NAJA_HIP(hipDeviceSynchronize());
+    free_acts(h);
+    const int cap = std::max(batch, 256);
+    NAJA_HIP(hipMalloc(&h->h1, sizeof(float) * cap * h->H1));
+    NAJA_HIP(hipMalloc(&h->h2, sizeof(float) * cap * h->H2));
+    NAJA_HIP(hipMalloc(&h->o, sizeof(float) * cap * h->O));
+    NAJA_HIP(hipMalloc(&h->d1, sizeof(float) * cap * h->H1));
+    NAJA_HIP(hipMalloc(&h->d2, sizeof(float) * cap * h->H2));
+    NAJA_HIP(hipMalloc(&h->dO, sizeof(float) * cap * h->O));
+    NAJA_HIP(hipMalloc(&h->lossb, sizeof(float) * cap));
+    h->cap = cap;
+}
+
+const bf16_t* wbf(const naja_lp* h, int64_t off) { return h->pbf + off; }
+
+// Layers 1-2 into h1, h2; layer 3 into out3 = alpha3 * (h2 W3^T + b3).
+void forward(naja_lp* h, const float* X, int B, hipStream_t s, float* out3, float alpha3)
+{
+    gemm<float, bf16_t, true, true, EPI_BIAS_RELU>(s, B, h->H1, h->F, X, h->F, wbf(h, h->off_w1), h->F, h->h1, h->H1,
+                                                   1.0f, h->p + h->off_b1);
+    gemm<float, bf16_t, true, true, EPI_BIAS_RELU>(s, B, h->H2, h->H1, h->h1, h->H1, wbf(h, h->off_w2), h->H1, h->h2,
+                                                   h->H2, 1.0f, h->p + h->off_b2);
+    gemm<float, bf16_t, true, true, EPI_BIAS>(s, B, h->O, h->H2, h->h2, h->H2, wbf(h, h->off_w3), h->H2, out3, h->O,
+                                              alpha3, h->p + h->off_b3);
+}
+
+void loss_and_backward(naja_lp* h, const float* X, const float* S, int B, float* loss, hipStream_t s)
+{
+    const size_t lds = sizeof(double) * (2 * (size_t)h->dim * 32 + 3 * 32 * 32);
+    hipLaunchKernelGGL((nll_kernel<32>), dim3(B), dim3(256), lds, s, h->o, (int64_t)h->O, S, h->dim, h->out_scale,
+                       1.0f / (float)B, loss ? loss : h->lossb, h->dO);
+    NAJA_HIP(hipGetLastError());
+    float* g = h->g;
+    // Layer 3: dW3 = dO^T h2, db3 = colsum dO, d2 = (dO W3) * relu'(z2).
+    gemm<float, float, false, false, EPI_NONE>(s, h->O, h->H2, B, h->dO, h->O, h->h2, h->H2, g + h->off_w3, h->H2);
+    hipLaunchKernelGGL(colsum_kernel, dim3((h->O + 255) / 256), dim3(256), 0, s, h->dO, B, h->O, g + h->off_b3);
+    gemm<float, bf16_t, true, false, EPI_MASK>(s, B, h->H2, h->O, h->dO, h->O, wbf(h, h->off_w3), h->H2, h->d2, h->H2,
+                                               1.0f, nullptr, h->h2, h->H2);
+    // Layer 2.
+    gemm<float, float, false, false, EPI_NONE>(s, h->H2, h->H1, B, h->d2, h->H2, h->h1, h->H1, g + h->off_w2, h->H1);
+    hipLaunchKernelGGL(colsum_kernel, dim3((h->H2 + 255) / 256), dim3(256), 0, s, h->d2, B, h->H2, g + h->off_b2);
+    gemm<float, bf16_t, true, false, EPI_MASK>(s, B, h->H1, h->H2, h->d2, h->H2, wbf(h, h->off_w2), h->H1, h->d1, h->H1,
+                                               1.0f, nullptr, h->h1, h->H1);
+    // Layer 1.
+    gemm<float, float, false, false, EPI_NONE>(s, h->H1, h->F, B, h->d1, h->H1, X, h->F, g + h->off_w1, h->F);
+    hipLaunchKernelGGL(colsum_kernel, dim3((h->H1 + 255) / 256), dim3(256), 0, s, h->d1, B, h->H1, g + h->off_b1);
+    NAJA_HIP(hipGetLastError());
+}
+
+void check_batch(const naja_lp* h, const float* X, int B)
+{
+    NAJA_REQUIRE(h && X && B > 0, NAJA_ERR_INVALID_ARGUMENT, "learned preconditioner: null handle/features or batch %d",
+                 B);
+}
+
+// splitmix64 stream for the initial weights.
+struct SplitMix {
+    uint64_t s;
+    double uniform()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+};
+
+}  // namespace
+
+extern "C" int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, uint64_t seed,
+                              float out_scale, naja_lp** out)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(out, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_create: null output");
+        NAJA_REQUIRE(n_features > 0 && dim > 0 && hidden1 > 0 && hidden2 > 0, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_lp_create: sizes must be positive");
+        NAJA_REQUIRE(rank == 32, NAJA_ERR_UNSUPPORTED, "naja_lp_create: rank %d (compiled for 32)", rank);
+        NAJA_REQUIRE(dim >= rank && dim <= kMaxNllDim, NAJA_ERR_UNSUPPORTED,
+                     "naja_lp_create: dim %d outside [%d, %d]", dim, rank, kMaxNllDim);
+        auto* h = new naja_lp();
+        h->F = n_features;
+        h->dim = dim;
+        h->rank = rank;
+        h->H1 = hidden1;
+        h->H2 = hidden2;
+        h->O = dim * rank;
+        h->out_scale = out_scale;
+        h->off_w1 = 0;
+        h->off_b1 = h->off_w1 + (int64_t)hidden1 * n_features;
+        h->off_w2 = h->off_b1 + hidden1;
+        h->off_b2 = h->off_w2 + (int64_t)hidden2 * hidden1;
+        h->off_w3 = h->off_b2 + hidden2;
+        h->off_b3 = h->off_w3 + (int64_t)h->O * hidden2;
+        h->n_params = h->off_b3 + h->O;
+        // He-uniform for the ReLU layers; the output layer starts 100x
+        // smaller so the initial map is close to the identity (V ~ 0).
+        std::vector<float> init(h->n_params, 0.0f);
+        SplitMix rng{seed};
+        auto fill = [&](int64_t off, int64_t count, double limit) {
+            for (int64_t i = 0; i < count; ++i) init[off + i] = (float)((2.0 * rng.uniform() - 1.0) * limit);
+        };
+        fill(h->off_w1, (int64_t)hidden1 * n_features, std::sqrt(6.0 / n_features));
+        fill(h->off_w2, (int64_t)hidden2 * hidden1, std::sqrt(6.0 / hidden1));
+        fill(h->off_w3, (int64_t)h->O * hidden2, 0.01 * std::sqrt(6.0 / hidden2));
+        try {
+            NAJA_HIP(hipMalloc(&h->p, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMalloc(&h->g, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMalloc(&h->m1, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMalloc(&h->m2, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMalloc(&h->pbf, sizeof(bf16_t) * h->n_params));
+            NAJA_HIP(hipMemcpy(h->p, init.data(), sizeof(float) * h->n_params, hipMemcpyHostToDevice));
+            NAJA_HIP(hipMemset(h->g, 0, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMemset(h->m1, 0, sizeof(float) * h->n_params));
+            NAJA_HIP(hipMemset(h->m2, 0, sizeof(float) * h->n_params));
+            hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, nullptr,
+                               h->p, h->pbf, h->n_params);
+            NAJA_HIP(hipGetLastError());
+            NAJA_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            for (void* q : {(void*)h->p, (void*)h->g, (void*)h->m1, (void*)h->m2, (void*)h->pbf})
+                if (q) (void)hipFree(q);
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+extern "C" int naja_lp_destroy(naja_lp* h)
+{
+    return guarded([&]() {
+        if (!h) return;
+        NAJA_HIP(hipDeviceSynchronize());
+        free_acts(h);
+        for (void* q : {(void*)h->p, (void*)h->g, (void*)h->m1, (void*)h->m2, (void*)h->pbf})
+            if (q) (void)hipFree(q);
+        delete h;
+    });
+}
+
+extern "C" int naja_lp_param_count(const naja_lp* h, int64_t* n)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(h && n, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_param_count: null argument");
+        *n = h->n_params;
+    });
+}
+
+extern "C" int naja_lp_get_params(const naja_lp* h, float* dst, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(h && dst, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_get_params: null argument");
+        NAJA_HIP(hipMemcpyAsync(dst, h->p, sizeof(float) * h->n_params, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    });
+}
+
+extern "C" int naja_lp_set_params(naja_lp* h, const float* src, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(h && src, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_set_params: null argument");
+        hipStream_t s = (hipStream_t)stream;
+        NAJA_HIP(hipMemcpyAsync(h->p, src, sizeof(float) * h->n_params, hipMemcpyDeviceToDevice, s));
+        NAJA_HIP(hipMemsetAsync(h->m1, 0, sizeof(float) * h->n_params, s));
+        NAJA_HIP(hipMemsetAsync(h->m2, 0, sizeof(float) * h->n_params, s));
+        hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, s, h->p, h->pbf,
+                           h->n_params);
+        NAJA_HIP(hipGetLastError());
+        h->step = 0;
+    });
+}
+
+extern "C" int naja_lp_forward(naja_lp* h, const float* features, int batch, float* V, void* stream)
+{
+    return guarded([&]() {
+        check_batch(h, features, batch);
+        NAJA_REQUIRE(V, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_forward: null output");
+        ensure_acts(h, batch);
+        forward(h, features, batch, (hipStream_t)stream, V, h->out_scale);
+    });
+}
+
+extern "C" int naja_lp_loss_grad(naja_lp* h, const float* features, const float* S, int batch, float* loss,
+                                 float* grad, void* stream)
+{
+    return guarded([&]() {
+        check_batch(h, features, batch);
+        NAJA_REQUIRE(S, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_loss_grad: null covariance");
+        ensure_acts(h, batch);
+        hipStream_t s = (hipStream_t)stream;
+        forward(h, features, batch, s, h->o, 1.0f);
+        loss_and_backward(h, features, S, batch, loss, s);
+        if (grad) NAJA_HIP(hipMemcpyAsync(grad, h->g, sizeof(float) * h->n_params, hipMemcpyDeviceToDevice, s));
+    });
+}
+
+extern "C" int naja_lp_train_step(naja_lp* h, const float* features, const float* S, int batch, float lr,
+                                  float* loss, void* stream)
+{
+    return guarded([&]() {
+        check_batch(h, features, batch);
+        NAJA_REQUIRE(S, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_train_step: null covariance");
+        NAJA_REQUIRE(lr > 0.0f, NAJA_ERR_INVALID_ARGUMENT, "naja_lp_train_step: learning rate must be > 0");
+        ensure_acts(h, batch);
+        hipStream_t s = (hipStream_t)stream;
+        forward(h, features, batch, s, h->o, 1.0f);
+        loss_and_backward(h, features, S, batch, loss, s);
+        h->step += 1;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, h->step)));
+        const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, h->step)));
+        hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((h->n_params + 255) / 256)), dim3(256), 0, s, h->p, h->g,
+                           h->m1, h->m2, h->pbf, h->n_params, lr, b1, b2, 1e-8f, c1, c2);
+        NAJA_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int naja_pilot_covariance(const float* samples, int n_variants, int64_t rows, int d, double scale,
+                                     float* S, double* mean, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(samples && S && mean && n_variants > 0 && d > 0, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_pilot_covariance: bad arguments");
+        NAJA_REQUIRE(rows >= 2, NAJA_ERR_INVALID_ARGUMENT, "naja_pilot_covariance: need at least 2 samples");
+        NAJA_REQUIRE(n_variants <= 65535, NAJA_ERR_UNSUPPORTED, "naja_pilot_covariance: at most 65535 variants");
+        hipStream_t s = (hipStream_t)stream;
+        hipLaunchKernelGGL(mean_kernel, dim3((d + 127) / 128, n_variants), dim3(128), 0, s, samples, rows, d, mean);
+        const int t = (d + 31) / 32;
+        hipLaunchKernelGGL(cov_kernel, dim3(t, t, n_variants), dim3(256), 0, s, samples, rows, d, mean, scale, S);
+        NAJA_HIP(hipGetLastError());
+    });
+}
+
+extern "C" int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, const float* A, int64_t lda,
+                              const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(A && B && C && M > 0 && N > 0 && K > 0, NAJA_ERR_INVALID_ARGUMENT, "naja_gemm_bf16: bad arguments");
+        hipStream_t s = (hipStream_t)stream;
+        // op(A) is M x K: row-major A (trans_a = 0) has k contiguous; op(B) is
+        // K x N: row-major B (trans_b = 0) has n contiguous.
+        if (!trans_a && !trans_b)
+            gemm<float, float, true, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+        else if (!trans_a && trans_b)
+            gemm<float, float, true, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+        else if (trans_a && !trans_b)
+            gemm<float, float, false, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+        else
+            gemm<float, float, false, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+    });
+}

@@ -212,10 +212,12 @@ int naja_precond_ellipsoid_batched(const double* A0, const double* b0, int m0, i
                                    const double* x_c, double* P, int nthreads);
 
 /*
- * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} from the learned
- * preconditioner's factors V (n_variants x d x rank). No reference counterpart.
+ * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion
+ * P_v = (I + V_v V_v^T)^{1/2} (expand = 1) from the learned preconditioner's
+ * factors V (n_variants x d x rank). No reference counterpart.
  */
-int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, double* P, int nthreads);
+int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, int expand, double* P,
+                         int nthreads);
 
 /* ---------------------------------------------------------------------------
  * Learned preconditioner (no reference counterpart: the reference's
@@ -231,8 +233,12 @@ int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, doubl
  * ------------------------------------------------------------------------- */
 typedef struct naja_lp naja_lp;
 
+/* Covariance models of naja_lp_create's `model`. */
+#define NAJA_LP_PRECISION 0  /* precision I + V V^T of S in base-polytope units; whitening (I + V V^T)^{-1/2} */
+#define NAJA_LP_COVARIANCE 1 /* S ~ tau (I + V V^T) in a base rounding's frame, tau free; update (I + V V^T)^{1/2} */
+
 /* rank must be 32 and rank <= dim <= 160 (NAJA_ERR_UNSUPPORTED otherwise). */
-int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, uint64_t seed,
+int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, int model, uint64_t seed,
                    float out_scale, naja_lp** out);
 int naja_lp_destroy(naja_lp* lp);
 int naja_lp_param_count(const naja_lp* lp, int64_t* n);
@@ -242,10 +248,11 @@ int naja_lp_set_params(naja_lp* lp, const float* src, void* stream);
 /* V: batch x dim x rank. */
 int naja_lp_forward(naja_lp* lp, const float* features, int batch, float* V, void* stream);
 /*
- * Gaussian negative log likelihood of the covariances S (batch x dim x dim,
- * in units of the base polytope's) under the precision I + V V^T:
- * loss[b] = tr(V^T S V) - logdet(I + V^T V); grad (nullable) = gradient of
- * the batch mean with respect to the parameter vector.
+ * Gaussian negative log likelihood of the covariances S (batch x dim x dim)
+ * under the handle's model, M = I + V^T V:
+ *   precision:  loss[b] = tr(V^T S V) - logdet M;
+ *   covariance: loss[b] = dim log(q / dim) + logdet M, q = tr S - tr(M^{-1} V^T S V).
+ * grad (nullable) = gradient of the batch mean with respect to the parameters.
  */
 int naja_lp_loss_grad(naja_lp* lp, const float* features, const float* S, int batch, float* loss,
                       float* grad, void* stream);

@@ -144,10 +144,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_dikin_batched.restype = i32
     lib.naja_precond_ellipsoid_batched.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, i32]
     lib.naja_precond_ellipsoid_batched.restype = i32
-    lib.naja_precond_lowrank.argtypes = [vp, i32, i32, i32, vp, i32]
+    lib.naja_precond_lowrank.argtypes = [vp, i32, i32, i32, i32, vp, i32]
     lib.naja_precond_lowrank.restype = i32
     f32 = ctypes.c_float
-    lib.naja_lp_create.argtypes = [i32, i32, i32, i32, i32, ctypes.c_uint64, f32, ctypes.POINTER(vp)]
+    lib.naja_lp_create.argtypes = [i32, i32, i32, i32, i32, i32, ctypes.c_uint64, f32, ctypes.POINTER(vp)]
     lib.naja_lp_destroy.argtypes = [vp]
     lib.naja_lp_param_count.argtypes = [vp, ctypes.POINTER(i64)]
     lib.naja_lp_get_params.argtypes = [vp, vp, vp]

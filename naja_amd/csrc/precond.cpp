@@ -9,8 +9,9 @@
 //   map y = c + L^{-T} z sends the Dikin ellipsoid {y : (y-c)^T H (y-c) <= 1}
 //   to the unit ball. The pilot chains of the learned preconditioner run in
 //   this frame.
-// * Low-rank whitening: P = (I + V V^T)^{-1/2} from a d x r factor V, through
-//   the r x r eigenproblem of V^T V (the learned preconditioner's output).
+// * Low-rank whitening: P = (I + V V^T)^{-1/2} (or the expansion
+//   (I + V V^T)^{1/2}) from a d x r factor V, through the r x r eigenproblem
+//   of V^T V (the learned preconditioner's output).
 //
 // All matrices are row-major FP64 in host memory. The sampler's rounding
 // transform takes P with y = c + P z, so every routine here returns that P.
@@ -87,14 +88,20 @@ void sym_eig(int n, const double* a, double* evals, double* evecs)
 
     // Implicit QL with Wilkinson-type shifts on the tridiagonal (diag, off),
     // off[i] coupling i and i+1; plane rotations accumulate into Q's columns.
+    // An off-diagonal entry counts as zero when it is negligible against the
+    // matrix's scale (not only against its neighbours: near-zero eigenvalues
+    // of a rank-deficient matrix would otherwise never deflate).
     const double eps = std::numeric_limits<double>::epsilon();
+    double tnorm = 0.0;
+    for (int i = 0; i < n; ++i)
+        tnorm = std::max(tnorm, std::fabs(diag[i]) + std::fabs(off[i]) + (i > 0 ? std::fabs(off[i - 1]) : 0.0));
     for (int l = 0; l < n; ++l) {
         int iter = 0;
         for (;;) {
             int m = l;
             for (; m + 1 < n; ++m) {
                 const double dd = std::fabs(diag[m]) + std::fabs(diag[m + 1]);
-                if (std::fabs(off[m]) <= eps * dd) break;
+                if (std::fabs(off[m]) <= eps * dd || std::fabs(off[m]) <= eps * tnorm) break;
             }
             if (m == l) break;
             NAJA_REQUIRE(++iter <= 64, NAJA_ERR_RUNTIME, "symmetric eigensolver did not converge (n=%d)", n);
@@ -333,9 +340,10 @@ void barrier_one(int m, int d, const double* A, const double* b, const double* x
         for (int c = 0; c < d; ++c) P[(size_t)r * d + c] = M[(size_t)c * d + r];
 }
 
-// P = (I + V V^T)^{-1/2} for V (d x r row-major): V^T V = R diag(lam) R^T,
-// Q = V R diag(lam)^{-1/2}, P = I + Q diag((1 + lam)^{-1/2} - 1) Q^T.
-void lowrank_one(int d, int r, const double* V, double* P)
+// P = (I + V V^T)^{p} for V (d x r row-major), p = -1/2 or +1/2:
+// V^T V = R diag(lam) R^T, Q = V R diag(lam)^{-1/2},
+// P = I + Q diag((1 + lam)^{p} - 1) Q^T.
+void lowrank_one(int d, int r, const double* V, double* P, bool expand)
 {
     std::vector<double> G((size_t)r * r, 0.0), lam(r), R((size_t)r * r);
     for (int i = 0; i < r; ++i)
@@ -357,7 +365,7 @@ void lowrank_one(int d, int r, const double* V, double* P)
             for (int t = 0; t < r; ++t) s += V[(size_t)k * r + t] * R[(size_t)t * r + j];
             q[k] = s * inv;
         }
-        const double coef = 1.0 / std::sqrt(1.0 + lam[j]) - 1.0;
+        const double coef = (expand ? std::sqrt(1.0 + lam[j]) : 1.0 / std::sqrt(1.0 + lam[j])) - 1.0;
         for (int a = 0; a < d; ++a) {
             const double ca = coef * q[a];
             for (int b = 0; b < d; ++b) P[(size_t)a * d + b] += ca * q[b];
@@ -482,13 +490,14 @@ extern "C" int naja_precond_ellipsoid_batched(const double* A0, const double* b0
     });
 }
 
-extern "C" int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, double* P, int nthreads)
+extern "C" int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, int expand, double* P,
+                                    int nthreads)
 {
     return guarded([&]() {
         NAJA_REQUIRE(V && P && n_variants > 0 && d > 0 && rank > 0 && rank <= d, NAJA_ERR_INVALID_ARGUMENT,
                      "naja_precond_lowrank: bad arguments");
         parallel_for(n_variants, nthreads, [&](int v) {
-            lowrank_one(d, rank, V + (size_t)v * d * rank, P + (size_t)v * d * d);
+            lowrank_one(d, rank, V + (size_t)v * d * rank, P + (size_t)v * d * d, expand != 0);
         });
     });
 }

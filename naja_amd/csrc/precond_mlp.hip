@@ -5,12 +5,12 @@
 //   z2 = h1 W2^T + b2, h2 = relu(z2)
 //   o  = h2 W3^T + b3, V = out_scale * reshape(o, dim, rank)
 //
-// The whitening it encodes is P = (I + V V^T)^{-1/2} (y = c + P z), i.e. the
-// precision model Pi = I + V V^T of the condition's covariance in units of
-// the base polytope's. Training minimises the Gaussian negative log
-// likelihood of the pilot covariance S under Pi,
-//   L(V) = tr(V^T S V) - logdet(I_r + V^T V)   (= tr(Pi S) - logdet Pi - tr S),
-// averaged over the batch; dL/dV = 2 S V - 2 V (I + V^T V)^{-1}.
+// Two covariance models are trained by the Gaussian negative log likelihood
+// of pilot covariances S (see nll_kernel), averaged over the batch:
+// * precision: Pi = I + V V^T in units of the base polytope; its whitening
+//   is P = (I + V V^T)^{-1/2};
+// * covariance: C = tau (I + V V^T) in the frame of a base rounding, tau
+//   profiled out (scale-free); its update of the base is P = (I + V V^T)^{1/2}.
 //
 // Every matrix product runs on v_mfma_f32_32x32x8bf16_1k: operands are
 // rounded to BF16 (round-to-nearest-even) as they are staged into LDS and
@@ -162,11 +162,19 @@ __global__ void to_bf16_kernel(const float* __restrict__ p, bf16_t* __restrict__
     if (i < n) pbf[i] = f2bf(p[i]);
 }
 
-// One workgroup per condition b: V = out_scale * O[b] (dim x R), S = S[b]
-// (dim x dim). loss[b] = tr(V^T S V) - logdet(M), M = I + V^T V;
-// dO[b] = out_scale * grad_scale * (2 S V - 2 V M^{-1}). FP64: M = L L^T by
-// Cholesky, L^{-1} column by column, M^{-1} = L^{-T} L^{-1}.
-template <int R>
+// One workgroup per condition b, FP64 throughout. V = out_scale * O[b]
+// (dim x R), S = S[b] (dim x dim), M = I + V^T V = L L^T (Cholesky),
+// M^{-1} = L^{-T} L^{-1} (L inverted column by column).
+//
+// MODEL 0, precision model Pi = I + V V^T (S in units of the base polytope):
+//   loss = tr(V^T S V) - logdet M,  dL/dV = 2 S V - 2 V M^{-1}.
+// MODEL 1, scale-free covariance model C = tau (I + V V^T), tau profiled out
+// (tau = q / dim):
+//   q = tr(C~^{-1} S) = tr S - tr(M^{-1} G), G = V^T S V,
+//   loss = dim log(q / dim) + logdet M,
+//   dL/dV = (dim / q) (2 V M^{-1} G M^{-1} - 2 S V M^{-1}) + 2 V M^{-1}.
+// dO[b] = out_scale * grad_scale * dL/dV.
+template <int R, int MODEL>
 __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, int64_t ldo,
                                                   const float* __restrict__ S, int dim, float out_scale,
                                                   float grad_scale, float* __restrict__ loss, float* __restrict__ dO)
@@ -177,7 +185,11 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, i
     double* L = SV + (size_t)dim * R;     // R x R (M, then its Cholesky factor)
     double* Li = L + R * R;               // R x R, L^{-1}
     double* Mi = Li + R * R;              // R x R, M^{-1}
+    double* G = Mi + R * R;               // R x R, V^T S V        (MODEL 1)
+    double* T = G + R * R;                // R x R, M^{-1} G       (MODEL 1)
+    double* K = T + R * R;                // R x R, M^{-1} G M^{-1} (MODEL 1)
     __shared__ double red[256];
+    __shared__ double red2[256];
     const int b = blockIdx.x, tid = threadIdx.x;
     const float* Ob = O + (int64_t)b * ldo;
     const float* Sb = S + (int64_t)b * dim * dim;
@@ -192,6 +204,9 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, i
         SV[i] = s;
         tr += V[i] * s;
     }
+    double trs = 0.0;
+    if (MODEL == 1)
+        for (int i = tid; i < dim; i += 256) trs += (double)Sb[(int64_t)i * dim + i];
     for (int i = tid; i < R * R; i += 256) {
         const int a = i / R, c = i % R;
         double s = (a == c) ? 1.0 : 0.0;
@@ -199,7 +214,16 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, i
         L[i] = s;
     }
     red[tid] = tr;
+    red2[tid] = trs;
     __syncthreads();
+    if (MODEL == 1) {
+        for (int i = tid; i < R * R; i += 256) {
+            const int a = i / R, c = i % R;
+            double s = 0.0;
+            for (int k = 0; k < dim; ++k) s += V[k * R + a] * SV[k * R + c];
+            G[i] = s;
+        }
+    }
     for (int j = 0; j < R; ++j) {
         if (tid == 0) {
             double s = L[j * R + j];
@@ -225,14 +249,14 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, i
         }
     }
     for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
+        if (tid < s) {
+            red[tid] += red[tid + s];
+            red2[tid] += red2[tid + s];
+        }
         __syncthreads();
     }
-    if (tid == 0) {
-        double ld = 0.0;
-        for (int j = 0; j < R; ++j) ld += 2.0 * log(L[j * R + j]);
-        loss[b] = (float)(red[0] - ld);
-    }
+    double logdet = 0.0;
+    for (int j = 0; j < R; ++j) logdet += 2.0 * log(L[j * R + j]);
     for (int i = tid; i < R * R; i += 256) {
         const int a = i / R, c = i % R;
         double s = 0.0;
@@ -242,11 +266,50 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O, i
     __syncthreads();
     const double gs = (double)out_scale * (double)grad_scale;
     float* out = dO + (int64_t)b * ldo;
-    for (int i = tid; i < dim * R; i += 256) {
-        const int r = i / R, c = i % R;
-        double x = 0.0;
-        for (int k = 0; k < R; ++k) x += V[r * R + k] * Mi[k * R + c];
-        out[i] = (float)(gs * (2.0 * SV[i] - 2.0 * x));
+    if (MODEL == 0) {
+        if (tid == 0) loss[b] = (float)(red[0] - logdet);
+        for (int i = tid; i < dim * R; i += 256) {
+            const int r = i / R, c = i % R;
+            double x = 0.0;
+            for (int k = 0; k < R; ++k) x += V[r * R + k] * Mi[k * R + c];
+            out[i] = (float)(gs * (2.0 * SV[i] - 2.0 * x));
+        }
+    } else {
+        for (int i = tid; i < R * R; i += 256) {
+            const int a = i / R, c = i % R;
+            double s = 0.0;
+            for (int k = 0; k < R; ++k) s += Mi[a * R + k] * G[k * R + c];
+            T[i] = s;
+        }
+        __syncthreads();
+        double tmg = 0.0;
+        for (int i = tid; i < R * R; i += 256) {
+            const int a = i / R, c = i % R;
+            double s = 0.0;
+            for (int k = 0; k < R; ++k) s += T[a * R + k] * Mi[k * R + c];
+            K[i] = s;
+            if (a == c) tmg += T[i];
+        }
+        __syncthreads();  // red[] is free again: the sums above were read before
+        red[tid] = tmg;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
+        }
+        const double q = red2[0] - red[0];
+        const double fd = (double)dim / q;
+        if (tid == 0) loss[b] = (float)((double)dim * log(q / (double)dim) + logdet);
+        for (int i = tid; i < dim * R; i += 256) {
+            const int r = i / R, c = i % R;
+            double vk = 0.0, svm = 0.0, vm = 0.0;
+            for (int k = 0; k < R; ++k) {
+                vk += V[r * R + k] * K[k * R + c];
+                svm += SV[r * R + k] * Mi[k * R + c];
+                vm += V[r * R + k] * Mi[k * R + c];
+            }
+            out[i] = (float)(gs * (fd * (2.0 * vk - 2.0 * svm) + 2.0 * vm));
+        }
     }
 }
 
@@ -311,6 +374,7 @@ using namespace naja::mlp;
 
 struct naja_lp {
     int F, dim, rank, H1, H2, O;
+    int model;
     float out_scale;
     int64_t n_params;
     int64_t off_w1, off_b1, off_w2, off_b2, off_w3, off_b3;
@@ -324,7 +388,7 @@ struct naja_lp {
 
 namespace {
 
-constexpr int kMaxNllDim = 160;  // LDS: (2 * dim + 3 * 32) * 32 doubles <= 106 KB
+constexpr int kMaxNllDim = 160;  // LDS: (2 * dim + 6 * 32) * 32 doubles <= 128 KB
 
 void free_acts(naja_lp* h)
 {
@@ -366,9 +430,10 @@ void forward(naja_lp* h, const float* X, int B, hipStream_t s, float* out3, floa
 
 void loss_and_backward(naja_lp* h, const float* X, const float* S, int B, float* loss, hipStream_t s)
 {
-    const size_t lds = sizeof(double) * (2 * (size_t)h->dim * 32 + 3 * 32 * 32);
-    hipLaunchKernelGGL((nll_kernel<32>), dim3(B), dim3(256), lds, s, h->o, (int64_t)h->O, S, h->dim, h->out_scale,
-                       1.0f / (float)B, loss ? loss : h->lossb, h->dO);
+    const size_t lds = sizeof(double) * (2 * (size_t)h->dim * 32 + 6 * 32 * 32);
+    auto fn = (h->model == NAJA_LP_COVARIANCE) ? nll_kernel<32, 1> : nll_kernel<32, 0>;
+    hipLaunchKernelGGL(fn, dim3(B), dim3(256), lds, s, h->o, (int64_t)h->O, S, h->dim, h->out_scale, 1.0f / (float)B,
+                       loss ? loss : h->lossb, h->dO);
     NAJA_HIP(hipGetLastError());
     float* g = h->g;
     // Layer 3: dW3 = dO^T h2, db3 = colsum dO, d2 = (dO W3) * relu'(z2).
@@ -408,7 +473,7 @@ struct SplitMix {
 
 }  // namespace
 
-extern "C" int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, uint64_t seed,
+extern "C" int naja_lp_create(int n_features, int dim, int rank, int hidden1, int hidden2, int model, uint64_t seed,
                               float out_scale, naja_lp** out)
 {
     return guarded([&]() {
@@ -416,6 +481,8 @@ extern "C" int naja_lp_create(int n_features, int dim, int rank, int hidden1, in
         NAJA_REQUIRE(n_features > 0 && dim > 0 && hidden1 > 0 && hidden2 > 0, NAJA_ERR_INVALID_ARGUMENT,
                      "naja_lp_create: sizes must be positive");
         NAJA_REQUIRE(rank == 32, NAJA_ERR_UNSUPPORTED, "naja_lp_create: rank %d (compiled for 32)", rank);
+        NAJA_REQUIRE(model == NAJA_LP_PRECISION || model == NAJA_LP_COVARIANCE, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_lp_create: unknown model %d", model);
         NAJA_REQUIRE(dim >= rank && dim <= kMaxNllDim, NAJA_ERR_UNSUPPORTED,
                      "naja_lp_create: dim %d outside [%d, %d]", dim, rank, kMaxNllDim);
         auto* h = new naja_lp();
@@ -426,6 +493,7 @@ extern "C" int naja_lp_create(int n_features, int dim, int rank, int hidden1, in
         h->H2 = hidden2;
         h->O = dim * rank;
         h->out_scale = out_scale;
+        h->model = model;
         h->off_w1 = 0;
         h->off_b1 = h->off_w1 + (int64_t)hidden1 * n_features;
         h->off_w2 = h->off_b1 + hidden1;

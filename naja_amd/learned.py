@@ -1,9 +1,10 @@
 """Learned preconditioner and pilot-chain statistics (ctypes over libnaja_amd.so).
 
 `LearnedPreconditioner` owns an MLP on the GPU: extra-bound features ->
-hidden1 -> hidden2 -> a dim x rank factor V, whose whitening
-P = (I + V V^T)^{-1/2} is applied by the sampler's rounding transform
-(y = c + P z). Training, inference and the covariance of pilot chains run in
+hidden1 -> hidden2 -> a dim x rank factor V. With the precision model its
+whitening is P = (I + V V^T)^{-1/2}; with the covariance model it updates a
+base rounding P0 as P0 (I + V V^T)^{1/2}. The sampler's rounding transform
+applies the result (y = c + P z). Training, inference and the covariance of pilot chains run in
 the library's own HIP kernels; tensors are torch device tensors.
 """
 from __future__ import annotations
@@ -14,13 +15,17 @@ from . import NajaInvalidArgument, NAJA_ERR_INVALID_ARGUMENT, _check, _dev, _str
 
 
 class LearnedPreconditioner:
-    def __init__(self, n_features=300, dim=100, rank=32, hidden1=512, hidden2=512, seed=0, out_scale=64.0):
+    PRECISION, COVARIANCE = 0, 1
+
+    def __init__(self, n_features=300, dim=100, rank=32, hidden1=512, hidden2=512, seed=0, out_scale=64.0,
+                 model=0):
         self.n_features, self.dim, self.rank = n_features, dim, rank
         self.hidden1, self.hidden2, self.out_scale = hidden1, hidden2, float(out_scale)
+        self.model = int(model)
         lib = load_library()
         h = ctypes.c_void_p()
-        _check(lib.naja_lp_create(n_features, dim, rank, hidden1, hidden2, ctypes.c_uint64(seed), self.out_scale,
-                                  ctypes.byref(h)))
+        _check(lib.naja_lp_create(n_features, dim, rank, hidden1, hidden2, self.model, ctypes.c_uint64(seed),
+                                  self.out_scale, ctypes.byref(h)))
         self._h = h
         n = ctypes.c_int64()
         _check(lib.naja_lp_param_count(h, ctypes.byref(n)))

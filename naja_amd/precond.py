@@ -4,7 +4,8 @@
   with the library's own symmetric eigensolver;
 * `ellipsoid_batched`: whitening by the full Dikin ellipsoid at the centre
   (H = A^T diag(1/s^2) A = L L^T, P = L^{-T});
-* `lowrank`: P = (I + V V^T)^{-1/2} from the learned preconditioner's factors.
+* `lowrank`: P = (I + V V^T)^{-1/2}, or the expansion (I + V V^T)^{1/2}, from
+  the learned preconditioner's factors.
 
 Every P returned maps the sampler's variable to the model's: y = x_c + P z,
 which is the form `coordinate_hit_and_run_batched(..., P=, c=)` takes.
@@ -83,10 +84,10 @@ def ellipsoid_batched(A0, b0, extra_A, extra_b, x_c=None, nthreads=0):
     return P
 
 
-def lowrank(V, nthreads=0):
-    """P_v = (I + V_v V_v^T)^{-1/2} for V (n_variants, d, rank)."""
+def lowrank(V, expand=False, nthreads=0):
+    """P_v = (I + V_v V_v^T)^{-1/2} (or ^{+1/2} with expand) for V (n_variants, d, rank)."""
     V = _f64(V)
     nv, d, r = V.shape
     P = np.empty((nv, d, d))
-    _check(load_library().naja_precond_lowrank(_ptr(V), nv, d, r, _ptr(P), int(nthreads)))
+    _check(load_library().naja_precond_lowrank(_ptr(V), nv, d, r, int(bool(expand)), _ptr(P), int(nthreads)))
     return P

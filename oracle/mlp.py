@@ -57,13 +57,36 @@ def nll(V, S):
     return loss, dV
 
 
-def loss_grad(p, X, S, F, H1, H2, dim, rank, out_scale, bf=True):
-    """(per-condition loss, gradient of the batch-mean loss w.r.t. p)."""
+def nll_cov(V, S):
+    """Scale-free covariance model C = tau (I + V V^T), tau profiled out:
+    q = tr S - tr(M^{-1} V^T S V), loss_b = d log(q/d) + logdet M, written
+    here through the explicit d x d inverse of I + V V^T (an independent route
+    to the same value), with the gradient
+    (d/q) (2 V M^{-1} G M^{-1} - 2 S V M^{-1}) + 2 V M^{-1}."""
+    V = np.asarray(V, dtype=np.float64)
+    S = np.asarray(S, dtype=np.float64)
+    B, d, r = V.shape
+    loss = np.empty(B)
+    dV = np.empty_like(V)
+    for b in range(B):
+        C = np.eye(d) + V[b] @ V[b].T
+        q = np.trace(np.linalg.solve(C, S[b]))
+        M = np.eye(r) + V[b].T @ V[b]
+        Mi = np.linalg.inv(M)
+        G = V[b].T @ S[b] @ V[b]
+        loss[b] = d * np.log(q / d) + np.linalg.slogdet(C)[1]
+        dV[b] = (d / q) * (2 * V[b] @ Mi @ G @ Mi - 2 * S[b] @ V[b] @ Mi) + 2 * V[b] @ Mi
+    return loss, dV
+
+
+def loss_grad(p, X, S, F, H1, H2, dim, rank, out_scale, bf=True, model=0):
+    """(per-condition loss, gradient of the batch-mean loss w.r.t. p);
+    model 0 = precision (nll), 1 = scale-free covariance (nll_cov)."""
     q = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     W1, b1, W2, b2, W3, b3 = split(p, F, H1, H2, dim * rank)
     V, c = forward(p, X, F, H1, H2, dim, rank, out_scale, bf)
     B = V.shape[0]
-    loss, dV = nll(V, S)
+    loss, dV = (nll_cov if model == 1 else nll)(V, S)
     dO = (out_scale / B) * dV.reshape(B, -1)
     gW3 = q(dO).T @ q(c["h2"])
     gb3 = dO.sum(0)

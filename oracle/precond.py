@@ -81,9 +81,9 @@ def ellipsoid(A, b, x_c=None):
     return np.linalg.inv(L).T
 
 
-def lowrank(V):
-    """(I + V V^T)^{-1/2} through the full d x d eigenproblem."""
+def lowrank(V, expand=False):
+    """(I + V V^T)^{-1/2} (or ^{+1/2}) through the full d x d eigenproblem."""
     V = np.asarray(V, dtype=np.float64)
     M = np.eye(V.shape[0]) + V @ V.T
     ev, Q = np.linalg.eigh(M)
-    return (Q / np.sqrt(ev)) @ Q.T
+    return (Q * (np.sqrt(ev) if expand else 1.0 / np.sqrt(ev))) @ Q.T

@@ -76,19 +76,27 @@ def test_forward_matches_oracle(gpu, net):
     assert err < 1e-4
 
 
-def test_loss_and_gradients_match_oracle(gpu, net):
+@pytest.mark.parametrize("model", [0, 1])
+def test_loss_and_gradients_match_oracle(gpu, net, model):
+    from naja_amd.learned import LearnedPreconditioner
+
     rng = np.random.default_rng(6)
     X, S = make_problem(rng, 24)
+    if model == 1:
+        S = (S * rng.uniform(0.1, 10.0, (24, 1, 1))).astype(np.float32)  # arbitrary scale per condition
+        net = LearnedPreconditioner(seed=12, model=1, out_scale=4.0)
     p = net.params().cpu().numpy()
     loss, g = net.loss_grad(gpu.from_numpy(X).cuda(), gpu.from_numpy(S).cuda())
     loss, g = loss.cpu().numpy(), g.cpu().numpy()
-    lo, go = orc.loss_grad(p, X, S, 300, 512, 512, 100, 32, net.out_scale, bf=True)
+    lo, go = orc.loss_grad(p, X, S, 300, 512, 512, 100, 32, net.out_scale, bf=True, model=model)
     assert rel_l2(loss, lo) < 1e-4
     names = ["W1", "b1", "W2", "b2", "W3", "b3"]
     for name, a, b in zip(names, net.split(g), net.split(go)):
         err = rel_l2(a, b)
-        print(f"grad {name} rel L2 {err:.2e}", flush=True)
+        print(f"model {model} grad {name} rel L2 {err:.2e}", flush=True)
         assert err < 2e-3, name
+    if model == 1:
+        net.close()
 
 
 def test_adam_step_matches_oracle(gpu, net):

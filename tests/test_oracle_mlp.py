@@ -77,3 +77,36 @@ def test_adam_first_step_is_signed_lr():
     g = np.array([3.0, -0.5, 1e-3, 0.0])
     p, _, _ = mlp.adam(np.zeros(4), g, np.zeros(4), np.zeros(4), 1, lr=1e-2)
     np.testing.assert_allclose(p, -1e-2 * g / (np.abs(g) + 1e-8))
+
+
+def test_nll_cov_minimiser_is_scale_free():
+    """For S = tau (I + U U^T) the loss is stationary at V = U for every tau;
+    scaling S shifts the loss by the constant d log(tau)."""
+    rng = np.random.default_rng(3)
+    d, r = 8, 3
+    U = rng.standard_normal((d, r))
+    base = np.eye(d) + U @ U.T
+    l1, g1 = mlp.nll_cov(U[None], base[None])
+    l2, g2 = mlp.nll_cov(U[None], (7.5 * base)[None])
+    assert np.abs(g1).max() < 1e-10 and np.abs(g2).max() < 1e-10
+    assert l2[0] - l1[0] == pytest.approx(d * np.log(7.5))
+    l3, _ = mlp.nll_cov((0.8 * U)[None], base[None])
+    assert l3[0] > l1[0]
+
+
+def test_nll_cov_gradient_finite_differences():
+    rng = np.random.default_rng(4)
+    B, d, r = 2, 6, 3
+    V = rng.standard_normal((B, d, r))
+    A = rng.standard_normal((B, d, d))
+    S = A @ A.transpose(0, 2, 1) / d + 0.2 * np.eye(d)
+    _, dV = mlp.nll_cov(V, S)
+    h = 1e-6
+    for b in range(B):
+        for i in range(d):
+            for j in range(r):
+                Vp, Vm = V.copy(), V.copy()
+                Vp[b, i, j] += h
+                Vm[b, i, j] -= h
+                fd = (mlp.nll_cov(Vp, S)[0][b] - mlp.nll_cov(Vm, S)[0][b]) / (2 * h)
+                assert dV[b, i, j] == pytest.approx(fd, rel=1e-6, abs=1e-7)

@@ -19,7 +19,7 @@ def conditioned_cube(d, nslab, w_lo, w_hi, seed):
 
 
 @pytest.mark.parametrize("n,kind", [(1, "rand"), (2, "rand"), (7, "rand"), (64, "rand"), (100, "rank3"),
-                                    (30, "repeated"), (16, "diag"), (40, "identity")])
+                                    (30, "repeated"), (16, "diag"), (40, "identity"), (100, "barrier")])
 def test_symmetric_eigen_matches_lapack(n, kind):
     rng = np.random.default_rng(n)
     if kind == "rand":
@@ -33,6 +33,12 @@ def test_symmetric_eigen_matches_lapack(n, kind):
         a = (Q * np.repeat([1.0, 2.0, 5.0], n // 3)) @ Q.T
     elif kind == "diag":
         a = np.diag(rng.standard_normal(n))
+    elif kind == "barrier":
+        # Rank-deficient Hessian with weights spanning 1e0..1e6 (a Dikin
+        # tight set): most eigenvalues are zero up to rounding.
+        N = rng.standard_normal((40, n))
+        N /= np.linalg.norm(N, axis=1, keepdims=True)
+        a = (N.T * 10 ** rng.uniform(0, 6, 40)) @ N
     else:
         a = np.eye(n)
     w, V = pc.symmetric_eigen(a)
@@ -145,14 +151,20 @@ def test_ellipsoid_rejects_exterior_centre():
         pc.ellipsoid_batched(A0, b0, [np.zeros((0, d))], [np.zeros(0)], x_c=np.array([[1.0, 0.0, 0.0]]))
 
 
+@pytest.mark.parametrize("expand", [False, True])
 @pytest.mark.parametrize("d,r,scale", [(20, 4, 1.0), (50, 32, 30.0), (8, 8, 1e-3)])
-def test_lowrank_inverse_sqrt(d, r, scale):
+def test_lowrank_sqrt(d, r, scale, expand):
+    """P = (I + V V^T)^{-1/2}: P P (I + V V^T) = I; the expansion ^{+1/2}:
+    P P = I + V V^T."""
     rng = np.random.default_rng(d + r)
     V = rng.standard_normal((3, d, r)) * scale
     V[1, :, r // 2:] = 0.0  # rank-deficient factor
-    P = pc.lowrank(V)
+    P = pc.lowrank(V, expand=expand)
     for v in range(3):
         M = np.eye(d) + V[v] @ V[v].T
-        np.testing.assert_allclose(P[v], P[v].T, atol=1e-12)
-        np.testing.assert_allclose(P[v] @ P[v] @ M, np.eye(d), atol=1e-9)
-        np.testing.assert_allclose(P[v], orc.lowrank(V[v]), atol=1e-10)
+        np.testing.assert_allclose(P[v], P[v].T, atol=1e-12 * max(1.0, np.abs(P[v]).max()))
+        if expand:
+            np.testing.assert_allclose(P[v] @ P[v], M, atol=1e-9 * np.abs(M).max())
+        else:
+            np.testing.assert_allclose(P[v] @ P[v] @ M, np.eye(d), atol=1e-9)
+        np.testing.assert_allclose(P[v], orc.lowrank(V[v], expand), atol=1e-10 * max(1.0, np.abs(P[v]).max()))

@@ -239,11 +239,13 @@ def genome_scale_plain(np, torch):
     sec = time.perf_counter() - t0
     chr_ms, bm_ms, dg_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
     min_ess = float(ess.min())
+    # Bound check on every sample: max over rows of A y - b (FP64).
+    viol = float((S.double() @ A.double().T - b[None, :]).max())
     b_cs = 4.0 * m * (2.0 + 1.0 + pp) + 4.0 * d / thin
     ach = b_cs * nch * nspc * thin / (chr_ms * 1e-3) / 1e9
     log(f"genome-scale plain: chr {chr_ms:.1f} ms, backmap {bm_ms:.2f} ms, diag {dg_ms:.2f} ms; min-ESS {min_ess:.2f}, "
-        f"max R-hat {float(rhat.max()):.3f}")
-    return {"value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3), "chr_ms": round(chr_ms, 3),
+        f"max R-hat {float(rhat.max()):.3f}, max(Ay-b) {viol:.2e}")
+    return {"max_bound_violation": viol, "value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3), "chr_ms": round(chr_ms, 3),
             "backmap_ms": round(bm_ms, 3), "diag_ms": round(dg_ms, 3), "min_ess": round(min_ess, 3),
             "rhat_max": round(float(rhat.max()), 4), "us_per_step": round(chr_ms * 1e3 / (nspc * thin), 4),
             "algorithmic_GBs": round(ach, 1), "bytes_per_chain_step": round(b_cs, 1),

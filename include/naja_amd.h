@@ -160,6 +160,16 @@ int naja_backmap(const float* T, const float* shift, const float* Y, float* V,
 int naja_diag(const float* samples, int dim, int n_total, int nchains, int layout,
               float* ess, float* rhat, float* ess_chain, void* stream);
 
+/*
+ * naja_diag over n_variants sample sets stored back to back (variant v at
+ * samples + v * n_total * dim), in one set of launches. ess, rhat:
+ * n_variants x dim; ess_min, rhat_max (nullable): per variant, the minimum
+ * over dims of ess and the maximum of rhat. No reference counterpart (the
+ * reference validates one sample file at a time, validate_cli.cpp:49-296).
+ */
+int naja_diag_batched(const float* samples, int n_variants, int dim, int n_total, int nchains, int layout,
+                      float* ess, float* rhat, float* ess_min, float* rhat_max, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Rounding preconditioners (host, FP64, row-major host memory). Every P
  * returned here is the sampler's rounding transform: y = x_c + P z.

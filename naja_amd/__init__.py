@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "naja_chr_plan_destroy",
     "naja_backmap",
     "naja_diag",
+    "naja_diag_batched",
     "naja_symmetric_eigen",
     "naja_precond_dikin",
     "naja_precond_dikin_batched",
@@ -134,6 +135,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_backmap.restype = i32
     lib.naja_diag.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
     lib.naja_diag.restype = i32
+    lib.naja_diag_batched.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp]
+    lib.naja_diag_batched.restype = i32
     f64 = ctypes.c_double
     lib.naja_symmetric_eigen.argtypes = [vp, i32, vp, vp]
     lib.naja_symmetric_eigen.restype = i32
@@ -422,3 +425,19 @@ def diagnostics(samples, nchains, layout=LAYOUT_INTERLEAVED, stream=None):
     _check(lib.naja_diag(S.data_ptr(), dim, n_total, nchains, layout, ess.data_ptr(), rhat.data_ptr(),
                          ess_chain.data_ptr(), _stream_ptr(stream)))
     return ess, rhat, ess_chain
+
+
+def diagnostics_batched(samples, nchains, layout=LAYOUT_INTERLEAVED, stream=None):
+    """diagnostics() of every variant of samples (n_variants, n_total, dim) in
+    one set of launches. Returns (ess[v, dim], rhat[v, dim], ess_min[v], rhat_max[v])."""
+    torch = _torch()
+    lib = load_library()
+    S = _dev(samples, torch.float32)
+    nv, n_total, dim = S.shape
+    ess = torch.empty((nv, dim), dtype=torch.float32, device=S.device)
+    rhat = torch.empty((nv, dim), dtype=torch.float32, device=S.device)
+    emin = torch.empty(nv, dtype=torch.float32, device=S.device)
+    rmax = torch.empty(nv, dtype=torch.float32, device=S.device)
+    _check(lib.naja_diag_batched(S.data_ptr(), nv, dim, n_total, nchains, layout, ess.data_ptr(), rhat.data_ptr(),
+                                 emin.data_ptr(), rmax.data_ptr(), _stream_ptr(stream)))
+    return ess, rhat, emin, rmax

@@ -34,7 +34,7 @@ import time
 
 import numpy as np
 
-from . import coordinate_hit_and_run_batched, diagnostics
+from . import coordinate_hit_and_run_batched, diagnostics_batched
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
@@ -94,16 +94,9 @@ def sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync=1000, fi
 
 
 def variant_diagnostics(Y, nchains):
-    """Per variant: (min over dims of the min-over-chains ESS, max split-R-hat)."""
-    import torch
-
-    nv = Y.shape[0]
-    ess_min = torch.empty(nv, dtype=torch.float32, device=Y.device)
-    rhat_max = torch.empty(nv, dtype=torch.float32, device=Y.device)
-    for v in range(nv):
-        e, r, _ = diagnostics(Y[v], nchains)
-        ess_min[v] = e.min()
-        rhat_max[v] = r.max()
+    """Per variant: (min over dims of the min-over-chains ESS, max split-R-hat),
+    all variants in one set of launches."""
+    _, _, ess_min, rhat_max = diagnostics_batched(Y, nchains)
     return ess_min.cpu().numpy(), rhat_max.cpu().numpy()
 
 

@@ -6,7 +6,10 @@ namespace naja {
 
 // Largest chain length the LDS-resident FFT handles (N/2 complex points in LDS).
 int diag_max_chain_length();
+// n_variants sample blocks of n_total x dim back to back; ess, rhat per
+// variant and dim; optional per-variant min ESS / max R-hat.
 void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
-                 float* ess_chain, hipStream_t stream);
+                 float* ess_chain, hipStream_t stream, int n_variants = 1, float* ess_min = nullptr,
+                 float* rhat_max = nullptr);
 
 }  // namespace naja

@@ -13,7 +13,11 @@
 //     (bit-reversed in, natural out), so no reordering pass is needed; the
 //     interleaved result is the unnormalised autocorrelation, scaled by
 //     1/(N n) as mcmc.cpp:49-53 scales kissfft's by 1/(2n n);
-//   * truncated pair-sum tau in float, sequential as mcmc.cpp:66-72.
+//   * truncated pair-sum tau (mcmc.cpp:66-72): the pair sums
+//     rho_{2k-1} + rho_{2k} are formed in parallel, the truncation point is
+//     the first negative pair (a workgroup minimum), and the pairs before it
+//     are summed in parallel in FP64 (the reference adds them one by one in
+//     float; the two agree to float rounding).
 // Split-R-hat (mcmc.cpp:126-194): FP64 sums per (dim, half chain), then the
 // reference's B / W / var_hat formula per dim.
 #include <hip/hip_runtime.h>
@@ -29,7 +33,7 @@ namespace naja {
 
 namespace {
 
-constexpr int kDiagThreads = 256;
+constexpr int kDiagThreads = 1024;  // 16 waves: the one workgroup per CU hides latency
 constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (128 KB)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
@@ -93,8 +97,11 @@ __device__ __forceinline__ int bitrev(int x, int bits)
 __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ samples, int dim,
                                                                   int nchains, int spc, int layout, int logH,
                                                                   const float2* __restrict__ tw,
-                                                                  float* __restrict__ ess_chain)
+                                                                  float* __restrict__ ess_chain, int64_t vstride)
 {
+    // Variant blockIdx.y: its samples start vstride floats further on.
+    samples += (int64_t)blockIdx.y * vstride;
+    ess_chain += (int64_t)blockIdx.y * dim * nchains;
     extern __shared__ float2 z[];
     __shared__ double red[kDiagThreads / 64];
     __shared__ float meanf_s;
@@ -176,31 +183,47 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     ifft_dit(z, H, logH, tw);
 
     // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
-    if (tid == 0) {
-        const float* raw = reinterpret_cast<const float*>(z);
-        const float norm = 1.0f / ((float)N * (float)n);
-        const float var = raw[0] * norm;
-        float e;
-        if (var < 1e-30f) {
-            e = 1.0f;
-        } else {
-            float tau = 1.0f;
-            for (int k = 1; k < n / 2; ++k) {
-                const float a1 = raw[2 * k - 1] * norm;
-                const float a2 = raw[2 * k] * norm;
-                const float pair_sum = a1 / var + a2 / var;
-                if (pair_sum < 0.0f) break;
-                tau += 2.0f * pair_sum;
-            }
-            tau = fmaxf(tau, 1.0f);
-            e = (float)n / tau;
+    __shared__ int kfirst[kDiagThreads / 64];
+    __shared__ double psum[kDiagThreads / 64];
+    const float* raw = reinterpret_cast<const float*>(z);
+    const float norm = 1.0f / ((float)N * (float)n);
+    const float var = raw[0] * norm;
+    const int kend = n / 2;  // pairs k = 1 .. kend-1
+    if (var < 1e-30f) {
+        if (tid == 0) ess_chain[(int64_t)d * nchains + c] = 1.0f;
+        return;  // uniform across the workgroup: no barrier follows for anyone
+    }
+    int kneg = kend;
+    for (int k = 1 + tid; k < kend; k += blockDim.x) {
+        const float pair_sum = (raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var;
+        if (pair_sum < 0.0f) {
+            kneg = k;
+            break;
         }
-        ess_chain[(int64_t)d * nchains + c] = e;
+    }
+    for (int off = 32; off > 0; off >>= 1) kneg = min(kneg, __shfl_xor(kneg, off));
+    if ((tid & 63) == 0) kfirst[tid >> 6] = kneg;
+    __syncthreads();
+    int K = kend;
+    for (int w = 0; w < kDiagThreads / 64; ++w) K = min(K, kfirst[w]);
+    double acc = 0.0;
+    for (int k = 1 + tid; k < K; k += blockDim.x)
+        acc += (double)((raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((tid & 63) == 0) psum[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kDiagThreads / 64; ++w) sum += psum[w];
+        const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
+        ess_chain[(int64_t)d * nchains + c] = (float)n / tau;
     }
 }
 
 __global__ void ess_min_kernel(const float* __restrict__ ess_chain, int dim, int nchains, float* __restrict__ ess)
 {
+    ess_chain += (int64_t)blockIdx.y * dim * nchains;
+    ess += (int64_t)blockIdx.y * dim;
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= dim) return;
     float m = 3.402823466e+38f;
@@ -210,8 +233,11 @@ __global__ void ess_min_kernel(const float* __restrict__ ess_chain, int dim, int
 
 // Sum and sum of squares (FP64) of 64 dims x one half chain per workgroup.
 __global__ __launch_bounds__(256) void rhat_moments_kernel(const float* __restrict__ samples, int dim, int nchains,
-                                                          int spc, int half, int layout, double* __restrict__ mom)
+                                                          int spc, int half, int layout, double* __restrict__ mom,
+                                                          int64_t vstride)
 {
+    samples += (int64_t)blockIdx.z * vstride;
+    mom += (int64_t)blockIdx.z * 2 * nchains * dim * 2;
     __shared__ double sh[2][4][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -242,6 +268,8 @@ __global__ __launch_bounds__(256) void rhat_moments_kernel(const float* __restri
 __global__ void rhat_final_kernel(const double* __restrict__ mom, int dim, int nchains, int half,
                                   float* __restrict__ rhat)
 {
+    mom += (int64_t)blockIdx.y * 2 * nchains * dim * 2;
+    rhat += (int64_t)blockIdx.y * dim;
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= dim) return;
     const int m = 2 * nchains;
@@ -265,12 +293,40 @@ __global__ void rhat_final_kernel(const double* __restrict__ mom, int dim, int n
     rhat[d] = (float)((W > 1e-30) ? sqrt(var_hat / W) : 1.0);
 }
 
+// Per variant: the minimum over dims of ess and the maximum of rhat.
+__global__ __launch_bounds__(256) void variant_summary_kernel(const float* __restrict__ ess,
+                                                             const float* __restrict__ rhat, int dim,
+                                                             float* __restrict__ ess_min, float* __restrict__ rhat_max)
+{
+    __shared__ float smin[256], smax[256];
+    const int v = blockIdx.x, tid = threadIdx.x;
+    float mn = 3.402823466e+38f, mx = -3.402823466e+38f;
+    for (int d = tid; d < dim; d += 256) {
+        mn = fminf(mn, ess[(int64_t)v * dim + d]);
+        mx = fmaxf(mx, rhat[(int64_t)v * dim + d]);
+    }
+    smin[tid] = mn;
+    smax[tid] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            smin[tid] = fminf(smin[tid], smin[tid + s]);
+            smax[tid] = fmaxf(smax[tid], smax[tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (ess_min) ess_min[v] = smin[0];
+        if (rhat_max) rhat_max[v] = smax[0];
+    }
+}
+
 }  // namespace
 
 int diag_max_chain_length() { return kMaxHalf; }
 
 void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
-                 float* ess_chain, hipStream_t stream)
+                 float* ess_chain, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
 {
     const int spc = n_total / nchains;
     int logH = 1;
@@ -278,7 +334,10 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const int H = 1 << logH;
     NAJA_REQUIRE(H <= kMaxHalf, NAJA_ERR_UNSUPPORTED, "chains of %d samples exceed the LDS FFT limit (%d)", spc,
                  kMaxHalf);
+    NAJA_REQUIRE(n_variants >= 1 && n_variants <= 65535, NAJA_ERR_UNSUPPORTED, "%d variants (1..65535)", n_variants);
     const int N = 2 * H;
+    const int64_t vstride = (int64_t)n_total * dim;
+    const unsigned nv = (unsigned)n_variants;
 
     std::vector<float2> tw_h((size_t)H);
     for (int k = 0; k < H; ++k) {
@@ -291,21 +350,25 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const int half = spc / 2;
     NAJA_HIP(hipMallocAsync((void**)&tw, sizeof(float2) * (size_t)H, stream));
     NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * (size_t)H, hipMemcpyHostToDevice, stream));
-    if (!ec) NAJA_HIP(hipMallocAsync((void**)&ec, sizeof(float) * (size_t)dim * nchains, stream));
-    NAJA_HIP(hipMallocAsync((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains, stream));
+    if (!ec) NAJA_HIP(hipMallocAsync((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream));
+    NAJA_HIP(hipMallocAsync((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream));
 
     const size_t lds = sizeof(float2) * (size_t)H;
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)(dim * nchains)), dim3(kDiagThreads), lds, stream, samples,
-                       dim, nchains, spc, layout, logH, tw, ec);
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)(dim * nchains), nv), dim3(kDiagThreads), lds, stream,
+                       samples, dim, nchains, spc, layout, logH, tw, ec, vstride);
     NAJA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256), dim3(256), 0, stream, ec, dim, nchains, ess);
+    hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
     NAJA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(rhat_moments_kernel, dim3((dim + 63) / 64, 2 * nchains), dim3(256), 0, stream, samples, dim,
-                       nchains, spc, half, layout, mom);
+    hipLaunchKernelGGL(rhat_moments_kernel, dim3((dim + 63) / 64, 2 * nchains, nv), dim3(256), 0, stream, samples,
+                       dim, nchains, spc, half, layout, mom, vstride);
     NAJA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(rhat_final_kernel, dim3((dim + 255) / 256), dim3(256), 0, stream, mom, dim, nchains, half,
+    hipLaunchKernelGGL(rhat_final_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, mom, dim, nchains, half,
                        rhat);
     NAJA_HIP(hipGetLastError());
+    if (ess_min || rhat_max) {
+        hipLaunchKernelGGL(variant_summary_kernel, dim3(nv), dim3(256), 0, stream, ess, rhat, dim, ess_min, rhat_max);
+        NAJA_HIP(hipGetLastError());
+    }
 
     NAJA_HIP(hipFreeAsync(tw, stream));
     NAJA_HIP(hipFreeAsync(mom, stream));

@@ -57,3 +57,23 @@ def test_diag_errors(gpu):
         naja_amd.diagnostics(t.zeros((10, 3)), 3)  # not divisible
     with pytest.raises(naja_amd.NajaInvalidArgument):
         naja_amd.diagnostics(t.zeros((6, 3)), 2)  # fewer than 4 per chain
+
+
+def test_batched_diagnostics_equal_per_variant_calls(gpu):
+    """naja_diag_batched over stacked variants equals naja_diag per variant,
+    and its per-variant summaries are the min ESS / max R-hat."""
+    import naja_amd
+
+    t = gpu
+    rng = np.random.default_rng(12)
+    nv, nch, spc, dim = 5, 4, 300, 7
+    X = np.cumsum(rng.standard_normal((nv, nch * spc, dim)), axis=1).astype(np.float32) * 0.1
+    X += rng.standard_normal((nv, nch * spc, dim)).astype(np.float32)
+    Xd = t.from_numpy(X).cuda()
+    ess, rhat, emin, rmax = naja_amd.diagnostics_batched(Xd, nch)
+    for v in range(nv):
+        e1, r1, _ = naja_amd.diagnostics(Xd[v], nch)
+        np.testing.assert_array_equal(ess[v].cpu().numpy(), e1.cpu().numpy())
+        np.testing.assert_array_equal(rhat[v].cpu().numpy(), r1.cpu().numpy())
+        assert float(emin[v]) == float(e1.min())
+        assert float(rmax[v]) == float(r1.max())

@@ -71,6 +71,11 @@ inline int chr_effective_resync(int requested, int d)
 struct ChrShape {
     int W, G, R;
     int rows() const { return 64 * W * R; }
+    // The large-polytope kernel (16 wavefronts, more than 8 rows per lane)
+    // forms reciprocals itself and reads no packed reciprocal column.
+    bool big() const { return W == 16 && R > 8; }
+    // Dynamic LDS: the G x d chain states.
+    size_t lds_bytes(int d) const { return sizeof(float) * (size_t)G * d; }
 };
 
 using ChrKernelFn = void (*)(ChrArgs);
@@ -89,6 +94,7 @@ ChrKernelTable chr_table_w4b();
 ChrKernelTable chr_table_w8();
 ChrKernelTable chr_table_w16();
 ChrKernelTable chr_table_g4();
+ChrKernelTable chr_table_big();  // W = 16, R = 10..22: the large-polytope kernel
 
 // Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
 // per group; -1 if none.

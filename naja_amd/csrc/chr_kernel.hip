@@ -122,7 +122,7 @@ namespace {
 const ChrKernelTable& all_kernels()
 {
     static const ChrKernelTable tables[] = {chr_table_w1(), chr_table_w4a(), chr_table_w4b(), chr_table_w8(),
-                                            chr_table_w16(), chr_table_g4()};
+                                            chr_table_w16(), chr_table_g4(), chr_table_big()};
     static std::vector<ChrKernelEntry> flat;
     static ChrKernelTable all{nullptr, 0};
     if (!all.e) {
@@ -182,8 +182,8 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
     for (int q = 0; q < t.n; ++q)
         if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R) fn = t.e[q].fn;
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d", sh.W, sh.G, sh.R);
-    const size_t lds = sizeof(float) * (size_t)sh.G * (size_t)args.d;
-    NAJA_REQUIRE(lds <= 96 * 1024, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
+    const size_t lds = sh.lds_bytes(args.d);
+    NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
                  args.d, sh.G);
     hipLaunchKernelGGL(fn, dim3(n_items), dim3(64 * sh.W), lds, stream, args);
     NAJA_HIP(hipGetLastError());

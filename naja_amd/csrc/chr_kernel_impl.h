@@ -505,5 +505,240 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     }
 }
 
+// Raw buffer loads with a descriptor built by hand: base address, stride 0,
+// num_records (bytes; loads beyond it read 0) and the gfx9 dword-3 format
+// word (32-bit data). The loads are the LLVM intrinsics, named as functions.
+typedef int chr_i32x4 __attribute__((ext_vector_type(4)));
+typedef float chr_f32x2 __attribute__((ext_vector_type(2)));
+__device__ float chr_raw_load_f32(chr_i32x4 rsrc, int voffset, int soffset,
+                                  int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
+__device__ chr_f32x2 chr_raw_load_v2f32(chr_i32x4 rsrc, int voffset, int soffset,
+                                        int aux) __asm("llvm.amdgcn.raw.buffer.load.v2f32");
+__device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes)
+{
+    const uint64_t addr = (uint64_t)p;
+    chr_i32x4 r;
+    // Wave-uniform by construction; readfirstlane keeps it in SGPRs.
+    r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
+    r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
+    r[2] = __builtin_amdgcn_readfirstlane((int)bytes);
+    r[3] = 0x00020000;
+    return r;
+}
+
+// Large polytopes (more rows than the register-resident kernel above holds):
+// one chain per workgroup of W wavefronts, R rows per lane, a register
+// budget of 3R: the slack s and two column arrays that swap roles every step
+// (the loop is unrolled by two). X holds this step's column A[:, i],
+// prefetched during the previous step; a pair move loads its partner
+// A[:, j] into Y at the start of the step and combines in place, X = ci X +
+// cj Y; then the next step's A[:, i] is prefetched into Y and stays in
+// flight during the chord, the reductions and the update. Buffer loads:
+// the variant's block is the resource, the lane's row the one VGPR offset,
+// the column / row-slot offset a wave-uniform SGPR, so the R loads of a
+// column cost no address registers. No reciprocal column: a coordinate move
+// forms RN(1/A[r,i]) with a correctly rounded division, which keeps the chord
+// bits of the packed-reciprocal kernel (q = RN(s * RN(1/A))) while reading
+// one column per step from memory instead of two. The FP64 resync keeps one
+// accumulator at a time (same c-ascending FMA order as above).
+template <int W, int R>
+__global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
+{
+    constexpr int T = 64 * W;
+    extern __shared__ float xs[];  // d: the chain's state
+    __shared__ uint2 part[2][W];
+    __shared__ int ks_idx[kKmax];
+    __shared__ float ks_coef[kKmax];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const ChrItem item = a.items[blockIdx.x];
+    const ChrVariant V = a.variants[item.variant];
+    const int d = a.d;
+    const int lda = V.lda;  // == T * R
+    const chr_i32x4 rsrc = chr_make_rsrc(a.A + V.a_off, (uint32_t)(4 * (int64_t)lda * d));
+    const int voff = 4 * tid;
+    auto ld = [&](int col, int k) -> float { return chr_raw_load_f32(rsrc, voff, 4 * (col * lda + k * T), 0); };
+    // b of the block through a buffer too (64-bit global addresses per row
+    // slot would be hoisted out of the step loop and held in VGPRs).
+    const chr_i32x4 rsrc_b = chr_make_rsrc(a.b + V.b_off, (uint32_t)(8 * lda));
+    auto ldb = [&](int k) -> double { return __builtin_bit_cast(double, chr_raw_load_v2f32(rsrc_b, 8 * tid, 8 * k * T, 0)); };
+    const uint32_t group = (uint32_t)item.c0;  // one chain per group
+    const uint32_t k0 = a.seed_lo;
+    const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
+    const int64_t nsteps = a.nsteps;
+
+    for (int c = tid; c < d; c += T) xs[c] = a.y0[(V.y0_col + item.c0) * (int64_t)d + c];
+
+    float s[R], X0[R], X1[R];
+    MoveBatch<1> mb;
+    gen_batch<1>(a, k0, k1, 0, group, item.c0, mb);
+    Move mv = read_move<1>(mb, 0);
+    {
+        const int ci = (mv.type == kMoveKsparse) ? 0 : mv.i;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            X0[k] = ld(ci, k);
+            X1[k] = 0.0f;
+        }
+    }
+    float last_i = 0.0f, last_j = 0.0f;
+    int prev_i = -1, prev_j = -1;
+    int buf = 0;
+    int until_save = 0;
+    int until_resync = 0;
+    int64_t sample_idx = 0;
+
+    auto step = [&](int64_t t, float (&X)[R], float (&Y)[R]) {
+        if (until_resync == 0) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                double acc = 0.0;
+                for (int c = 0; c < d; ++c) acc = fma((double)ld(c, k), (double)xs[c], acc);
+                s[k] = (float)(ldb(k) - acc);
+            }
+            prev_i = -1;
+            prev_j = -1;
+            until_resync = a.resync > 0 ? a.resync : -1;
+        }
+
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t < a.nlog && tid == 0) {
+            a.move_log[t * 3 + 0] = mv.type;
+            a.move_log[t * 3 + 1] = mv.i;
+            a.move_log[t * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+        }
+
+        const int L = (int)(t & 63);
+        const float U = readlane_f32(mb.u[0], L);
+        Move nx = mv;
+        const bool has_next = t + 1 < nsteps;
+        if (has_next) {
+            if (L == 63) gen_batch<1>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
+            nx = read_move<1>(mb, (L + 1) & 63);
+        }
+
+        // This step's direction column in X.
+        const int kq = a.ks_k;
+        if (mv.type == kMovePair) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) Y[k] = ld(mv.j, k);
+#pragma unroll
+            for (int k = 0; k < R; ++k) X[k] = fmaf(X[k], mv.ci, Y[k] * mv.cj);
+        } else if (mv.type == kMoveKsparse) {
+            __syncthreads();
+            if (tid < kq) {
+                ks_idx[tid] = ksparse_index(a, k0, k1, (uint64_t)t, group, tid);
+                ks_coef[tid] = ((mv.w3 >> tid) & 1u) ? -a.ks_mag : a.ks_mag;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < R; ++k) X[k] = 0.0f;
+            for (int q = 0; q < kq; ++q) {
+                const int col = __builtin_amdgcn_readfirstlane(ks_idx[q]);
+                const float cq = ks_coef[q];
+#pragma unroll
+                for (int k = 0; k < R; ++k) X[k] = fmaf(ld(col, k), cq, X[k]);
+            }
+        }
+        // The next step's first column, in flight for the rest of this step.
+        {
+            const int ni = (has_next && nx.type != kMoveKsparse) ? nx.i : 0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) Y[k] = ld(ni, k);
+        }
+
+        uint32_t hb = 0xFFFFFFFFu;
+        int32_t lb = 0x7FFFFFFF;
+        if (mv.type == kMoveCoord) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const uint32_t qb = chord_bits_rcp(1.0f / X[k], s[k]);
+                hb = umin32(hb, qb);
+                lb = smin32(lb, (int32_t)qb);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const uint32_t qb = chord_bits(X[k], s[k]);
+                hb = umin32(hb, qb);
+                lb = smin32(lb, (int32_t)qb);
+            }
+        }
+        wave_min2(hb, lb);
+        if (lane == 0) part[buf][wave] = make_uint2(hb, (uint32_t)lb);
+
+        float oi = 0.0f, oj = 0.0f;
+        if (mv.type != kMoveKsparse) {
+            oi = (mv.i == prev_j) ? last_j : ((mv.i == prev_i) ? last_i : xs[mv.i]);
+            if (mv.type == kMovePair) oj = (mv.j == prev_j) ? last_j : ((mv.j == prev_i) ? last_i : xs[mv.j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint2 p = part[buf][w];
+            hb = umin32(hb, p.x);
+            lb = smin32(lb, (int32_t)p.y);
+        }
+        const float hi = __builtin_bit_cast(float, hb);
+        const float lo = __builtin_bit_cast(float, lb);
+        const float alpha = fmaf(U, hi - lo, lo);
+
+        const float nal = -alpha;
+#pragma unroll
+        for (int k = 0; k < R; ++k) s[k] = fmaf(nal, X[k], s[k]);
+
+        if (mv.type != kMoveKsparse) {
+            const float ni = fmaf(alpha, mv.ci, oi);
+            last_i = ni;
+            if (mv.type == kMovePair) {
+                const float base = (mv.j == mv.i) ? ni : oj;
+                last_j = fmaf(alpha, mv.cj, base);
+            }
+            prev_i = mv.i;
+            prev_j = (mv.type == kMovePair) ? mv.j : -1;
+            if (tid == 0) {
+                xs[mv.i] = last_i;
+                if (mv.type == kMovePair) xs[mv.j] = last_j;
+            }
+        } else {
+            __syncthreads();
+            if (tid == 0)
+                for (int q = 0; q < kq; ++q) xs[ks_idx[q]] = fmaf(alpha, ks_coef[q], xs[ks_idx[q]]);
+            __syncthreads();
+            prev_i = -1;
+            prev_j = -1;
+        }
+
+        if (until_save == 0) {
+            const int64_t col = sample_idx * a.nchains + item.c0;
+            float* dst = a.samples + V.sample_off + col * (int64_t)d;
+            for (int c = tid; c < d; c += T) {
+                float v = xs[c];
+                if (c == prev_i) v = last_i;
+                if (c == prev_j) v = last_j;
+                dst[c] = v;
+            }
+            ++sample_idx;
+            until_save = a.thin;
+        }
+        --until_save;
+        if (until_resync > 0) --until_resync;
+        mv = nx;
+        buf ^= 1;
+    };
+    int64_t t = 0;
+    for (; t + 1 < nsteps; t += 2) {
+        step(t, X0, X1);
+        step(t + 1, X1, X0);
+    }
+    if (t < nsteps) step(t, X0, X1);
+
+    __syncthreads();
+    if (a.y_final)
+        for (int c = tid; c < d; c += T) a.y_final[(V.y0_col + item.c0) * (int64_t)d + c] = xs[c];
+}
+
 }  // namespace chr_detail
 }  // namespace naja

@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-genome", action="store_true", help="skip the genome-scale plain run")
+    ap.add_argument("--no-large", action="store_true", help="skip the large-scale run (m = 21200, d = 2400)")
+    ap.add_argument("--large-nspc", type=int, default=2000, help="samples per chain of the large-scale run")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after timing, copy a buffer of this size (a known-size kernel for counter scaling)")
     return ap.parse_args()
@@ -153,6 +155,7 @@ def main():
                 "note": "kernel_ms brackets the main sampling call (pack + transform + CHR + backmap)"}
 
     genome = None if args.no_genome else genome_scale_plain(np, torch)
+    large = None if args.no_large else large_scale_plain(np, torch, args.large_nspc)
 
     if args.copy_probe_mb > 0:
         nel = args.copy_probe_mb * (1 << 20) // 4
@@ -201,6 +204,7 @@ def main():
             "pilot_seconds": round(t_pilot, 3), "training_seconds": round(t_train, 3),
             "training_nll_first_last": [round(hist[0], 3), round(hist[-1], 3)],
             "genome_scale_plain": genome,
+            "large_scale_plain": large,
             "parallelism": "single GPU; variants x chains as workgroups of one batched launch",
         },
         "roofline": roofline,
@@ -249,6 +253,59 @@ def genome_scale_plain(np, torch):
             "backmap_ms": round(bm_ms, 3), "diag_ms": round(dg_ms, 3), "min_ess": round(min_ess, 3),
             "rhat_max": round(float(rhat.max()), 4), "us_per_step": round(chr_ms * 1e3 / (nspc * thin), 4),
             "algorithmic_GBs": round(ach, 1), "bytes_per_chain_step": round(b_cs, 1),
+            "backmap_TFLOPs": round(2.0 * d * n * nch * nspc / (bm_ms * 1e-3) / 1e12, 2)}
+
+
+def large_scale_plain(np, torch, nspc):
+    """Plain CHR at the large-scale shape: n = 10600 reactions, d = 2400,
+    A = [T; -T] (m = 21200, 203.5 MB FP32), 1024 chains x nspc x thinning 400,
+    pair-prob 0.3; the FP32 MFMA backmap to n = 10600 and the GPU ESS /
+    split-R-hat over all 2400 dims. One run, timed with HIP events."""
+    import naja_amd
+    from naja_amd import workloads
+
+    n, d, nch, thin, pp = 10600, 2400, 1024, 400, 0.3
+    g = workloads.genome_scale(n=n, d=d, seed=0)
+    m = g["A"].shape[0]
+    A = torch.from_numpy(g["A"].astype(np.float32)).cuda()
+    b = torch.from_numpy(g["b"]).cuda()
+    Y0 = torch.zeros((nch, d), dtype=torch.float32, device="cuda")
+    T = torch.from_numpy(g["T"].astype(np.float32)).cuda()
+    shift = torch.from_numpy(g["shift"].astype(np.float32)).cuda()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    log(f"large-scale plain: m={m} d={d}, {nch} chains x {nspc} x thin {thin} ({nspc * thin} steps per chain)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    S = naja_amd.coordinate_hit_and_run(A, b, Y0, nspc, thin, nch, seed=0, pair_prob=pp)
+    ev[1].record(stream)
+    V = naja_amd.backmap(T, shift, S)
+    ev[2].record(stream)
+    ess, rhat, _ = naja_amd.diagnostics(S, nch)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    del V
+    chr_ms, bm_ms, dg_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
+    steps = nspc * thin
+    chain_steps = nch * steps
+    min_ess = float(ess.min())
+    # Algorithmic bytes per chain-step (slack read + write, kbar columns,
+    # amortised sample write) and the column bytes that actually leave the
+    # chip here (the slack stays in registers; one column per coordinate
+    # move, two per pair move).
+    b_cs = 4.0 * m * (2.0 + 1.0 + pp) + 4.0 * d / thin
+    col_bytes = 4.0 * m * (1.0 + pp) + 4.0 * d / thin
+    log(f"large-scale plain: chr {chr_ms / 1e3:.2f} s ({chr_ms * 1e3 / steps:.2f} us per step for all chains), "
+        f"backmap {bm_ms:.1f} ms, diag {dg_ms:.1f} ms; min-ESS {min_ess:.2f}, max R-hat {float(rhat.max()):.3f}")
+    return {"value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3), "chr_s": round(chr_ms / 1e3, 3),
+            "backmap_ms": round(bm_ms, 2), "diag_ms": round(dg_ms, 2), "min_ess": round(min_ess, 3),
+            "rhat_max": round(float(rhat.max()), 4), "chains": nch, "nspc": nspc, "thin": thin,
+            "chain_steps_per_s": round(chain_steps / (chr_ms * 1e-3), 1),
+            "algorithmic_GBs": round(b_cs * chain_steps / (chr_ms * 1e-3) / 1e9, 1),
+            "column_GBs": round(col_bytes * chain_steps / (chr_ms * 1e-3) / 1e9, 1),
+            "bytes_per_chain_step": round(b_cs, 1),
             "backmap_TFLOPs": round(2.0 * d * n * nch * nspc / (bm_ms * 1e-3) / 1e12, 2)}
 
 

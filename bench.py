@@ -31,6 +31,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+# Bytes the main learned-rounded launch moves between L2 and the memory fabric
+# (256 variants x 4 chains x 200000 steps): FETCH_SIZE and WRITE_SIZE passes
+# scaled by a 1 GiB copy probe (profiles/chr_step_pmc_traffic.md).
+MAIN_LAUNCH_TRAFFIC_BYTES = 812.9e9 + 5.32e9
 METRIC = "min-ESS/sec at split-R-hat<1.05, learned-rounded vs plain CHR, 1 MI355X"
 
 
@@ -47,6 +51,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-genome", action="store_true", help="skip the genome-scale plain run")
     ap.add_argument("--no-large", action="store_true", help="skip the large-scale run (m = 21200, d = 2400)")
+    ap.add_argument("--only-large", action="store_true",
+                    help="run only the large-scale line and print it (for profiling its kernel)")
     ap.add_argument("--large-nspc", type=int, default=2000, help="samples per chain of the large-scale run")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after timing, copy a buffer of this size (a known-size kernel for counter scaling)")
@@ -65,6 +71,10 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+
+    if args.only_large:
+        print(json.dumps({"large_scale_plain": large_scale_plain(np, torch, args.large_nspc)}), flush=True)
+        return
 
     from naja_amd import conditioning as cd
     from naja_amd import precond
@@ -147,8 +157,9 @@ def main():
     steps_per_chain = nspc * thin
     total_bytes = float(np.sum(nch * steps_per_chain * (4.0 * m_v * 3.0 + 4.0 * d / thin)))
     achieved = total_bytes / (main_ms * 1e-3) / 1e9
+    traffic = MAIN_LAUNCH_TRAFFIC_BYTES if (nv, nch, nspc, thin) == (256, 4, 12500, 16) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(main_ms, 3),
                 "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),

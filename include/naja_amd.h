@@ -140,6 +140,23 @@ int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, f
 int naja_chr_plan_destroy(naja_chr_plan* plan);
 
 /*
+ * Streamed sampling (CoordinateHitAndRunStreamed, chr.cu:101-230): the
+ * chains run in chunks of chunk_nspc samples; each chunk starts from Y
+ * (nchains x d device FP32, updated in place: the reference's persist_state)
+ * at the global step chunk_start * thinning, so the Philox stream continues
+ * and the chunks reproduce one long run bit for bit when chunk_nspc *
+ * thinning is a multiple of the effective resync interval. With T (n x d)
+ * and shift (n) each chunk is backmapped on the device. Chunk k is copied to
+ * pinned host memory on a second stream while chunk k + 1 computes, then
+ * handed to sink(rows, n_rows, n_cols, first_row, user) with rows in
+ * samples.npy order (row s * nchains + c); a non-zero sink return aborts.
+ */
+typedef int (*naja_sample_sink)(const float* rows, int64_t n_rows, int n_cols, int64_t first_row, void* user);
+int naja_chr_sample_streamed(const float* A, const double* b, int m, int d, float* Y, const naja_chr_params* params,
+                             int chunk_nspc, const float* T, const float* shift, int n, naja_sample_sink sink,
+                             void* user, void* stream);
+
+/*
  * backmap_into (backmap.cu:21-48): V = T Y + shift, FP32 MFMA.
  *   T     : n x d row-major (reaction rows, as the CSV stores T)
  *   shift : n
@@ -220,6 +237,19 @@ int naja_precond_dikin_batched(const double* A0, const double* b0, int m0, int d
 int naja_precond_ellipsoid_batched(const double* A0, const double* b0, int m0, int d, int n_variants,
                                    const int* extra_rows, const double* extra_A, const double* extra_b,
                                    const double* x_c, double* P, int nthreads);
+
+/*
+ * naja_precond_ellipsoid_batched on the GPU, FP64 inside, for d x d and the
+ * variant's rows in LDS ((d^2 + d + rows) x 8 B <= 150 KB, else
+ * NAJA_ERR_UNSUPPORTED). Device memory, the batched-sampler layout:
+ * A0 column-major (m0 x d), b0 FP64, extra_A the variants' column-major
+ * k_v x d blocks back to back, extra_b FP64; extra_rows on the host; x_c
+ * (nullable) n_variants x d; P out: n_variants x d x d FP32 row-major.
+ * Synchronises the stream (per-variant status is checked).
+ */
+int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int d, int n_variants,
+                                  const int* extra_rows, const float* extra_A, const double* extra_b,
+                                  const float* x_c, float* P, void* stream);
 
 /*
  * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion

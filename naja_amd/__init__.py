@@ -43,6 +43,7 @@ EXPORTED_SYMBOLS = (
     "naja_chr_plan_shape",
     "naja_chr_plan_launch",
     "naja_chr_plan_destroy",
+    "naja_chr_sample_streamed",
     "naja_backmap",
     "naja_diag",
     "naja_diag_batched",
@@ -51,6 +52,7 @@ EXPORTED_SYMBOLS = (
     "naja_precond_dikin_batched",
     "naja_precond_ellipsoid_batched",
     "naja_precond_lowrank",
+    "naja_precond_ellipsoid_device",
     "naja_lp_create",
     "naja_lp_destroy",
     "naja_lp_param_count",
@@ -62,6 +64,10 @@ EXPORTED_SYMBOLS = (
     "naja_pilot_covariance",
     "naja_gemm_bf16",
 )
+
+
+# int sink(const float* rows, int64_t n_rows, int n_cols, int64_t first_row, void* user)
+SINK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p)
 
 
 class NajaError(RuntimeError):
@@ -131,6 +137,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_chr_plan_launch.restype = i32
     lib.naja_chr_plan_destroy.argtypes = [vp]
     lib.naja_chr_plan_destroy.restype = i32
+    lib.naja_chr_sample_streamed.argtypes = [vp, vp, i32, i32, vp, ctypes.POINTER(ChrParams), i32, vp, vp, i32,
+                                             SINK, vp, vp]
+    lib.naja_chr_sample_streamed.restype = i32
     lib.naja_backmap.argtypes = [vp, vp, vp, vp, i32, i32, i64, vp]
     lib.naja_backmap.restype = i32
     lib.naja_diag.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
@@ -149,6 +158,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_ellipsoid_batched.restype = i32
     lib.naja_precond_lowrank.argtypes = [vp, i32, i32, i32, i32, vp, i32]
     lib.naja_precond_lowrank.restype = i32
+    lib.naja_precond_ellipsoid_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]
+    lib.naja_precond_ellipsoid_device.restype = i32
     f32 = ctypes.c_float
     lib.naja_lp_create.argtypes = [i32, i32, i32, i32, i32, i32, ctypes.c_uint64, f32, ctypes.POINTER(vp)]
     lib.naja_lp_destroy.argtypes = [vp]
@@ -276,15 +287,61 @@ def coordinate_hit_and_run(A, b, X0, nspc, thinning, nchains, tpb_ss=-1, seed=0,
 
 
 def _batch_inputs(A, b, extra_A, extra_b):
+    if isinstance(A, VariantBatch):
+        return A.A_cm, A.b, A.d, A.m0, A.ks, A.Ae, A.be
     torch = _torch()
     A_cm = _colmajor_f32(A)
     d, m0 = A_cm.shape
     b_d = _dev(b, torch.float64)
     ks = [int(e.shape[0]) for e in extra_A]
+    if all(not isinstance(e, torch.Tensor) for e in extra_A):
+        # Host blocks: transpose and concatenate on the host, one upload.
+        import numpy as np
+
+        blocks = [np.asarray(e, dtype=np.float32).reshape(-1, d).T.ravel() for e in extra_A if len(e) > 0]
+        Ae = torch.from_numpy(np.concatenate(blocks)).cuda() if blocks else None
+        rhs = [np.asarray(f, dtype=np.float64).reshape(-1) for f in extra_b if len(f) > 0]
+        be = torch.from_numpy(np.concatenate(rhs)).cuda() if blocks else None
+        return A_cm, b_d, d, m0, ks, Ae, be
     blocks = [_colmajor_f32(e).reshape(-1) for e in extra_A if e.shape[0] > 0]
     Ae = torch.cat(blocks) if blocks else None
     be = torch.cat([_dev(f, torch.float64).reshape(-1) for f in extra_b if f.numel() > 0]) if blocks else None
     return A_cm, b_d, d, m0, ks, Ae, be
+
+
+class VariantBatch:
+    """A base polytope A y <= b and its conditioned variants [A; E_v] y <=
+    [b; f_v], uploaded once in the batched sampler's layout (A column-major
+    FP32, b FP64, the variants' extra rows as column-major blocks back to
+    back). Pass it as `A` (with extra_A / extra_b None) to the batched
+    sampler and the GPU roundings, so repeated launches upload nothing."""
+
+    def __init__(self, A, b, extra_A, extra_b):
+        self.A_cm, self.b, self.d, self.m0, self.ks, self.Ae, self.be = _batch_inputs(A, b, extra_A, extra_b)
+
+    @property
+    def n_variants(self) -> int:
+        return len(self.ks)
+
+    def slice(self, lo: int, hi: int) -> "VariantBatch":
+        """Variants lo..hi-1 (views of the device buffers)."""
+        out = VariantBatch.__new__(VariantBatch)
+        out.A_cm, out.b, out.d, out.m0 = self.A_cm, self.b, self.d, self.m0
+        r0, r1 = sum(self.ks[:lo]), sum(self.ks[:hi])
+        out.ks = list(self.ks[lo:hi])
+        out.Ae = self.Ae[r0 * self.d:r1 * self.d] if (self.Ae is not None and r1 > r0) else None
+        out.be = self.be[r0:r1] if (self.be is not None and r1 > r0) else None
+        return out
+
+    def repeat(self, times: int) -> "VariantBatch":
+        """The variant list repeated `times` times (device buffers concatenated)."""
+        torch = _torch()
+        out = VariantBatch.__new__(VariantBatch)
+        out.A_cm, out.b, out.d, out.m0 = self.A_cm, self.b, self.d, self.m0
+        out.ks = list(self.ks) * times
+        out.Ae = torch.cat([self.Ae] * times) if self.Ae is not None else None
+        out.be = torch.cat([self.be] * times) if self.be is not None else None
+        return out
 
 
 def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, nchains, tpb_ss=-1,
@@ -294,7 +351,8 @@ def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, n
     """All conditioned variants [A; E_v] in one launch.
 
     extra_A / extra_b: lists (one entry per variant) of (k_v, d) and (k_v,)
-    tensors, k_v >= 0. X0: (n_variants, nchains, d) starts. With a rounding
+    arrays, k_v >= 0; or A a VariantBatch (already on the device) with b,
+    extra_A, extra_b None. X0: (n_variants, nchains, d) starts. With a rounding
     transform P (n_variants, d, d) and c (n_variants, d), variant v samples z
     with y = c_v + P_v z, X0 holds z-space starts and the samples are returned
     in y. Returns samples (n_variants, nchains*nspc, d).
@@ -441,3 +499,64 @@ def diagnostics_batched(samples, nchains, layout=LAYOUT_INTERLEAVED, stream=None
     _check(lib.naja_diag_batched(S.data_ptr(), nv, dim, n_total, nchains, layout, ess.data_ptr(), rhat.data_ptr(),
                                  emin.data_ptr(), rmax.data_ptr(), _stream_ptr(stream)))
     return ess, rhat, emin, rmax
+
+
+def ellipsoid_rounding(A, b, extra_A, extra_b, x_c=None, stream=None):
+    """Dikin-ellipsoid rounding of every variant [A; E_v] on the GPU:
+    P (n_variants, d, d) FP32 with P P^T = H^{-1} (see naja_amd.precond for
+    the FP64 host routine)."""
+    torch = _torch()
+    lib = load_library()
+    A_cm, b_d, d, m0, ks, Ae, be = _batch_inputs(A, b, extra_A, extra_b)
+    nv = len(ks)
+    karr = (ctypes.c_int * nv)(*ks)
+    xc = _dev(x_c, torch.float32) if x_c is not None else None
+    P = torch.empty((nv, d, d), dtype=torch.float32, device=A_cm.device)
+    _check(lib.naja_precond_ellipsoid_device(A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr,
+                                             Ae.data_ptr() if Ae is not None else None,
+                                             be.data_ptr() if be is not None else None,
+                                             xc.data_ptr() if xc is not None else None, P.data_ptr(),
+                                             _stream_ptr(stream)))
+    return P
+
+
+def coordinate_hit_and_run_streamed(A, b, X, nspc, thinning, nchains, chunk_nspc, sink, T=None, shift=None,
+                                    tpb_ss=-1, seed=0, pair_prob=0.0, resync_interval=0, ksparse_prob=0.0,
+                                    ksparse_k=8, pair_mode=1, chains_per_group=1, stream=None):
+    """CoordinateHitAndRunStreamed (chr.cu:101-230): chunks of chunk_nspc
+    samples per chain, each handed to sink(rows: np.ndarray (n_rows, n_cols),
+    first_row: int) on the host as soon as it is copied back (rows in
+    samples.npy order). X (nchains, d) is the start and is updated in place
+    to the final state. With T, shift the chunks are backmapped first."""
+    import numpy as np
+
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    if not (isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float32 and X.is_contiguous()):
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, "X must be a contiguous float32 CUDA tensor")
+    Td = _dev(T, torch.float32) if T is not None else None
+    sd = _dev(shift, torch.float32) if shift is not None else None
+    n = int(Td.shape[0]) if Td is not None else 0
+    failure = []
+
+    def _sink(ptr, n_rows, n_cols, first_row, user):
+        try:
+            buf = (ctypes.c_float * (n_rows * n_cols)).from_address(ptr)
+            sink(np.ctypeslib.as_array(buf).reshape(n_rows, n_cols).copy(), int(first_row))
+            return 0
+        except BaseException as e:  # surfaced after the call returns
+            failure.append(e)
+            return 1
+
+    cb = SINK(_sink)
+    p = chr_params(nspc, thinning, nchains, tpb_ss, seed, pair_prob, resync_interval, ksparse_prob, ksparse_k,
+                   pair_mode, None, chains_per_group)
+    rc = lib.naja_chr_sample_streamed(A_cm.data_ptr(), b_d.data_ptr(), m, d, X.data_ptr(), ctypes.byref(p),
+                                      int(chunk_nspc), Td.data_ptr() if Td is not None else None,
+                                      sd.data_ptr() if sd is not None else None, n, cb, None, _stream_ptr(stream))
+    if failure:
+        raise failure[0]
+    _check(rc)

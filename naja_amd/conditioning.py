@@ -34,7 +34,8 @@ import time
 
 import numpy as np
 
-from . import coordinate_hit_and_run_batched, diagnostics_batched
+from . import (NAJA_ERR_UNSUPPORTED, NajaError, VariantBatch, coordinate_hit_and_run_batched, diagnostics_batched,
+               ellipsoid_rounding)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
@@ -76,21 +77,29 @@ def _sync():
     torch.cuda.synchronize()
 
 
-def sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync=1000, first_variant=0):
+def _shape(A0, EA):
+    if isinstance(A0, VariantBatch):
+        return A0.n_variants, A0.d
+    return len(EA), A0.shape[1]
+
+
+def sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync=1000):
     """Batched CHR of every variant in z with y = P_v z (P None: plain).
-    Returns y samples (n_variants, nchains * nspc, d) on the GPU."""
+    A0 may be a VariantBatch (b0, EA, Eb None). Returns y samples
+    (n_variants, nchains * nspc, d) on the GPU."""
     import torch
 
-    nv, d = len(EA), A0.shape[1]
+    nv, d = _shape(A0, EA)
     X0 = torch.zeros((nv, nchains, d), dtype=torch.float32, device="cuda")
     kw = {}
     if P is not None:
-        kw = dict(P=torch.as_tensor(np.asarray(P, np.float32)).cuda(),
+        Pt = P if torch.is_tensor(P) else torch.as_tensor(np.asarray(P, np.float32))
+        kw = dict(P=Pt.to(device="cuda", dtype=torch.float32).contiguous(),
                   c=torch.zeros((nv, d), dtype=torch.float32, device="cuda"))
-    return coordinate_hit_and_run_batched(
-        torch.as_tensor(A0, dtype=torch.float32).cuda(), torch.as_tensor(b0, dtype=torch.float64).cuda(),
-        [torch.as_tensor(e, dtype=torch.float32) for e in EA], [torch.as_tensor(f, dtype=torch.float64) for f in Eb],
-        X0, nspc, thin, nchains, seed=seed, resync_interval=resync, **kw)
+    if not isinstance(A0, VariantBatch):
+        A0 = VariantBatch(A0, b0, EA, Eb)
+    return coordinate_hit_and_run_batched(A0, None, None, None, X0, nspc, thin, nchains, seed=seed,
+                                          resync_interval=resync, **kw)
 
 
 def variant_diagnostics(Y, nchains):
@@ -101,10 +110,18 @@ def variant_diagnostics(Y, nchains):
 
 
 def base_rounding(A0, b0, EA, Eb):
-    """The Dikin-ellipsoid rounding of every variant (FP64 Cholesky of the
-    log-barrier Hessian at the start point): the learned map's base and its
-    fallback."""
-    return precond.ellipsoid_batched(A0, b0, EA, Eb)
+    """The Dikin-ellipsoid rounding of every variant (Cholesky of the
+    log-barrier Hessian at the start point, FP64): the learned map's base and
+    its fallback. On the GPU (P as an FP32 device tensor); the host FP64
+    routine when d or the rows exceed the kernel's LDS budget."""
+    import torch
+
+    try:
+        return ellipsoid_rounding(A0, b0, EA, Eb)
+    except NajaError as e:
+        if e.code != NAJA_ERR_UNSUPPORTED or isinstance(A0, VariantBatch):
+            raise
+        return torch.as_tensor(precond.ellipsoid_batched(A0, b0, EA, Eb), dtype=torch.float32).cuda()
 
 
 def pilot_covariances(A0, b0, EA, Eb, P0=None, nchains=32, steps=2000, thin=4, seed=1, chunk=512):
@@ -117,13 +134,15 @@ def pilot_covariances(A0, b0, EA, Eb, P0=None, nchains=32, steps=2000, thin=4, s
     nv, d = len(EA), A0.shape[1]
     if P0 is None:
         P0 = base_rounding(A0, b0, EA, Eb)
+    A0 = VariantBatch(A0, b0, EA, Eb)
     S = torch.empty((nv, d, d), dtype=torch.float32, device="cuda")
     for lo in range(0, nv, chunk):
         hi = min(nv, lo + chunk)
-        Y = sample_rounded(A0, b0, EA[lo:hi], Eb[lo:hi], P0[lo:hi], nchains, steps // thin, thin, seed + lo)
+        Y = sample_rounded(A0.slice(lo, hi), None, None, None, P0[lo:hi], nchains, steps // thin, thin, seed + lo)
         Sy, _ = pilot_covariance(Y)
         del Y
-        Pinv = np.linalg.inv(P0[lo:hi])
+        P0c = P0[lo:hi].double().cpu().numpy() if torch.is_tensor(P0) else np.asarray(P0[lo:hi], np.float64)
+        Pinv = np.linalg.inv(P0c)
         Sz = Pinv @ Sy.double().cpu().numpy() @ Pinv.transpose(0, 2, 1)
         S[lo:hi] = torch.as_tensor(Sz, dtype=torch.float32).cuda()
     return S
@@ -152,11 +171,13 @@ def train(lp: LearnedPreconditioner, F, S, epochs=150, batch=256, lr=1e-3, seed=
 
 def learned_maps(lp: LearnedPreconditioner, F, P0, expand=True):
     """P_v = P0_v (I + V_v V_v^T)^{1/2} (expand) or P0_v (I + V_v V_v^T)^{-1/2}:
-    the learned low-rank update of the base."""
+    the learned low-rank update of the base (P0 a device tensor). The r x r
+    eigenproblems run on the host in FP64; the product on the device."""
     import torch
 
     V = lp.forward(torch.as_tensor(F, dtype=torch.float32).cuda())
-    return P0 @ precond.lowrank(V.double().cpu().numpy(), expand=expand)
+    L = torch.as_tensor(precond.lowrank(V.double().cpu().numpy(), expand=expand), dtype=torch.float32).cuda()
+    return torch.bmm(P0, L)
 
 
 def gate(ess_l, rhat_l, ess_b, rhat_b):
@@ -196,31 +217,37 @@ def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, probe
         _sync()
         t0 = time.perf_counter()
         P, extra = make_P()
-        Y = sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync)
+        Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
         finish(name, t0, Y, extra)
         del Y
 
     nv = len(EA)
+    # The polytopes go to the device once, before any method is timed.
+    batch = VariantBatch(A0, b0, EA, Eb)
+    batch2 = batch.repeat(2)
     if "plain" in methods:
         timed("plain", lambda: (None, None))
     if "dikin" in methods:
         timed("dikin", lambda: (precond.dikin_batched(A0, b0, EA, Eb)[0], None))
     if "ellipsoid" in methods:
-        timed("ellipsoid", lambda: (base_rounding(A0, b0, EA, Eb), None))
+        timed("ellipsoid", lambda: (base_rounding(batch, None, None, None), None))
 
     def gated():
-        P0 = base_rounding(A0, b0, EA, Eb)
+        import torch
+
+        P0 = base_rounding(batch, None, None, None)
         Pl = learned_maps(lp, F, P0, expand)
         # One probe launch holds both maps of every variant.
-        Yp = sample_rounded(A0, b0, EA + EA, Eb + Eb, np.concatenate([Pl, P0]), nchains, probe_nspc, thin,
-                            seed + 1, resync)
+        Yp = sample_rounded(batch2, None, None, None, torch.cat([Pl, P0]), nchains, probe_nspc, thin, seed + 1,
+                            resync)
         ep, rp = variant_diagnostics(Yp, nchains)
         del Yp
         acc = gate(ep[:nv], rp[:nv], ep[nv:], rp[nv:])
-        return np.where(acc[:, None, None], Pl, P0), dict(accepted=int(acc.sum()), accepted_mask=acc)
+        sel = torch.as_tensor(acc, device=P0.device)[:, None, None]
+        return torch.where(sel, Pl, P0), dict(accepted=int(acc.sum()), accepted_mask=acc)
 
     if "learned" in methods:
         timed("learned", gated)
     if "learned_ungated" in methods:
-        timed("learned_ungated", lambda: (learned_maps(lp, F, base_rounding(A0, b0, EA, Eb), expand), None))
+        timed("learned_ungated", lambda: (learned_maps(lp, F, base_rounding(batch, None, None, None), expand), None))
     return out

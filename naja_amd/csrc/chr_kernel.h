@@ -54,6 +54,7 @@ struct ChrArgs {
     int resync;
     int32_t* move_log;
     int64_t nlog;
+    int64_t t0;  // global index of the first step (Philox counter); a multiple of thin
 };
 
 // FP64 resync window actually used: R > 0 as given; 0 = automatic

@@ -231,9 +231,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // Row r = k*T + tid lives in slot k of lane tid.
     float s[G][R];
     float ca[R], cb[R];
+    // Steps t0 .. t0 + nsteps - 1 (t0 > 0 continues an earlier launch).
+    const int64_t t_begin = a.t0;
+    const int64_t t_end = a.t0 + a.nsteps;
     MoveBatch<G> mb;
-    gen_batch<G>(a, k0, k1, 0, group, item.c0, mb);
-    Move mv = read_move<G>(mb, 0);
+    gen_batch<G>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
+    Move mv = read_move<G>(mb, (int)(t_begin & 63));
     {
         // (ca, cb) = (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j])
         // for a pair move; a k-sparse move loads a dummy pair.
@@ -258,7 +261,6 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     int until_save = 0;    // steps until the next saved sample (t % thin == 0)
     int until_resync = 0;  // steps until the next FP64 slack computation
     int64_t sample_idx = 0;
-    const int64_t nsteps = a.nsteps;
 
     // One CHR step. The two register sets (ca, cb) / (na, nb) alternate
     // between consecutive steps (the loop below is unrolled by two), so the
@@ -299,10 +301,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : -1;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t < a.nlog && tid == 0) {
-            a.move_log[t * 3 + 0] = mv.type;
-            a.move_log[t * 3 + 1] = mv.i;
-            a.move_log[t * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t - t_begin < a.nlog && tid == 0) {
+            const int64_t tl = t - t_begin;
+            a.move_log[tl * 3 + 0] = mv.type;
+            a.move_log[tl * 3 + 1] = mv.i;
+            a.move_log[tl * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
         }
 
         // Alpha uniforms of step t, then the draw of step t+1 (a new batch of
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 #pragma unroll
         for (int g = 0; g < G; ++g) U[g] = readlane_f32(mb.u[g], L);
         nx = mv;
-        const bool has_next = t + 1 < nsteps;
+        const bool has_next = t + 1 < t_end;
         if (has_next) {
             if (L == 63) gen_batch<G>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
             nx = read_move<G>(mb, (L + 1) & 63);
@@ -491,12 +494,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         na[k] = 0.0f;
         nb[k] = 0.0f;
     }
-    int64_t t = 0;
-    for (; t + 1 < nsteps; t += 2) {
+    int64_t t = t_begin;
+    for (; t + 1 < t_end; t += 2) {
         step(t, ca, cb, na, nb);
         step(t + 1, na, nb, ca, cb);
     }
-    if (t < nsteps) step(t, ca, cb, na, nb);
+    if (t < t_end) step(t, ca, cb, na, nb);
 
     __syncthreads();
     if (a.y_final) {
@@ -567,14 +570,15 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
     const uint32_t group = (uint32_t)item.c0;  // one chain per group
     const uint32_t k0 = a.seed_lo;
     const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
-    const int64_t nsteps = a.nsteps;
+    const int64_t t_begin = a.t0;
+    const int64_t t_end = a.t0 + a.nsteps;
 
     for (int c = tid; c < d; c += T) xs[c] = a.y0[(V.y0_col + item.c0) * (int64_t)d + c];
 
     float s[R], X0[R], X1[R];
     MoveBatch<1> mb;
-    gen_batch<1>(a, k0, k1, 0, group, item.c0, mb);
-    Move mv = read_move<1>(mb, 0);
+    gen_batch<1>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
+    Move mv = read_move<1>(mb, (int)(t_begin & 63));
     {
         const int ci = (mv.type == kMoveKsparse) ? 0 : mv.i;
 #pragma unroll
@@ -604,16 +608,17 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : -1;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t < a.nlog && tid == 0) {
-            a.move_log[t * 3 + 0] = mv.type;
-            a.move_log[t * 3 + 1] = mv.i;
-            a.move_log[t * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t - t_begin < a.nlog && tid == 0) {
+            const int64_t tl = t - t_begin;
+            a.move_log[tl * 3 + 0] = mv.type;
+            a.move_log[tl * 3 + 1] = mv.i;
+            a.move_log[tl * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
         }
 
         const int L = (int)(t & 63);
         const float U = readlane_f32(mb.u[0], L);
         Move nx = mv;
-        const bool has_next = t + 1 < nsteps;
+        const bool has_next = t + 1 < t_end;
         if (has_next) {
             if (L == 63) gen_batch<1>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
             nx = read_move<1>(mb, (L + 1) & 63);
@@ -728,12 +733,12 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
         mv = nx;
         buf ^= 1;
     };
-    int64_t t = 0;
-    for (; t + 1 < nsteps; t += 2) {
+    int64_t t = t_begin;
+    for (; t + 1 < t_end; t += 2) {
         step(t, X0, X1);
         step(t + 1, X1, X0);
     }
-    if (t < nsteps) step(t, X0, X1);
+    if (t < t_end) step(t, X0, X1);
 
     __syncthreads();
     if (a.y_final)

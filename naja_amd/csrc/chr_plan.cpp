@@ -159,7 +159,8 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
     pl.shape = sh;
 }
 
-ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps)
+ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps,
+                  int64_t t0 = 0)
 {
     const naja_chr_params* p = &pl.p;
     ChrArgs a;
@@ -198,13 +199,15 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
     a.resync = chr_effective_resync(p->resync_interval, pl.d);
     a.move_log = p->move_log;
     a.nlog = p->move_log ? p->nlog : 0;
+    a.t0 = t0;
     return a;
 }
 
-void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s)
+void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s,
+            int64_t t0 = 0)
 {
     float* out = pl.P ? pl.zbuf : samples;
-    const ChrArgs a = make_args(pl, y0, out, y_final, nsteps);
+    const ChrArgs a = make_args(pl, y0, out, y_final, nsteps, t0);
     chr_launch(a, (int)pl.items.size(), pl.shape, s);
     if (pl.P) {
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
@@ -415,6 +418,88 @@ int naja_chr_sample(const float* A, const double* b, int m, int d, const float* 
                     float* samples, float* y_final, void* stream)
 {
     return naja_chr_sample_batched(A, b, m, d, 1, nullptr, nullptr, nullptr, y0, p, samples, y_final, stream);
+}
+
+int naja_chr_sample_streamed(const float* A, const double* b, int m, int d, float* Y, const naja_chr_params* p,
+                             int chunk_nspc, const float* T, const float* shift, int n, naja_sample_sink sink,
+                             void* user, void* stream)
+{
+    using namespace naja;
+    naja_chr_plan* pl = nullptr;
+    float* dbuf[2] = {nullptr, nullptr};
+    float* vbuf[2] = {nullptr, nullptr};
+    float* hbuf[2] = {nullptr, nullptr};
+    hipStream_t copy = nullptr;
+    hipEvent_t computed[2] = {nullptr, nullptr}, copied[2] = {nullptr, nullptr};
+    const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int rc = guarded([&] {
+        NAJA_REQUIRE(A && b && Y && p && sink, NAJA_ERR_INVALID_ARGUMENT, "NULL A, b, Y, params or sink");
+        NAJA_REQUIRE(chunk_nspc > 0, NAJA_ERR_INVALID_ARGUMENT, "chunk_nspc must be > 0");
+        NAJA_REQUIRE((T == nullptr) == (shift == nullptr) && (T == nullptr || n > 0), NAJA_ERR_INVALID_ARGUMENT,
+                     "a backmap needs T, shift and n > 0");
+        check_params(p, d);
+    });
+    if (rc) return rc;
+    naja_chr_params pc = *p;
+    pc.nspc = std::min(chunk_nspc, p->nspc);
+    rc = naja_chr_plan_create(A, b, m, d, 1, nullptr, nullptr, nullptr, nullptr, nullptr, &pc, stream, &pl);
+    if (rc) return rc;
+    rc = guarded([&] {
+        const int nch = p->nchains;
+        const int cols = T ? n : d;
+        const size_t chunk_rows = (size_t)pc.nspc * nch;
+        for (int k = 0; k < 2; ++k) {
+            NAJA_HIP(hipMalloc(&dbuf[k], sizeof(float) * chunk_rows * d));
+            if (T) NAJA_HIP(hipMalloc(&vbuf[k], sizeof(float) * chunk_rows * n));
+            NAJA_HIP(hipHostMalloc((void**)&hbuf[k], sizeof(float) * chunk_rows * cols, hipHostMallocDefault));
+            NAJA_HIP(hipEventCreateWithFlags(&computed[k], hipEventDisableTiming));
+            NAJA_HIP(hipEventCreateWithFlags(&copied[k], hipEventDisableTiming));
+        }
+        NAJA_HIP(hipStreamCreateWithFlags(&copy, hipStreamNonBlocking));
+        const int64_t nchunks = ((int64_t)p->nspc + pc.nspc - 1) / pc.nspc;
+        std::vector<int64_t> rows(2, 0), first(2, 0);
+        auto deliver = [&](int slot) {
+            NAJA_HIP(hipEventSynchronize(copied[slot]));
+            const int src = sink(hbuf[slot], rows[slot], cols, first[slot], user);
+            NAJA_REQUIRE(src == 0, NAJA_ERR_RUNTIME, "sample sink returned %d", src);
+        };
+        for (int64_t k = 0; k < nchunks; ++k) {
+            const int slot = (int)(k & 1);
+            const int64_t s0 = k * pc.nspc;
+            const int ns = (int)std::min<int64_t>(pc.nspc, p->nspc - s0);
+            // The copy of chunk k - 2 must have drained this slot.
+            if (k >= 2) NAJA_HIP(hipStreamWaitEvent(s, copied[slot], 0));
+            // Chunk k continues the chains from Y (state persisted in place)
+            // at global step s0 * thin, so the Philox stream runs on.
+            launch(*pl, Y, dbuf[slot], Y, (int64_t)ns * p->thinning, s, s0 * p->thinning);
+            const float* src = dbuf[slot];
+            if (T) {
+                backmap_launch(T, shift, dbuf[slot], vbuf[slot], n, d, (int64_t)ns * nch, s);
+                src = vbuf[slot];
+            }
+            NAJA_HIP(hipEventRecord(computed[slot], s));
+            NAJA_HIP(hipStreamWaitEvent(copy, computed[slot], 0));
+            NAJA_HIP(hipMemcpyAsync(hbuf[slot], src, sizeof(float) * (size_t)ns * nch * cols, hipMemcpyDeviceToHost,
+                                    copy));
+            NAJA_HIP(hipEventRecord(copied[slot], copy));
+            rows[slot] = (int64_t)ns * nch;
+            first[slot] = s0 * nch;
+            // Hand chunk k - 1 to the sink while chunk k computes.
+            if (k >= 1) deliver(slot ^ 1);
+        }
+        deliver((int)((nchunks - 1) & 1));
+        NAJA_HIP(hipStreamSynchronize(s));
+    });
+    for (int k = 0; k < 2; ++k) {
+        if (dbuf[k]) (void)hipFree(dbuf[k]);
+        if (vbuf[k]) (void)hipFree(vbuf[k]);
+        if (hbuf[k]) (void)hipHostFree(hbuf[k]);
+        if (computed[k]) (void)hipEventDestroy(computed[k]);
+        if (copied[k]) (void)hipEventDestroy(copied[k]);
+    }
+    if (copy) (void)hipStreamDestroy(copy);
+    const int rc2 = naja_chr_plan_destroy(pl);
+    return rc ? rc : rc2;
 }
 
 }  // extern "C"

@@ -2,13 +2,16 @@
 // compute_ess / compute_split_rhat (mcmc.cpp:91-194) and their vendored
 // kissfft dependency.
 //
-// ESS (mcmc.cpp:17-76): one workgroup per (dim, chain) series of length n.
+// ESS (mcmc.cpp:17-76): the samples are first gathered into contiguous
+// series (one tiled transpose), then one workgroup per (variant, chain, dim)
+// series of length n.
 //   * mean summed in FP64, cast to float; series centred in float;
 //   * autocovariance by a hand-written FFT: the real series of length
 //     N = 2^ceil(log2(2n)) (zero padded; linear, not circular, correlation)
 //     is packed as N/2 complex points z_j = x_2j + i x_2j+1 and transformed
-//     in LDS by an in-place radix-2 decimation-in-frequency FFT (natural in,
-//     bit-reversed out); the real spectrum X_k is split out, |X_k|^2 is
+//     in LDS by an in-place decimation-in-frequency FFT, two radix-2 stages
+//     per LDS pass (natural in, bit-reversed out; twiddles from two small
+//     LDS tables); the real spectrum X_k is split out, |X_k|^2 is
 //     re-packed in place and inverted by a decimation-in-time FFT
 //     (bit-reversed in, natural out), so no reordering pass is needed; the
 //     interleaved result is the unnormalised autocorrelation, scaled by
@@ -46,44 +49,93 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// In-place radix-2 FFTs over H points; tw[k] = exp(-2 pi i k / N), N = 2H,
-// so the twiddle of a length-L butterfly block is W_L^p = tw[p * N / L].
-// Decimation in frequency: natural-order input, bit-reversed output.
-__device__ void fft_dif(float2* z, int H, int logH, const float2* __restrict__ tw)
+// Twiddles W^k = exp(-2 pi i k / N), N = 2H, from two small LDS tables:
+// k = hi * B + lo, W^k = thi[hi] * tlo[lo] (B = 2^tb, both tables <= 128
+// entries for H <= 16384), each entry rounded once from FP64 on the host.
+struct Twiddles {
+    const float2* tlo;
+    const float2* thi;
+    int tb;
+    __device__ __forceinline__ float2 at(int k) const { return cmul(thi[k >> tb], tlo[k & ((1 << tb) - 1)]); }
+};
+
+// (x) * (-i) and (x) * (+i).
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// In-place FFT over H points, decimation in frequency (natural in,
+// bit-reversed out), two radix-2 stages per LDS pass (radix 2^2): stage st
+// pairs (i, i + 2^(st-1)) with twiddle W_{2^st}^pos = W^(pos * (N >> st)).
+__device__ void fft_dif(float2* z, int H, int logH, const Twiddles& tw)
 {
-    for (int st = logH; st >= 1; --st) {
-        const int half = 1 << (st - 1);
-        const int tstride = (H >> st) * 2;
+    const int logN = logH + 1;
+    int st = logH;
+    for (; st >= 2; st -= 2) {
+        const int q = 1 << (st - 2);
+        for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
+            const int p0 = g & (q - 1);
+            const int j = ((g >> (st - 2)) << st) + p0;
+            const float2 a = z[j], b = z[j + q], c = z[j + 2 * q], e = z[j + 3 * q];
+            const float2 w1 = tw.at(p0 << (logN - st));        // W_{4q}^{p0}
+            const float2 w2 = tw.at(p0 << (logN - st + 1));    // W_{2q}^{p0}
+            const float2 a1 = make_float2(a.x + c.x, a.y + c.y);
+            const float2 c1 = cmul(make_float2(a.x - c.x, a.y - c.y), w1);
+            const float2 b1 = make_float2(b.x + e.x, b.y + e.y);
+            const float2 e1 = cmul(mul_mi(make_float2(b.x - e.x, b.y - e.y)), w1);  // W_{4q}^{p0+q}
+            z[j] = make_float2(a1.x + b1.x, a1.y + b1.y);
+            z[j + q] = cmul(make_float2(a1.x - b1.x, a1.y - b1.y), w2);
+            z[j + 2 * q] = make_float2(c1.x + e1.x, c1.y + e1.y);
+            z[j + 3 * q] = cmul(make_float2(c1.x - e1.x, c1.y - e1.y), w2);
+        }
+        __syncthreads();
+    }
+    if (st == 1) {
         for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
-            const int pos = b & (half - 1);
-            const int i0 = ((b >> (st - 1)) << st) + pos;
-            const int i1 = i0 + half;
-            const float2 u = z[i0];
-            const float2 v = z[i1];
-            z[i0] = make_float2(u.x + v.x, u.y + v.y);
-            z[i1] = cmul(make_float2(u.x - v.x, u.y - v.y), tw[pos * tstride]);
+            const float2 u = z[2 * b], v = z[2 * b + 1];
+            z[2 * b] = make_float2(u.x + v.x, u.y + v.y);
+            z[2 * b + 1] = make_float2(u.x - v.x, u.y - v.y);
         }
         __syncthreads();
     }
 }
 
-// Decimation in time with conjugate twiddles (unnormalised inverse):
-// bit-reversed input, natural-order output.
-__device__ void ifft_dit(float2* z, int H, int logH, const float2* __restrict__ tw)
+// Unnormalised inverse, decimation in time with conjugate twiddles
+// (bit-reversed in, natural out), stages 1..logH two per LDS pass.
+__device__ void ifft_dit(float2* z, int H, int logH, const Twiddles& tw)
 {
-    for (int st = 1; st <= logH; ++st) {
+    const int logN = logH + 1;
+    int st = 1;
+    for (; st + 1 <= logH; st += 2) {
+        const int q = 1 << (st - 1);
+        for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
+            const int p0 = g & (q - 1);
+            const int j = ((g >> (st - 1)) << (st + 1)) + p0;
+            const float2 a = z[j], b = z[j + q], c = z[j + 2 * q], e = z[j + 3 * q];
+            const float2 w1 = conjf2(tw.at(p0 << (logN - st)));      // conj W_{2q}^{p0}
+            const float2 w2 = conjf2(tw.at(p0 << (logN - st - 1)));  // conj W_{4q}^{p0}
+            const float2 b1 = cmul(w1, b), e1 = cmul(w1, e);
+            const float2 a1 = make_float2(a.x + b1.x, a.y + b1.y), bb = make_float2(a.x - b1.x, a.y - b1.y);
+            const float2 c1 = make_float2(c.x + e1.x, c.y + e1.y), ee = make_float2(c.x - e1.x, c.y - e1.y);
+            const float2 c2 = cmul(w2, c1);
+            const float2 e2 = mul_pi(cmul(w2, ee));  // conj W_{4q}^{p0+q}
+            z[j] = make_float2(a1.x + c2.x, a1.y + c2.y);
+            z[j + 2 * q] = make_float2(a1.x - c2.x, a1.y - c2.y);
+            z[j + q] = make_float2(bb.x + e2.x, bb.y + e2.y);
+            z[j + 3 * q] = make_float2(bb.x - e2.x, bb.y - e2.y);
+        }
+        __syncthreads();
+    }
+    if (st == logH) {
         const int half = 1 << (st - 1);
-        const int tstride = (H >> st) * 2;
         for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
             const int pos = b & (half - 1);
             const int i0 = ((b >> (st - 1)) << st) + pos;
-            const int i1 = i0 + half;
-            float2 w = tw[pos * tstride];
-            w.y = -w.y;
+            const float2 w = conjf2(tw.at(pos << (logN - st)));
             const float2 u = z[i0];
-            const float2 v = cmul(w, z[i1]);
+            const float2 v = cmul(w, z[i0 + half]);
             z[i0] = make_float2(u.x + v.x, u.y + v.y);
-            z[i1] = make_float2(u.x - v.x, u.y - v.y);
+            z[i0 + half] = make_float2(u.x - v.x, u.y - v.y);
         }
         __syncthreads();
     }
@@ -94,34 +146,74 @@ __device__ __forceinline__ int bitrev(int x, int bits)
     return (int)(__builtin_bitreverse32((uint32_t)x) >> (32 - bits));
 }
 
-__global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ samples, int dim,
-                                                                  int nchains, int spc, int layout, int logH,
-                                                                  const float2* __restrict__ tw,
-                                                                  float* __restrict__ ess_chain, int64_t vstride)
+// Gathers every (variant, chain) series into contiguous rows:
+// out[((v * nchains + c) * dim + d) * spc + s] = sample s of chain c, dim d,
+// through 64 x 64 LDS tiles (coalesced on both sides). Strided per-series
+// reads would touch one cache line per element.
+__global__ __launch_bounds__(256) void series_transpose_kernel(const float* __restrict__ samples, int dim,
+                                                              int nchains, int spc, int layout, int64_t vstride,
+                                                              int s_tiles, float* __restrict__ out)
 {
-    // Variant blockIdx.y: its samples start vstride floats further on.
-    samples += (int64_t)blockIdx.y * vstride;
-    ess_chain += (int64_t)blockIdx.y * dim * nchains;
+    __shared__ float tile[64][65];
+    const int vc = blockIdx.x / s_tiles;  // v * nchains + c
+    const int st = blockIdx.x % s_tiles;
+    const int v = vc / nchains, c = vc % nchains;
+    const int s0 = st * 64, d0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const float* src = samples + (int64_t)v * vstride;
+    for (int i = ty; i < 64; i += 4) {
+        const int s = s0 + i, d = d0 + tx;
+        tile[i][tx] = (s < spc && d < dim) ? src[sample_row(layout, s, c, nchains, spc) * dim + d] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int d = d0 + i, s = s0 + tx;
+        if (s < spc && d < dim) out[((int64_t)vc * dim + d) * spc + s] = tile[tx][i];
+    }
+}
+
+// ESS of one contiguous series per workgroup (blockIdx.x = (v * nchains + c) * dim + d).
+__global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ series, int dim,
+                                                                  int nchains, int spc, int logH,
+                                                                  const float2* __restrict__ tw_tables, int tb,
+                                                                  float* __restrict__ ess_chain)
+{
     extern __shared__ float2 z[];
+    __shared__ float2 tlo_s[128], thi_s[128];
     __shared__ double red[kDiagThreads / 64];
     __shared__ float meanf_s;
 
-    const int series = blockIdx.x;  // series = c * dim + d
-    const int d = series % dim;
-    const int c = series / dim;
+    const int64_t sid = blockIdx.x;
+    const int d = (int)(sid % dim);
+    const int vc = (int)(sid / dim);
+    const int c = vc % nchains;
+    const int v = vc / nchains;
+    float* out = ess_chain + (int64_t)v * dim * nchains + (int64_t)d * nchains + c;
+    const float* x = series + sid * spc;
     const int n = spc;
     const int H = 1 << logH;
     const int N = 2 * H;
     const int tid = threadIdx.x;
 
     if (n < 4) {
-        if (tid == 0) ess_chain[(int64_t)d * nchains + c] = (float)n;
+        if (tid == 0) *out = (float)n;
         return;
     }
+    const int nlo = 1 << tb, nhi = H >> tb;
+    for (int i = tid; i < nlo; i += blockDim.x) tlo_s[i] = tw_tables[i];
+    for (int i = tid; i < nhi; i += blockDim.x) thi_s[i] = tw_tables[nlo + i];
+    const Twiddles tw{tlo_s, thi_s, tb};
 
-    // Mean in FP64 (mcmc.cpp:25-27).
+    // One pass over the series: raw values into the packed complex layout
+    // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside (mcmc.cpp:25-27).
     double part = 0.0;
-    for (int s = tid; s < n; s += blockDim.x) part += (double)samples[sample_row(layout, s, c, nchains, spc) * dim + d];
+    for (int j = tid; j < H; j += blockDim.x) {
+        const int s0 = 2 * j, s1 = 2 * j + 1;
+        const float x0 = s0 < n ? x[s0] : 0.0f;
+        const float x1 = s1 < n ? x[s1] : 0.0f;
+        part += (double)x0 + (double)x1;
+        z[j] = make_float2(x0, x1);
+    }
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if ((tid & 63) == 0) red[tid >> 6] = part;
     __syncthreads();
@@ -132,13 +224,13 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     }
     __syncthreads();
     const float meanf = meanf_s;
-
-    // Pack z_j = x_2j + i x_2j+1 (centred, zero padded), natural order.
+    // Centre in float; the padding stays zero.
     for (int j = tid; j < H; j += blockDim.x) {
         const int s0 = 2 * j, s1 = 2 * j + 1;
-        const float x0 = s0 < n ? samples[sample_row(layout, s0, c, nchains, spc) * dim + d] - meanf : 0.0f;
-        const float x1 = s1 < n ? samples[sample_row(layout, s1, c, nchains, spc) * dim + d] - meanf : 0.0f;
-        z[j] = make_float2(x0, x1);
+        float2 p = z[j];
+        if (s0 < n) p.x -= meanf;
+        if (s1 < n) p.y -= meanf;
+        z[j] = p;
     }
     __syncthreads();
     fft_dif(z, H, logH, tw);  // Z_k now sits at z[bitrev(k)]
@@ -161,8 +253,8 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
         // X_k = E_k + W^k O_k, W = e^{-2 pi i/N}; the partner of k = 0 is
         // X_H = E_0 + W^H O_0 with W^H = -1.
-        const float2 wk = tw[k];
-        const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw[H - k];
+        const float2 wk = tw.at(k);
+        const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw.at(H - k);
         const float2 Xk = make_float2(Ek.x + (wk.x * Ok.x - wk.y * Ok.y), Ek.y + (wk.x * Ok.y + wk.y * Ok.x));
         const float2 Xb = make_float2(Eb.x + (wb.x * Ob.x - wb.y * Ob.y), Eb.y + (wb.x * Ob.y + wb.y * Ob.x));
         const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
@@ -190,7 +282,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     const float var = raw[0] * norm;
     const int kend = n / 2;  // pairs k = 1 .. kend-1
     if (var < 1e-30f) {
-        if (tid == 0) ess_chain[(int64_t)d * nchains + c] = 1.0f;
+        if (tid == 0) *out = 1.0f;
         return;  // uniform across the workgroup: no barrier follows for anyone
     }
     int kneg = kend;
@@ -216,7 +308,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         double sum = 0.0;
         for (int w = 0; w < kDiagThreads / 64; ++w) sum += psum[w];
         const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
-        ess_chain[(int64_t)d * nchains + c] = (float)n / tau;
+        *out = (float)n / tau;
     }
 }
 
@@ -339,23 +431,39 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const int64_t vstride = (int64_t)n_total * dim;
     const unsigned nv = (unsigned)n_variants;
 
-    std::vector<float2> tw_h((size_t)H);
-    for (int k = 0; k < H; ++k) {
-        const double ang = -2.0 * M_PI * (double)k / (double)N;
-        tw_h[(size_t)k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    // Twiddle tables: tlo[l] = W^l (l < 2^tb), thi[h] = W^(h 2^tb) (h < H / 2^tb).
+    const int tb = (logH + 1) / 2;
+    const int nlo = 1 << tb, nhi = H >> tb;
+    std::vector<float2> tw_h((size_t)(nlo + nhi));
+    for (int l = 0; l < nlo; ++l) {
+        const double ang = -2.0 * M_PI * (double)l / (double)N;
+        tw_h[(size_t)l] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    for (int h = 0; h < nhi; ++h) {
+        const double ang = -2.0 * M_PI * (double)h * (double)nlo / (double)N;
+        tw_h[(size_t)(nlo + h)] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     float2* tw = nullptr;
     double* mom = nullptr;
+    float* ser = nullptr;
     float* ec = ess_chain;
     const int half = spc / 2;
-    NAJA_HIP(hipMallocAsync((void**)&tw, sizeof(float2) * (size_t)H, stream));
-    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * (size_t)H, hipMemcpyHostToDevice, stream));
+    const int64_t n_series = (int64_t)dim * nchains * nv;
+    NAJA_HIP(hipMallocAsync((void**)&ser, sizeof(float) * (size_t)n_series * spc, stream));
+    NAJA_HIP(hipMallocAsync((void**)&tw, sizeof(float2) * tw_h.size(), stream));
+    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
     if (!ec) NAJA_HIP(hipMallocAsync((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream));
     NAJA_HIP(hipMallocAsync((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream));
 
+    const int s_tiles = (spc + 63) / 64;
+    NAJA_REQUIRE((int64_t)s_tiles * nchains * nv < (int64_t)1 << 31 && n_series < (int64_t)1 << 31,
+                 NAJA_ERR_UNSUPPORTED, "too many series for one diagnostics launch");
+    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)(s_tiles * nchains * nv), (unsigned)((dim + 63) / 64)),
+                       dim3(256), 0, stream, samples, dim, nchains, spc, layout, vstride, s_tiles, ser);
+    NAJA_HIP(hipGetLastError());
     const size_t lds = sizeof(float2) * (size_t)H;
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)(dim * nchains), nv), dim3(kDiagThreads), lds, stream,
-                       samples, dim, nchains, spc, layout, logH, tw, ec, vstride);
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), lds, stream, ser, dim, nchains,
+                       spc, logH, tw, tb, ec);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
     NAJA_HIP(hipGetLastError());
@@ -370,6 +478,7 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
         NAJA_HIP(hipGetLastError());
     }
 
+    NAJA_HIP(hipFreeAsync(ser, stream));
     NAJA_HIP(hipFreeAsync(tw, stream));
     NAJA_HIP(hipFreeAsync(mom, stream));
     if (ec != ess_chain) NAJA_HIP(hipFreeAsync(ec, stream));

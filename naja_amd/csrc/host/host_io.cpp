@@ -86,6 +86,44 @@ void save_npy_f64(const std::string& path, const double* data, int64_t rows, int
               sizeof(double) * (size_t)rows * (size_t)cols);
 }
 
+NpyStreamWriter::NpyStreamWriter(const std::string& path, int64_t rows, int64_t cols)
+    : path_(path), rows_(rows), cols_(cols)
+{
+    f_ = std::fopen(path.c_str(), "wb");
+    if (!f_) throw std::runtime_error("Cannot write file: " + path);
+    const std::string h = npy_header("<f4", std::to_string(rows) + ", " + std::to_string(cols));
+    const uint16_t hl = (uint16_t)h.size();
+    const char magic[8] = {'\x93', 'N', 'U', 'M', 'P', 'Y', 0x01, 0x00};
+    if (std::fwrite(magic, 1, 8, f_) != 8 || std::fwrite(&hl, 2, 1, f_) != 1 ||
+        std::fwrite(h.data(), 1, h.size(), f_) != h.size())
+        throw std::runtime_error("write failed: " + path);
+}
+
+NpyStreamWriter::~NpyStreamWriter()
+{
+    if (f_) std::fclose(f_);
+}
+
+void NpyStreamWriter::append(const float* data, int64_t n_rows)
+{
+    if (!f_) throw std::runtime_error("append to a closed NPY stream: " + path_);
+    if (written_ + n_rows > rows_) throw std::runtime_error("NPY stream overflow: " + path_);
+    const size_t n = (size_t)n_rows * (size_t)cols_;
+    if (std::fwrite(data, sizeof(float), n, f_) != n) throw std::runtime_error("write failed: " + path_);
+    written_ += n_rows;
+}
+
+void NpyStreamWriter::close()
+{
+    if (!f_) return;
+    const bool full = written_ == rows_;
+    const bool ok = std::fclose(f_) == 0;
+    f_ = nullptr;
+    if (!full) throw std::runtime_error("NPY stream closed after " + std::to_string(written_) + " of " +
+                                        std::to_string(rows_) + " rows: " + path_);
+    if (!ok) throw std::runtime_error("write failed: " + path_);
+}
+
 void save_npy_u8(const std::string& path, const std::vector<uint8_t>& v)
 {
     write_npy(path, "|u1", std::to_string(v.size()) + ",", v.data(), v.size());

@@ -8,6 +8,7 @@
 //     (npy.h:15-53 float64, npy.h:56-84 uint8, npy.h:96-141 float32 samples);
 //   * status lines "> phase" and "key :: value" (status.h:8-16).
 #pragma once
+#include <cstdio>
 
 #include <cstdint>
 #include <string>
@@ -34,6 +35,22 @@ std::string npy_header(const std::string& descr, const std::string& shape);
 void save_npy_f32(const std::string& path, const float* data, int64_t rows, int64_t cols);
 void save_npy_f64(const std::string& path, const double* data, int64_t rows, int64_t cols);
 void save_npy_u8(const std::string& path, const std::vector<uint8_t>& v);
+
+// (rows, cols) float32 NPY written in pieces: the header (final shape) at
+// open, rows appended as they arrive, close checks that all rows came.
+class NpyStreamWriter {
+public:
+    NpyStreamWriter(const std::string& path, int64_t rows, int64_t cols);
+    ~NpyStreamWriter();
+    void append(const float* data, int64_t n_rows);
+    void close();
+    int64_t rows_written() const { return written_; }
+
+private:
+    std::string path_;
+    std::FILE* f_ = nullptr;
+    int64_t rows_, cols_, written_ = 0;
+};
 
 struct NpyArray {
     int64_t rows = 0, cols = 0;

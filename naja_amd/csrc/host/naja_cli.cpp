@@ -21,6 +21,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../../include/naja_amd.h"
@@ -71,6 +72,7 @@ struct RunConfig {
     int N_CHAINS = -1, N_SAMPLES = -1, TPB_SS = 128, GPU_DEVICE = 0, THINNING = 0;
     bool BACK_TRANSFORM = false, WRITE_DATA = false, VERBOSE = false, STATUS = true;
     bool BOUNDS_FILTER = false, WRITE_SAMPLES_VALID = false;
+    int STREAM_CHUNK = 0;  // > 0: stream samples.npy in chunks of this many samples per chain
     double BOUNDS_EPS = 1e-6, PAIR_PROB = 0.0, EXTRA_CONSTRAINT_EPS = 0.0, CONSTRAINT_EPS = 0.0;
     double AFFINE_HULL_TOL = 0.0, KSPARSE_PROB = 0.0;
     int RESYNC_INTERVAL = 0, ITER_ROUNDING_PASSES = 0, ITER_ROUNDING_WARMUP = 0, KSPARSE_K = 8;
@@ -99,7 +101,7 @@ struct RunConfig {
                  "auto|ignore|require] [--extra-constraint-eps <eps>] [--constraint-eps <eps>] [--pair-prob <p>] "
                  "[--pair-schedule <csv>] [--ksparse-prob <p>] [--ksparse-k <k>] [--resync-interval <n>] "
                  "[--iter-rounding-passes <n>] [--iter-rounding-warmup <n>] [--affine-hull-tol <tol>] [--backmap] "
-                 "[--write-npy] [--bounds-policy ignore|filter] [--bounds-eps <eps>] [--write-samples-valid] "
+                 "[--write-npy] [--stream-chunk <n>] [--bounds-policy ignore|filter] [--bounds-eps <eps>] [--write-samples-valid] "
                  "[--seed <s>] [--chains-per-group 1|4] [--verbose] [--quiet] [--dry-run] [--print-config]\n";
     std::cerr << "  naja validate --samples <file.npy> [--model-dir <dir>] [--n-chains 4] "
                  "[--chain-layout interleaved|naja-blocks]\n";
@@ -677,6 +679,56 @@ int run_sampling_job(RunConfig c)
         p.n_pairs = (int)sched.i.size();
     }
     const int64_t ns = (int64_t)c.N_CHAINS * c.N_SAMPLES;
+    if (c.STREAM_CHUNK > 0) {
+        // Chunked sampling straight into samples.npy (chr.cu:101-230 streaming;
+        // the file is written as chunks arrive, never held whole in memory).
+        std::unique_ptr<DevBuf<float>> T_d, s_d;
+        int out_cols = d;
+        if (c.BACK_TRANSFORM) {
+            const Matrix T = load_csv_matrix(c.file("T"));
+            const std::vector<double> shift = load_csv_vector(c.file("shift"));
+            if (T.cols != d)
+                throw std::runtime_error("dimension mismatch: transform matrix cols (" + std::to_string(T.cols) +
+                                         ") != reduced dim (" + std::to_string(d) + ")");
+            if ((int)shift.size() != T.rows)
+                throw std::invalid_argument("backmap_into: shift length mismatch with transformation rows");
+            std::vector<float> Tf(T.v.begin(), T.v.end()), sf(shift.begin(), shift.end());
+            T_d.reset(new DevBuf<float>(Tf.data(), Tf.size()));
+            s_d.reset(new DevBuf<float>(sf.data(), sf.size()));
+            out_cols = T.rows;
+        }
+        prof.original_dim = out_cols;
+        status_phase(c.STATUS, "gpu sampling (streamed)");
+        t0 = now_seconds();
+        std::unique_ptr<NpyStreamWriter> w;
+        if (c.WRITE_DATA) w.reset(new NpyStreamWriter(c.OUT_DIR + "/samples.npy", ns, out_cols));
+        struct Ctx {
+            NpyStreamWriter* w;
+        } ctx{w.get()};
+        auto sink = [](const float* rows, int64_t n_rows, int, int64_t, void* user) -> int {
+            Ctx* cx = static_cast<Ctx*>(user);
+            try {
+                if (cx->w) cx->w->append(rows, n_rows);
+            } catch (const std::exception& e) {
+                std::cerr << e.what() << std::endl;
+                return 1;
+            }
+            return 0;
+        };
+        napi(naja_chr_sample_streamed(A_d.p, b_d.p, m, d, X_d.p, &p, c.STREAM_CHUNK, T_d ? T_d->p : nullptr,
+                                      s_d ? s_d->p : nullptr, out_cols, sink, &ctx, nullptr),
+             "CoordinateHitAndRunStreamed");
+        if (w) w->close();
+        prof.sampling = now_seconds() - t0;
+        prof.throughput = (double)c.N_CHAINS * c.N_SAMPLES / prof.sampling;
+        prof.total = now_seconds() - t_total;
+        write_profile_json(c.OUT_DIR + "/profile.json", prof, c);
+        write_config_snapshot(c, prof);
+        if (c.STATUS)
+            std::cout << "> done (streamed): " << std::fixed << std::setprecision(1) << prof.sampling << "s"
+                      << std::defaultfloat << std::endl;
+        return 0;
+    }
     DevBuf<float> Y_d((size_t)ns * d);
     std::vector<float> out;
     int out_dim = d;
@@ -769,6 +821,7 @@ int cmd_run(int argc, char** argv)
         else if (a == "--affine-hull-tol") c.AFFINE_HULL_TOL = std::stod(next_arg(i, argc, argv, a));
         else if (a == "--backmap") c.BACK_TRANSFORM = true;
         else if (a == "--write-npy") c.WRITE_DATA = true;
+        else if (a == "--stream-chunk") c.STREAM_CHUNK = std::stoi(next_arg(i, argc, argv, a));
         else if (a == "--verbose") c.VERBOSE = true;
         else if (a == "--bounds-policy") bounds_policy = next_arg(i, argc, argv, a);
         else if (a == "--bounds-eps") c.BOUNDS_EPS = std::stod(next_arg(i, argc, argv, a));
@@ -799,6 +852,9 @@ int cmd_run(int argc, char** argv)
     if (c.EXTRA_CONSTRAINTS != "auto" && c.EXTRA_CONSTRAINTS != "ignore" && c.EXTRA_CONSTRAINTS != "require")
         die_usage("invalid --extra-constraints (auto|ignore|require)");
     if (bounds_policy != "ignore" && bounds_policy != "filter") die_usage("invalid --bounds-policy: " + bounds_policy);
+    if (c.STREAM_CHUNK < 0) die_usage("--stream-chunk must be >= 0");
+    if (c.STREAM_CHUNK > 0 && bounds_policy == "filter")
+        die_usage("--stream-chunk writes samples.npy in pieces; --bounds-policy filter needs them all in memory");
     if (c.CHAINS_PER_GROUP != 1 && c.CHAINS_PER_GROUP != 4) die_usage("invalid --chains-per-group (1|4)");
     if (bounds_policy == "filter" && !c.BACK_TRANSFORM) throw std::runtime_error("bounds-policy=filter requires --backmap");
     c.STATUS = !quiet;

@@ -657,3 +657,153 @@ extern "C" int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, con
             gemm<float, float, false, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
     });
 }
+
+// ---------------------------------------------------------------------------
+// Dikin-ellipsoid rounding on the GPU (FP64): one workgroup per variant of a
+// ragged batch [A0; E_v] y <= [b0; f_v]. H = sum_r a_r a_r^T / s_r^2 at x_c
+// (lower triangle in LDS), H = L L^T in place, M = L^{-1} column by column
+// into the free upper triangle (diagonal kept apart), P = M^T written as FP32.
+// The host routine of the same name in precond.cpp does the same in FP64.
+// ---------------------------------------------------------------------------
+namespace naja {
+namespace mlp {
+
+__global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict__ A0, const double* __restrict__ b0,
+                                                        int m0, int d, const float* __restrict__ Ae,
+                                                        const double* __restrict__ be,
+                                                        const int64_t* __restrict__ eoff,
+                                                        const int* __restrict__ ek, const float* __restrict__ xc,
+                                                        float* __restrict__ P, int* __restrict__ status)
+{
+    extern __shared__ double sh[];
+    double* H = sh;                   // d x d
+    double* Md = H + (size_t)d * d;   // d: diagonal of M
+    double* w = Md + d;               // m_v: 1 / s_r^2
+    const int v = blockIdx.x, tid = threadIdx.x;
+    const int k = ek[v];
+    const int64_t off = eoff[v];
+    const int m = m0 + k;
+    const float* x = xc ? xc + (int64_t)v * d : nullptr;
+    // Row r, column c of [A0; E_v] (both blocks column-major).
+    auto arow = [&](int r, int c) -> double {
+        return r < m0 ? (double)A0[(int64_t)c * m0 + r] : (double)Ae[off * d + (int64_t)c * k + (r - m0)];
+    };
+    __shared__ int bad;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int r = tid; r < m; r += 256) {
+        double ax = 0.0;
+        if (x)
+            for (int c = 0; c < d; ++c) ax = fma(arow(r, c), (double)x[c], ax);
+        const double s = (r < m0 ? b0[r] : be[off + (r - m0)]) - ax;
+        if (!(s > 0.0)) bad = 1;
+        w[r] = 1.0 / (s * s);
+    }
+    __syncthreads();
+    if (bad) {
+        if (tid == 0) status[v] = NAJA_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    const int ntri = d * (d + 1) / 2;
+    for (int e = tid; e < ntri; e += 256) {
+        // e -> (i, j), i >= j, row-major over the lower triangle.
+        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        while (i * (i + 1) / 2 > e) --i;
+        const int j = e - i * (i + 1) / 2;
+        double acc = 0.0;
+        for (int r = 0; r < m; ++r) acc = fma(w[r] * arow(r, i), arow(r, j), acc);
+        H[(size_t)i * d + j] = acc;
+    }
+    __syncthreads();
+    __shared__ int singular;
+    if (tid == 0) singular = 0;
+    for (int j = 0; j < d; ++j) {
+        if (tid == 0) {
+            double s = H[(size_t)j * d + j];
+            for (int q = 0; q < j; ++q) s -= H[(size_t)j * d + q] * H[(size_t)j * d + q];
+            if (!(s > 0.0)) singular = 1;
+            H[(size_t)j * d + j] = sqrt(s > 0.0 ? s : 1.0);
+        }
+        __syncthreads();
+        const double ljj = H[(size_t)j * d + j];
+        for (int i = j + 1 + tid; i < d; i += 256) {
+            double s = H[(size_t)i * d + j];
+            for (int q = 0; q < j; ++q) s -= H[(size_t)i * d + q] * H[(size_t)j * d + q];
+            H[(size_t)i * d + j] = s / ljj;
+        }
+        __syncthreads();
+    }
+    if (singular) {
+        if (tid == 0) status[v] = NAJA_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    // Column c of M = L^{-1}: M[i][c] (i > c) at H[c][i], M[c][c] at Md[c].
+    for (int c = tid; c < d; c += 256) {
+        Md[c] = 1.0 / H[(size_t)c * d + c];
+        for (int i = c + 1; i < d; ++i) {
+            double s = H[(size_t)i * d + c] * Md[c];
+            for (int q = c + 1; q < i; ++q) s += H[(size_t)i * d + q] * H[(size_t)c * d + q];
+            H[(size_t)c * d + i] = -s / H[(size_t)i * d + i];
+        }
+    }
+    __syncthreads();
+    float* Pv = P + (int64_t)v * d * d;
+    for (int e = tid; e < d * d; e += 256) {
+        const int r = e / d, c = e % d;
+        const double val = (c == r) ? Md[r] : (c > r ? H[(size_t)r * d + c] : 0.0);
+        Pv[e] = (float)val;
+    }
+    if (tid == 0) status[v] = NAJA_OK;
+}
+
+}  // namespace mlp
+}  // namespace naja
+
+extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int d, int n_variants,
+                                             const int* extra_rows, const float* extra_A, const double* extra_b,
+                                             const float* x_c, float* P, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(A0 && b0 && P && m0 > 0 && d > 0 && n_variants > 0 && extra_rows, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_precond_ellipsoid_device: bad arguments");
+        std::vector<int64_t> off(n_variants);
+        int64_t tot = 0;
+        int kmax = 0;
+        for (int v = 0; v < n_variants; ++v) {
+            NAJA_REQUIRE(extra_rows[v] >= 0, NAJA_ERR_INVALID_ARGUMENT, "variant %d has %d extra rows", v,
+                         extra_rows[v]);
+            off[v] = tot;
+            tot += extra_rows[v];
+            kmax = std::max(kmax, extra_rows[v]);
+        }
+        NAJA_REQUIRE(tot == 0 || (extra_A && extra_b), NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_precond_ellipsoid_device: extra rows without extra_A/extra_b");
+        const size_t lds = sizeof(double) * ((size_t)d * d + d + (size_t)(m0 + kmax));
+        NAJA_REQUIRE(lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
+                     "naja_precond_ellipsoid_device: d=%d with %d rows exceeds the LDS budget (use the host routine)",
+                     d, m0 + kmax);
+        hipStream_t s = (hipStream_t)stream;
+        int64_t* eoff = nullptr;
+        int* ek = nullptr;
+        int* status = nullptr;
+        NAJA_HIP(hipMallocAsync((void**)&eoff, sizeof(int64_t) * n_variants, s));
+        NAJA_HIP(hipMallocAsync((void**)&ek, sizeof(int) * n_variants, s));
+        NAJA_HIP(hipMallocAsync((void**)&status, sizeof(int) * n_variants, s));
+        NAJA_HIP(hipMemcpyAsync(eoff, off.data(), sizeof(int64_t) * n_variants, hipMemcpyHostToDevice, s));
+        NAJA_HIP(hipMemcpyAsync(ek, extra_rows, sizeof(int) * n_variants, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(mlp::ellipsoid_kernel, dim3(n_variants), dim3(256), lds, s, A0, b0, m0, d, extra_A, extra_b,
+                           eoff, ek, x_c, P, status);
+        NAJA_HIP(hipGetLastError());
+        std::vector<int> st(n_variants);
+        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
+        NAJA_HIP(hipStreamSynchronize(s));
+        (void)hipFreeAsync(eoff, s);
+        (void)hipFreeAsync(ek, s);
+        (void)hipFreeAsync(status, s);
+        for (int v = 0; v < n_variants; ++v)
+            NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_INVALID_ARGUMENT,
+                         "Dikin ellipsoid of variant %d: the centre is not interior or the barrier Hessian is singular",
+                         v);
+    });
+}

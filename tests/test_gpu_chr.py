@@ -260,3 +260,37 @@ def test_plan_autotune_relaunch_and_graph_replay(gpu):
         t.cuda.synchronize()
         np.testing.assert_array_equal(static_out.cpu().numpy()[0], ref)
     plan.close()
+
+
+def test_streamed_chunks_reproduce_one_run(gpu):
+    """CoordinateHitAndRunStreamed semantics: chunks continue the Philox
+    stream and the chain state; with chunk steps a multiple of the resync
+    interval the concatenated chunks equal one long run bit for bit, the
+    state is persisted, and chunk backmaps equal the backmap of the run."""
+    import naja_amd
+
+    t = gpu
+    A, b, _ = random_polytope(12, 300, seed=17)
+    nch, nspc, thin, chunk = 3, 250, 4, 60  # 240 steps per chunk, resync 120
+    At, bt = t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda()
+    kw = dict(seed=4, pair_prob=0.3, resync_interval=120)
+    X0 = t.zeros((nch, 12), dtype=t.float32, device="cuda")
+    full, yfin = naja_amd.coordinate_hit_and_run(At, bt, X0, nspc, thin, nch, return_final=True, **kw)
+    t.cuda.synchronize()
+    parts = {}
+    X = X0.clone()
+    naja_amd.coordinate_hit_and_run_streamed(At, bt, X, nspc, thin, nch, chunk,
+                                             lambda rows, first: parts.__setitem__(first, rows), **kw)
+    got = np.concatenate([parts[k] for k in sorted(parts)])
+    assert sorted(parts) == [k * chunk * nch for k in range(5)]
+    np.testing.assert_array_equal(got, full.cpu().numpy())
+    np.testing.assert_array_equal(X.cpu().numpy(), yfin.cpu().numpy())
+    rng = np.random.default_rng(2)
+    T = t.from_numpy(rng.standard_normal((7, 12)).astype(np.float32)).cuda()
+    sh = t.from_numpy(rng.standard_normal(7).astype(np.float32)).cuda()
+    V = naja_amd.backmap(T, sh, full).cpu().numpy()
+    parts.clear()
+    X = X0.clone()
+    naja_amd.coordinate_hit_and_run_streamed(At, bt, X, nspc, thin, nch, chunk,
+                                             lambda rows, first: parts.__setitem__(first, rows), T=T, shift=sh, **kw)
+    np.testing.assert_array_equal(np.concatenate([parts[k] for k in sorted(parts)]), V)

@@ -103,3 +103,22 @@ def test_sample_run_backmap_filter_and_validate(gpu, tmp_path):
     vb = subprocess.run([NAJA, "validate", "--samples", str(red), "--n-chains", "4", "--chain-layout",
                          "naja-blocks"], capture_output=True, text=True, timeout=120)
     assert vb.returncode == 0 and json.loads(vb.stdout)["chain_layout"] == "naja-blocks"
+
+
+@pytest.mark.parametrize("backmap", [False, True])
+def test_sample_run_streamed_equals_whole(gpu, tmp_path, backmap):
+    """`--stream-chunk` writes samples.npy chunk by chunk; with chunk steps a
+    multiple of the resync window the file equals the one-shot run's."""
+    md, A, b, T, shift = make_model(tmp_path)
+    base = [NAJA, "sample", "run", "--model-dir", str(md), "--gpu", "0", "--n-chains", "3", "--n-samples", "400",
+            "--thinning", "4", "--pair-prob", "0.3", "--seed", "77", "--write-npy", "--resync-interval", "200"]
+    if backmap:
+        base.append("--backmap")
+    outs = []
+    for extra in ([], ["--stream-chunk", "50"]):
+        out = tmp_path / ("out" + "".join(extra))
+        r = subprocess.run(base + ["--out-root", str(out)] + extra, capture_output=True, text=True, timeout=120)
+        print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+        assert r.returncode == 0
+        outs.append((run_dir(out) / "samples.npy").read_bytes())
+    assert outs[0] == outs[1]

@@ -61,3 +61,28 @@ def test_rounded_samples_stay_feasible(gpu):
             worst = max(worst, float((Y[v].double() @ A.T - b).max()))
         print(f"{name}: max(Ay - b) = {worst:.2e}", flush=True)
         assert worst <= 1e-5, name
+
+
+def test_device_ellipsoid_matches_host(gpu):
+    """The GPU Dikin-ellipsoid rounding equals the FP64 host routine (on the
+    same FP32 row values) to FP32 rounding, and P P^T H = I."""
+    import naja_amd
+    from naja_amd import conditioning as cd
+    from naja_amd import precond
+
+    d = 100
+    A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
+    pool = cd.make_pool(d)
+    EA, Eb, _, _ = cd.sample_conditions(pool, 24, 300, 1e-3, 1e-1, seed=4)
+    EA32 = [e.astype(np.float32).astype(np.float64) for e in EA]
+    Pd = naja_amd.ellipsoid_rounding(A0, b0, EA32, Eb).cpu().numpy().astype(np.float64)
+    Ph = precond.ellipsoid_batched(A0, b0, EA32, Eb)
+    for v in range(len(EA)):
+        A = np.vstack([A0, EA32[v]])
+        b = np.concatenate([b0, Eb[v]])
+        H = (A.T / b ** 2) @ A
+        scale = np.abs(Ph[v]).max()
+        assert np.abs(Pd[v] - Ph[v]).max() <= 1e-5 * scale, v
+        # P is the FP32 rounding of the FP64 factor; H has condition numbers
+        # up to ~1e6 here, so the identity holds to ~1e-3.
+        np.testing.assert_allclose(Pd[v] @ Pd[v].T @ H, np.eye(d), atol=1e-2)

@@ -188,6 +188,63 @@ int naja_diag_batched(const float* samples, int n_variants, int dim, int n_total
                       float* ess, float* rhat, float* ess_min, float* rhat_max, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Affine-hull reduction (host, FP64, row-major host memory), replacing the
+ * Eigen::FullPivLU block of run_sampling_job (job_sampling.cu:196-235).
+ * ------------------------------------------------------------------------- */
+
+/*
+ * Kernel basis of Aeq (r x n) by full-pivot LU with relative pivot threshold
+ * `threshold` (a pivot counts when |u_kk| > threshold * max |pivot|; the
+ * reference uses 1e-10, job_sampling.cu:211). basis: caller buffer of n*n
+ * doubles; receives n x dim row-major (leading dimension dim), one column per
+ * non-pivot column f of the permuted matrix, with entry 1 at that original
+ * column and 0 at the other free columns. r = 0 gives the identity.
+ */
+int naja_kernel_basis(const double* Aeq, int r, int n, double threshold, double* basis, int* dim_out);
+
+/*
+ * The equality set of {x : A x <= b} at x0 is the rows with b_i - a_i.x0 <= tol
+ * (job_sampling.cu:199-204); basis (n*n buffer) receives its kernel basis as in
+ * naja_kernel_basis, dim_out its dimension, n_tight_out (nullable) the row
+ * count. The caller reduces when 0 < dim < n: A' = A basis, b' = b - A x0,
+ * z0 = 0, x = x0 + basis z (job_sampling.cu:213-223).
+ */
+int naja_affine_hull(const double* A, const double* b, const double* x0, int m, int n, double tol,
+                     double threshold, double* basis, int* dim_out, int* n_tight_out);
+
+/* ---------------------------------------------------------------------------
+ * Producers of conditioned variants (host, FP64, row-major host memory).
+ * ------------------------------------------------------------------------- */
+
+/*
+ * Start point of a polytope: the centre of the largest axis-aligned cube
+ * inside {x : A x <= b}, i.e. the LP  max r  s.t.  A x + ||a_i||_1 r <= b,
+ * r >= 0 (axis_aligned_cube_center_lp_gurobi, feasible_start_lp.cpp:33-87,
+ * which calls Gurobi). Own interior-point solver; x_hint (nullable) seeds it.
+ * r_out is the exact radius min_i (b_i - a_i.x) / ||a_i||_1 at the returned x.
+ * When the optimal centre is not unique, x is an interior point of the
+ * optimal face (the reference returns Gurobi's choice). Errors:
+ * NAJA_ERR_INVALID_ARGUMENT for an empty or non-finite input,
+ * NAJA_ERR_RUNTIME for an infeasible or unbounded LP (the reference throws
+ * runtime_error "LP status ..."). nthreads <= 0: all host threads.
+ */
+int naja_cube_center(const double* A, const double* b, int m, int n, const double* x_hint, double* x_out,
+                     double* r_out, int nthreads);
+
+/*
+ * build_extra_from_bounds (cmd_prepare.cpp:111-156): for each flux i of the
+ * rounded model x = T y + shift (T n x d), a tightened upper bound
+ * (ub_new < ub_base - eps) adds the row T[i,:] with rhs ub_new - shift, a
+ * tightened lower bound (lb_new > lb_base + eps) the row -T[i,:] with rhs
+ * -lb_new + shift, in flux order (the reference uses eps = 1e-9). Both bound
+ * sets are checked as validate_bounds does (cmd_prepare.cpp:80-93).
+ * extra_A: caller buffer of 2 n d doubles (k x d row-major out); extra_b: 2 n.
+ */
+int naja_extra_from_bounds(const double* T, const double* shift, int n, int d, const double* lb_base,
+                           const double* ub_base, const double* lb_new, const double* ub_new, double eps,
+                           double* extra_A, double* extra_b, int* k_out);
+
+/* ---------------------------------------------------------------------------
  * Rounding preconditioners (host, FP64, row-major host memory). Every P
  * returned here is the sampler's rounding transform: y = x_c + P z.
  * ------------------------------------------------------------------------- */

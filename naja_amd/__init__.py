@@ -47,6 +47,10 @@ EXPORTED_SYMBOLS = (
     "naja_backmap",
     "naja_diag",
     "naja_diag_batched",
+    "naja_kernel_basis",
+    "naja_cube_center",
+    "naja_extra_from_bounds",
+    "naja_affine_hull",
     "naja_symmetric_eigen",
     "naja_precond_dikin",
     "naja_precond_dikin_batched",
@@ -147,6 +151,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_diag_batched.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp]
     lib.naja_diag_batched.restype = i32
     f64 = ctypes.c_double
+    lib.naja_kernel_basis.argtypes = [vp, i32, i32, f64, vp, vp]
+    lib.naja_cube_center.argtypes = [vp, vp, i32, i32, vp, vp, vp, i32]
+    lib.naja_cube_center.restype = i32
+    lib.naja_extra_from_bounds.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, f64, vp, vp, vp]
+    lib.naja_extra_from_bounds.restype = i32
+    lib.naja_kernel_basis.restype = i32
+    lib.naja_affine_hull.argtypes = [vp, vp, vp, i32, i32, f64, f64, vp, vp, vp]
+    lib.naja_affine_hull.restype = i32
     lib.naja_symmetric_eigen.argtypes = [vp, i32, vp, vp]
     lib.naja_symmetric_eigen.restype = i32
     lib.naja_precond_dikin.argtypes = [vp, vp, vp, i32, i32, i32, f64, f64, f64, vp, vp, vp, vp, vp]

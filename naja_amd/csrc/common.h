@@ -7,7 +7,10 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/naja_amd.h"
 
@@ -56,6 +59,31 @@ int guarded(F&& body)
         set_error("unknown C++ exception");
         return NAJA_ERR_RUNTIME;
     }
+}
+
+// Runs body(v) for v in [0, n) over up to nthreads host threads.
+template <typename F>
+void parallel_for(int n, int nthreads, F&& body)
+{
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    nthreads = std::max(1, std::min(nthreads, n));
+    std::vector<std::thread> pool;
+    std::vector<int> codes(nthreads, NAJA_OK);
+    std::vector<std::string> msgs(nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        pool.emplace_back([&, t]() {
+            codes[t] = guarded([&]() {
+                for (int v = t; v < n; v += nthreads) body(v);
+            });
+            if (codes[t] != NAJA_OK) msgs[t] = naja_last_error();
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (int t = 0; t < nthreads; ++t)
+        if (codes[t] != NAJA_OK) {
+            set_error("%s", msgs[t].c_str());
+            throw Error(codes[t]);
+        }
 }
 
 }  // namespace naja

@@ -2,6 +2,7 @@
 //
 //   naja sample run   (cmd_run.cpp:44-229 + run_sampling_job, job_sampling.cu:82-443)
 //   naja validate     (validate_cli.cpp:49-296)
+//   naja prepare      (cmd_prepare.cpp:213-457; prepare_cli.cpp)
 //
 // Flags, the model-directory contract, the run-directory layout and the
 // output files (samples.npy, profile.json, config_generated.txt,
@@ -561,6 +562,63 @@ void bounds_filter_and_write(const RunConfig& c, const std::vector<float>& V, in
     f << "  ]\n}\n";
 }
 
+// Affine hull of the start point: n_file x k basis (row-major) and x0.
+struct Hull {
+    bool enabled = false;
+    std::vector<double> basis, shift;
+};
+
+// The map applied to the sampled variables before output: the model
+// transform (T, shift) when --backmap is set, composed with the hull as
+// (T B, T x0 + shift) (job_sampling.cu:288-302); (B, x0) for a hull without
+// --backmap (job_sampling.cu:343-348); nothing otherwise.
+struct OutputTransform {
+    bool on = false;
+    int rows = 0;
+    std::vector<float> T, shift;
+};
+
+OutputTransform output_transform(const RunConfig& c, int d, int d_file, const Hull& hull)
+{
+    OutputTransform o;
+    if (c.BACK_TRANSFORM) {
+        status_phase(c.STATUS, "loading transform");
+        const Matrix T = load_csv_matrix(c.file("T"));
+        const std::vector<double> shift = load_csv_vector(c.file("shift"));
+        const int expected = hull.enabled ? d_file : d;
+        if (T.cols != expected)
+            throw std::runtime_error("dimension mismatch: transform matrix cols (" + std::to_string(T.cols) +
+                                     ") != reduced dim (" + std::to_string(expected) + ")");
+        if ((int)shift.size() != T.rows)
+            throw std::invalid_argument("backmap_into: shift length mismatch with transformation rows");
+        o.on = true;
+        o.rows = T.rows;
+        o.T.resize((size_t)T.rows * d);
+        o.shift.resize((size_t)T.rows);
+        for (int i = 0; i < T.rows; ++i) {
+            if (hull.enabled) {
+                double sh = shift[(size_t)i];
+                for (int j = 0; j < d_file; ++j) sh += T.at(i, j) * hull.shift[(size_t)j];
+                o.shift[(size_t)i] = (float)sh;
+                for (int l = 0; l < d; ++l) {
+                    double acc = 0.0;
+                    for (int j = 0; j < d_file; ++j) acc += T.at(i, j) * hull.basis[(size_t)j * d + l];
+                    o.T[(size_t)i * d + l] = (float)acc;
+                }
+            } else {
+                o.shift[(size_t)i] = (float)shift[(size_t)i];
+                for (int l = 0; l < d; ++l) o.T[(size_t)i * d + l] = (float)T.at(i, l);
+            }
+        }
+    } else if (hull.enabled) {
+        o.on = true;
+        o.rows = d_file;
+        o.T.assign(hull.basis.begin(), hull.basis.end());
+        o.shift.assign(hull.shift.begin(), hull.shift.end());
+    }
+    return o;
+}
+
 // ------------------------------------------------------------ sampling job
 
 int run_sampling_job(RunConfig c)
@@ -588,15 +646,24 @@ int run_sampling_job(RunConfig c)
     if (c.CONSTRAINT_EPS > 0.0)
         for (double& v : P.b) v += c.CONSTRAINT_EPS;
     prof.load = now_seconds() - t0;
-    const int m = P.A.rows, d = P.A.cols;
+    int m = P.A.rows, d = P.A.cols;
+    const int d_file = d;
     if ((int)P.x0.size() != d)
         throw std::runtime_error("dimension mismatch: start point size (" + std::to_string(P.x0.size()) +
                                  ") != cols (" + std::to_string(d) + ")");
     if ((int)P.b.size() != m)
         throw std::runtime_error("dimension mismatch: rhs vector size (" + std::to_string(P.b.size()) +
                                  ") != rows (" + std::to_string(m) + ")");
-    if (c.START_POLICY == "cube_center")
-        throw std::runtime_error("start policy cube_center needs the Gurobi LP solver, which this build does not ship");
+    if (c.START_POLICY == "cube_center") {
+        // job_sampling.cu:162-167: the centre of the largest axis-aligned cube
+        // (own interior-point LP in place of Gurobi).
+        status_phase(c.STATUS, "start policy: cube_center");
+        std::vector<double> xc((size_t)d);
+        double r = 0.0;
+        napi(naja_cube_center(P.A.v.data(), P.b.data(), m, d, nullptr, xc.data(), &r, 0), "axis_aligned_cube_center");
+        P.x0 = xc;
+        if (c.VERBOSE) std::cout << "  start_cube_r     " << std::setprecision(6) << r << "\n" << std::defaultfloat;
+    }
     require_feasible_start(P, 1e-9, extra_loaded ? "A+extra" : "A");
     {
         const std::vector<double> r = residual(P);
@@ -609,8 +676,44 @@ int run_sampling_job(RunConfig c)
                       << "\n  The feasible set may be a single point. Samples will likely be identical.\n"
                       << "  Check rounding quality (tight constraint rank vs dimension).\n\n";
     }
-    if (c.AFFINE_HULL_TOL > 0.0)
-        throw std::runtime_error("--affine-hull-tol is not supported by this build yet");
+    // Affine-hull reduction (job_sampling.cu:196-235): rows tight at x0 become
+    // equalities and sampling runs in their kernel, x = x0 + B z.
+    Hull hull;
+    if (c.AFFINE_HULL_TOL > 0.0 && d > 0) {
+        std::vector<double> B((size_t)d * d);
+        int k = 0, n_tight = 0;
+        napi(naja_affine_hull(P.A.v.data(), P.b.data(), P.x0.data(), m, d, c.AFFINE_HULL_TOL, 1e-10, B.data(), &k,
+                              &n_tight),
+             "affine hull");
+        if (n_tight > 0 && k > 0 && k < d) {
+            hull.enabled = true;
+            hull.basis.assign(B.begin(), B.begin() + (size_t)d * k);
+            hull.shift = P.x0;
+            const std::vector<double> r = residual(P);
+            Matrix Ar;
+            Ar.rows = m;
+            Ar.cols = k;
+            Ar.v.assign((size_t)m * k, 0.0);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < d; ++j) {
+                    const double a = P.A.at(i, j);
+                    for (int l = 0; l < k; ++l) Ar.at(i, l) += a * hull.basis[(size_t)j * k + l];
+                }
+            for (int i = 0; i < m; ++i) P.b[(size_t)i] = -r[(size_t)i];
+            P.A = std::move(Ar);
+            P.x0.assign((size_t)k, 0.0);
+            d = k;
+            if (c.VERBOSE)
+                std::cout << "  affine_hull_tol  " << std::setprecision(12) << c.AFFINE_HULL_TOL
+                          << "\n  affine_hull_rows " << n_tight << "\n  affine_hull_dim  " << k << " (from " << d_file
+                          << ")\n" << std::defaultfloat;
+            require_feasible_start(P, 1e-9, "affine-hull reduced");
+        } else if (c.VERBOSE) {
+            std::cout << "  affine_hull_tol  " << std::setprecision(12) << c.AFFINE_HULL_TOL
+                      << (n_tight == 0 ? " (no tight rows)" : " (no reduction; kernel dim=" + std::to_string(k) + ")")
+                      << "\n" << std::defaultfloat;
+        }
+    }
 
     const int thin = c.THINNING > 0 ? c.THINNING : std::max(d / 6, 1);
     if (c.VERBOSE)
@@ -684,18 +787,11 @@ int run_sampling_job(RunConfig c)
         // the file is written as chunks arrive, never held whole in memory).
         std::unique_ptr<DevBuf<float>> T_d, s_d;
         int out_cols = d;
-        if (c.BACK_TRANSFORM) {
-            const Matrix T = load_csv_matrix(c.file("T"));
-            const std::vector<double> shift = load_csv_vector(c.file("shift"));
-            if (T.cols != d)
-                throw std::runtime_error("dimension mismatch: transform matrix cols (" + std::to_string(T.cols) +
-                                         ") != reduced dim (" + std::to_string(d) + ")");
-            if ((int)shift.size() != T.rows)
-                throw std::invalid_argument("backmap_into: shift length mismatch with transformation rows");
-            std::vector<float> Tf(T.v.begin(), T.v.end()), sf(shift.begin(), shift.end());
-            T_d.reset(new DevBuf<float>(Tf.data(), Tf.size()));
-            s_d.reset(new DevBuf<float>(sf.data(), sf.size()));
-            out_cols = T.rows;
+        const OutputTransform tr = output_transform(c, d, d_file, hull);
+        if (tr.on) {
+            T_d.reset(new DevBuf<float>(tr.T.data(), tr.T.size()));
+            s_d.reset(new DevBuf<float>(tr.shift.data(), tr.shift.size()));
+            out_cols = tr.rows;
         }
         prof.original_dim = out_cols;
         status_phase(c.STATUS, "gpu sampling (streamed)");
@@ -732,31 +828,24 @@ int run_sampling_job(RunConfig c)
     DevBuf<float> Y_d((size_t)ns * d);
     std::vector<float> out;
     int out_dim = d;
-    if (c.BACK_TRANSFORM) {
-        status_phase(c.STATUS, "loading transform");
-        const Matrix T = load_csv_matrix(c.file("T"));
-        const std::vector<double> shift = load_csv_vector(c.file("shift"));
-        if (T.cols != d)
-            throw std::runtime_error("dimension mismatch: transform matrix cols (" + std::to_string(T.cols) +
-                                     ") != reduced dim (" + std::to_string(d) + ")");
-        if ((int)shift.size() != T.rows)
-            throw std::invalid_argument("backmap_into: shift length mismatch with transformation rows");
-        prof.original_dim = T.rows;
-        std::vector<float> Tf(T.v.begin(), T.v.end()), sf(shift.begin(), shift.end());
-        DevBuf<float> T_d(Tf.data(), Tf.size()), s_d(sf.data(), sf.size());
-        DevBuf<float> V_d((size_t)ns * T.rows);
-        status_phase(c.STATUS, "gpu sampling+backmap");
+    const OutputTransform tr = output_transform(c, d, d_file, hull);
+    if (tr.on) {
+        prof.original_dim = tr.rows;
+        DevBuf<float> T_d(tr.T.data(), tr.T.size()), s_d(tr.shift.data(), tr.shift.size());
+        DevBuf<float> V_d((size_t)ns * tr.rows);
+        status_phase(c.STATUS, hull.enabled && !c.BACK_TRANSFORM ? "gpu sampling+backmap (affine hull)"
+                                                                   : "gpu sampling+backmap");
         t0 = now_seconds();
         napi(naja_chr_sample(A_d.p, b_d.p, m, d, X_d.p, &p, Y_d.p, nullptr, nullptr), "CoordinateHitAndRun");
-        napi(naja_backmap(T_d.p, s_d.p, Y_d.p, V_d.p, T.rows, d, ns, nullptr), "backmap_into");
+        napi(naja_backmap(T_d.p, s_d.p, Y_d.p, V_d.p, tr.rows, d, ns, nullptr), "backmap_into");
         HIPCHK(hipDeviceSynchronize());
         prof.sampling = now_seconds() - t0;
         status_phase(c.STATUS, "downloading");
         t0 = now_seconds();
-        out.resize((size_t)ns * T.rows);
+        out.resize((size_t)ns * tr.rows);
         HIPCHK(hipMemcpy(out.data(), V_d.p, sizeof(float) * out.size(), hipMemcpyDeviceToHost));
         prof.download = now_seconds() - t0;
-        out_dim = T.rows;
+        out_dim = tr.rows;
     } else {
         status_phase(c.STATUS, "gpu sampling");
         t0 = now_seconds();
@@ -1132,17 +1221,21 @@ int cmd_validate(int argc, char** argv)
 
 }  // namespace
 
+int cmd_prepare(int argc, char** argv);
+
 int main(int argc, char** argv)
 {
     try {
         if (argc >= 3 && std::string(argv[1]) == "sample" && std::string(argv[2]) == "run")
             return cmd_run(argc - 3, argv + 3);
         if (argc >= 2 && std::string(argv[1]) == "validate") return cmd_validate(argc - 2, argv + 2);
+        if (argc >= 2 && std::string(argv[1]) == "prepare") return cmd_prepare(argc - 2, argv + 2);
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
     }
     std::cerr << "usage:\n  naja sample run <args...>\n  naja validate --samples <file.npy> [--model-dir <dir>] "
-                 "[--n-chains 4] [--chain-layout interleaved|naja-blocks]\n";
+                 "[--n-chains 4] [--chain-layout interleaved|naja-blocks]\n  naja prepare --models-root <dir> "
+                 "--model-list <file> [--base-model-dir <dir>] ...\n";
     return 2;
 }

@@ -373,31 +373,6 @@ void lowrank_one(int d, int r, const double* V, double* P, bool expand)
     }
 }
 
-// Runs body(v) for v in [0, n) over up to nthreads host threads.
-template <typename F>
-void parallel_for(int n, int nthreads, F&& body)
-{
-    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
-    nthreads = std::max(1, std::min(nthreads, n));
-    std::vector<std::thread> pool;
-    std::vector<int> codes(nthreads, NAJA_OK);
-    std::vector<std::string> msgs(nthreads);
-    for (int t = 0; t < nthreads; ++t) {
-        pool.emplace_back([&, t]() {
-            codes[t] = guarded([&]() {
-                for (int v = t; v < n; v += nthreads) body(v);
-            });
-            if (codes[t] != NAJA_OK) msgs[t] = naja_last_error();
-        });
-    }
-    for (auto& th : pool) th.join();
-    for (int t = 0; t < nthreads; ++t)
-        if (codes[t] != NAJA_OK) {
-            set_error("%s", msgs[t].c_str());
-            throw Error(codes[t]);
-        }
-}
-
 // Stacks [A0; E_v] and [b0; f_v] for variant v of a ragged batch.
 void stack_variant(int m0, int d, const double* A0, const double* b0, const int* ks, const int64_t* koff,
                    const double* Ae, const double* be, int v, std::vector<double>& A, std::vector<double>& b)

@@ -122,3 +122,68 @@ def test_sample_run_streamed_equals_whole(gpu, tmp_path, backmap):
         assert r.returncode == 0
         outs.append((run_dir(out) / "samples.npy").read_bytes())
     assert outs[0] == outs[1]
+
+
+def make_hull_model(tmp_path):
+    """Box [-1, 1]^4 with the equality y0 + y1 - y3 = 0.2 written as two
+    opposite rows, a start on it, and a random transform to 7 fluxes."""
+    rng = np.random.default_rng(5)
+    d, n = 4, 7
+    e = np.array([[1.0, 1.0, 0.0, -1.0]])
+    A = np.vstack([np.eye(d), -np.eye(d), e, -e])
+    b = np.concatenate([np.ones(2 * d), [0.2, -0.2]])
+    x0 = np.array([0.1, 0.05, 0.3, -0.05])
+    T = rng.standard_normal((n, d))
+    shift = rng.standard_normal(n)
+    md = tmp_path / "models" / "HULL"
+    (md / "rounding").mkdir(parents=True)
+    (md / "gem").mkdir()
+    pre = md / "rounding" / "HULL_rounding_"
+    write_csv(str(pre) + "A.csv", A)
+    write_csv(str(pre) + "b.csv", b[:, None])
+    write_csv(str(pre) + "start.csv", x0[:, None])
+    write_csv(str(pre) + "T.csv", T)
+    write_csv(str(pre) + "shift.csv", shift[:, None])
+    return md, A, b, x0, T, shift
+
+
+@pytest.mark.parametrize("backmap", [False, True])
+def test_sample_run_affine_hull(gpu, tmp_path, backmap):
+    """--affine-hull-tol (job_sampling.cu:196-235): sampling runs in the kernel
+    of the tight rows and the output maps back through (B, x0), composed with
+    (T, shift) under --backmap; equals the oracle chain on the reduced polytope."""
+    import oracle.hull as ohull
+
+    md, A, b, x0, T, shift = make_hull_model(tmp_path)
+    out = tmp_path / "out"
+    cmd = [NAJA, "sample", "run", "--model-dir", str(md), "--out-root", str(out), "--gpu", "0", "--n-chains", "4",
+           "--n-samples", "600", "--seed", "21", "--write-npy", "--affine-hull-tol", "1e-9", "--verbose"]
+    if backmap:
+        cmd.append("--backmap")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "affine_hull_dim  3 (from 4)" in r.stdout
+    rd = run_dir(out)
+    V = np.load(rd / "samples.npy")
+    prof = json.loads((rd / "profile.json").read_text())
+    rows, B = ohull.affine_hull(A, b, x0, 1e-9)
+    assert rows == [8, 9] and B.shape == (4, 3)
+    assert prof["config"]["thinning"] == 1  # max(3 / 6, 1) on the reduced dimension
+    # the reduced polytope in the CLI's own FP64 summation order (sequential dots)
+    Ar = np.array([[sum((A[i, j] * B[j, l] for j in range(4)), 0.0) for l in range(3)] for i in range(10)])
+    br = np.array([b[i] - sum((A[i, j] * x0[j] for j in range(4)), 0.0) for i in range(10)])
+    ref = oracle.chr_run_f32(Ar.astype(np.float32), br, np.zeros((4, 3), np.float32), 600, 1, seed=21)
+    Z = ref["samples"]
+    if backmap:
+        Tc, sc = T @ B, T @ x0 + shift
+        assert V.shape == (2400, 7)
+    else:
+        Tc, sc = B, x0
+        assert V.shape == (2400, 4)
+        # every sample stays on the equality and inside the box
+        assert np.abs(V @ np.array([1.0, 1.0, 0.0, -1.0]) - 0.2).max() <= 2e-6
+        assert np.abs(V).max() <= 1.0 + 1e-6
+        assert V[:, 0].std() > 0.2  # the chain moves along the face
+    Vref = oracle.backmap(Tc.astype(np.float32).astype(np.float64), sc.astype(np.float32).astype(np.float64), Z)
+    assert np.abs(V - Vref).max() <= 1e-5 * max(1.0, np.abs(Vref).max())

@@ -3,6 +3,7 @@
 //   naja sample run   (cmd_run.cpp:44-229 + run_sampling_job, job_sampling.cu:82-443)
 //   naja validate     (validate_cli.cpp:49-296)
 //   naja prepare      (cmd_prepare.cpp:213-457; prepare_cli.cpp)
+//   naja sample calibrate-rounding (cmd_calibrate_rounding.cpp:24-195)
 //
 // Flags, the model-directory contract, the run-directory layout and the
 // output files (samples.npy, profile.json, config_generated.txt,
@@ -1221,6 +1222,210 @@ int cmd_validate(int argc, char** argv)
 
 }  // namespace
 
+// Per-pair population variance/covariance of one warmup chain (rows of d),
+// accumulated into the sums over models (cmd_calibrate_rounding.cpp:136-156).
+void accumulate_pair_moments(const float* W, int d, int warmup, const PairSchedule& S, std::vector<double>& vi_sum,
+                             std::vector<double>& vj_sum, std::vector<double>& cov_sum)
+{
+    for (size_t k = 0; k < S.i.size(); ++k) {
+        const int ii = S.i[k], jj = S.j[k];
+        double mi = 0, mj = 0;
+        for (int t = 0; t < warmup; ++t) {
+            mi += W[(size_t)t * d + ii];
+            mj += W[(size_t)t * d + jj];
+        }
+        mi /= warmup;
+        mj /= warmup;
+        double vi = 0, vj = 0, cij = 0;
+        for (int t = 0; t < warmup; ++t) {
+            const double di = W[(size_t)t * d + ii] - mi, dj = W[(size_t)t * d + jj] - mj;
+            vi += di * di;
+            vj += dj * dj;
+            cij += di * dj;
+        }
+        vi_sum[k] += vi / warmup;
+        vj_sum[k] += vj / warmup;
+        cov_sum[k] += cij / warmup;
+    }
+}
+
+// cmd_calibrate_rounding.cpp:24-195: one Jacobi pair schedule for a set of
+// models. Disjoint pairs from a seeded shuffle; each pass runs one warmup
+// chain per model on rotated pairs (pair mode 2), averages the per-pair
+// moments over models and sets each pair's rotation. Models that share A and
+// b (the conditioned variants of one base model) run as one batched launch
+// per pass (variant = model index, seed ^ (pass + 1)); otherwise one launch
+// per model with the reference's seed ^ model ^ (pass + 1).
+int cmd_calibrate_rounding(int argc, char** argv)
+{
+    std::string models_root, model_list, out_csv;
+    int warmup = 2000, passes = 2, seed = 1, max_models = 0, gpu = 0;
+    double pair_prob = 0.3, extra_eps = 0.0;
+    for (int i = 0; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a == "--models-root") models_root = next_arg(i, argc, argv, a);
+        else if (a == "--model-list") model_list = next_arg(i, argc, argv, a);
+        else if (a == "--out") out_csv = next_arg(i, argc, argv, a);
+        else if (a == "--warmup") warmup = std::stoi(next_arg(i, argc, argv, a));
+        else if (a == "--passes") passes = std::stoi(next_arg(i, argc, argv, a));
+        else if (a == "--pair-prob") pair_prob = std::stod(next_arg(i, argc, argv, a));
+        else if (a == "--extra-constraint-eps") extra_eps = std::stod(next_arg(i, argc, argv, a));
+        else if (a == "--seed") seed = std::stoi(next_arg(i, argc, argv, a));
+        else if (a == "--max-models") max_models = std::stoi(next_arg(i, argc, argv, a));
+        else if (a == "--gpu") gpu = std::stoi(next_arg(i, argc, argv, a));
+        else die_usage("unknown flag: " + a);
+    }
+    if (models_root.empty()) die_usage("missing --models-root");
+    if (model_list.empty()) die_usage("missing --model-list");
+    if (out_csv.empty()) die_usage("missing --out");
+    if (warmup <= 0) die_usage("invalid --warmup");
+    if (passes <= 0) die_usage("invalid --passes");
+    if (!(pair_prob > 0.0 && pair_prob <= 1.0)) die_usage("invalid --pair-prob (0,1]");
+    if (extra_eps < 0.0) die_usage("invalid --extra-constraint-eps (must be >=0)");
+    if (max_models < 0) die_usage("invalid --max-models (must be >=0)");
+    if (seed == 0) seed = 1;
+    std::vector<std::string> names;
+    for (const auto& l : read_nonempty_trimmed_lines(model_list))
+        if (l[0] != '#') names.push_back(l);
+    if (names.empty()) throw std::runtime_error("no models in list: " + model_list);
+    if (max_models > 0 && (int)names.size() > max_models) names.resize((size_t)max_models);
+
+    // Load every model: base rows, extra rows (+ eps), start.
+    struct Model {
+        Matrix A;
+        std::vector<double> b, x0;
+        Matrix E;
+        std::vector<double> f;
+    };
+    std::vector<Model> ms;
+    for (const auto& name : names) {
+        std::string md = make_absolute_path(models_root + "/" + name);
+        if (!md.empty() && md.back() == '/') md.pop_back();
+        const std::string mname = md.substr(md.find_last_of('/') + 1);
+        const std::string pre = md + "/rounding/" + mname + "_rounding_";
+        Model m;
+        m.A = load_csv_matrix(pre + "A.csv");
+        m.b = load_csv_vector(pre + "b.csv");
+        m.x0 = load_csv_vector(pre + "start.csv");
+        if (path_exists(pre + "extra_A.csv") && path_exists(pre + "extra_b.csv")) {
+            m.E = load_csv_matrix(pre + "extra_A.csv");
+            m.f = load_csv_vector(pre + "extra_b.csv");
+            for (double& v : m.f) v += extra_eps;
+            if ((size_t)m.E.rows != m.f.size() || m.E.cols != m.A.cols)
+                throw std::runtime_error("extra constraint matrices have mismatched dimensions: " + pre);
+        } else {
+            m.E.cols = m.A.cols;
+        }
+        if ((int)m.x0.size() != m.A.cols || (int)m.b.size() != m.A.rows)
+            throw std::runtime_error("dimension mismatch in " + pre);
+        ms.push_back(std::move(m));
+    }
+    const int d = ms.front().A.cols;
+    if (d < 2) throw std::runtime_error("calibrate-rounding requires reduced_dim >= 2");
+    for (const auto& m : ms)
+        if (m.A.cols != d) throw std::runtime_error("calibrate-rounding: models differ in reduced dimension");
+    bool shared = true;
+    for (const auto& m : ms) shared = shared && m.A.rows == ms[0].A.rows && m.A.v == ms[0].A.v && m.b == ms[0].b;
+
+    std::vector<int> perm((size_t)d);
+    for (int i = 0; i < d; ++i) perm[(size_t)i] = i;
+    std::mt19937_64 rng((uint64_t)seed);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    const int np = d / 2;
+    PairSchedule S;
+    std::vector<double> sc((size_t)np, 1.0), ss((size_t)np, 0.0);  // FP64 angles as the reference keeps them
+    for (int k = 0; k < np; ++k) {
+        S.i.push_back(perm[(size_t)(2 * k)]);
+        S.j.push_back(perm[(size_t)(2 * k + 1)]);
+        S.c.push_back(1.0f);
+        S.s.push_back(0.0f);
+    }
+    HIPCHK(hipSetDevice(gpu));
+    const int nm = (int)ms.size();
+    const std::vector<float> A0cm = colmajor_f32(ms[0].A);
+    DevBuf<float> A0_d(A0cm.data(), A0cm.size());
+    DevBuf<double> b0_d(ms[0].b.data(), ms[0].b.size());
+    // batched layout of every model's extra rows and starts
+    std::vector<int> ks;
+    std::vector<float> Ecm, X0;
+    std::vector<double> fe;
+    for (const auto& m : ms) {
+        ks.push_back(m.E.rows);
+        const std::vector<float> e = colmajor_f32(m.E);
+        Ecm.insert(Ecm.end(), e.begin(), e.end());
+        fe.insert(fe.end(), m.f.begin(), m.f.end());
+        for (double v : m.x0) X0.push_back((float)v);
+    }
+    DevBuf<float> E_d(Ecm.data(), Ecm.size()), X_d(X0.data(), X0.size());
+    DevBuf<double> f_d(fe.data(), fe.size());
+    DevBuf<float> W_d((size_t)(shared ? nm : 1) * warmup * d);
+    std::vector<float> W((size_t)nm * warmup * d);
+    for (int pass = 0; pass < passes; ++pass) {
+        DevBuf<int> pi(S.i.data(), S.i.size()), pj(S.j.data(), S.j.size());
+        DevBuf<float> pc(S.c.data(), S.c.size()), ps(S.s.data(), S.s.size());
+        naja_chr_params p{};
+        p.nspc = warmup;
+        p.thinning = 1;
+        p.nchains = 1;
+        p.tpb = 128;
+        p.pair_prob = pair_prob;
+        p.ksparse_k = 8;
+        p.pair_mode = 2;
+        p.pair_i = pi.p;
+        p.pair_j = pj.p;
+        p.pair_c = pc.p;
+        p.pair_s = ps.p;
+        p.n_pairs = np;
+        p.chains_per_group = 1;
+        if (shared) {
+            p.seed = (uint64_t)(int64_t)(seed ^ (pass + 1));
+            napi(naja_chr_sample_batched(A0_d.p, b0_d.p, ms[0].A.rows, d, nm, ks.data(), Ecm.empty() ? nullptr : E_d.p,
+                                         fe.empty() ? nullptr : f_d.p, X_d.p, &p, W_d.p, nullptr, nullptr),
+                 "warmup CHR (batched)");
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipMemcpy(W.data(), W_d.p, sizeof(float) * W.size(), hipMemcpyDeviceToHost));
+        } else {
+            for (int mi = 0; mi < nm; ++mi) {
+                Matrix A = ms[(size_t)mi].A;
+                std::vector<double> b = ms[(size_t)mi].b;
+                A.v.insert(A.v.end(), ms[(size_t)mi].E.v.begin(), ms[(size_t)mi].E.v.end());
+                A.rows += ms[(size_t)mi].E.rows;
+                b.insert(b.end(), ms[(size_t)mi].f.begin(), ms[(size_t)mi].f.end());
+                const std::vector<float> Acm = colmajor_f32(A);
+                DevBuf<float> Ad(Acm.data(), Acm.size());
+                DevBuf<double> bd(b.data(), b.size());
+                p.seed = (uint64_t)(int64_t)(seed ^ mi ^ (pass + 1));
+                napi(naja_chr_sample(Ad.p, bd.p, A.rows, d, X_d.p + (size_t)mi * d, &p, W_d.p, nullptr, nullptr),
+                     "warmup CHR");
+                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipMemcpy(W.data() + (size_t)mi * warmup * d, W_d.p, sizeof(float) * (size_t)warmup * d,
+                                 hipMemcpyDeviceToHost));
+            }
+        }
+        std::vector<double> vi((size_t)np, 0.0), vj((size_t)np, 0.0), cv((size_t)np, 0.0);
+        for (int mi = 0; mi < nm; ++mi)
+            accumulate_pair_moments(W.data() + (size_t)mi * warmup * d, d, warmup, S, vi, vj, cv);
+        for (int k = 0; k < np; ++k) {
+            const auto cs = jacobi_rotation_cs(vi[(size_t)k] / nm, vj[(size_t)k] / nm, cv[(size_t)k] / nm);
+            sc[(size_t)k] = cs.first;
+            ss[(size_t)k] = cs.second;
+            S.c[(size_t)k] = (float)cs.first;
+            S.s[(size_t)k] = (float)cs.second;
+        }
+    }
+    // schedule_io.cpp:77-89: i,j,c,s per line at the stream's default precision
+    std::ofstream f(out_csv);
+    if (!f.is_open()) throw std::runtime_error("cannot write pair schedule: " + out_csv);
+    for (int k = 0; k < np; ++k)
+        f << S.i[(size_t)k] << "," << S.j[(size_t)k] << "," << sc[(size_t)k] << "," << ss[(size_t)k] << "\n";
+    f.close();
+    std::cout << "OK\nout              :: " << make_absolute_path(out_csv) << "\nmodels           :: " << nm
+              << "\nreduced_dim      :: " << d << "\npairs            :: " << np << "\npasses           :: " << passes
+              << "\nwarmup           :: " << warmup << "\npair_prob        :: " << pair_prob
+              << "\nextra_eps        :: " << extra_eps << "\nbatched          :: " << (shared ? "yes" : "no") << "\n";
+    return 0;
+}
+
 int cmd_prepare(int argc, char** argv);
 
 int main(int argc, char** argv)
@@ -1230,12 +1435,15 @@ int main(int argc, char** argv)
             return cmd_run(argc - 3, argv + 3);
         if (argc >= 2 && std::string(argv[1]) == "validate") return cmd_validate(argc - 2, argv + 2);
         if (argc >= 2 && std::string(argv[1]) == "prepare") return cmd_prepare(argc - 2, argv + 2);
+        if (argc >= 3 && std::string(argv[1]) == "sample" && std::string(argv[2]) == "calibrate-rounding")
+            return cmd_calibrate_rounding(argc - 3, argv + 3);
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
     }
     std::cerr << "usage:\n  naja sample run <args...>\n  naja validate --samples <file.npy> [--model-dir <dir>] "
                  "[--n-chains 4] [--chain-layout interleaved|naja-blocks]\n  naja prepare --models-root <dir> "
-                 "--model-list <file> [--base-model-dir <dir>] ...\n";
+                 "--model-list <file> [--base-model-dir <dir>] ...\n  naja sample calibrate-rounding --models-root <dir> "
+                 "--model-list <file> --out <csv> [--warmup 2000] [--passes 2] [--pair-prob 0.3] [--seed 1] ...\n";
     return 2;
 }

@@ -8,8 +8,9 @@ namespace naja {
 
 // One conditioned variant, packed by chr_pack_kernel: rows [A; E_v] in a
 // column-major block with lda = 64 W R (the launch shape's rows); padding
-// rows hold A = 0 and b = +inf, so their inverse distance is 0/inf = 0 and
-// they never move the chord of a bounded polytope.
+// rows hold A = 0 and b = +inf, so their slack stays +inf and their chord
+// ratio (inf / 0, or inf * 0 when the register kernel's ranged loads read the
+// padding as 0) never bounds the chord of a bounded polytope.
 struct ChrVariant {
     int m;               // real rows m0 + k
     int lda;             // padded rows (64 W R)
@@ -83,6 +84,9 @@ using ChrKernelFn = void (*)(ChrArgs);
 struct ChrKernelEntry {
     int W, G, R;
     ChrKernelFn fn;
+    // Same shape specialised for launches whose every move is a coordinate
+    // move (no pair or k-sparse probability); null when not compiled.
+    ChrKernelFn fn_coord = nullptr;
 };
 struct ChrKernelTable {
     const ChrKernelEntry* e;

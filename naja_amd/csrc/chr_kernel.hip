@@ -179,8 +179,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 {
     const ChrKernelTable& t = all_kernels();
     ChrKernelFn fn = nullptr;
+    const bool coord_only = args.thr_k == 0 && !args.has_pair;
     for (int q = 0; q < t.n; ++q)
-        if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R) fn = t.e[q].fn;
+        if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R)
+            fn = coord_only && t.e[q].fn_coord ? t.e[q].fn_coord : t.e[q].fn;
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d", sh.W, sh.G, sh.R);
     const size_t lds = sh.lds_bytes(args.d);
     NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",

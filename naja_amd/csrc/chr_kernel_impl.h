@@ -98,25 +98,38 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
 __device__ __forceinline__ uint32_t umin32(uint32_t a, uint32_t b) { return a < b ? a : b; }
 __device__ __forceinline__ int32_t smin32(int32_t a, int32_t b) { return a < b ? a : b; }
 
-// Unsigned and signed minimum over the 64 lanes of a wavefront: DPP within
-// each row of 16 (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
-// row_mirror), then the four row results through readlane.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
+{
+    // bound_ctrl set and full masks: every lane of these patterns has a
+    // source, and the move folds into the consuming min as a DPP operand.
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+// Unsigned and signed minimum over the 64 lanes of a wavefront, left in every
+// lane: DPP within each row of 16 (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), then rows 0/1 and 2/3 through
+// v_permlane16_swap and the two halves through v_permlane32_swap (gfx950),
+// all in VALU with no trip through scalar registers.
 __device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
 {
-    hb = umin32(hb, dpp_u32<0xB1>(hb));
-    lb = smin32(lb, (int32_t)dpp_u32<0xB1>((uint32_t)lb));
-    hb = umin32(hb, dpp_u32<0x4E>(hb));
-    lb = smin32(lb, (int32_t)dpp_u32<0x4E>((uint32_t)lb));
-    hb = umin32(hb, dpp_u32<0x141>(hb));
-    lb = smin32(lb, (int32_t)dpp_u32<0x141>((uint32_t)lb));
-    hb = umin32(hb, dpp_u32<0x140>(hb));
-    lb = smin32(lb, (int32_t)dpp_u32<0x140>((uint32_t)lb));
-    const uint32_t h0 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 0), h1 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 16);
-    const uint32_t h2 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 32), h3 = (uint32_t)__builtin_amdgcn_readlane((int)hb, 48);
-    const int32_t l0 = __builtin_amdgcn_readlane(lb, 0), l1 = __builtin_amdgcn_readlane(lb, 16);
-    const int32_t l2 = __builtin_amdgcn_readlane(lb, 32), l3 = __builtin_amdgcn_readlane(lb, 48);
-    hb = umin32(umin32(h0, h1), umin32(h2, h3));
-    lb = smin32(smin32(l0, l1), smin32(l2, l3));
+    uint32_t l = (uint32_t)lb;
+    hb = umin32(hb, dpp_mov<0xB1>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0xB1>(l));
+    hb = umin32(hb, dpp_mov<0x4E>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0x4E>(l));
+    hb = umin32(hb, dpp_mov<0x141>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0x141>(l));
+    hb = umin32(hb, dpp_mov<0x140>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0x140>(l));
+    const auto h16 = __builtin_amdgcn_permlane16_swap(hb, hb, false, false);
+    const auto l16 = __builtin_amdgcn_permlane16_swap(l, l, false, false);
+    hb = umin32(h16[0], h16[1]);
+    l = (uint32_t)smin32((int32_t)l16[0], (int32_t)l16[1]);
+    const auto h32 = __builtin_amdgcn_permlane32_swap(hb, hb, false, false);
+    const auto l32 = __builtin_amdgcn_permlane32_swap(l, l, false, false);
+    hb = umin32(h32[0], h32[1]);
+    lb = smin32((int32_t)l32[0], (int32_t)l32[1]);
 }
 
 // Chord ends in the direct form q = s / ae:
@@ -128,19 +141,25 @@ __device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
 // reference's ends (chr.cu:416-438), each rounded once here instead of twice.
 // A slack that FP32 rounding left at or below zero is replaced by +0, so a
 // point just outside a facet is treated as on it (the FP64 reference has no
-// such clamp).
+// such clamp). The clamp is a signed integer maximum with 0 over the bit
+// pattern (one instruction): negative values and -0 become +0, positive
+// values pass, and so does a NaN with the sign clear, whose quotient is a NaN
+// that never wins either minimum (no finite input produces one).
+__device__ __forceinline__ float clamp_slack(float s)
+{
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, s), 0));
+}
+
 __device__ __forceinline__ uint32_t chord_bits(float ae, float s)
 {
-    const float sp = (s > 0.0f) ? s : 0.0f;
-    return __builtin_bit_cast(uint32_t, sp / ae);
+    return __builtin_bit_cast(uint32_t, clamp_slack(s) / ae);
 }
 
 // Coordinate moves use the packed reciprocal column rinv = RN(1/A[r,i]):
 // q = RN(s * rinv), one multiply instead of an IEEE division.
 __device__ __forceinline__ uint32_t chord_bits_rcp(float rinv, float s)
 {
-    const float sp = (s > 0.0f) ? s : 0.0f;
-    return __builtin_bit_cast(uint32_t, sp * rinv);
+    return __builtin_bit_cast(uint32_t, clamp_slack(s) * rinv);
 }
 
 // Lane l of every wavefront draws the move and the alpha uniforms of step
@@ -155,16 +174,23 @@ struct MoveBatch {
     float u[G];  // alpha uniform of chain c0 + g
 };
 
-template <int G>
+// CO: the launch has no pair or k-sparse moves (every draw is a coordinate
+// move), so only the index is kept; the draws are the same.
+template <int G, bool CO>
 __device__ __forceinline__ void gen_batch(const ChrArgs& a, uint32_t k0, uint32_t k1, uint64_t tb, uint32_t group,
                                           int c0, MoveBatch<G>& mb)
 {
     const uint64_t t = tb + (uint64_t)(threadIdx.x & 63);
-    const Move mv = draw_move(a, k0, k1, t, group);
-    mb.tij = (mv.type << 30) | mv.i;
-    mb.j = (mv.type == kMoveKsparse) ? (int)mv.w3 : mv.j;
-    mb.ci = mv.ci;
-    mb.cj = mv.cj;
+    if (CO) {
+        const U32x4 w = philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), group, 0u, k0, k1);
+        mb.tij = (int)rng_index(w.y, (uint32_t)a.d);
+    } else {
+        const Move mv = draw_move(a, k0, k1, t, group);
+        mb.tij = (mv.type << 30) | mv.i;
+        mb.j = (mv.type == kMoveKsparse) ? (int)mv.w3 : mv.j;
+        mb.ci = mv.ci;
+        mb.cj = mv.cj;
+    }
     uint32_t ublk[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int g = 0; g < G; ++g) {
@@ -183,11 +209,20 @@ __device__ __forceinline__ void gen_batch(const ChrArgs& a, uint32_t k0, uint32_
 
 __device__ __forceinline__ int readlane_i32(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
-template <int G>
+template <int G, bool CO>
 __device__ __forceinline__ Move read_move(const MoveBatch<G>& mb, int lane)
 {
     Move mv;
     const int tij = readlane_i32(mb.tij, lane);
+    if (CO) {
+        mv.type = kMoveCoord;
+        mv.i = tij;
+        mv.j = -1;
+        mv.w3 = 0u;
+        mv.ci = 1.0f;
+        mv.cj = 0.0f;
+        return mv;
+    }
     mv.type = (int)((uint32_t)tij >> 30);
     mv.i = tij & 0x3FFFFFFF;
     const int jv = readlane_i32(mb.j, lane);
@@ -198,9 +233,34 @@ __device__ __forceinline__ Move read_move(const MoveBatch<G>& mb, int lane)
     return mv;
 }
 
-template <int W, int G, int R>
+// Raw buffer loads with a descriptor built by hand: base address, stride 0,
+// num_records (bytes; loads beyond it read 0) and the gfx9 dword-3 format
+// word (32-bit data). The loads are the LLVM intrinsics, named as functions.
+typedef int chr_i32x4 __attribute__((ext_vector_type(4)));
+typedef float chr_f32x2 __attribute__((ext_vector_type(2)));
+__device__ float chr_raw_load_f32(chr_i32x4 rsrc, int voffset, int soffset,
+                                  int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
+__device__ chr_f32x2 chr_raw_load_v2f32(chr_i32x4 rsrc, int voffset, int soffset,
+                                        int aux) __asm("llvm.amdgcn.raw.buffer.load.v2f32");
+__device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes)
+{
+    const uint64_t addr = (uint64_t)p;
+    chr_i32x4 r;
+    // Wave-uniform by construction; readfirstlane keeps it in SGPRs.
+    r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
+    r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
+    r[2] = __builtin_amdgcn_readfirstlane((int)bytes);
+    r[3] = 0x00020000;
+    return r;
+}
+
+// PF: prefetch distance in steps. The columns of step t + PF are loaded
+// during step t into one of PF + 1 rotating register sets, so each load has
+// PF steps to arrive (moves do not depend on the state).
+template <int W, int G, int R, bool CO = false, int PF = 1>
 __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 {
+    static_assert(PF == 1 || PF == 2, "prefetch distance 1 or 2");
     constexpr int T = 64 * W;
     extern __shared__ float xs[];  // G x d current states
     __shared__ uint2 part[2][G][W];
@@ -216,7 +276,6 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const int d = a.d;
     const int lda = V.lda;  // == T * R: every slot of every lane is a real or a padding row
     const float* Av = a.A + V.a_off + tid;
-    const float* Ainv = a.Ainv + V.a_off + tid;
     const int geff = min(G, a.nchains - item.c0);
     const uint32_t group = (uint32_t)(item.c0 / G);
     const uint32_t k0 = a.seed_lo;
@@ -235,19 +294,45 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const int64_t t_begin = a.t0;
     const int64_t t_end = a.t0 + a.nsteps;
     MoveBatch<G> mb;
-    gen_batch<G>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
-    Move mv = read_move<G>(mb, (int)(t_begin & 63));
-    {
-        // (ca, cb) = (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j])
-        // for a pair move; a k-sparse move loads a dummy pair.
-        const float* pi = Av + (int64_t)(mv.type == kMoveKsparse ? 0 : mv.i) * lda;
-        const float* pj = (mv.type == kMovePair) ? Av + (int64_t)mv.j * lda
-                                                 : Ainv + (int64_t)(mv.type == kMoveKsparse ? 0 : mv.i) * lda;
+    gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
+    const int L0 = (int)(t_begin & 63);
+    Move mv = read_move<G, CO>(mb, L0);
+    // (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j]) for a pair
+    // move; a k-sparse move (or no move) loads a dummy pair. Buffer loads over
+    // one column each, ranged to the variant's real rows: padding rows read 0
+    // without touching memory (their slack is +inf, so their chord ratio is
+    // inf * 0 or inf / 0 and never bounds the chord).
+    const float* Acol0 = a.A + V.a_off;
+    const float* Icol0 = a.Ainv + V.a_off;
+    const uint32_t col_bytes = 4u * (uint32_t)V.m;
+    auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
+        const int mt = CO ? kMoveCoord : m.type;
+        const bool real = real_move && mt != kMoveKsparse;
+        const bool pair = real && mt == kMovePair;
+        const float* pi = Acol0 + (int64_t)(real ? m.i : 0) * lda;
+        const float* pj = pair ? Acol0 + (int64_t)m.j * lda : Icol0 + (int64_t)(real ? m.i : 0) * lda;
+        const chr_i32x4 ri = chr_make_rsrc(pi, col_bytes);
+        const chr_i32x4 rj = chr_make_rsrc(pj, col_bytes);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            ca[k] = pi[k * T];
-            cb[k] = pj[k * T];
+            xa[k] = chr_raw_load_f32(ri, 4 * (k * T + tid), 0, 0);
+            xb[k] = chr_raw_load_f32(rj, 4 * (k * T + tid), 0, 0);
         }
+    };
+    load_cols(mv, true, ca, cb);
+    float Ucur[G];  // alpha uniforms of the current step (PF = 2 reads them a step early)
+#pragma unroll
+    for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], L0);
+    // Columns of set 1 for PF = 2: the move of t_begin + 1, and from here on
+    // mb holds the batch of the step after the current one.
+    float c1a[R], c1b[R];
+    Move nx = mv;
+    if (PF == 2) {
+        if (t_begin + 1 < t_end) {
+            if (L0 == 63) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + 1), group, item.c0, mb);
+            nx = read_move<G, CO>(mb, (L0 + 1) & 63);
+        }
+        load_cols(nx, t_begin + 1 < t_end, c1a, c1b);
     }
 
     float last_i[G], last_j[G];
@@ -262,12 +347,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     int until_resync = 0;  // steps until the next FP64 slack computation
     int64_t sample_idx = 0;
 
-    // One CHR step. The two register sets (ca, cb) / (na, nb) alternate
-    // between consecutive steps (the loop below is unrolled by two), so the
-    // prefetched columns are never copied and their loads stay in flight
-    // across the rest of the step.
-    Move nx = mv;
+    // One CHR step on the columns (ca, cb), prefetching the columns of step
+    // t + PF into (na, nb). The register sets rotate between consecutive
+    // steps (the loop below is unrolled by PF + 1), so the prefetched columns
+    // are never copied and their loads stay in flight across PF steps.
     auto step = [&](int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+        const int mtype = CO ? kMoveCoord : mv.type;
         if (until_resync == 0) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
             __syncthreads();
@@ -303,47 +388,49 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 
         if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t - t_begin < a.nlog && tid == 0) {
             const int64_t tl = t - t_begin;
-            a.move_log[tl * 3 + 0] = mv.type;
+            a.move_log[tl * 3 + 0] = mtype;
             a.move_log[tl * 3 + 1] = mv.i;
-            a.move_log[tl * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+            a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
         }
 
-        // Alpha uniforms of step t, then the draw of step t+1 (a new batch of
-        // 64 steps when t+1 crosses a batch boundary).
+        // Alpha uniforms of step t and the draw of step t + PF (a new batch of
+        // 64 steps when t + PF crosses a batch boundary; with PF = 2 the
+        // uniforms of step t + 1 are read first, before that batch replaces
+        // the one holding them).
         const int L = (int)(t & 63);
         float U[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) U[g] = readlane_f32(mb.u[g], L);
-        nx = mv;
-        const bool has_next = t + 1 < t_end;
-        if (has_next) {
-            if (L == 63) gen_batch<G>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
-            nx = read_move<G>(mb, (L + 1) & 63);
-        }
-
-        // Prefetch the columns of step t+1 into the other register set
-        // (moves do not depend on state).
-        // Every step issues exactly 2R loads, so the in-order vmcnt wait for
-        // this step's columns never waits on the prefetch of the next.
-        {
-            const bool real = has_next && nx.type != kMoveKsparse;
-            const bool pair = real && nx.type == kMovePair;
-            const float* pi = Av + (int64_t)(real ? nx.i : 0) * lda;
-            const float* pj = pair ? Av + (int64_t)nx.j * lda : Ainv + (int64_t)(real ? nx.i : 0) * lda;
+        for (int g = 0; g < G; ++g) U[g] = Ucur[g];
+        Move nf = mv;  // the move of step t + PF
+        const bool has_pf = t + PF < t_end;
+        if (PF == 1) {
+            if (has_pf) {
+                if (L == 63) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
+                nf = read_move<G, CO>(mb, (L + 1) & 63);
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                na[k] = pi[k * T];
-                nb[k] = pj[k * T];
+                for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], (L + 1) & 63);
+            }
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], (L + 1) & 63);
+            if (has_pf) {
+                if (L == 62) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + 2), group, item.c0, mb);
+                nf = read_move<G, CO>(mb, (L + 2) & 63);
             }
         }
+
+        // Prefetch the columns of step t + PF. Every step issues exactly 2R
+        // loads, so the in-order vmcnt wait for this step's columns never
+        // waits on the later prefetches.
+        load_cols(nf, has_pf, na, nb);
 
         // ae = (A u)_r for this lane's rows.
         float ae[R];
         const int kq = a.ks_k;
-        if (mv.type == kMoveCoord) {
+        if (mtype == kMoveCoord) {
 #pragma unroll
             for (int k = 0; k < R; ++k) ae[k] = ca[k];
-        } else if (mv.type == kMovePair) {
+        } else if (mtype == kMovePair) {
 #pragma unroll
             for (int k = 0; k < R; ++k) ae[k] = fmaf(ca[k], mv.ci, cb[k] * mv.cj);
         } else {
@@ -364,6 +451,26 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
         }
 
+        // Pre-update coordinates. With one wavefront its own LDS write of the
+        // previous update is ordered before this read, so the read is issued
+        // here, early; with several, registers carry the previous update
+        // because the owner's LDS write of it is not yet ordered.
+        float oi[G], oj[G];
+        if (mtype != kMoveKsparse) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                if (W == 1) {
+                    oi[g] = xs[g * d + mv.i];
+                    oj[g] = (mtype == kMovePair) ? xs[g * d + mv.j] : 0.0f;
+                } else {
+                    oi[g] = (mv.i == prev_j) ? last_j[g] : ((mv.i == prev_i) ? last_i[g] : xs[g * d + mv.i]);
+                    oj[g] = 0.0f;
+                    if (mtype == kMovePair)
+                        oj[g] = (mv.j == prev_j) ? last_j[g] : ((mv.j == prev_i) ? last_i[g] : xs[g * d + mv.j]);
+                }
+            }
+        }
+
         // Chord ends per chain.
         uint32_t rhi[G];
         int32_t rlo[G];
@@ -371,7 +478,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) {
             uint32_t hb = 0xFFFFFFFFu;
             int32_t lb = 0x7FFFFFFF;
-            if (mv.type == kMoveCoord) {
+            if (mtype == kMoveCoord) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     const uint32_t qb = chord_bits_rcp(cb[k], s[g][k]);
@@ -390,19 +497,6 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             rhi[g] = hb;
             rlo[g] = lb;
             if (W > 1 && lane == 0) part[buf][g][wave] = make_uint2(hb, (uint32_t)lb);
-        }
-
-        // Pre-update coordinates (registers carry the previous step's update
-        // because the owner's LDS write of it is not yet ordered).
-        float oi[G], oj[G];
-        if (mv.type != kMoveKsparse) {
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                oi[g] = (mv.i == prev_j) ? last_j[g] : ((mv.i == prev_i) ? last_i[g] : xs[g * d + mv.i]);
-                oj[g] = 0.0f;
-                if (mv.type == kMovePair)
-                    oj[g] = (mv.j == prev_j) ? last_j[g] : ((mv.j == prev_i) ? last_i[g] : xs[g * d + mv.j]);
-            }
         }
 
         if (W > 1) __syncthreads();
@@ -433,23 +527,23 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             for (int k = 0; k < R; ++k) s[g][k] = fmaf(nal, ae[k], s[g][k]);
         }
 
-        if (mv.type != kMoveKsparse) {
+        if (mtype != kMoveKsparse) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const float ni = fmaf(alpha[g], mv.ci, oi[g]);
                 last_i[g] = ni;
-                if (mv.type == kMovePair) {
+                if (mtype == kMovePair) {
                     const float base = (mv.j == mv.i) ? ni : oj[g];
                     last_j[g] = fmaf(alpha[g], mv.cj, base);
                 }
             }
             prev_i = mv.i;
-            prev_j = (mv.type == kMovePair) ? mv.j : -1;
+            prev_j = (mtype == kMovePair) ? mv.j : -1;
             if (tid == 0) {
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     xs[g * d + mv.i] = last_i[g];
-                    if (mv.type == kMovePair) xs[g * d + mv.j] = last_j[g];
+                    if (mtype == kMovePair) xs[g * d + mv.j] = last_j[g];
                 }
             }
         } else {
@@ -485,48 +579,37 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         --until_save;
         if (until_resync > 0) --until_resync;
 
-    mv = nx;
+    if (PF == 1) {
+        mv = nf;
+    } else {
+        mv = nx;
+        nx = nf;
+    }
     buf ^= 1;
     };
-    float na[R], nb[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        na[k] = 0.0f;
-        nb[k] = 0.0f;
-    }
     int64_t t = t_begin;
-    for (; t + 1 < t_end; t += 2) {
-        step(t, ca, cb, na, nb);
-        step(t + 1, na, nb, ca, cb);
+    if (PF == 1) {
+        for (; t + 1 < t_end; t += 2) {
+            step(t, ca, cb, c1a, c1b);
+            step(t + 1, c1a, c1b, ca, cb);
+        }
+        if (t < t_end) step(t, ca, cb, c1a, c1b);
+    } else {
+        float c2a[R], c2b[R];
+        for (; t + 2 < t_end; t += 3) {
+            step(t, ca, cb, c2a, c2b);
+            step(t + 1, c1a, c1b, ca, cb);
+            step(t + 2, c2a, c2b, c1a, c1b);
+        }
+        if (t < t_end) step(t, ca, cb, c2a, c2b);
+        if (t + 1 < t_end) step(t + 1, c1a, c1b, ca, cb);
     }
-    if (t < t_end) step(t, ca, cb, na, nb);
 
     __syncthreads();
     if (a.y_final) {
         for (int g = 0; g < geff; ++g)
             for (int c = tid; c < d; c += T) a.y_final[(V.y0_col + item.c0 + g) * (int64_t)d + c] = xs[g * d + c];
     }
-}
-
-// Raw buffer loads with a descriptor built by hand: base address, stride 0,
-// num_records (bytes; loads beyond it read 0) and the gfx9 dword-3 format
-// word (32-bit data). The loads are the LLVM intrinsics, named as functions.
-typedef int chr_i32x4 __attribute__((ext_vector_type(4)));
-typedef float chr_f32x2 __attribute__((ext_vector_type(2)));
-__device__ float chr_raw_load_f32(chr_i32x4 rsrc, int voffset, int soffset,
-                                  int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
-__device__ chr_f32x2 chr_raw_load_v2f32(chr_i32x4 rsrc, int voffset, int soffset,
-                                        int aux) __asm("llvm.amdgcn.raw.buffer.load.v2f32");
-__device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes)
-{
-    const uint64_t addr = (uint64_t)p;
-    chr_i32x4 r;
-    // Wave-uniform by construction; readfirstlane keeps it in SGPRs.
-    r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
-    r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
-    r[2] = __builtin_amdgcn_readfirstlane((int)bytes);
-    r[3] = 0x00020000;
-    return r;
 }
 
 // Large polytopes (more rows than the register-resident kernel above holds):
@@ -577,8 +660,8 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
 
     float s[R], X0[R], X1[R];
     MoveBatch<1> mb;
-    gen_batch<1>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
-    Move mv = read_move<1>(mb, (int)(t_begin & 63));
+    gen_batch<1, false>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
+    Move mv = read_move<1, false>(mb, (int)(t_begin & 63));
     {
         const int ci = (mv.type == kMoveKsparse) ? 0 : mv.i;
 #pragma unroll
@@ -620,8 +703,8 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
         Move nx = mv;
         const bool has_next = t + 1 < t_end;
         if (has_next) {
-            if (L == 63) gen_batch<1>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
-            nx = read_move<1>(mb, (L + 1) & 63);
+            if (L == 63) gen_batch<1, false>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
+            nx = read_move<1, false>(mb, (L + 1) & 63);
         }
 
         // This step's direction column in X.

@@ -217,7 +217,13 @@ static void* orc_chain_worker32(void* arg)
                     for (int q = 0; q < mv.k; ++q) a = fmaf(J->A[(size_t)mv.idx[q] * m + r], mv.coef[q], a);
                 }
                 ae[r] = a;
-                const float sp = (s[r] > 0.0f) ? s[r] : 0.0f;
+                /* s <= 0 (and -0) -> +0 as a signed integer max over the bits; a
+                   NaN with the sign clear passes and never wins a minimum */
+                int32_t sbits;
+                memcpy(&sbits, &s[r], 4);
+                if (sbits < 0) sbits = 0;
+                float sp;
+                memcpy(&sp, &sbits, 4);
                 /* coordinate moves: q = RN(s * RN(1/a)); other moves: RN(s / ae) */
                 const float q = (mv.type == ORC_MOVE_COORD) ? sp * (1.0f / a) : sp / a;
                 uint32_t qb;

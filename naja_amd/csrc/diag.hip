@@ -52,11 +52,18 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 // Twiddles W^k = exp(-2 pi i k / N), N = 2H, from two small LDS tables:
 // k = hi * B + lo, W^k = thi[hi] * tlo[lo] (B = 2^tb, both tables <= 128
 // entries for H <= 16384), each entry rounded once from FP64 on the host.
+// A pass over N' = N / 2^sh points reads W_{N'}^k = W_N^(k 2^sh) from the same
+// tables.
 struct Twiddles {
     const float2* tlo;
     const float2* thi;
     int tb;
-    __device__ __forceinline__ float2 at(int k) const { return cmul(thi[k >> tb], tlo[k & ((1 << tb) - 1)]); }
+    int sh;
+    __device__ __forceinline__ float2 at(int k) const
+    {
+        const int e = k << sh;
+        return cmul(thi[e >> tb], tlo[e & ((1 << tb) - 1)]);
+    }
 };
 
 // (x) * (-i) and (x) * (+i).
@@ -173,9 +180,15 @@ __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __re
 }
 
 // ESS of one contiguous series per workgroup (blockIdx.x = (v * nchains + c) * dim + d).
+// mode 0: the full transform (N >= 2n, every lag). mode 1: a shorter
+// transform, N < 2n: zero padding to N keeps the correlation exact for lags
+// k <= N - n, so the result is final when the truncation point (the first
+// negative pair, lags 2k-1 and 2k) lies within them; otherwise the series is
+// marked -1 for mode 2, which redoes only the marked series at full size.
 __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ series, int dim,
                                                                   int nchains, int spc, int logH,
                                                                   const float2* __restrict__ tw_tables, int tb,
+                                                                  int tw_shift, int mode,
                                                                   float* __restrict__ ess_chain)
 {
     extern __shared__ float2 z[];
@@ -195,14 +208,15 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     const int N = 2 * H;
     const int tid = threadIdx.x;
 
+    if (mode == 2 && *out >= 0.0f) return;  // settled by the short pass (uniform per workgroup)
     if (n < 4) {
         if (tid == 0) *out = (float)n;
         return;
     }
-    const int nlo = 1 << tb, nhi = H >> tb;
+    const int nlo = 1 << tb, nhi = (H << tw_shift) >> tb;
     for (int i = tid; i < nlo; i += blockDim.x) tlo_s[i] = tw_tables[i];
     for (int i = tid; i < nhi; i += blockDim.x) thi_s[i] = tw_tables[nlo + i];
-    const Twiddles tw{tlo_s, thi_s, tb};
+    const Twiddles tw{tlo_s, thi_s, tb, tw_shift};
 
     // One pass over the series: raw values into the packed complex layout
     // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside (mcmc.cpp:25-27).
@@ -285,8 +299,10 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         if (tid == 0) *out = 1.0f;
         return;  // uniform across the workgroup: no barrier follows for anyone
     }
-    int kneg = kend;
-    for (int k = 1 + tid; k < kend; k += blockDim.x) {
+    // Pairs whose lags are exact in this transform: 2k <= N - n.
+    const int klim = (mode == 1) ? min(kend, (N - n) / 2 + 1) : kend;
+    int kneg = klim;
+    for (int k = 1 + tid; k < klim; k += blockDim.x) {
         const float pair_sum = (raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var;
         if (pair_sum < 0.0f) {
             kneg = k;
@@ -296,8 +312,12 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     for (int off = 32; off > 0; off >>= 1) kneg = min(kneg, __shfl_xor(kneg, off));
     if ((tid & 63) == 0) kfirst[tid >> 6] = kneg;
     __syncthreads();
-    int K = kend;
+    int K = klim;
     for (int w = 0; w < kDiagThreads / 64; ++w) K = min(K, kfirst[w]);
+    if (K == klim && klim < kend) {
+        if (tid == 0) *out = -1.0f;  // truncation beyond the exact lags: redo at full size
+        return;
+    }
     double acc = 0.0;
     for (int k = 1 + tid; k < K; k += blockDim.x)
         acc += (double)((raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var);
@@ -461,9 +481,17 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)(s_tiles * nchains * nv), (unsigned)((dim + 63) / 64)),
                        dim3(256), 0, stream, samples, dim, nchains, spc, layout, vstride, s_tiles, ser);
     NAJA_HIP(hipGetLastError());
-    const size_t lds = sizeof(float2) * (size_t)H;
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), lds, stream, ser, dim, nchains,
-                       spc, logH, tw, tb, ec);
+    // A half-size pass first when it leaves at least 1/8 of the chain as exact
+    // lags (N/2 - n >= n/8); the full pass then redoes only the series whose
+    // truncation point lies beyond them.
+    const bool two_pass = N / 2 - spc >= spc / 8;
+    if (two_pass) {
+        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)(H / 2),
+                           stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec);
+        NAJA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)H,
+                       stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
     NAJA_HIP(hipGetLastError());

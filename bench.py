@@ -327,25 +327,30 @@ def large_scale_plain(np, torch, nspc):
 def cpu_baseline(np, A0, b0, EA, Eb, P, nch, nspc, thin, seed, resync, ess_pass_sum, budget_s):
     """The FP32 oracle (same Philox stream and arithmetic as the GPU kernel)
     on the host cores, over a bounded sample of the learned-rounded main run:
-    a few variants, 16 threads (4 variants at a time x 4 chains), a prefix of
-    the steps; extrapolated linearly in chain-steps to the whole job."""
+    whole variants (every chain, every step) chosen at random, as many as fill
+    about budget_s seconds on 16 threads (4 variants at a time x 4 chains);
+    extrapolated linearly in chain-steps to all variants."""
     import concurrent.futures as cf
 
     import oracle
 
-    rng = np.random.default_rng(0)
-    pick = rng.choice(len(EA), size=8, replace=False)
     y0 = np.zeros((nch, A0.shape[1]), np.float32)
-    jobs = []
-    for v in pick:
+
+    def job(v):
         A = np.vstack([A0, EA[v]]).astype(np.float32)
         b = np.concatenate([b0, Eb[v]])
         At, bt = oracle.transform(A, b, P[v].astype(np.float32), np.zeros(A.shape[1], np.float32))
-        jobs.append((At, bt, int(v)))
-    probe = oracle.chr_run_f32(jobs[0][0], jobs[0][1], y0, 20, thin, seed=seed, variant=jobs[0][2],
+        return At, bt, int(v)
+
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(EA))
+    first = job(order[0])
+    probe = oracle.chr_run_f32(first[0], first[1], y0, 50, thin, seed=seed, variant=first[2],
                                resync_interval=resync, nthreads=nch)
-    per_sample = probe["seconds"] / 20  # all chains of one variant
-    ns = int(max(20, min(nspc, budget_s * 4 / len(jobs) / max(per_sample, 1e-9))))
+    per_variant = probe["seconds"] / 50 * nspc  # all chains of one variant, whole run
+    n_pick = int(min(len(EA), max(4, round(budget_s * 4 / max(per_variant, 1e-9)))))
+    jobs = [first] + [job(v) for v in order[1:n_pick]]
+    ns = nspc
 
     def one(job):
         At, bt, v = job
@@ -362,9 +367,9 @@ def cpu_baseline(np, A0, b0, EA, Eb, P, nch, nspc, thin, seed, resync, ess_pass_
     t_total = total_chain_steps / rate
     return {"value": round(ess_pass_sum / t_total, 4), "unit": "min-ESS/s", "cores": 16, "kind": "port",
             "sample": f"oracle CHR (FP32 slack, FP64 resync, same Philox stream) on the learned-rounded "
-                      f"polytopes of {len(jobs)} variants x {nch} chains x {ns} samples x thin {thin} "
-                      f"({wall:.1f} s wall, 4 variants at a time x 4 chain threads, {rate:.0f} chain-steps/s), "
-                      f"extrapolated to all {len(EA)} variants x {nspc} samples: {t_total:.0f} s; "
+                      f"polytopes of {len(jobs)} of the {len(EA)} variants, whole runs ({nch} chains x {ns} "
+                      f"samples x thin {thin}; {wall:.1f} s wall, 4 variants at a time x 4 chain threads, "
+                      f"{rate:.0f} chain-steps/s), extrapolated to all {len(EA)} variants: {t_total:.1f} s; "
                       f"diagnostics and the rounding solves not included",
             "chain_steps_per_s": round(rate, 1), "seconds_total": round(t_total, 1)}
 

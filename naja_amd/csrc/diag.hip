@@ -21,8 +21,9 @@
 //     the first negative pair (a workgroup minimum), and the pairs before it
 //     are summed in parallel in FP64 (the reference adds them one by one in
 //     float; the two agree to float rounding).
-// Split-R-hat (mcmc.cpp:126-194): FP64 sums per (dim, half chain), then the
-// reference's B / W / var_hat formula per dim.
+// Split-R-hat (mcmc.cpp:126-194): FP64 sums per (dim, half chain), taken by
+// the first ESS pass while it reads each series, then the reference's
+// B / W / var_hat formula per dim.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -189,11 +190,12 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
                                                                   int nchains, int spc, int logH,
                                                                   const float2* __restrict__ tw_tables, int tb,
                                                                   int tw_shift, int mode,
-                                                                  float* __restrict__ ess_chain)
+                                                                  float* __restrict__ ess_chain, int half,
+                                                                  double* __restrict__ mom)
 {
     extern __shared__ float2 z[];
     __shared__ float2 tlo_s[128], thi_s[128];
-    __shared__ double red[kDiagThreads / 64];
+    __shared__ double red[5][kDiagThreads / 64];
     __shared__ float meanf_s;
 
     const int64_t sid = blockIdx.x;
@@ -219,22 +221,44 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     const Twiddles tw{tlo_s, thi_s, tb, tw_shift};
 
     // One pass over the series: raw values into the packed complex layout
-    // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside (mcmc.cpp:25-27).
-    double part = 0.0;
+    // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside
+    // (mcmc.cpp:25-27), and the split-R-hat moments of its two halves
+    // (sum and sum of squares of samples [0, half) and [half, 2 half) in FP64,
+    // mcmc.cpp:150-165), written for the R-hat pass by the first ESS pass.
+    double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // total, half-0 sum, sq, half-1 sum, sq
     for (int j = tid; j < H; j += blockDim.x) {
         const int s0 = 2 * j, s1 = 2 * j + 1;
         const float x0 = s0 < n ? x[s0] : 0.0f;
         const float x1 = s1 < n ? x[s1] : 0.0f;
-        part += (double)x0 + (double)x1;
+        const double d0 = (double)x0, d1 = (double)x1;
+        part[0] += d0 + d1;
+        if (s0 < 2 * half) {
+            const int h = s0 >= half;
+            part[1 + 2 * h] += d0;
+            part[2 + 2 * h] += d0 * d0;
+        }
+        if (s1 < 2 * half) {
+            const int h = s1 >= half;
+            part[1 + 2 * h] += d1;
+            part[2 + 2 * h] += d1 * d1;
+        }
         z[j] = make_float2(x0, x1);
     }
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-    if ((tid & 63) == 0) red[tid >> 6] = part;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        for (int off = 32; off > 0; off >>= 1) part[q] += __shfl_xor(part[q], off);
+        if ((tid & 63) == 0) red[q][tid >> 6] = part[q];
+    }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < 5) {
         double sum = 0.0;
-        for (int w = 0; w < kDiagThreads / 64; ++w) sum += red[w];
-        meanf_s = (float)(sum / n);
+        for (int w = 0; w < kDiagThreads / 64; ++w) sum += red[tid][w];
+        if (tid == 0) meanf_s = (float)(sum / n);
+        else if (mom && mode != 2) {
+            // mom[v][2c + h][d][moment]
+            const int h = (tid - 1) >> 1, q = (tid - 1) & 1;
+            mom[(((int64_t)v * 2 * nchains + 2 * c + h) * dim + d) * 2 + q] = sum;
+        }
     }
     __syncthreads();
     const float meanf = meanf_s;
@@ -341,39 +365,6 @@ __global__ void ess_min_kernel(const float* __restrict__ ess_chain, int dim, int
     float m = 3.402823466e+38f;
     for (int c = 0; c < nchains; ++c) m = fminf(m, ess_chain[(int64_t)d * nchains + c]);
     ess[d] = m;
-}
-
-// Sum and sum of squares (FP64) of 64 dims x one half chain per workgroup.
-__global__ __launch_bounds__(256) void rhat_moments_kernel(const float* __restrict__ samples, int dim, int nchains,
-                                                          int spc, int half, int layout, double* __restrict__ mom,
-                                                          int64_t vstride)
-{
-    samples += (int64_t)blockIdx.z * vstride;
-    mom += (int64_t)blockIdx.z * 2 * nchains * dim * 2;
-    __shared__ double sh[2][4][64];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int d = blockIdx.x * 64 + lane;
-    const int hc = blockIdx.y;  // half chain = 2*c + h
-    const int c = hc >> 1;
-    const int h = hc & 1;
-    double s = 0.0, s2 = 0.0;
-    if (d < dim) {
-        for (int i = wave; i < half; i += 4) {
-            const double v = (double)samples[sample_row(layout, h * half + i, c, nchains, spc) * dim + d];
-            s += v;
-            s2 += v * v;
-        }
-    }
-    sh[0][wave][lane] = s;
-    sh[1][wave][lane] = s2;
-    __syncthreads();
-    if (wave == 0 && d < dim) {
-        const double S = sh[0][0][lane] + sh[0][1][lane] + sh[0][2][lane] + sh[0][3][lane];
-        const double S2 = sh[1][0][lane] + sh[1][1][lane] + sh[1][2][lane] + sh[1][3][lane];
-        mom[((int64_t)hc * dim + d) * 2 + 0] = S;
-        mom[((int64_t)hc * dim + d) * 2 + 1] = S2;
-    }
 }
 
 // mcmc.cpp:150-186 per dim.
@@ -487,16 +478,13 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const bool two_pass = N / 2 - spc >= spc / 8;
     if (two_pass) {
         hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)(H / 2),
-                           stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec);
+                           stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec, half, mom);
         NAJA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)H,
-                       stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec);
+                       stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
-    NAJA_HIP(hipGetLastError());
-    hipLaunchKernelGGL(rhat_moments_kernel, dim3((dim + 63) / 64, 2 * nchains, nv), dim3(256), 0, stream, samples,
-                       dim, nchains, spc, half, layout, mom, vstride);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(rhat_final_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, mom, dim, nchains, half,
                        rhat);

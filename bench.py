@@ -32,9 +32,9 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 # Bytes the main learned-rounded launch moves between L2 and the memory fabric
-# (256 variants x 4 chains x 200000 steps): FETCH_SIZE and WRITE_SIZE passes
-# scaled by a 1 GiB copy probe (profiles/chr_step_pmc_traffic.md).
-MAIN_LAUNCH_TRAFFIC_BYTES = 812.9e9 + 5.32e9
+# (256 variants x 4 chains x 200000 steps): rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes, scaled by what the same counters read for a 1 GiB copy.
+MAIN_LAUNCH_TRAFFIC_BYTES = 580.4e9 + 5.32e9
 METRIC = "min-ESS/sec at split-R-hat<1.05, learned-rounded vs plain CHR, 1 MI355X"
 
 
@@ -76,6 +76,7 @@ def main():
         print(json.dumps({"large_scale_plain": large_scale_plain(np, torch, args.large_nspc)}), flush=True)
         return
 
+    import naja_amd
     from naja_amd import conditioning as cd
     from naja_amd import precond
     from naja_amd.learned import LearnedPreconditioner
@@ -129,11 +130,19 @@ def main():
     torch.cuda.synchronize()
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
     torch.cuda.synchronize()
+    # HIP events around every CHR step-kernel launch (recorded by the library
+    # on the launch stream): the main launch's kernel time for the roofline.
+    naja_amd.chr_kernel_times()
+    naja_amd.chr_kernel_timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
         Y, e, r, acc = learned_step(events[k])
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
+    naja_amd.chr_kernel_timing(False)
+    launches = naja_amd.chr_kernel_times()
+    chr_ms = [ms for items, ms in launches if items == nv * nch]
+    chr_kernel_ms = float(np.mean(chr_ms)) if chr_ms else float("nan")
     main_ms = float(np.mean([ev[0].elapsed_time(ev[1]) for ev in events]))
     ok = r < 1.05
     value = float(e[ok].sum()) / sec
@@ -160,14 +169,16 @@ def main():
     m_v = 2 * d + np.array([e_.shape[0] for e_ in EAh])
     steps_per_chain = nspc * thin
     total_bytes = float(np.sum(nch * steps_per_chain * (4.0 * m_v * 3.0 + 4.0 * d / thin)))
-    achieved = total_bytes / (main_ms * 1e-3) / 1e9
+    achieved = total_bytes / (chr_kernel_ms * 1e-3) / 1e9
     traffic = MAIN_LAUNCH_TRAFFIC_BYTES if (nv, nch, nspc, thin) == (256, 4, 12500, 16) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(main_ms, 3),
+                "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(chr_kernel_ms, 3),
+                "launches_timed": len(chr_ms), "main_call_ms": round(main_ms, 3),
                 "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),
-                "note": "kernel_ms brackets the main sampling call (pack + transform + CHR + backmap)"}
+                "note": "kernel_ms: HIP events the library records around the main CHR launch on its stream; "
+                        "main_call_ms brackets the whole sampling call (pack + transform + CHR + backmap)"}
 
     genome = None if args.no_genome else genome_scale_plain(np, torch)
     large = None if args.no_large else large_scale_plain(np, torch, args.large_nspc)

@@ -187,6 +187,16 @@ int naja_diag(const float* samples, int dim, int n_total, int nchains, int layou
 int naja_diag_batched(const float* samples, int n_variants, int dim, int n_total, int nchains, int layout,
                       float* ess, float* rhat, float* ess_min, float* rhat_max, void* stream);
 
+/*
+ * Measurement hook (no reference counterpart; for benchmarks): while enabled,
+ * every CHR step-kernel launch records a pair of HIP events on its stream.
+ * naja_chr_kernel_times waits for and returns up to max recorded launches
+ * (work items, i.e. chain groups, and milliseconds), oldest first, writes how
+ * many it returned to *count, and clears the record. Off by default.
+ */
+int naja_chr_timing(int enable);
+int naja_chr_kernel_times(int* items, float* ms, int max, int* count);
+
 /* ---------------------------------------------------------------------------
  * Affine-hull reduction (host, FP64, row-major host memory), replacing the
  * Eigen::FullPivLU block of run_sampling_job (job_sampling.cu:196-235).

@@ -47,6 +47,8 @@ EXPORTED_SYMBOLS = (
     "naja_backmap",
     "naja_diag",
     "naja_diag_batched",
+    "naja_chr_timing",
+    "naja_chr_kernel_times",
     "naja_kernel_basis",
     "naja_cube_center",
     "naja_extra_from_bounds",
@@ -152,6 +154,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_diag_batched.restype = i32
     f64 = ctypes.c_double
     lib.naja_kernel_basis.argtypes = [vp, i32, i32, f64, vp, vp]
+    lib.naja_chr_timing.argtypes = [i32]
+    lib.naja_chr_timing.restype = i32
+    lib.naja_chr_kernel_times.argtypes = [vp, vp, i32, vp]
+    lib.naja_chr_kernel_times.restype = i32
     lib.naja_cube_center.argtypes = [vp, vp, i32, i32, vp, vp, vp, i32]
     lib.naja_cube_center.restype = i32
     lib.naja_extra_from_bounds.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, f64, vp, vp, vp]
@@ -572,3 +578,17 @@ def coordinate_hit_and_run_streamed(A, b, X, nspc, thinning, nchains, chunk_nspc
     if failure:
         raise failure[0]
     _check(rc)
+
+
+def chr_kernel_timing(enable: bool) -> None:
+    """Record HIP events around every CHR step-kernel launch (measurement)."""
+    _check(load_library().naja_chr_timing(1 if enable else 0))
+
+
+def chr_kernel_times(max_launches: int = 4096):
+    """[(work items, ms)] of the launches recorded since the last call, oldest first."""
+    items = (ctypes.c_int * max_launches)()
+    ms = (ctypes.c_float * max_launches)()
+    n = ctypes.c_int(0)
+    _check(load_library().naja_chr_kernel_times(items, ms, max_launches, ctypes.byref(n)))
+    return [(int(items[i]), float(ms[i])) for i in range(n.value)]

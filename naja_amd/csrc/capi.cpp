@@ -93,3 +93,18 @@ int naja_diag_batched(const float* samples, int n_variants, int dim, int n_total
 }
 
 }  // extern "C"
+
+extern "C" int naja_chr_timing(int enable)
+{
+    using namespace naja;
+    return guarded([&]() { chr_timing_enable(enable != 0); });
+}
+
+extern "C" int naja_chr_kernel_times(int* items, float* ms, int max, int* count)
+{
+    using namespace naja;
+    return guarded([&]() {
+        NAJA_REQUIRE(max >= 0 && count, NAJA_ERR_INVALID_ARGUMENT, "naja_chr_kernel_times: bad arguments");
+        *count = chr_timing_collect(items, ms, max);
+    });
+}

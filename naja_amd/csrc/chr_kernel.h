@@ -109,6 +109,11 @@ int chr_smallest_r(int W, int G, int rows);
 // reference's --tpb, taken as a preference).
 ChrShape chr_choose_shape(int m_max, int G, int tpb);
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream);
+// Launch timing for measurement: while enabled, chr_launch records HIP events
+// around every step-kernel launch; collect returns (work items, ms) of up to
+// max recorded launches, oldest first, and drops the record.
+void chr_timing_enable(bool on);
+int chr_timing_collect(int* items, float* ms, int max);
 
 // Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks.
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,

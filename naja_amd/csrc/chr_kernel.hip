@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "chr_kernel.h"
@@ -175,6 +176,49 @@ ChrShape chr_choose_shape(int m_max, int G, int tpb)
     return ChrShape{W, G, R};
 }
 
+namespace {
+
+// Launch timing (off unless enabled): a pair of HIP events on the launch's
+// stream around every CHR step-kernel launch.
+struct TimedLaunch {
+    hipEvent_t start, stop;
+    int items;
+};
+std::mutex g_timing_mu;
+bool g_timing = false;
+std::vector<TimedLaunch> g_timed;
+
+}  // namespace
+
+void chr_timing_enable(bool on)
+{
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    g_timing = on;
+}
+
+int chr_timing_collect(int* items, float* ms, int max)
+{
+    std::vector<TimedLaunch> recs;
+    {
+        std::lock_guard<std::mutex> g(g_timing_mu);
+        recs.swap(g_timed);
+    }
+    int n = 0;
+    for (TimedLaunch& r : recs) {
+        if (n < max) {
+            NAJA_HIP(hipEventSynchronize(r.stop));
+            float t = 0.0f;
+            NAJA_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+            if (items) items[n] = r.items;
+            if (ms) ms[n] = t;
+            ++n;
+        }
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    return n;
+}
+
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream)
 {
     const ChrKernelTable& t = all_kernels();
@@ -187,8 +231,24 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
     const size_t lds = sh.lds_bytes(args.d);
     NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
                  args.d, sh.G);
+    bool timed;
+    {
+        std::lock_guard<std::mutex> g(g_timing_mu);
+        timed = g_timing;
+    }
+    TimedLaunch rec{nullptr, nullptr, n_items};
+    if (timed) {
+        NAJA_HIP(hipEventCreate(&rec.start));
+        NAJA_HIP(hipEventCreate(&rec.stop));
+        NAJA_HIP(hipEventRecord(rec.start, stream));
+    }
     hipLaunchKernelGGL(fn, dim3(n_items), dim3(64 * sh.W), lds, stream, args);
     NAJA_HIP(hipGetLastError());
+    if (timed) {
+        NAJA_HIP(hipEventRecord(rec.stop, stream));
+        std::lock_guard<std::mutex> g(g_timing_mu);
+        g_timed.push_back(rec);
+    }
 }
 
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,

@@ -294,3 +294,21 @@ def test_streamed_chunks_reproduce_one_run(gpu):
     naja_amd.coordinate_hit_and_run_streamed(At, bt, X, nspc, thin, nch, chunk,
                                              lambda rows, first: parts.__setitem__(first, rows), T=T, shift=sh, **kw)
     np.testing.assert_array_equal(np.concatenate([parts[k] for k in sorted(parts)]), V)
+
+
+def test_kernel_timing_hook(gpu):
+    """naja_chr_timing / naja_chr_kernel_times: one record per CHR launch while
+    enabled (work items, positive ms), none when disabled, cleared on read."""
+    import naja_amd
+
+    t = gpu
+    A, b, _ = cube(6)
+    X0 = t.zeros((3, 6))
+    naja_amd.chr_kernel_times()
+    naja_amd.chr_kernel_timing(True)
+    naja_amd.coordinate_hit_and_run(t.from_numpy(A.astype(np.float32)), t.from_numpy(b), X0, 50, 2, 3, seed=1)
+    naja_amd.chr_kernel_timing(False)
+    naja_amd.coordinate_hit_and_run(t.from_numpy(A.astype(np.float32)), t.from_numpy(b), X0, 50, 2, 3, seed=1)
+    rec = naja_amd.chr_kernel_times()
+    assert len(rec) == 1 and rec[0][0] == 3 and rec[0][1] > 0.0
+    assert naja_amd.chr_kernel_times() == []

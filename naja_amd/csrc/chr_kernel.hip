@@ -85,14 +85,34 @@ __global__ void chr_transform_kernel(const float* Asrc, const double* bsrc, cons
     const ChrVariant V = variants[v];
     const float* A = Asrc + V.a_off;
     const float* Pv = P + (int64_t)v * d * d;
-    const int64_t total = (int64_t)V.lda * d;
+    // Each thread owns one row r and a block of kJB columns, so every load of
+    // A[k, r] feeds kJB products; the per-element order (k ascending, fmaf)
+    // is unchanged. A wavefront shares its column block (lda is a multiple of
+    // 64), so the P reads are uniform.
+    constexpr int kJB = 8;
+    const int njb = (d + kJB - 1) / kJB;
+    const int64_t total = (int64_t)V.lda * njb;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e % V.lda);
-        const int j = (int)(e / V.lda);
-        float acc = 0.0f;
-        for (int k = 0; k < d; ++k) acc = fmaf(A[(int64_t)k * V.lda + r], Pv[(int64_t)k * d + j], acc);
-        Adst[V.a_off + e] = acc;
-        Ainv[V.a_off + e] = 1.0f / acc;
+        const int j0 = (int)(e / V.lda) * kJB;
+        const int nj = min(kJB, d - j0);
+        float acc[kJB];
+#pragma unroll
+        for (int q = 0; q < kJB; ++q) acc[q] = 0.0f;
+        for (int k = 0; k < d; ++k) {
+            const float a = A[(int64_t)k * V.lda + r];
+            const float* pk = Pv + (int64_t)k * d + j0;
+#pragma unroll
+            for (int q = 0; q < kJB; ++q)
+                if (q < nj) acc[q] = fmaf(a, pk[q], acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kJB; ++q)
+            if (q < nj) {
+                const int64_t o = V.a_off + (int64_t)(j0 + q) * V.lda + r;
+                Adst[o] = acc[q];
+                Ainv[o] = 1.0f / acc[q];
+            }
     }
     if (blockIdx.x == 0) {
         const float* cv = c + (int64_t)v * d;
@@ -110,7 +130,7 @@ void chr_transform(const float* A_pack, const double* b_pack, const ChrVariant* 
                    int lda, const float* P, const float* c, float* A_out, float* Ainv_out, double* b_out,
                    hipStream_t stream)
 {
-    const int64_t per = (int64_t)lda * d;
+    const int64_t per = (int64_t)lda * ((d + 7) / 8);
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
     hipLaunchKernelGGL(chr_transform_kernel, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack, variants_dev, d,

@@ -26,6 +26,7 @@
 // B / W / var_hat formula per dim.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -37,7 +38,10 @@ namespace naja {
 
 namespace {
 
-constexpr int kDiagThreads = 1024;  // 16 waves: the one workgroup per CU hides latency
+// Threads per ESS workgroup: H / 2 (each thread owns two butterflies of a
+// radix-2^2 pass), between 64 and 1024.
+constexpr int kDiagThreads = 1024;
+inline int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H / 2)); }
 constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (128 KB)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     __syncthreads();
     if (tid < 5) {
         double sum = 0.0;
-        for (int w = 0; w < kDiagThreads / 64; ++w) sum += red[tid][w];
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += red[tid][w];
         if (tid == 0) meanf_s = (float)(sum / n);
         else if (mom && mode != 2) {
             // mom[v][2c + h][d][moment]
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     if ((tid & 63) == 0) kfirst[tid >> 6] = kneg;
     __syncthreads();
     int K = klim;
-    for (int w = 0; w < kDiagThreads / 64; ++w) K = min(K, kfirst[w]);
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) K = min(K, kfirst[w]);
     if (K == klim && klim < kend) {
         if (tid == 0) *out = -1.0f;  // truncation beyond the exact lags: redo at full size
         return;
@@ -350,7 +354,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     __syncthreads();
     if (tid == 0) {
         double sum = 0.0;
-        for (int w = 0; w < kDiagThreads / 64; ++w) sum += psum[w];
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += psum[w];
         const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
         *out = (float)n / tau;
     }
@@ -477,11 +481,11 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     // truncation point lies beyond them.
     const bool two_pass = N / 2 - spc >= spc / 8;
     if (two_pass) {
-        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)(H / 2),
+        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)), sizeof(float2) * (size_t)(H / 2),
                            stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec, half, mom);
         NAJA_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(kDiagThreads), sizeof(float2) * (size_t)H,
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), sizeof(float2) * (size_t)H,
                        stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);

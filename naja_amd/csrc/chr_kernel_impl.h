@@ -242,6 +242,9 @@ __device__ float chr_raw_load_f32(chr_i32x4 rsrc, int voffset, int soffset,
                                   int aux) __asm("llvm.amdgcn.raw.buffer.load.f32");
 __device__ chr_f32x2 chr_raw_load_v2f32(chr_i32x4 rsrc, int voffset, int soffset,
                                         int aux) __asm("llvm.amdgcn.raw.buffer.load.v2f32");
+typedef float chr_f32x4 __attribute__((ext_vector_type(4)));
+__device__ chr_f32x4 chr_raw_load_v4f32(chr_i32x4 rsrc, int voffset, int soffset,
+                                        int aux) __asm("llvm.amdgcn.raw.buffer.load.v4f32");
 __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes)
 {
     const uint64_t addr = (uint64_t)p;
@@ -275,7 +278,13 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const ChrVariant V = a.variants[item.variant];
     const int d = a.d;
     const int lda = V.lda;  // == T * R: every slot of every lane is a real or a padding row
-    const float* Av = a.A + V.a_off + tid;
+    // Row layout: lane tid holds its rows in runs of VEC consecutive rows,
+    // run q at rows q*VEC*T + tid*VEC .. + VEC-1, so one column's slots are
+    // R/VEC vector loads per lane and a wavefront's load covers VEC*T
+    // consecutive rows. Padding rows (A = 0, b = +inf) fill every run past m.
+    constexpr int VEC = (R % 4 == 0) ? 4 : ((R % 2 == 0) ? 2 : 1);
+    auto row_of = [&](int k) { return (k / VEC) * (VEC * T) + tid * VEC + (k % VEC); };
+    const float* Av = a.A + V.a_off;
     const int geff = min(G, a.nchains - item.c0);
     const uint32_t group = (uint32_t)(item.c0 / G);
     const uint32_t k0 = a.seed_lo;
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         }
     }
 
-    // Row r = k*T + tid lives in slot k of lane tid.
+    // Slot k of lane tid holds row row_of(k).
     float s[G][R];
     float ca[R], cb[R];
     // Steps t0 .. t0 + nsteps - 1 (t0 > 0 continues an earlier launch).
@@ -304,7 +313,9 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // inf * 0 or inf / 0 and never bounds the chord).
     const float* Acol0 = a.A + V.a_off;
     const float* Icol0 = a.Ainv + V.a_off;
-    const uint32_t col_bytes = 4u * (uint32_t)V.m;
+    // Whole runs in range or out of range: m rounded up to VEC rows (the rows
+    // past m in memory are padding).
+    const uint32_t col_bytes = 4u * (uint32_t)((V.m + VEC - 1) / VEC * VEC);
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
         const int mt = CO ? kMoveCoord : m.type;
         const bool real = real_move && mt != kMoveKsparse;
@@ -314,9 +325,27 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         const chr_i32x4 ri = chr_make_rsrc(pi, col_bytes);
         const chr_i32x4 rj = chr_make_rsrc(pj, col_bytes);
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            xa[k] = chr_raw_load_f32(ri, 4 * (k * T + tid), 0, 0);
-            xb[k] = chr_raw_load_f32(rj, 4 * (k * T + tid), 0, 0);
+        for (int q = 0; q < R / VEC; ++q) {
+            const int off = 4 * (q * VEC * T + tid * VEC);
+            if (VEC == 4) {
+                const chr_f32x4 u = chr_raw_load_v4f32(ri, off, 0, 0);
+                const chr_f32x4 w = chr_raw_load_v4f32(rj, off, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    xa[q * 4 + e] = u[e];
+                    xb[q * 4 + e] = w[e];
+                }
+            } else if (VEC == 2) {
+                const chr_f32x2 u = chr_raw_load_v2f32(ri, off, 0, 0);
+                const chr_f32x2 w = chr_raw_load_v2f32(rj, off, 0, 0);
+                xa[q * 2] = u[0];
+                xa[q * 2 + 1] = u[1];
+                xb[q * 2] = w[0];
+                xb[q * 2 + 1] = w[1];
+            } else {
+                xa[q] = chr_raw_load_f32(ri, off, 0, 0);
+                xb[q] = chr_raw_load_f32(rj, off, 0, 0);
+            }
         }
     };
     load_cols(mv, true, ca, cb);
@@ -368,16 +397,16 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                 for (int g = 0; g < G; ++g) xv[g] = (double)xs[g * d + c];
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
-                    const double av = (double)pc[k * T];
+                    const double av = (double)pc[row_of(k)];
 #pragma unroll
                     for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xv[g], acc[g][k]);
                 }
                 pc += lda;
             }
-            const double* bv = a.b + V.b_off + tid;
+            const double* bv = a.b + V.b_off;
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                const double bb = bv[k * T];
+                const double bb = bv[row_of(k)];
 #pragma unroll
                 for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
             }
@@ -447,7 +476,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                 const float* pq = Av + (int64_t)ks_idx[q] * lda;
                 const float cq = ks_coef[q];
 #pragma unroll
-                for (int k = 0; k < R; ++k) ae[k] = fmaf(pq[k * T], cq, ae[k]);
+                for (int k = 0; k < R; ++k) ae[k] = fmaf(pq[row_of(k)], cq, ae[k]);
             }
         }
 

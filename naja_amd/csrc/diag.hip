@@ -42,7 +42,7 @@ namespace {
 // radix-2^2 pass), between 64 and 1024.
 constexpr int kDiagThreads = 1024;
 inline int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H / 2)); }
-constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (128 KB)
+constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (136 KB with the padding)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
 {
@@ -71,6 +71,21 @@ struct Twiddles {
     }
 };
 
+// The transform's LDS array with one float2 of padding after every 16
+// (128 bytes): the power-of-two strides of the radix-2^2 passes then spread
+// over all LDS banks instead of a few.
+struct PaddedZ {
+    float2* p;
+    __device__ __forceinline__ float2& operator[](int i) const { return p[i + (i >> 4)]; }
+    // Element m of the interleaved real view (x_0, y_0, x_1, y_1, ...).
+    __device__ __forceinline__ float real(int m) const
+    {
+        const float2 v = (*this)[m >> 1];
+        return (m & 1) ? v.y : v.x;
+    }
+};
+__host__ __device__ constexpr size_t padded_lds_bytes(int H) { return sizeof(float2) * (size_t)(H + H / 16); }
+
 // (x) * (-i) and (x) * (+i).
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 __device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
@@ -79,7 +94,7 @@ __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.
 // In-place FFT over H points, decimation in frequency (natural in,
 // bit-reversed out), two radix-2 stages per LDS pass (radix 2^2): stage st
 // pairs (i, i + 2^(st-1)) with twiddle W_{2^st}^pos = W^(pos * (N >> st)).
-__device__ void fft_dif(float2* z, int H, int logH, const Twiddles& tw)
+__device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
     int st = logH;
@@ -114,7 +129,7 @@ __device__ void fft_dif(float2* z, int H, int logH, const Twiddles& tw)
 
 // Unnormalised inverse, decimation in time with conjugate twiddles
 // (bit-reversed in, natural out), stages 1..logH two per LDS pass.
-__device__ void ifft_dit(float2* z, int H, int logH, const Twiddles& tw)
+__device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
     int st = 1;
@@ -197,7 +212,8 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
                                                                   float* __restrict__ ess_chain, int half,
                                                                   double* __restrict__ mom)
 {
-    extern __shared__ float2 z[];
+    extern __shared__ float2 z_lds[];
+    const PaddedZ z{z_lds};
     __shared__ float2 tlo_s[128], thi_s[128];
     __shared__ double red[5][kDiagThreads / 64];
     __shared__ float meanf_s;
@@ -281,10 +297,14 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     // half-size inverse: A_k = P_k + P_{H-k}, B_k = (P_k - P_{H-k}) e^{+2 pi i k/N},
     // Z'_k = A_k + i B_k, written back to z[bitrev(k)] (the slot it came
     // from, which is where the DIT inverse wants it). Each thread owns the
-    // pair {k, H-k}, so no other thread touches those two slots.
-    for (int k = tid; k <= H / 2; k += blockDim.x) {
+    // pair {k, H-k}, so no other thread touches those two slots. Threads
+    // walk the slots rather than k, so that the z[pk] accesses of a
+    // wavefront are adjacent: the even slots hold every k < H/2 (the top bit
+    // of k is the low bit of its slot) and slot 1 holds k = H/2.
+    for (int i = tid; i <= H / 2; i += blockDim.x) {
+        const int pk = (i < H / 2) ? 2 * i : 1;
+        const int k = bitrev(pk, logH);
         const int kb = (H - k) & (H - 1);
-        const int pk = bitrev(k, logH);
         const int pb = bitrev(kb, logH);
         const float2 zk = z[pk];
         const float2 zb = z[pb];
@@ -319,9 +339,8 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
     __shared__ int kfirst[kDiagThreads / 64];
     __shared__ double psum[kDiagThreads / 64];
-    const float* raw = reinterpret_cast<const float*>(z);
     const float norm = 1.0f / ((float)N * (float)n);
-    const float var = raw[0] * norm;
+    const float var = z.real(0) * norm;
     const int kend = n / 2;  // pairs k = 1 .. kend-1
     if (var < 1e-30f) {
         if (tid == 0) *out = 1.0f;
@@ -331,7 +350,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     const int klim = (mode == 1) ? min(kend, (N - n) / 2 + 1) : kend;
     int kneg = klim;
     for (int k = 1 + tid; k < klim; k += blockDim.x) {
-        const float pair_sum = (raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var;
+        const float pair_sum = (z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var;
         if (pair_sum < 0.0f) {
             kneg = k;
             break;
@@ -348,7 +367,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     }
     double acc = 0.0;
     for (int k = 1 + tid; k < K; k += blockDim.x)
-        acc += (double)((raw[2 * k - 1] * norm) / var + (raw[2 * k] * norm) / var);
+        acc += (double)((z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((tid & 63) == 0) psum[tid >> 6] = acc;
     __syncthreads();
@@ -481,11 +500,11 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     // truncation point lies beyond them.
     const bool two_pass = N / 2 - spc >= spc / 8;
     if (two_pass) {
-        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)), sizeof(float2) * (size_t)(H / 2),
+        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)), padded_lds_bytes(H / 2),
                            stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec, half, mom);
         NAJA_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), sizeof(float2) * (size_t)H,
+    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), padded_lds_bytes(H),
                        stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);

@@ -16,7 +16,7 @@ def gpu_diag(t, S, nch, layout=0):
     return e.cpu().numpy(), r.cpu().numpy(), ec.cpu().numpy()
 
 
-@pytest.mark.parametrize("spc,nch,dim,phi", [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99),
+@pytest.mark.parametrize("spc,nch,dim,phi", [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99), (2000, 4, 8, 0.6),
                                              (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7)])
 def test_diag_matches_oracle(gpu, spc, nch, dim, phi):
     rng = np.random.default_rng(spc + nch)

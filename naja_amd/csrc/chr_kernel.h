@@ -107,7 +107,7 @@ int chr_smallest_r(int W, int G, int rows);
 
 // Shape for a batch whose largest variant has m_max rows (tpb > 0 is the
 // reference's --tpb, taken as a preference).
-ChrShape chr_choose_shape(int m_max, int G, int tpb);
+ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items);
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to

@@ -167,7 +167,7 @@ int chr_smallest_r(int W, int G, int rows)
     return best;
 }
 
-ChrShape chr_choose_shape(int m_max, int G, int tpb)
+ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items)
 {
     // A step costs (rows per lane x ~15 VALU + ~100 VALU of per-wave chord,
     // alpha and bookkeeping work) x wavefronts per SIMD. One wavefront needs
@@ -176,6 +176,10 @@ ChrShape chr_choose_shape(int m_max, int G, int tpb)
     // per-wave work best; eight when the rows would not fit in VGPRs.
     const int rows = (m_max + 63) / 64 * 64;
     int W = rows <= 64 * 8 ? 1 : (rows <= 256 * 32 ? 4 : 8);
+    // A launch of few chain groups leaves most SIMDs idle: eight wavefronts
+    // per group then shorten each step without costing parallelism (the
+    // 4-chain genome-scale run: 1.21 -> 1.10 us per step).
+    if (W == 4 && n_items > 0 && n_items * 8 <= 1024 && chr_smallest_r(8, G, rows) > 0) W = 8;
     if (tpb > 0) {
         // The reference's --tpb (threads per chain) is a preference here.
         const int w = std::max(1, tpb / 64);

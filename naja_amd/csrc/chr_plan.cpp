@@ -278,7 +278,7 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_v
                                 hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(pl->ek_d, pl->ek.data(), sizeof(int) * pl->ek.size(), hipMemcpyHostToDevice, s));
         if (P) alloc_async(&pl->zbuf, (size_t)n_variants * p->nchains * p->nspc * d, s);
-        ChrShape sh = chr_choose_shape(pl->m_max, pl->G, p->tpb);
+        ChrShape sh = chr_choose_shape(pl->m_max, pl->G, p->tpb, (int)pl->items.size());
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tuned.find(tune_key(*pl));

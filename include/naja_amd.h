@@ -326,6 +326,19 @@ int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int
 int naja_precond_lowrank(const double* V, int n_variants, int d, int rank, int expand, double* P,
                          int nthreads);
 
+/*
+ * The learned map P_v = P0_v (I + V_v V_v^T)^{1/2} (expand = 1) or
+ * P0_v (I + V_v V_v^T)^{-1/2} (expand = 0) on the GPU, FP64 inside: the
+ * r x r matrix function of V^T V by a Newton-Schulz square root, then
+ * P = P0 + (P0 V) M V^T. Device memory: P0 and P n_variants x d x d, V
+ * n_variants x d x rank, all FP32 row-major; P must not alias P0.
+ * rank <= 32 and d <= 160, else NAJA_ERR_UNSUPPORTED. Synchronises the
+ * stream (per-variant convergence is checked: NAJA_ERR_RUNTIME).
+ * No reference counterpart.
+ */
+int naja_lowrank_maps_device(const float* P0, const float* V, int n_variants, int d, int rank, int expand,
+                             float* P, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Learned preconditioner (no reference counterpart: the reference's
  * preconditioners are the Dikin rounding above and the Jacobi pair schedule,

@@ -59,6 +59,7 @@ EXPORTED_SYMBOLS = (
     "naja_precond_ellipsoid_batched",
     "naja_precond_lowrank",
     "naja_precond_ellipsoid_device",
+    "naja_lowrank_maps_device",
     "naja_lp_create",
     "naja_lp_destroy",
     "naja_lp_param_count",
@@ -178,6 +179,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_lowrank.restype = i32
     lib.naja_precond_ellipsoid_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]
     lib.naja_precond_ellipsoid_device.restype = i32
+    lib.naja_lowrank_maps_device.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
+    lib.naja_lowrank_maps_device.restype = i32
     f32 = ctypes.c_float
     lib.naja_lp_create.argtypes = [i32, i32, i32, i32, i32, i32, ctypes.c_uint64, f32, ctypes.POINTER(vp)]
     lib.naja_lp_destroy.argtypes = [vp]
@@ -535,6 +538,26 @@ def ellipsoid_rounding(A, b, extra_A, extra_b, x_c=None, stream=None):
                                              be.data_ptr() if be is not None else None,
                                              xc.data_ptr() if xc is not None else None, P.data_ptr(),
                                              _stream_ptr(stream)))
+    return P
+
+
+def lowrank_maps(P0, V, expand=True, stream=None):
+    """P_v = P0_v (I + V_v V_v^T)^{+1/2} (expand) or ^{-1/2} on the GPU, FP64
+    inside: P0 (n_variants, d, d) and V (n_variants, d, rank) FP32 CUDA
+    tensors; returns P (n_variants, d, d) FP32 (see naja_amd.precond.lowrank
+    for the FP64 host routine of the update alone)."""
+    torch = _torch()
+    lib = load_library()
+    for name, t in (("P0", P0), ("V", V)):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.dim() == 3):
+            raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"{name} must be a 3-d float32 CUDA tensor")
+    nv, d, r = V.shape
+    if tuple(P0.shape) != (nv, d, d):
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"P0 {tuple(P0.shape)} does not match V {tuple(V.shape)}")
+    P0c, Vc = P0.contiguous(), V.contiguous()
+    P = torch.empty_like(P0c)
+    _check(lib.naja_lowrank_maps_device(P0c.data_ptr(), Vc.data_ptr(), nv, d, r, int(bool(expand)), P.data_ptr(),
+                                        _stream_ptr(stream)))
     return P
 
 

@@ -35,7 +35,7 @@ import time
 import numpy as np
 
 from . import (NAJA_ERR_UNSUPPORTED, NajaError, VariantBatch, coordinate_hit_and_run_batched, diagnostics_batched,
-               ellipsoid_rounding)
+               ellipsoid_rounding, lowrank_maps)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
@@ -171,13 +171,13 @@ def train(lp: LearnedPreconditioner, F, S, epochs=150, batch=256, lr=1e-3, seed=
 
 def learned_maps(lp: LearnedPreconditioner, F, P0, expand=True):
     """P_v = P0_v (I + V_v V_v^T)^{1/2} (expand) or P0_v (I + V_v V_v^T)^{-1/2}:
-    the learned low-rank update of the base (P0 a device tensor). The r x r
-    eigenproblems run on the host in FP64; the product on the device."""
+    the learned low-rank update of the base (P0 a device tensor). MLP
+    inference and the FP64 matrix function plus product (naja_amd.lowrank_maps)
+    both on the device."""
     import torch
 
     V = lp.forward(torch.as_tensor(F, dtype=torch.float32).cuda())
-    L = torch.as_tensor(precond.lowrank(V.double().cpu().numpy(), expand=expand), dtype=torch.float32).cuda()
-    return torch.bmm(P0, L)
+    return lowrank_maps(P0, V, expand=expand)
 
 
 def gate(ess_l, rhat_l, ess_b, rhat_b):

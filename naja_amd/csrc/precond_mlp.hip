@@ -807,3 +807,230 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
                          v);
     });
 }
+
+// ---------------------------------------------------------------------------
+// Learned map on the GPU: P_v = P0_v (I + V_v V_v^T)^{p}, p = +1/2 (expand)
+// or -1/2, FP64 inside, one workgroup per variant. With G = V^T V (r x r),
+// (I + V V^T)^{p} = I + V M V^T, M = f(G), f(l) = ((1 + l)^p - 1) / l on
+// G's eigenvalues, i.e. with S = (I + G)^{1/2}:
+//   expand:  M = (I + S)^{-1};   shrink:  M = -S^{-1} (I + S)^{-1}.
+// S and S^{-1} come from the coupled Newton-Schulz iteration (Higham,
+// Functions of Matrices, 2008, eq. 6.35) on A = (I + G) / c with
+// c = ||I + G||_F, so A's eigenvalues lie in (0, 1]:
+//   T = (3 I - Z Y) / 2,  Y <- Y T,  Z <- T Z;   Y -> A^{1/2}, Z -> A^{-1/2},
+// stopped once ||I - Z Y||_F <= 1e-12 or its quadratic decrease stalls at
+// rounding level (a variant that meets neither within kNsMaxIter iterations
+// reports NAJA_ERR_RUNTIME). (I + S)^{-1} by
+// Gauss-Jordan without pivoting (I + S is SPD with eigenvalues >= 2). Then
+// Q = M V^T (r x d), W = P0 V (d x r) and P = P0 + W Q, written as FP32.
+// The host routine naja_precond_lowrank forms the same matrix function from
+// an eigendecomposition of G.
+// ---------------------------------------------------------------------------
+namespace naja {
+namespace mlp {
+
+constexpr int kLrThreads = 256;
+constexpr int kNsMaxIter = 100;
+
+__device__ double lr_block_sum(double x, double* red)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < kLrThreads / 64; ++w) s += red[w];
+    return s;
+}
+
+// C = A B for r x r row-major LDS matrices; entries e = tid + 256 q go to out[q].
+__device__ __forceinline__ void lr_matmul(const double* A, const double* B, int r, double out[4])
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = threadIdx.x + kLrThreads * q;
+        double s = 0.0;
+        if (e < r * r) {
+            const int i = e / r, j = e % r;
+            for (int t = 0; t < r; ++t) s = fma(A[i * r + t], B[t * r + j], s);
+        }
+        out[q] = s;
+    }
+}
+
+__global__ __launch_bounds__(kLrThreads) void lowrank_map_kernel(const float* __restrict__ P0,
+                                                                 const float* __restrict__ V, int d, int r, int expand,
+                                                                 float* __restrict__ P, int* __restrict__ status)
+{
+    extern __shared__ double lr_sh[];
+    __shared__ double red[kLrThreads / 64];
+    const int v = blockIdx.x, tid = threadIdx.x;
+    const int rr = r * r;
+    double* Vs = lr_sh;            // d x r
+    double* M = Vs + d * r;        // r x r
+    double* R2 = M + rr;           // max(5 r^2, 2 d r)
+    double* Y = R2;
+    double* Z = Y + rr;
+    double* T = Z + rr;
+    double* B = T + rr;            // I + S, then reduced to I
+    double* Bi = B + rr;           // (I + S)^{-1}
+    const float* Vv = V + (int64_t)v * d * r;
+    const float* P0v = P0 + (int64_t)v * d * d;
+
+    for (int e = tid; e < d * r; e += kLrThreads) Vs[e] = (double)Vv[e];
+    __syncthreads();
+    // A = I + G and its Frobenius norm.
+    double fro = 0.0;
+    for (int e = tid; e < rr; e += kLrThreads) {
+        const int i = e / r, j = e % r;
+        double s = (i == j) ? 1.0 : 0.0;
+        for (int k = 0; k < d; ++k) s = fma(Vs[k * r + i], Vs[k * r + j], s);
+        T[e] = s;
+        fro = fma(s, s, fro);
+    }
+    const double c = sqrt(lr_block_sum(fro, red));
+    for (int e = tid; e < rr; e += kLrThreads) {
+        Y[e] = T[e] / c;
+        Z[e] = (e / r == e % r) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    bool converged = false;
+    double prev = 1.0;
+    for (int it = 0; it < kNsMaxIter; ++it) {
+        double zy[4];
+        lr_matmul(Z, Y, r, zy);
+        double err = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + kLrThreads * q;
+            if (e < rr) {
+                const double dlt = ((e / r == e % r) ? 1.0 : 0.0) - zy[q];
+                err = fma(dlt, dlt, err);
+                T[e] = 0.5 * (((e / r == e % r) ? 3.0 : 0.0) - zy[q]);
+            }
+        }
+        // The sum ends with a barrier: T is visible to every thread. Stop at
+        // 1e-12, or once the quadratic convergence stalls at rounding level
+        // (the residual no longer halves after falling below 1e-6).
+        const double res = sqrt(lr_block_sum(err, red));
+        if (!(res == res) || res > 1e300) break;
+        if (res <= 1e-12 || (prev < 1e-6 && res > 0.5 * prev)) {
+            converged = true;
+            break;
+        }
+        prev = res;
+        double yn[4], zn[4];
+        lr_matmul(Y, T, r, yn);
+        lr_matmul(T, Z, r, zn);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + kLrThreads * q;
+            if (e < rr) {
+                Y[e] = yn[q];
+                Z[e] = zn[q];
+            }
+        }
+        __syncthreads();
+    }
+    if (!converged) {
+        if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+        return;  // uniform across the workgroup
+    }
+    // S = sqrt(c) Y, S^{-1} = Z / sqrt(c). Gauss-Jordan on [I + S | I].
+    const double sc = sqrt(c);
+    for (int e = tid; e < rr; e += kLrThreads) {
+        const bool diag = (e / r == e % r);
+        B[e] = (diag ? 1.0 : 0.0) + sc * Y[e];
+        Bi[e] = diag ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    __shared__ double rowp[64];  // pivot row of B / pivot, then of Bi / pivot
+    __shared__ double fcol[32];
+    for (int p = 0; p < r; ++p) {
+        const double piv = B[p * r + p];
+        if (tid < r) {
+            rowp[tid] = B[p * r + tid] / piv;
+            rowp[r + tid] = Bi[p * r + tid] / piv;
+            fcol[tid] = B[tid * r + p];
+        }
+        __syncthreads();
+        for (int e = tid; e < rr; e += kLrThreads) {
+            const int i = e / r, j = e % r;
+            if (i == p) {
+                B[e] = rowp[j];
+                Bi[e] = rowp[r + j];
+            } else {
+                B[e] = fma(-fcol[i], rowp[j], B[e]);
+                Bi[e] = fma(-fcol[i], rowp[r + j], Bi[e]);
+            }
+        }
+        __syncthreads();
+    }
+    if (expand) {
+        for (int e = tid; e < rr; e += kLrThreads) M[e] = Bi[e];
+    } else {
+        double m[4];
+        lr_matmul(Z, Bi, r, m);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + kLrThreads * q;
+            if (e < rr) M[e] = -m[q] / sc;
+        }
+    }
+    __syncthreads();
+    double* Q = R2;          // r x d
+    double* W = R2 + r * d;  // d x r
+    for (int e = tid; e < r * d; e += kLrThreads) {
+        const int t = e / d, b = e % d;
+        double s = 0.0;
+        for (int u = 0; u < r; ++u) s = fma(M[t * r + u], Vs[b * r + u], s);
+        Q[e] = s;
+    }
+    for (int e = tid; e < d * r; e += kLrThreads) {
+        const int a = e / r, t = e % r;
+        double s = 0.0;
+        for (int k = 0; k < d; ++k) s = fma((double)P0v[(int64_t)a * d + k], Vs[k * r + t], s);
+        W[e] = s;
+    }
+    __syncthreads();
+    float* Pv = P + (int64_t)v * d * d;
+    for (int e = tid; e < d * d; e += kLrThreads) {
+        const int a = e / d, b = e % d;
+        double s = (double)P0v[e];
+        for (int t = 0; t < r; ++t) s = fma(W[a * r + t], Q[t * d + b], s);
+        Pv[e] = (float)s;
+    }
+    if (tid == 0) status[v] = NAJA_OK;
+}
+
+}  // namespace mlp
+}  // namespace naja
+
+extern "C" int naja_lowrank_maps_device(const float* P0, const float* V, int n_variants, int d, int rank, int expand,
+                                        float* P, void* stream)
+{
+    return guarded([&]() {
+        NAJA_REQUIRE(P0 && V && P && n_variants > 0 && rank > 0 && rank <= d, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_lowrank_maps_device: bad arguments");
+        NAJA_REQUIRE(rank <= 32 && d <= 160, NAJA_ERR_UNSUPPORTED,
+                     "naja_lowrank_maps_device: rank %d, d %d (rank <= 32, d <= 160)", rank, d);
+        NAJA_REQUIRE(P != P0, NAJA_ERR_INVALID_ARGUMENT, "naja_lowrank_maps_device: P must not alias P0");
+        const size_t rr = (size_t)rank * rank;
+        const size_t lds = sizeof(double) * ((size_t)d * rank + rr + std::max(5 * rr, 2 * (size_t)d * rank));
+        hipStream_t s = (hipStream_t)stream;
+        int* status = nullptr;
+        NAJA_HIP(hipMallocAsync((void**)&status, sizeof(int) * n_variants, s));
+        hipLaunchKernelGGL(mlp::lowrank_map_kernel, dim3(n_variants), dim3(mlp::kLrThreads), lds, s, P0, V, d, rank,
+                           expand, P, status);
+        NAJA_HIP(hipGetLastError());
+        std::vector<int> st(n_variants);
+        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
+        NAJA_HIP(hipStreamSynchronize(s));
+        (void)hipFreeAsync(status, s);
+        for (int v = 0; v < n_variants; ++v)
+            NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_RUNTIME,
+                         "low-rank map of variant %d: the Newton-Schulz square root did not converge", v);
+    });
+}

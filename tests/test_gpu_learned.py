@@ -159,3 +159,47 @@ def test_invalid_arguments(gpu):
         LearnedPreconditioner(rank=16)
     with pytest.raises(NajaError, match="dim"):
         LearnedPreconditioner(dim=400)
+
+
+@pytest.mark.parametrize("expand", [True, False])
+@pytest.mark.parametrize("d,r,scale", [(100, 32, 0.3), (100, 32, 3.0), (50, 32, 30.0), (20, 4, 1.0), (160, 32, 1.0),
+                                       (8, 8, 1e-3), (3, 1, 0.0)])
+def test_lowrank_maps_device_matches_oracle(gpu, d, r, scale, expand):
+    """P0 (I + V V^T)^{+-1/2} on the GPU (Newton-Schulz, FP64 inside) against
+    P0 times the oracle's full d x d eigen-decomposition, including a
+    rank-deficient factor and V = 0 (P = P0)."""
+    import naja_amd
+    from oracle import precond as orp
+
+    rng = np.random.default_rng(d * 31 + r)
+    nv = 4
+    V = (rng.standard_normal((nv, d, r)) * scale).astype(np.float32)
+    V[1, :, r // 2:] = 0.0
+    P0 = rng.standard_normal((nv, d, d)).astype(np.float32)
+    P = naja_amd.lowrank_maps(gpu.from_numpy(P0).cuda(), gpu.from_numpy(V).cuda(), expand=expand).cpu().numpy()
+    for v in range(nv):
+        ref = P0[v].astype(np.float64) @ orp.lowrank(V[v].astype(np.float64), expand)
+        tol = 2e-6 * np.abs(P0[v]).astype(np.float64) @ np.abs(orp.lowrank(V[v].astype(np.float64), expand)) + 1e-30
+        err = np.abs(P[v] - ref)
+        assert np.all(err <= tol), f"variant {v}: max err {err.max():.3e}, worst ratio {(err / tol).max():.2f}"
+
+
+def test_lowrank_maps_device_two_calls_and_errors(gpu):
+    """The map agrees with the host FP64 routine on consecutive calls of
+    different sizes; bad shapes are rejected before any launch."""
+    import naja_amd
+    from naja_amd import precond
+
+    rng = np.random.default_rng(5)
+    for nv, d in ((256, 100), (3, 40)):
+        V = (rng.standard_normal((nv, d, 32)) * 0.5).astype(np.float32)
+        P0 = np.broadcast_to(np.eye(d, dtype=np.float32), (nv, d, d)).copy()
+        P = naja_amd.lowrank_maps(gpu.from_numpy(P0).cuda(), gpu.from_numpy(V).cuda()).cpu().numpy()
+        ref = precond.lowrank(V.astype(np.float64), expand=True)
+        np.testing.assert_allclose(P, ref, atol=2e-6 * np.abs(ref).max())
+    with pytest.raises(naja_amd.NajaError):
+        naja_amd.lowrank_maps(gpu.zeros((2, 5, 5)).cuda(), gpu.zeros((2, 5, 33)).cuda())
+    with pytest.raises(naja_amd.NajaError):
+        naja_amd.lowrank_maps(gpu.zeros((2, 200, 200)).cuda(), gpu.zeros((2, 200, 32)).cuda())
+    with pytest.raises(naja_amd.NajaError):
+        naja_amd.lowrank_maps(gpu.zeros((2, 6, 6)).cuda(), gpu.zeros((3, 6, 2)).cuda())

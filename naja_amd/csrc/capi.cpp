@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/naja_amd.h"
@@ -37,6 +38,43 @@ void set_error(const char* fmt, ...)
 }
 
 void clear_error() { g_error[0] = 0; }
+
+namespace {
+std::mutex g_pool_mu;
+std::vector<hipMemPool_t> g_pools;  // per device, created on first use
+}  // namespace
+
+static hipMemPool_t scratch_pool()
+{
+    int dev = 0;
+    NAJA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if ((int)g_pools.size() <= dev) g_pools.resize((size_t)dev + 1, nullptr);
+    if (!g_pools[(size_t)dev]) {
+        hipMemPoolProps props;
+        std::memset(&props, 0, sizeof(props));
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        hipMemPool_t pool = nullptr;
+        NAJA_HIP(hipMemPoolCreate(&pool, &props));
+        uint64_t keep = kScratchKeepBytes;
+        NAJA_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+        g_pools[(size_t)dev] = pool;
+    }
+    return g_pools[(size_t)dev];
+}
+
+void scratch_alloc(void** ptr, size_t bytes, hipStream_t s)
+{
+    NAJA_HIP(hipMallocFromPoolAsync(ptr, bytes ? bytes : 1, scratch_pool(), s));
+}
+
+void scratch_free(void* ptr, hipStream_t s)
+{
+    if (ptr) NAJA_HIP(hipFreeAsync(ptr, s));
+}
 
 }  // namespace naja
 

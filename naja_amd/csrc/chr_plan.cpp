@@ -69,15 +69,12 @@ TuneKey tune_key(const naja_chr_plan& pl)
     return TuneKey{(pl.m_max + 63) / 64 * 64, pl.d, pl.G, (int)pl.items.size(), pl.p.tpb > 0 ? pl.p.tpb : 0};
 }
 
-void free_async(void* ptr, hipStream_t s)
-{
-    if (ptr) NAJA_HIP(hipFreeAsync(ptr, s));
-}
+void free_async(void* ptr, hipStream_t s) { scratch_free(ptr, s); }
 
 template <typename T>
 void alloc_async(T** ptr, size_t count, hipStream_t s)
 {
-    NAJA_HIP(hipMallocAsync((void**)ptr, sizeof(T) * (count ? count : 1), s));
+    scratch_alloc(ptr, count, s);
 }
 
 void check_params(const naja_chr_params* p, int d)

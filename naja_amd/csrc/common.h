@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <algorithm>
 #include <string>
@@ -59,6 +60,20 @@ int guarded(F&& body)
         set_error("unknown C++ exception");
         return NAJA_ERR_RUNTIME;
     }
+}
+
+// Stream-ordered device scratch from a library-owned pool (one per device)
+// that keeps up to kScratchKeepBytes of freed memory mapped between calls.
+// The default pool returns freed memory at every synchronisation, so a call
+// that needs a multi-GB buffer would map it again each time (about a
+// millisecond per 5 GB). scratch_free(nullptr, s) is a no-op.
+constexpr uint64_t kScratchKeepBytes = 16ull << 30;
+void scratch_alloc(void** ptr, size_t bytes, hipStream_t s);
+void scratch_free(void* ptr, hipStream_t s);
+template <typename T>
+void scratch_alloc(T** ptr, size_t count, hipStream_t s)
+{
+    scratch_alloc(reinterpret_cast<void**>(ptr), sizeof(T) * (count ? count : 1), s);
 }
 
 // Runs body(v) for v in [0, n) over up to nthreads host threads.

@@ -483,15 +483,15 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     float* ec = ess_chain;
     const int half = spc / 2;
     const int64_t n_series = (int64_t)dim * nchains * nv;
-    NAJA_HIP(hipMallocAsync((void**)&ser, sizeof(float) * (size_t)n_series * spc, stream));
-    NAJA_HIP(hipMallocAsync((void**)&tw, sizeof(float2) * tw_h.size(), stream));
-    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
-    if (!ec) NAJA_HIP(hipMallocAsync((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream));
-    NAJA_HIP(hipMallocAsync((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream));
-
     const int s_tiles = (spc + 63) / 64;
     NAJA_REQUIRE((int64_t)s_tiles * nchains * nv < (int64_t)1 << 31 && n_series < (int64_t)1 << 31,
                  NAJA_ERR_UNSUPPORTED, "too many series for one diagnostics launch");
+    scratch_alloc((void**)&ser, sizeof(float) * (size_t)n_series * spc, stream);
+    scratch_alloc((void**)&tw, sizeof(float2) * tw_h.size(), stream);
+    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
+    if (!ec) scratch_alloc((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream);
+    scratch_alloc((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream);
+
     hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)(s_tiles * nchains * nv), (unsigned)((dim + 63) / 64)),
                        dim3(256), 0, stream, samples, dim, nchains, spc, layout, vstride, s_tiles, ser);
     NAJA_HIP(hipGetLastError());
@@ -517,10 +517,10 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
         NAJA_HIP(hipGetLastError());
     }
 
-    NAJA_HIP(hipFreeAsync(ser, stream));
-    NAJA_HIP(hipFreeAsync(tw, stream));
-    NAJA_HIP(hipFreeAsync(mom, stream));
-    if (ec != ess_chain) NAJA_HIP(hipFreeAsync(ec, stream));
+    scratch_free(ser, stream);
+    scratch_free(tw, stream);
+    scratch_free(mom, stream);
+    if (ec != ess_chain) scratch_free(ec, stream);
 }
 
 }  // namespace naja

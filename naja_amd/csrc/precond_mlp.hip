@@ -787,9 +787,9 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         int64_t* eoff = nullptr;
         int* ek = nullptr;
         int* status = nullptr;
-        NAJA_HIP(hipMallocAsync((void**)&eoff, sizeof(int64_t) * n_variants, s));
-        NAJA_HIP(hipMallocAsync((void**)&ek, sizeof(int) * n_variants, s));
-        NAJA_HIP(hipMallocAsync((void**)&status, sizeof(int) * n_variants, s));
+        scratch_alloc((void**)&eoff, sizeof(int64_t) * n_variants, s);
+        scratch_alloc((void**)&ek, sizeof(int) * n_variants, s);
+        scratch_alloc((void**)&status, sizeof(int) * n_variants, s);
         NAJA_HIP(hipMemcpyAsync(eoff, off.data(), sizeof(int64_t) * n_variants, hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(ek, extra_rows, sizeof(int) * n_variants, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(mlp::ellipsoid_kernel, dim3(n_variants), dim3(256), lds, s, A0, b0, m0, d, extra_A, extra_b,
@@ -798,9 +798,9 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
-        (void)hipFreeAsync(eoff, s);
-        (void)hipFreeAsync(ek, s);
-        (void)hipFreeAsync(status, s);
+        scratch_free(eoff, s);
+        scratch_free(ek, s);
+        scratch_free(status, s);
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_INVALID_ARGUMENT,
                          "Dikin ellipsoid of variant %d: the centre is not interior or the barrier Hessian is singular",
@@ -1021,14 +1021,14 @@ extern "C" int naja_lowrank_maps_device(const float* P0, const float* V, int n_v
         const size_t lds = sizeof(double) * ((size_t)d * rank + rr + std::max(5 * rr, 2 * (size_t)d * rank));
         hipStream_t s = (hipStream_t)stream;
         int* status = nullptr;
-        NAJA_HIP(hipMallocAsync((void**)&status, sizeof(int) * n_variants, s));
+        scratch_alloc((void**)&status, sizeof(int) * n_variants, s);
         hipLaunchKernelGGL(mlp::lowrank_map_kernel, dim3(n_variants), dim3(mlp::kLrThreads), lds, s, P0, V, d, rank,
                            expand, P, status);
         NAJA_HIP(hipGetLastError());
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
-        (void)hipFreeAsync(status, s);
+        scratch_free(status, s);
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_RUNTIME,
                          "low-rank map of variant %d: the Newton-Schulz square root did not converge", v);

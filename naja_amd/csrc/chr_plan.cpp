@@ -77,6 +77,22 @@ void alloc_async(T** ptr, size_t count, hipStream_t s)
     scratch_alloc(ptr, count, s);
 }
 
+// Workgroup b of a launch is dispatched to XCD b % 8 (round robin), and each
+// XCD has its own L2. Workgroup b therefore runs logical item
+// (b % 8) * per + b / 8, per = n / 8: every XCD gets one contiguous range of
+// items, so the chains of one variant share an L2 and each L2 holds the
+// columns of an eighth of the variants instead of reading from all of them.
+// Items past 8 * per keep their place. Results do not depend on the order:
+// an item's samples depend only on its variant and chains.
+std::vector<ChrItem> xcd_order(const std::vector<ChrItem>& logical)
+{
+    constexpr int kXcds = 8;
+    const int n = (int)logical.size(), per = n / kXcds;
+    std::vector<ChrItem> out(logical);
+    for (int b = 0; b < per * kXcds; ++b) out[(size_t)b] = logical[(size_t)((b % kXcds) * per + b / kXcds)];
+    return out;
+}
+
 void check_params(const naja_chr_params* p, int d)
 {
     NAJA_REQUIRE(p != nullptr, NAJA_ERR_INVALID_ARGUMENT, "params is NULL");
@@ -262,8 +278,10 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_v
             e_off += k;
             pl->m_max = std::max(pl->m_max, m0 + k);
         }
+        std::vector<ChrItem> logical;
         for (int v = 0; v < n_variants; ++v)
-            for (int c0 = 0; c0 < p->nchains; c0 += pl->G) pl->items.push_back(ChrItem{v, c0});
+            for (int c0 = 0; c0 < p->nchains; c0 += pl->G) logical.push_back(ChrItem{v, c0});
+        pl->items = xcd_order(logical);
         const hipStream_t s = pl->stream;
         alloc_async(&pl->var_d, pl->var.size(), s);
         alloc_async(&pl->items_d, pl->items.size(), s);

@@ -78,8 +78,10 @@ __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, 
 // y = c_v + P_v z, so its rows become A' = A P_v (FP32, fmaf over k in
 // order) and b' = b - A c_v (FP64). Padding rows stay A' = 0, b' = +inf.
 // Also writes the IEEE reciprocal of A'.
-__global__ void chr_transform_kernel(const float* Asrc, const double* bsrc, const ChrVariant* variants, int d,
-                                     const float* P, const float* c, float* Adst, float* Ainv, double* bdst)
+__global__ void chr_transform_kernel(const float* __restrict__ Asrc, const double* __restrict__ bsrc,
+                                     const ChrVariant* __restrict__ variants, int d, const float* __restrict__ P,
+                                     const float* __restrict__ c, float* __restrict__ Adst,
+                                     float* __restrict__ Ainv, double* __restrict__ bdst)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
@@ -88,23 +90,34 @@ __global__ void chr_transform_kernel(const float* Asrc, const double* bsrc, cons
     // Each thread owns one row r and a block of kJB columns, so every load of
     // A[k, r] feeds kJB products; the per-element order (k ascending, fmaf)
     // is unchanged. A wavefront shares its column block (lda is a multiple of
-    // 64), so the P reads are uniform.
+    // 64), so the P reads are uniform: readfirstlane states it, and they
+    // become scalar loads.
     constexpr int kJB = 8;
     const int njb = (d + kJB - 1) / kJB;
     const int64_t total = (int64_t)V.lda * njb;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e % V.lda);
-        const int j0 = (int)(e / V.lda) * kJB;
+        const int j0 = __builtin_amdgcn_readfirstlane((int)(e / V.lda) * kJB);
         const int nj = min(kJB, d - j0);
         float acc[kJB];
 #pragma unroll
         for (int q = 0; q < kJB; ++q) acc[q] = 0.0f;
-        for (int k = 0; k < d; ++k) {
-            const float a = A[(int64_t)k * V.lda + r];
-            const float* pk = Pv + (int64_t)k * d + j0;
+        if (nj == kJB) {
+#pragma unroll 4
+            for (int k = 0; k < d; ++k) {
+                const float a = A[(int64_t)k * V.lda + r];
+                const float* pk = Pv + (int64_t)k * d + j0;
 #pragma unroll
-            for (int q = 0; q < kJB; ++q)
-                if (q < nj) acc[q] = fmaf(a, pk[q], acc[q]);
+                for (int q = 0; q < kJB; ++q) acc[q] = fmaf(a, pk[q], acc[q]);
+            }
+        } else {
+            for (int k = 0; k < d; ++k) {
+                const float a = A[(int64_t)k * V.lda + r];
+                const float* pk = Pv + (int64_t)k * d + j0;
+#pragma unroll
+                for (int q = 0; q < kJB; ++q)
+                    if (q < nj) acc[q] = fmaf(a, pk[q], acc[q]);
+            }
         }
 #pragma unroll
         for (int q = 0; q < kJB; ++q)

@@ -660,13 +660,26 @@ extern "C" int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, con
 
 // ---------------------------------------------------------------------------
 // Dikin-ellipsoid rounding on the GPU (FP64): one workgroup per variant of a
-// ragged batch [A0; E_v] y <= [b0; f_v]. H = sum_r a_r a_r^T / s_r^2 at x_c
-// (lower triangle in LDS), H = L L^T in place, M = L^{-1} column by column
+// ragged batch [A0; E_v] y <= [b0; f_v]. H = sum_r u_r u_r^T, u_r = a_r / s_r
+// at x_c: rows staged kEllRows at a time in LDS (read down the column-major
+// blocks), each thread accumulating up to three 4 x 4 tiles of the lower
+// triangle in registers; then H = L L^T in place, M = L^{-1} column by column
 // into the free upper triangle (diagonal kept apart), P = M^T written as FP32.
 // The host routine of the same name in precond.cpp does the same in FP64.
 // ---------------------------------------------------------------------------
 namespace naja {
 namespace mlp {
+
+constexpr int kEllRows = 16;
+constexpr int kEllTiles = 3;  // 4 x 4 tiles of H per thread: d <= 4 * 38 = 152
+
+// Doubles of the H region: H itself, or one staged row chunk when larger.
+__host__ __device__ inline size_t ellipsoid_h_doubles(int d)
+{
+    const size_t dpad = (size_t)((d + 3) & ~3);
+    const size_t hd = (size_t)d * d, ch = (size_t)kEllRows * dpad;
+    return hd > ch ? hd : ch;
+}
 
 __global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict__ A0, const double* __restrict__ b0,
                                                         int m0, int d, const float* __restrict__ Ae,
@@ -676,9 +689,10 @@ __global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict_
                                                         float* __restrict__ P, int* __restrict__ status)
 {
     extern __shared__ double sh[];
-    double* H = sh;                   // d x d
-    double* Md = H + (size_t)d * d;   // d: diagonal of M
-    double* w = Md + d;               // m_v: 1 / s_r^2
+    double* H = sh;                                // d x d (first the staged row chunks)
+    double* Ub = sh;                               // kEllRows x dpad
+    double* Md = sh + ellipsoid_h_doubles(d);      // d: diagonal of M
+    double* w = Md + d;                            // m_v: 1 / s_r
     const int v = blockIdx.x, tid = threadIdx.x;
     const int k = ek[v];
     const int64_t off = eoff[v];
@@ -697,23 +711,76 @@ __global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict_
             for (int c = 0; c < d; ++c) ax = fma(arow(r, c), (double)x[c], ax);
         const double s = (r < m0 ? b0[r] : be[off + (r - m0)]) - ax;
         if (!(s > 0.0)) bad = 1;
-        w[r] = 1.0 / (s * s);
+        w[r] = 1.0 / s;
     }
     __syncthreads();
     if (bad) {
         if (tid == 0) status[v] = NAJA_ERR_INVALID_ARGUMENT;
         return;
     }
-    const int ntri = d * (d + 1) / 2;
-    for (int e = tid; e < ntri; e += 256) {
-        // e -> (i, j), i >= j, row-major over the lower triangle.
-        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        while ((i + 1) * (i + 2) / 2 <= e) ++i;
-        while (i * (i + 1) / 2 > e) --i;
-        const int j = e - i * (i + 1) / 2;
-        double acc = 0.0;
-        for (int r = 0; r < m; ++r) acc = fma(w[r] * arow(r, i), arow(r, j), acc);
-        H[(size_t)i * d + j] = acc;
+    const int dpad = (d + 3) & ~3;
+    const int nt = dpad / 4;
+    const int ntiles = nt * (nt + 1) / 2;
+    int ti[kEllTiles], tj[kEllTiles];
+#pragma unroll
+    for (int q = 0; q < kEllTiles; ++q) {
+        // Tile t -> (I, J), I >= J, row-major over the lower triangle of tiles.
+        const int t = tid + 256 * q;
+        ti[q] = -1;
+        tj[q] = 0;
+        if (t < ntiles) {
+            int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+            while ((I + 1) * (I + 2) / 2 <= t) ++I;
+            while (I * (I + 1) / 2 > t) --I;
+            ti[q] = I;
+            tj[q] = t - I * (I + 1) / 2;
+        }
+    }
+    double acc[kEllTiles][4][4];
+#pragma unroll
+    for (int q = 0; q < kEllTiles; ++q)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[q][x][y] = 0.0;
+    for (int r0 = 0; r0 < m; r0 += kEllRows) {
+        const int nr = min(kEllRows, m - r0);
+        __syncthreads();  // the previous chunk has been read
+        for (int e = tid; e < kEllRows * dpad; e += 256) {
+            const int rr = e % kEllRows, c = e / kEllRows;  // rows fastest: reads run down a column
+            Ub[rr * dpad + c] = (rr < nr && c < d) ? arow(r0 + rr, c) * w[r0 + rr] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kEllTiles; ++q) {
+            if (ti[q] < 0) continue;
+            const double* ui = Ub + 4 * ti[q];
+            const double* uj = Ub + 4 * tj[q];
+            for (int rr = 0; rr < nr; ++rr) {
+                double a4[4], b4[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    a4[x] = ui[rr * dpad + x];
+                    b4[x] = uj[rr * dpad + x];
+                }
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) acc[q][x][y] = fma(a4[x], b4[y], acc[q][x][y]);
+            }
+        }
+    }
+    __syncthreads();  // every chunk read: H may now overwrite them
+#pragma unroll
+    for (int q = 0; q < kEllTiles; ++q) {
+        if (ti[q] < 0) continue;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int i = 4 * ti[q] + x, j = 4 * tj[q] + y;
+                if (i < d && j <= i) H[(size_t)i * d + j] = acc[q][x][y];
+            }
     }
     __syncthreads();
     __shared__ int singular;
@@ -779,8 +846,8 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         }
         NAJA_REQUIRE(tot == 0 || (extra_A && extra_b), NAJA_ERR_INVALID_ARGUMENT,
                      "naja_precond_ellipsoid_device: extra rows without extra_A/extra_b");
-        const size_t lds = sizeof(double) * ((size_t)d * d + d + (size_t)(m0 + kmax));
-        NAJA_REQUIRE(lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
+        const size_t lds = sizeof(double) * (mlp::ellipsoid_h_doubles(d) + d + (size_t)(m0 + kmax));
+        NAJA_REQUIRE(lds <= 150 * 1024 && d <= 4 * 38, NAJA_ERR_UNSUPPORTED,
                      "naja_precond_ellipsoid_device: d=%d with %d rows exceeds the LDS budget (use the host routine)",
                      d, m0 + kmax);
         hipStream_t s = (hipStream_t)stream;

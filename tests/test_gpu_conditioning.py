@@ -63,24 +63,30 @@ def test_rounded_samples_stay_feasible(gpu):
         assert worst <= 1e-5, name
 
 
-def test_device_ellipsoid_matches_host(gpu):
+@pytest.mark.parametrize("d,w_lo,off_centre", [(100, 1e-3, False), (7, 1e-2, True), (33, 1e-2, True),
+                                                (128, 1e-2, False)])
+def test_device_ellipsoid_matches_host(gpu, d, w_lo, off_centre):
     """The GPU Dikin-ellipsoid rounding equals the FP64 host routine (on the
-    same FP32 row values) to FP32 rounding, and P P^T H = I."""
+    same FP32 row values) to FP32 rounding, and P P^T H = I; d not a multiple
+    of the 4 x 4 Hessian tiles, an off-centre point and the largest tiled d."""
     import naja_amd
     from naja_amd import conditioning as cd
     from naja_amd import precond
 
-    d = 100
     A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
     pool = cd.make_pool(d)
-    EA, Eb, _, _ = cd.sample_conditions(pool, 24, 300, 1e-3, 1e-1, seed=4)
+    EA, Eb, _, _ = cd.sample_conditions(pool, 24, 300, w_lo, 1e-1, seed=4)
     EA32 = [e.astype(np.float32).astype(np.float64) for e in EA]
-    Pd = naja_amd.ellipsoid_rounding(A0, b0, EA32, Eb).cpu().numpy().astype(np.float64)
-    Ph = precond.ellipsoid_batched(A0, b0, EA32, Eb)
+    xc = None
+    if off_centre:
+        xc = (1e-4 * np.random.default_rng(d).standard_normal((len(EA), d))).astype(np.float32).astype(np.float64)
+    Pd = naja_amd.ellipsoid_rounding(A0, b0, EA32, Eb, x_c=xc).cpu().numpy().astype(np.float64)
+    Ph = precond.ellipsoid_batched(A0, b0, EA32, Eb, x_c=xc)
     for v in range(len(EA)):
         A = np.vstack([A0, EA32[v]])
         b = np.concatenate([b0, Eb[v]])
-        H = (A.T / b ** 2) @ A
+        s = b - (A @ xc[v] if xc is not None else 0.0)
+        H = (A.T / s ** 2) @ A
         scale = np.abs(Ph[v]).max()
         assert np.abs(Pd[v] - Ph[v]).max() <= 1e-5 * scale, v
         # P is the FP32 rounding of the FP64 factor; H has condition numbers

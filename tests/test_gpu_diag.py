@@ -17,7 +17,8 @@ def gpu_diag(t, S, nch, layout=0):
 
 
 @pytest.mark.parametrize("spc,nch,dim,phi", [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99), (2000, 4, 8, 0.6),
-                                             (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7)])
+                                             (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7),
+                                             (200, 37, 300, 0.5), (77, 4, 100, 0.8)])
 def test_diag_matches_oracle(gpu, spc, nch, dim, phi):
     rng = np.random.default_rng(spc + nch)
     series = ar1(spc, phi, rng, nseries=nch * dim).reshape(nch, dim, spc) + rng.standard_normal((nch, dim, 1))
@@ -38,13 +39,16 @@ def test_diag_matches_oracle(gpu, spc, nch, dim, phi):
     np.testing.assert_array_equal(e, ec.min(1))
 
 
-def test_diag_blocks_layout(gpu):
+@pytest.mark.parametrize("n,nch,dim", [(4000, 4, 3), (1400, 7, 150)])
+def test_diag_blocks_layout(gpu, n, nch, dim):
+    """Chain-major sample rows (layout 1), including several chain and dim
+    tiles of the series transpose."""
     rng = np.random.default_rng(1)
-    S = ar1(4000, 0.8, rng, nseries=3).T.copy()
-    e0, r0, _ = gpu_diag(gpu, S, 4, layout=0)
-    e1, r1, _ = gpu_diag(gpu, S, 4, layout=1)
-    e1_ref, _ = oracle.ess(S, 4, layout=1)
-    r1_ref, _, _ = oracle.split_rhat(S, 4, layout=1)
+    S = ar1(n, 0.8, rng, nseries=dim).T.copy()
+    e0, r0, _ = gpu_diag(gpu, S, nch, layout=0)
+    e1, r1, _ = gpu_diag(gpu, S, nch, layout=1)
+    e1_ref, _ = oracle.ess(S, nch, layout=1)
+    r1_ref, _, _ = oracle.split_rhat(S, nch, layout=1)
     assert np.abs(r1 - r1_ref).max() <= 1e-5
     assert (np.abs(e1 - e1_ref) / e1_ref).max() <= 1e-3
 

@@ -9,9 +9,10 @@
 //   * autocovariance by a hand-written FFT: the real series of length
 //     N = 2^ceil(log2(2n)) (zero padded; linear, not circular, correlation)
 //     is packed as N/2 complex points z_j = x_2j + i x_2j+1 and transformed
-//     in LDS by an in-place decimation-in-frequency FFT, two radix-2 stages
-//     per LDS pass (natural in, bit-reversed out; twiddles from two small
-//     LDS tables); the real spectrum X_k is split out, |X_k|^2 is
+//     in LDS by an in-place decimation-in-frequency FFT of radix-16 passes,
+//     16 points per thread in registers (natural in, bit-reversed out;
+//     twiddles from two small LDS tables); the real spectrum X_k is split
+//     out, |X_k|^2 is
 //     re-packed in place and inverted by a decimation-in-time FFT
 //     (bit-reversed in, natural out), so no reordering pass is needed; the
 //     interleaved result is the unnormalised autocorrelation, scaled by
@@ -38,10 +39,10 @@ namespace naja {
 
 namespace {
 
-// Threads per ESS workgroup: H / 2 (each thread owns two butterflies of a
-// radix-2^2 pass), between 64 and 1024.
+// Threads per ESS workgroup: H / 16 (each thread owns one 16-point group of
+// a radix-16 pass), between 64 and 1024.
 constexpr int kDiagThreads = 1024;
-inline int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H / 2)); }
+inline int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H / 16)); }
 constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (136 KB with the padding)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
@@ -72,7 +73,7 @@ struct Twiddles {
 };
 
 // The transform's LDS array with one float2 of padding after every 16
-// (128 bytes): the power-of-two strides of the radix-2^2 passes then spread
+// (128 bytes): the power-of-two strides of the radix-16 passes then spread
 // over all LDS banks instead of a few.
 struct PaddedZ {
     float2* p;
@@ -91,79 +92,194 @@ __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.
 __device__ __forceinline__ float2 mul_pi(float2 a) { return make_float2(-a.y, a.x); }
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// 4-point DFT in place, natural order (INV: the unnormalised inverse).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& u0, float2& u1, float2& u2, float2& u3)
+{
+    const float2 t0 = cadd(u0, u2), t1 = csub(u0, u2), t2 = cadd(u1, u3);
+    const float2 t3 = INV ? mul_pi(csub(u1, u3)) : mul_mi(csub(u1, u3));
+    u0 = cadd(t0, t2);
+    u1 = cadd(t1, t3);
+    u2 = csub(t0, t2);
+    u3 = csub(t1, t3);
+}
+
+// x * W_16^k (conjugated for INV), k in {1, 2, 3, 4, 6, 9}.
+template <bool INV, int K>
+__device__ __forceinline__ float2 w16(float2 x)
+{
+    constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, h = 0.70710678118654752f;
+    float2 w;
+    if (K == 1) w = make_float2(c1, -s1);
+    if (K == 2) w = make_float2(h, -h);
+    if (K == 3) w = make_float2(s1, -c1);
+    if (K == 4) return INV ? mul_pi(x) : mul_mi(x);
+    if (K == 6) w = make_float2(-h, -h);
+    if (K == 9) w = make_float2(-c1, s1);
+    if (INV) w.y = -w.y;
+    return cmul(x, w);
+}
+
+// 16-point DFT of x[0..15] (natural order) in registers as 4 x 4:
+// DFT4 over m1 of x[4 m1 + m2], twiddle W_16^(m2 r1), DFT4 over m2.
+// Output X_r ends in x[4 (r & 3) + (r >> 2)].
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&x)[16])
+{
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) dft4<INV>(x[m2], x[4 + m2], x[8 + m2], x[12 + m2]);
+    x[5] = w16<INV, 1>(x[5]);
+    x[9] = w16<INV, 2>(x[9]);
+    x[13] = w16<INV, 3>(x[13]);
+    x[6] = w16<INV, 2>(x[6]);
+    x[10] = w16<INV, 4>(x[10]);
+    x[14] = w16<INV, 6>(x[14]);
+    x[7] = w16<INV, 3>(x[7]);
+    x[11] = w16<INV, 6>(x[11]);
+    x[15] = w16<INV, 9>(x[15]);
+#pragma unroll
+    for (int r1 = 0; r1 < 4; ++r1) dft4<INV>(x[4 * r1], x[4 * r1 + 1], x[4 * r1 + 2], x[4 * r1 + 3]);
+}
+
+__host__ __device__ constexpr int bitrev4(int r) { return ((r & 1) << 3) | ((r & 2) << 1) | ((r & 4) >> 1) | ((r & 8) >> 3); }
+
+// w[r] = W_L^(p0 r), r = 0..15, from w1 = W_L^p0 and w4 = W_L^(4 p0) (both
+// from the tables; every other power is at most three products away).
+__device__ __forceinline__ void twiddle_powers(float2 w1, float2 w4, float2 (&w)[16])
+{
+    w[0] = make_float2(1.0f, 0.0f);
+    w[1] = w1;
+    w[2] = cmul(w1, w1);
+    w[3] = cmul(w[2], w1);
+    w[4] = w4;
+    w[8] = cmul(w4, w4);
+    w[12] = cmul(w[8], w4);
+#pragma unroll
+    for (int b = 4; b < 16; b += 4)
+#pragma unroll
+        for (int k = 1; k < 4; ++k) w[b + k] = cmul(w[b], w[k]);
+}
+
+// One radix-16 pass of the in-place DIF transform over blocks of L = 16 q
+// points (q = 2^logq): per block offset p0 < q, the DFT16 of the points
+// p0 + m q, output r scaled by W_L^(p0 r) and stored at p0 + bitrev4(r) q.
+// Passes at L = H, H/16, ... leave the spectrum in bit-reversed order.
+__device__ void dif16_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
+{
+    const int q = 1 << logq;
+    const int sh = logN - logq - 4;  // W_L^j = W_N^(j << sh)
+    for (int g = threadIdx.x; g < H / 16; g += blockDim.x) {
+        const int p0 = g & (q - 1);
+        const int base = ((g >> logq) << (logq + 4)) + p0;
+        float2 x[16], w[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) x[m] = z[base + m * q];
+        dft16<false>(x);
+        twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) z[base + bitrev4(r) * q] = cmul(x[4 * (r & 3) + (r >> 2)], w[r]);
+    }
+}
+
+// The inverse of dif16_pass (times 16): conjugate twiddles on the
+// bit-reversed inputs, inverse DFT16, natural-order outputs.
+__device__ void dit16_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
+{
+    const int q = 1 << logq;
+    const int sh = logN - logq - 4;
+    for (int g = threadIdx.x; g < H / 16; g += blockDim.x) {
+        const int p0 = g & (q - 1);
+        const int base = ((g >> logq) << (logq + 4)) + p0;
+        float2 x[16], w[16];
+        twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = cmul(z[base + bitrev4(r) * q], conjf2(w[r]));
+        dft16<true>(x);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) z[base + m * q] = x[4 * (m & 3) + (m >> 2)];
+    }
+}
+
+// The last DIF pass (first DIT pass) when logH is not a multiple of 4:
+// DFTs of 2^rem contiguous points, no twiddles; DIF stores output r at
+// bitrev_rem(r), DIT reads input r from there.
+template <bool INV>
+__device__ void tail_pass(const PaddedZ& z, int H, int rem)
+{
+    const int L = 1 << rem;
+    for (int g = threadIdx.x; g < H / L; g += blockDim.x) {
+        const int o = g * L;
+        if (rem == 1) {
+            const float2 u = z[o], v = z[o + 1];
+            z[o] = cadd(u, v);
+            z[o + 1] = csub(u, v);
+        } else if (rem == 2) {
+            // bitrev2: slots 0, 1, 2, 3 hold outputs 0, 2, 1, 3
+            float2 u0 = z[o], u1 = z[o + (INV ? 2 : 1)], u2 = z[o + (INV ? 1 : 2)], u3 = z[o + 3];
+            dft4<INV>(u0, u1, u2, u3);
+            z[o] = u0;
+            z[o + (INV ? 1 : 2)] = u1;
+            z[o + (INV ? 2 : 1)] = u2;
+            z[o + 3] = u3;
+        } else {
+            // 8 points: DFT4 over m1 of x[2 m1 + m2], twiddle W_8^(m2 r1),
+            // DFT2 over m2; output r1 + 4 r2; bitrev3 = {0, 4, 2, 6, 1, 5, 3, 7}.
+            constexpr int br3[8] = {0, 4, 2, 6, 1, 5, 3, 7};
+            float2 x[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) x[m] = z[o + (INV ? br3[m] : m)];
+            dft4<INV>(x[0], x[2], x[4], x[6]);
+            dft4<INV>(x[1], x[3], x[5], x[7]);
+            constexpr float h = 0.70710678118654752f;
+            // x[2 r1 + 1] *= W_8^r1
+            x[3] = cmul(x[3], make_float2(h, INV ? h : -h));
+            x[5] = INV ? mul_pi(x[5]) : mul_mi(x[5]);
+            x[7] = cmul(x[7], make_float2(-h, INV ? h : -h));
+            float2 X[8];
+#pragma unroll
+            for (int r1 = 0; r1 < 4; ++r1) {
+                X[r1] = cadd(x[2 * r1], x[2 * r1 + 1]);
+                X[r1 + 4] = csub(x[2 * r1], x[2 * r1 + 1]);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) z[o + (INV ? r : br3[r])] = X[r];
+        }
+    }
+}
+
 // In-place FFT over H points, decimation in frequency (natural in,
-// bit-reversed out), two radix-2 stages per LDS pass (radix 2^2): stage st
-// pairs (i, i + 2^(st-1)) with twiddle W_{2^st}^pos = W^(pos * (N >> st)).
+// bit-reversed out): radix-16 passes at block sizes H, H/16, ... while
+// >= 16, then one radix-2/4/8 pass. Each pass holds 16 (or 2^rem) points
+// per thread in registers, so an 8192-point transform takes 4 LDS passes.
 __device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
-    int st = logH;
-    for (; st >= 2; st -= 2) {
-        const int q = 1 << (st - 2);
-        for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
-            const int p0 = g & (q - 1);
-            const int j = ((g >> (st - 2)) << st) + p0;
-            const float2 a = z[j], b = z[j + q], c = z[j + 2 * q], e = z[j + 3 * q];
-            const float2 w1 = tw.at(p0 << (logN - st));        // W_{4q}^{p0}
-            const float2 w2 = tw.at(p0 << (logN - st + 1));    // W_{2q}^{p0}
-            const float2 a1 = make_float2(a.x + c.x, a.y + c.y);
-            const float2 c1 = cmul(make_float2(a.x - c.x, a.y - c.y), w1);
-            const float2 b1 = make_float2(b.x + e.x, b.y + e.y);
-            const float2 e1 = cmul(mul_mi(make_float2(b.x - e.x, b.y - e.y)), w1);  // W_{4q}^{p0+q}
-            z[j] = make_float2(a1.x + b1.x, a1.y + b1.y);
-            z[j + q] = cmul(make_float2(a1.x - b1.x, a1.y - b1.y), w2);
-            z[j + 2 * q] = make_float2(c1.x + e1.x, c1.y + e1.y);
-            z[j + 3 * q] = cmul(make_float2(c1.x - e1.x, c1.y - e1.y), w2);
-        }
+    int logL = logH;
+    for (; logL >= 4; logL -= 4) {
+        dif16_pass(z, H, logL - 4, logN, tw);
         __syncthreads();
     }
-    if (st == 1) {
-        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
-            const float2 u = z[2 * b], v = z[2 * b + 1];
-            z[2 * b] = make_float2(u.x + v.x, u.y + v.y);
-            z[2 * b + 1] = make_float2(u.x - v.x, u.y - v.y);
-        }
+    if (logL > 0) {
+        tail_pass<false>(z, H, logL);
         __syncthreads();
     }
 }
 
-// Unnormalised inverse, decimation in time with conjugate twiddles
-// (bit-reversed in, natural out), stages 1..logH two per LDS pass.
+// Unnormalised inverse of fft_dif's output order (bit-reversed in, natural
+// out): the passes of fft_dif undone in reverse order.
 __device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
-    int st = 1;
-    for (; st + 1 <= logH; st += 2) {
-        const int q = 1 << (st - 1);
-        for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
-            const int p0 = g & (q - 1);
-            const int j = ((g >> (st - 1)) << (st + 1)) + p0;
-            const float2 a = z[j], b = z[j + q], c = z[j + 2 * q], e = z[j + 3 * q];
-            const float2 w1 = conjf2(tw.at(p0 << (logN - st)));      // conj W_{2q}^{p0}
-            const float2 w2 = conjf2(tw.at(p0 << (logN - st - 1)));  // conj W_{4q}^{p0}
-            const float2 b1 = cmul(w1, b), e1 = cmul(w1, e);
-            const float2 a1 = make_float2(a.x + b1.x, a.y + b1.y), bb = make_float2(a.x - b1.x, a.y - b1.y);
-            const float2 c1 = make_float2(c.x + e1.x, c.y + e1.y), ee = make_float2(c.x - e1.x, c.y - e1.y);
-            const float2 c2 = cmul(w2, c1);
-            const float2 e2 = mul_pi(cmul(w2, ee));  // conj W_{4q}^{p0+q}
-            z[j] = make_float2(a1.x + c2.x, a1.y + c2.y);
-            z[j + 2 * q] = make_float2(a1.x - c2.x, a1.y - c2.y);
-            z[j + q] = make_float2(bb.x + e2.x, bb.y + e2.y);
-            z[j + 3 * q] = make_float2(bb.x - e2.x, bb.y - e2.y);
-        }
+    const int rem = logH & 3;
+    if (rem > 0) {
+        tail_pass<true>(z, H, rem);
         __syncthreads();
     }
-    if (st == logH) {
-        const int half = 1 << (st - 1);
-        for (int b = threadIdx.x; b < H / 2; b += blockDim.x) {
-            const int pos = b & (half - 1);
-            const int i0 = ((b >> (st - 1)) << st) + pos;
-            const float2 w = conjf2(tw.at(pos << (logN - st)));
-            const float2 u = z[i0];
-            const float2 v = cmul(w, z[i0 + half]);
-            z[i0] = make_float2(u.x + v.x, u.y + v.y);
-            z[i0 + half] = make_float2(u.x - v.x, u.y - v.y);
-        }
+    for (int logL = rem + 4; logL <= logH; logL += 4) {
+        dit16_pass(z, H, logL - 4, logN, tw);
         __syncthreads();
     }
 }

@@ -594,11 +594,14 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             for (int g = 0; g < G; ++g) {
                 if (g < geff) {
                     float* dst = a.samples + V.sample_off + (col + g) * (int64_t)d;
+                    // Streaming stores: the samples are not read again by
+                    // this launch, so they should not displace the columns
+                    // of A in L2.
                     for (int c = tid; c < d; c += T) {
                         float v = xs[g * d + c];
                         if (c == prev_i) v = last_i[g];
                         if (c == prev_j) v = last_j[g];
-                        dst[c] = v;
+                        __builtin_nontemporal_store(v, dst + c);
                     }
                 }
             }
@@ -835,7 +838,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
                 float v = xs[c];
                 if (c == prev_i) v = last_i;
                 if (c == prev_j) v = last_j;
-                dst[c] = v;
+                __builtin_nontemporal_store(v, dst + c);  // streaming, as above
             }
             ++sample_idx;
             until_save = a.thin;

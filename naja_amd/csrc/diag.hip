@@ -289,30 +289,58 @@ __device__ __forceinline__ int bitrev(int x, int bits)
     return (int)(__builtin_bitreverse32((uint32_t)x) >> (32 - bits));
 }
 
-// Gathers every (variant, chain) series into contiguous rows:
-// out[((v * nchains + c) * dim + d) * spc + s] = sample s of chain c, dim d,
-// through 64 x 64 LDS tiles (coalesced on both sides). Strided per-series
-// reads would touch one cache line per element.
+// Gathers every (variant, chain) series into rows of stride spcp (spc
+// rounded up to 64 floats, so every row starts on a 256-byte boundary):
+// out[((v * nchains + c) * dim + d) * spcp + s] = sample s of chain c, dim d.
+// A workgroup moves kTs samples x td dims of one chain through LDS (row
+// stride td + 1, so the column reads hit distinct banks) and writes, per dim,
+// kTs consecutive samples (one aligned 256-byte run). Block ids run over
+// (chain, sample tile) so that the chains of 8 consecutive sample tiles are
+// 8 apart: they land on the same XCD (blocks go to XCDs round robin) close
+// in time, and the cache lines their interleaved sample rows share are read
+// from HBM once.
+constexpr int kTs = 64;
+constexpr int kTd = 128;
+
 __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __restrict__ samples, int dim,
-                                                              int nchains, int spc, int layout, int64_t vstride,
-                                                              int s_tiles, float* __restrict__ out)
+                                                              int nchains, int spc, int spcp, int layout,
+                                                              int64_t vstride, int s_tiles, int d_tiles,
+                                                              float* __restrict__ out)
 {
-    __shared__ float tile[64][65];
-    const int vc = blockIdx.x / s_tiles;  // v * nchains + c
-    const int st = blockIdx.x % s_tiles;
-    const int v = vc / nchains, c = vc % nchains;
-    const int s0 = st * 64, d0 = blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    __shared__ float tile[kTs * (kTd + 1)];
+    const int64_t b = blockIdx.x;
+    const int64_t grp = b / (8 * nchains);
+    const int within = (int)(b % (8 * nchains));
+    const int c = within / 8;
+    const int s_groups = (s_tiles + 7) / 8;
+    const int st = (int)(grp % s_groups) * 8 + within % 8;
+    const int64_t rest = grp / s_groups;
+    const int dt = (int)(rest % d_tiles);
+    const int v = (int)(rest / d_tiles);
+    if (st >= s_tiles) return;  // padding blocks of the last sample group
+    const int s0 = st * kTs, d0 = dt * kTd;
+    const int tdv = min(kTd, dim - d0);
     const float* src = samples + (int64_t)v * vstride;
-    for (int i = ty; i < 64; i += 4) {
-        const int s = s0 + i, d = d0 + tx;
-        tile[i][tx] = (s < spc && d < dim) ? src[sample_row(layout, s, c, nchains, spc) * dim + d] : 0.0f;
+    // Reads: lanes along dims of one sample row (runs of tdv floats), two
+    // rows per pass of the 256 threads; all loads issued before the LDS
+    // stores.
+    constexpr int kRows = kTs / (256 / kTd);
+    const int dr = threadIdx.x % kTd, sr = threadIdx.x / kTd;
+    float vals[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const int ss = s0 + sr + k * (256 / kTd);
+        vals[k] = (dr < tdv && ss < spc) ? src[sample_row(layout, ss, c, nchains, spc) * dim + d0 + dr] : 0.0f;
     }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) tile[(sr + k * (256 / kTd)) * (kTd + 1) + dr] = vals[k];
     __syncthreads();
-    for (int i = ty; i < 64; i += 4) {
-        const int d = d0 + i, s = s0 + tx;
-        if (s < spc && d < dim) out[((int64_t)vc * dim + d) * spc + s] = tile[tx][i];
-    }
+    // Writes: lanes along samples of one series.
+    const int s = threadIdx.x % kTs;
+    const int ss = s0 + s;
+    if (ss >= spc) return;  // no barrier follows
+    for (int d = threadIdx.x / kTs; d < tdv; d += 256 / kTs)
+        out[(((int64_t)v * nchains + c) * dim + d0 + d) * spcp + ss] = tile[s * (kTd + 1) + d];
 }
 
 // ESS of one contiguous series per workgroup (blockIdx.x = (v * nchains + c) * dim + d).
@@ -322,7 +350,7 @@ __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __re
 // negative pair, lags 2k-1 and 2k) lies within them; otherwise the series is
 // marked -1 for mode 2, which redoes only the marked series at full size.
 __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ series, int dim,
-                                                                  int nchains, int spc, int logH,
+                                                                  int nchains, int spc, int spcp, int logH,
                                                                   const float2* __restrict__ tw_tables, int tb,
                                                                   int tw_shift, int mode,
                                                                   float* __restrict__ ess_chain, int half,
@@ -340,7 +368,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     const int c = vc % nchains;
     const int v = vc / nchains;
     float* out = ess_chain + (int64_t)v * dim * nchains + (int64_t)d * nchains + c;
-    const float* x = series + sid * spc;
+    const float* x = series + sid * spcp;
     const int n = spc;
     const int H = 1 << logH;
     const int N = 2 * H;
@@ -361,11 +389,26 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
     // (mcmc.cpp:25-27), and the split-R-hat moments of its two halves
     // (sum and sum of squares of samples [0, half) and [half, 2 half) in FP64,
     // mcmc.cpp:150-165), written for the R-hat pass by the first ESS pass.
+    // A thread owns at most kPerThread points (H <= 16 * blockDim); their
+    // loads are all issued before any of them is used.
     double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // total, half-0 sum, sq, half-1 sum, sq
-    for (int j = tid; j < H; j += blockDim.x) {
+    constexpr int kPerThread = 16;
+    float2 xv[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int s0 = 2 * (tid + q * (int)blockDim.x);
+        xv[q] = make_float2(0.0f, 0.0f);
+        if (s0 + 1 < n)
+            xv[q] = *reinterpret_cast<const float2*>(x + s0);  // rows are 256-byte aligned, s0 even
+        else if (s0 < n)
+            xv[q].x = x[s0];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int j = tid + q * (int)blockDim.x;
+        if (j >= H) break;
         const int s0 = 2 * j, s1 = 2 * j + 1;
-        const float x0 = s0 < n ? x[s0] : 0.0f;
-        const float x1 = s1 < n ? x[s1] : 0.0f;
+        const float x0 = xv[q].x, x1 = xv[q].y;
         const double d0 = (double)x0, d1 = (double)x1;
         part[0] += d0 + d1;
         if (s0 < 2 * half) {
@@ -599,17 +642,20 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     float* ec = ess_chain;
     const int half = spc / 2;
     const int64_t n_series = (int64_t)dim * nchains * nv;
-    const int s_tiles = (spc + 63) / 64;
-    NAJA_REQUIRE((int64_t)s_tiles * nchains * nv < (int64_t)1 << 31 && n_series < (int64_t)1 << 31,
-                 NAJA_ERR_UNSUPPORTED, "too many series for one diagnostics launch");
-    scratch_alloc((void**)&ser, sizeof(float) * (size_t)n_series * spc, stream);
+    const int spcp = (spc + kTs - 1) / kTs * kTs;  // series row stride (256-byte aligned rows)
+    const int s_tiles = (spc + kTs - 1) / kTs;
+    const int d_tiles = (dim + kTd - 1) / kTd;
+    const int64_t t_blocks = (int64_t)((s_tiles + 7) / 8) * 8 * nchains * d_tiles * nv;
+    NAJA_REQUIRE(t_blocks < (int64_t)1 << 31 && n_series < (int64_t)1 << 31, NAJA_ERR_UNSUPPORTED,
+                 "too many series for one diagnostics launch");
+    scratch_alloc((void**)&ser, sizeof(float) * (size_t)n_series * spcp, stream);
     scratch_alloc((void**)&tw, sizeof(float2) * tw_h.size(), stream);
     NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
     if (!ec) scratch_alloc((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream);
     scratch_alloc((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream);
 
-    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)(s_tiles * nchains * nv), (unsigned)((dim + 63) / 64)),
-                       dim3(256), 0, stream, samples, dim, nchains, spc, layout, vstride, s_tiles, ser);
+    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)t_blocks), dim3(256), 0, stream, samples, dim, nchains,
+                       spc, spcp, layout, vstride, s_tiles, d_tiles, ser);
     NAJA_HIP(hipGetLastError());
     // A half-size pass first when it leaves at least 1/8 of the chain as exact
     // lags (N/2 - n >= n/8); the full pass then redoes only the series whose
@@ -617,11 +663,11 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const bool two_pass = N / 2 - spc >= spc / 8;
     if (two_pass) {
         hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)), padded_lds_bytes(H / 2),
-                           stream, ser, dim, nchains, spc, logH - 1, tw, tb, 1, 1, ec, half, mom);
+                           stream, ser, dim, nchains, spc, spcp, logH - 1, tw, tb, 1, 1, ec, half, mom);
         NAJA_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), padded_lds_bytes(H),
-                       stream, ser, dim, nchains, spc, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
+                       stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
     NAJA_HIP(hipGetLastError());

@@ -11,6 +11,7 @@
 // feed one MFMA operand read consecutive words.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "backmap.h"
@@ -128,12 +129,220 @@ __global__ __launch_bounds__(256) void backmap_mfma_kernel(const float* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Small operands (n <= 128, d <= 128, d % 4 == 0): the rounding backmap of
+// batched variants, y = c + P z with n = d = 100 for the learned-rounded
+// batch. A persistent workgroup takes a run of 128-row sample tiles of one
+// batch item and keeps T in LDS for all of them. Each wave owns 32 rows x
+// 16 NB columns as 2 x NB blocks of v_mfma_f32_16x16x4f32; K runs in groups
+// of 16 where lane group g = lane / 16 takes k = kk + 4 g .. kk + 4 g + 3
+// over four consecutive MFMAs, so each operand is one 16-byte LDS read
+// (A and B use the same k order, so the sums are unchanged). The LDS row
+// stride S >= d has S % 8 == 4: the 8 lanes of a 16-byte read phase then
+// hit distinct bank groups. The next tile's rows are loaded into registers
+// while the current tile's MFMAs run; results go back through LDS so every
+// tile's output block V[s0 .. s0 + 128) is written as one contiguous run.
+// ---------------------------------------------------------------------------
+constexpr int kSmRows = 128;
+constexpr int kSmMaxN = 128;
+constexpr int kSmMaxD = 128;
+constexpr int kSmPer = kSmRows * kSmMaxD / 4 / 256;  // float4 loads per thread per tile (<= 16)
+
+__host__ __device__ inline int small_stride(int d)
+{
+    const int s = (d + 3) / 4 * 4;
+    return (s % 8 == 4) ? s : s + 4;
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restrict__ T,
+                                                           const float* __restrict__ shift,
+                                                           const float* __restrict__ Y, float* __restrict__ V, int n,
+                                                           int d, int64_t ns, int tiles_per_wg, int64_t sT,
+                                                           int64_t sShift, int64_t sY, int64_t sV)
+{
+    extern __shared__ float lds[];
+    const int S = small_stride(d);
+    float* Ts = lds;                              // 16 NB rows x S (rows >= n zero)
+    float* Ys = lds + 16 * NB * S;                // kSmRows x S, then the output tile (kSmRows x n)
+    T += sT * blockIdx.z;
+    shift += sShift * blockIdx.z;
+    Y += sY * blockIdx.z;
+    V += sV * blockIdx.z;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n_tiles = (ns + kSmRows - 1) / kSmRows;
+    const int64_t t_begin = (int64_t)blockIdx.x * tiles_per_wg;
+    const int64_t t_end = t_begin + tiles_per_wg < n_tiles ? t_begin + tiles_per_wg : n_tiles;
+    if (t_begin >= t_end) return;  // uniform
+
+    // T into LDS (rows >= n and columns >= d zero).
+    for (int e = tid; e < 16 * NB * S; e += 256) {
+        const int j = e / S, k = e % S;
+        Ts[e] = (j < n && k < d) ? T[(int64_t)j * d + k] : 0.0f;
+    }
+    float sh[NB];
+#pragma unroll
+    for (int jb = 0; jb < NB; ++jb) {
+        const int col = 16 * jb + (lane & 15);
+        sh[jb] = col < n ? shift[col] : 0.0f;
+    }
+
+    // A tile of rows [s0, s0 + rows) is the contiguous float run
+    // Y[s0 d .. (s0 + rows) d): thread tid loads float4 q = tid + 256 i.
+    const int q_tile = kSmRows * d / 4;
+    auto load_tile = [&](int64_t t, float4 (&r)[kSmPer]) {
+        const int64_t s0 = t * kSmRows;
+        const int64_t rows = ns - s0 < kSmRows ? ns - s0 : kSmRows;
+        const int q_valid = (int)(rows * d / 4);
+        const float4* src = reinterpret_cast<const float4*>(Y + s0 * d);
+#pragma unroll
+        for (int i = 0; i < kSmPer; ++i) {
+            const int q = tid + 256 * i;
+            r[i] = (q < q_valid) ? src[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    };
+    float4 next[kSmPer];
+    load_tile(t_begin, next);
+
+    for (int64_t t = t_begin; t < t_end; ++t) {
+        __syncthreads();  // the previous output tile has been written out
+#pragma unroll
+        for (int i = 0; i < kSmPer; ++i) {
+            const int q = tid + 256 * i;
+            if (q < q_tile) {
+                const int e = 4 * q;
+                const int li = (S == d) ? e : (e / d) * S + e % d;  // d % 4 == 0: a float4 stays in one row
+                *reinterpret_cast<float4*>(Ys + li) = next[i];
+            }
+        }
+        __syncthreads();
+        if (t + 1 < t_end) load_tile(t + 1, next);  // in flight during the MFMAs
+
+        floatx4 acc[2][NB];
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) acc[ib][jb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        const int g = lane >> 4, r16 = lane & 15;
+        const float* ya = Ys + (32 * wave + r16) * S + 4 * g;
+        const float* tb = Ts + r16 * S + 4 * g;
+        for (int kk = 0; kk < d; kk += 16) {
+            const bool full = kk + 16 <= d;
+            float4 a[2], b[NB];
+#pragma unroll
+            for (int ib = 0; ib < 2; ++ib) a[ib] = *reinterpret_cast<const float4*>(ya + ib * 16 * S + kk);
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) b[jb] = *reinterpret_cast<const float4*>(tb + jb * 16 * S + kk);
+            if (!full) {
+                // k = kk + 4 g + st past d: both operands zero (whatever the
+                // LDS holds there).
+                const int k0 = kk + 4 * g;
+#pragma unroll
+                for (int ib = 0; ib < 2; ++ib) {
+                    if (k0 + 0 >= d) a[ib].x = 0.0f;
+                    if (k0 + 1 >= d) a[ib].y = 0.0f;
+                    if (k0 + 2 >= d) a[ib].z = 0.0f;
+                    if (k0 + 3 >= d) a[ib].w = 0.0f;
+                }
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) {
+                    if (k0 + 0 >= d) b[jb].x = 0.0f;
+                    if (k0 + 1 >= d) b[jb].y = 0.0f;
+                    if (k0 + 2 >= d) b[jb].z = 0.0f;
+                    if (k0 + 3 >= d) b[jb].w = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb) {
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].x, b[jb].x, acc[ib][jb], 0, 0, 0);
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].y, b[jb].y, acc[ib][jb], 0, 0, 0);
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].z, b[jb].z, acc[ib][jb], 0, 0, 0);
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].w, b[jb].w, acc[ib][jb], 0, 0, 0);
+                }
+        }
+        __syncthreads();  // every wave is done reading Ys
+        // D layout of a 16 x 16 block: lane holds column lane % 16, register
+        // e holds row 4 (lane / 16) + e. Output tile kSmRows x n, row-major.
+        float* Os = Ys;
+#pragma unroll
+        for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) {
+                const int col = 16 * jb + r16;
+                if (col < n) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int row = 32 * wave + 16 * ib + 4 * g + e;
+                        Os[row * n + col] = acc[ib][jb][e] + sh[jb];
+                    }
+                }
+            }
+        __syncthreads();
+        const int64_t s0 = t * kSmRows;
+        const int64_t rows = ns - s0 < kSmRows ? ns - s0 : kSmRows;
+        const int len = (int)(rows * n);
+        float* dst = V + s0 * n;
+        if ((n & 3) == 0) {
+            for (int q = tid; q < len / 4; q += 256)
+                reinterpret_cast<float4*>(dst)[q] = reinterpret_cast<const float4*>(Os)[q];
+        } else {
+            for (int e = tid; e < len; e += 256) dst[e] = Os[e];
+        }
+    }
+}
+
+size_t small_lds_bytes(int n, int d)
+{
+    const int NB = (n + 15) / 16;
+    const size_t S = (size_t)small_stride(d);
+    const size_t ys = std::max((size_t)kSmRows * S, (size_t)kSmRows * n);
+    // + 16 floats so the masked 16-byte reads past the last T row stay in
+    // the allocation.
+    return sizeof(float) * (16 * (size_t)NB * S + ys + 16);
+}
+
+template <int NB>
+void launch_small(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns, int batch,
+                  int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
+{
+    const int64_t n_tiles = (ns + kSmRows - 1) / kSmRows;
+    // About two workgroups per CU in all (one fits at a time at this LDS
+    // size); each takes a contiguous run of tiles of one batch item.
+    const int64_t wpb = std::max<int64_t>(1, std::min<int64_t>(n_tiles, (512 + batch - 1) / batch));
+    const int tpw = (int)((n_tiles + wpb - 1) / wpb);
+    const int64_t gx = (n_tiles + tpw - 1) / tpw;
+    hipLaunchKernelGGL(backmap_small_kernel<NB>, dim3((unsigned)gx, 1, (unsigned)batch), dim3(256),
+                       small_lds_bytes(n, d), stream, T, shift, Y, V, n, d, ns, tpw, sT, sShift, sY, sV);
+    NAJA_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void backmap_launch_batched(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
                             int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
 {
     if (ns == 0 || batch == 0) return;
+    // The small-operand kernel needs 16-byte rows and bases for its float4
+    // tile loads and stores.
+    const bool small = n <= kSmMaxN && d <= kSmMaxD && d % 4 == 0 && n >= 1 &&
+                       reinterpret_cast<uintptr_t>(Y) % 16 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
+                       sY % 4 == 0 && sV % 4 == 0 && batch <= 65535;
+    if (small) {
+        switch ((n + 15) / 16) {
+        case 1: launch_small<1>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 2: launch_small<2>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 3: launch_small<3>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 4: launch_small<4>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 5: launch_small<5>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 6: launch_small<6>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 7: launch_small<7>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        default: launch_small<8>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        }
+    }
     const int64_t gx = (ns + kBM - 1) / kBM;
     NAJA_REQUIRE(gx <= 0x7FFFFFFF, NAJA_ERR_UNSUPPORTED, "too many sample rows for one backmap launch");
     NAJA_REQUIRE(batch <= 65535, NAJA_ERR_UNSUPPORTED, "too many backmap batch items (%d)", batch);

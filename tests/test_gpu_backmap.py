@@ -7,7 +7,8 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 5, 1000), (37, 7, 129), (2583, 512, 4000), (130, 64, 257), (1, 1, 1), (300, 33, 0),
-          (100, 100, 5003), (128, 128, 333), (64, 127, 100), (128, 4, 64)]
+          (100, 100, 5003), (128, 128, 333), (64, 127, 100), (128, 4, 64), (12, 8, 1000), (17, 36, 300),
+          (100, 100, 128), (100, 100, 129), (129, 100, 300)]
 
 
 @pytest.mark.parametrize("n,d,ns", SHAPES)

@@ -39,10 +39,11 @@ namespace naja {
 
 namespace {
 
-// Threads per ESS workgroup: H / 16 (each thread owns one 16-point group of
-// a radix-16 pass), between 64 and 1024.
+// Threads per ESS workgroup: one per 16-point group of a radix-16 pass
+// (H >= 1024) or per 4-point group of a radix-4 pass (shorter transforms),
+// between 64 and 1024.
 constexpr int kDiagThreads = 1024;
-inline int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H / 16)); }
+int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H >= 1024 ? H / 16 : H / 4)); }
 constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (136 KB with the padding)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
@@ -86,6 +87,29 @@ struct PaddedZ {
     }
 };
 __host__ __device__ constexpr size_t padded_lds_bytes(int H) { return sizeof(float2) * (size_t)(H + H / 16); }
+
+int ess_threads(int H);
+
+// Workgroups of the full-size redo pass, which loops over all series and
+// redoes those the short pass could not settle: two rounds of what the CUs
+// hold at once (by waves, 32 per CU, and by LDS, 160 KB per CU with about
+// 4 KB of static arrays per workgroup). Dispatching one workgroup per series
+// only to have most of them exit costs milliseconds for millions of short
+// series.
+unsigned ess_grid(int H, int64_t n_series)
+{
+    static const int n_cu = [] {
+        int dev = 0, cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0)
+            cu = 256;
+        return cu;
+    }();
+    const int by_waves = std::max(1, 32 / (ess_threads(H) / 64));
+    const int by_lds = std::max(1, (int)((160 * 1024) / (padded_lds_bytes(H) + 4096)));
+    const int64_t g = (int64_t)n_cu * std::min(by_waves, by_lds) * 2;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(n_series, g));
+}
 
 // (x) * (-i) and (x) * (+i).
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
@@ -203,6 +227,47 @@ __device__ void dit16_pass(const PaddedZ& z, int H, int logq, int logN, const Tw
     }
 }
 
+// Radix-4 passes of the same scheme for short transforms (H < 1024, where
+// H / 16 groups would leave most of a 64-thread workgroup idle): per block
+// of L = 4 q points, DFT4 of p0 + m q, output r scaled by W_L^(p0 r) and
+// stored at p0 + bitrev2(r) q (bitrev2 swaps outputs 1 and 2).
+__device__ void dif4_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
+{
+    const int q = 1 << logq;
+    const int sh = logN - logq - 2;
+    for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
+        const int p0 = g & (q - 1);
+        const int base = ((g >> logq) << (logq + 2)) + p0;
+        float2 x0 = z[base], x1 = z[base + q], x2 = z[base + 2 * q], x3 = z[base + 3 * q];
+        dft4<false>(x0, x1, x2, x3);
+        const float2 w1 = tw.at(p0 << sh), w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        z[base] = x0;
+        z[base + 2 * q] = cmul(x1, w1);
+        z[base + q] = cmul(x2, w2);
+        z[base + 3 * q] = cmul(x3, w3);
+    }
+}
+
+__device__ void dit4_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
+{
+    const int q = 1 << logq;
+    const int sh = logN - logq - 2;
+    for (int g = threadIdx.x; g < H / 4; g += blockDim.x) {
+        const int p0 = g & (q - 1);
+        const int base = ((g >> logq) << (logq + 2)) + p0;
+        const float2 w1 = tw.at(p0 << sh), w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        float2 y0 = z[base], y1 = cmul(z[base + 2 * q], conjf2(w1)), y2 = cmul(z[base + q], conjf2(w2)),
+               y3 = cmul(z[base + 3 * q], conjf2(w3));
+        dft4<true>(y0, y1, y2, y3);
+        z[base] = y0;
+        z[base + q] = y1;
+        z[base + 2 * q] = y2;
+        z[base + 3 * q] = y3;
+    }
+}
+
+constexpr int kRadix16MinH = 1024;  // radix-16 passes from here (H / 16 >= 64 groups)
+
 // The last DIF pass (first DIT pass) when logH is not a multiple of 4:
 // DFTs of 2^rem contiguous points, no twiddles; DIF stores output r at
 // bitrev_rem(r), DIT reads input r from there.
@@ -258,6 +323,17 @@ __device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
     int logL = logH;
+    if (H < kRadix16MinH) {
+        for (; logL >= 2; logL -= 2) {
+            dif4_pass(z, H, logL - 2, logN, tw);
+            __syncthreads();
+        }
+        if (logL == 1) {
+            tail_pass<false>(z, H, 1);
+            __syncthreads();
+        }
+        return;
+    }
     for (; logL >= 4; logL -= 4) {
         dif16_pass(z, H, logL - 4, logN, tw);
         __syncthreads();
@@ -273,6 +349,18 @@ __device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 __device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw)
 {
     const int logN = logH + 1;
+    if (H < kRadix16MinH) {
+        const int rem2 = logH & 1;
+        if (rem2) {
+            tail_pass<true>(z, H, 1);
+            __syncthreads();
+        }
+        for (int logL = rem2 + 2; logL <= logH; logL += 2) {
+            dit4_pass(z, H, logL - 2, logN, tw);
+            __syncthreads();
+        }
+        return;
+    }
     const int rem = logH & 3;
     if (rem > 0) {
         tail_pass<true>(z, H, rem);
@@ -349,192 +437,209 @@ __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __re
 // k <= N - n, so the result is final when the truncation point (the first
 // negative pair, lags 2k-1 and 2k) lies within them; otherwise the series is
 // marked -1 for mode 2, which redoes only the marked series at full size.
+// Workgroup b takes series b, b + gridDim.x, ... one at a time, with the
+// twiddle tables loaded once: the short and single passes run one workgroup
+// per series, the full-size redo pass (mode 2) a bounded grid that skips the
+// series the short pass settled.
 __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* __restrict__ series, int dim,
                                                                   int nchains, int spc, int spcp, int logH,
                                                                   const float2* __restrict__ tw_tables, int tb,
                                                                   int tw_shift, int mode,
                                                                   float* __restrict__ ess_chain, int half,
-                                                                  double* __restrict__ mom)
+                                                                  double* __restrict__ mom, int64_t n_series)
 {
     extern __shared__ float2 z_lds[];
     const PaddedZ z{z_lds};
     __shared__ float2 tlo_s[128], thi_s[128];
     __shared__ double red[5][kDiagThreads / 64];
     __shared__ float meanf_s;
+    __shared__ int kfirst[kDiagThreads / 64];
+    __shared__ double psum[kDiagThreads / 64];
 
-    const int64_t sid = blockIdx.x;
-    const int d = (int)(sid % dim);
-    const int vc = (int)(sid / dim);
-    const int c = vc % nchains;
-    const int v = vc / nchains;
-    float* out = ess_chain + (int64_t)v * dim * nchains + (int64_t)d * nchains + c;
-    const float* x = series + sid * spcp;
     const int n = spc;
     const int H = 1 << logH;
     const int N = 2 * H;
     const int tid = threadIdx.x;
-
-    if (mode == 2 && *out >= 0.0f) return;  // settled by the short pass (uniform per workgroup)
-    if (n < 4) {
-        if (tid == 0) *out = (float)n;
-        return;
-    }
     const int nlo = 1 << tb, nhi = (H << tw_shift) >> tb;
     for (int i = tid; i < nlo; i += blockDim.x) tlo_s[i] = tw_tables[i];
     for (int i = tid; i < nhi; i += blockDim.x) thi_s[i] = tw_tables[nlo + i];
     const Twiddles tw{tlo_s, thi_s, tb, tw_shift};
+    __syncthreads();
 
-    // One pass over the series: raw values into the packed complex layout
-    // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside
-    // (mcmc.cpp:25-27), and the split-R-hat moments of its two halves
-    // (sum and sum of squares of samples [0, half) and [half, 2 half) in FP64,
-    // mcmc.cpp:150-165), written for the R-hat pass by the first ESS pass.
-    // A thread owns at most kPerThread points (H <= 16 * blockDim); their
-    // loads are all issued before any of them is used.
-    double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // total, half-0 sum, sq, half-1 sum, sq
-    constexpr int kPerThread = 16;
-    float2 xv[kPerThread];
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-        const int s0 = 2 * (tid + q * (int)blockDim.x);
-        xv[q] = make_float2(0.0f, 0.0f);
-        if (s0 + 1 < n)
-            xv[q] = *reinterpret_cast<const float2*>(x + s0);  // rows are 256-byte aligned, s0 even
-        else if (s0 < n)
-            xv[q].x = x[s0];
-    }
-#pragma unroll
-    for (int q = 0; q < kPerThread; ++q) {
-        const int j = tid + q * (int)blockDim.x;
-        if (j >= H) break;
-        const int s0 = 2 * j, s1 = 2 * j + 1;
-        const float x0 = xv[q].x, x1 = xv[q].y;
-        const double d0 = (double)x0, d1 = (double)x1;
-        part[0] += d0 + d1;
-        if (s0 < 2 * half) {
-            const int h = s0 >= half;
-            part[1 + 2 * h] += d0;
-            part[2 + 2 * h] += d0 * d0;
-        }
-        if (s1 < 2 * half) {
-            const int h = s1 >= half;
-            part[1 + 2 * h] += d1;
-            part[2 + 2 * h] += d1 * d1;
-        }
-        z[j] = make_float2(x0, x1);
-    }
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        for (int off = 32; off > 0; off >>= 1) part[q] += __shfl_xor(part[q], off);
-        if ((tid & 63) == 0) red[q][tid >> 6] = part[q];
-    }
-    __syncthreads();
-    if (tid < 5) {
-        double sum = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += red[tid][w];
-        if (tid == 0) meanf_s = (float)(sum / n);
-        else if (mom && mode != 2) {
-            // mom[v][2c + h][d][moment]
-            const int h = (tid - 1) >> 1, q = (tid - 1) & 1;
-            mom[(((int64_t)v * 2 * nchains + 2 * c + h) * dim + d) * 2 + q] = sum;
-        }
-    }
-    __syncthreads();
-    const float meanf = meanf_s;
-    // Centre in float; the padding stays zero.
-    for (int j = tid; j < H; j += blockDim.x) {
-        const int s0 = 2 * j, s1 = 2 * j + 1;
-        float2 p = z[j];
-        if (s0 < n) p.x -= meanf;
-        if (s1 < n) p.y -= meanf;
-        z[j] = p;
-    }
-    __syncthreads();
-    fft_dif(z, H, logH, tw);  // Z_k now sits at z[bitrev(k)]
+    // Every early return below is uniform across the workgroup.
+    auto one = [&](const int64_t sid) {
+        const int d = (int)(sid % dim);
+        const int vc = (int)(sid / dim);
+        const int c = vc % nchains;
+        const int v = vc / nchains;
+        float* out = ess_chain + (int64_t)v * dim * nchains + (int64_t)d * nchains + c;
+        const float* x = series + sid * spcp;
 
-    // Split the real spectrum, form P = |X|^2 and re-pack it for the
-    // half-size inverse: A_k = P_k + P_{H-k}, B_k = (P_k - P_{H-k}) e^{+2 pi i k/N},
-    // Z'_k = A_k + i B_k, written back to z[bitrev(k)] (the slot it came
-    // from, which is where the DIT inverse wants it). Each thread owns the
-    // pair {k, H-k}, so no other thread touches those two slots. Threads
-    // walk the slots rather than k, so that the z[pk] accesses of a
-    // wavefront are adjacent: the even slots hold every k < H/2 (the top bit
-    // of k is the low bit of its slot) and slot 1 holds k = H/2.
-    for (int i = tid; i <= H / 2; i += blockDim.x) {
-        const int pk = (i < H / 2) ? 2 * i : 1;
-        const int k = bitrev(pk, logH);
-        const int kb = (H - k) & (H - 1);
-        const int pb = bitrev(kb, logH);
-        const float2 zk = z[pk];
-        const float2 zb = z[pb];
-        // E = (Z_k + conj Z_{H-k})/2, O = (Z_k - conj Z_{H-k})/(2i)
-        const float2 Ek = make_float2(0.5f * (zk.x + zb.x), 0.5f * (zk.y - zb.y));
-        const float2 Ok = make_float2(0.5f * (zk.y + zb.y), -0.5f * (zk.x - zb.x));
-        const float2 Eb = make_float2(0.5f * (zb.x + zk.x), 0.5f * (zb.y - zk.y));
-        const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
-        // X_k = E_k + W^k O_k, W = e^{-2 pi i/N}; the partner of k = 0 is
-        // X_H = E_0 + W^H O_0 with W^H = -1.
-        const float2 wk = tw.at(k);
-        const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw.at(H - k);
-        const float2 Xk = make_float2(Ek.x + (wk.x * Ok.x - wk.y * Ok.y), Ek.y + (wk.x * Ok.y + wk.y * Ok.x));
-        const float2 Xb = make_float2(Eb.x + (wb.x * Ob.x - wb.y * Ob.y), Eb.y + (wb.x * Ob.y + wb.y * Ob.x));
-        const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
-        const float Pb = Xb.x * Xb.x + Xb.y * Xb.y;
-        {
-            const float Ak = Pk + Pb;
-            const float Dk = Pk - Pb;
-            // e^{+2 pi i k/N} = conj(tw[k])
-            z[pk] = make_float2(Ak + Dk * wk.y, Dk * wk.x);
+        if (mode == 2 && *out >= 0.0f) return;  // settled by the short pass
+        if (n < 4) {
+            if (tid == 0) *out = (float)n;
+            return;
         }
-        if (k != 0 && kb != k) {
-            const float Ab = Pb + Pk;
-            const float Db = Pb - Pk;
-            z[pb] = make_float2(Ab + Db * wb.y, Db * wb.x);
-        }
-    }
-    __syncthreads();
-    ifft_dit(z, H, logH, tw);
 
-    // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
-    __shared__ int kfirst[kDiagThreads / 64];
-    __shared__ double psum[kDiagThreads / 64];
-    const float norm = 1.0f / ((float)N * (float)n);
-    const float var = z.real(0) * norm;
-    const int kend = n / 2;  // pairs k = 1 .. kend-1
-    if (var < 1e-30f) {
-        if (tid == 0) *out = 1.0f;
-        return;  // uniform across the workgroup: no barrier follows for anyone
-    }
-    // Pairs whose lags are exact in this transform: 2k <= N - n.
-    const int klim = (mode == 1) ? min(kend, (N - n) / 2 + 1) : kend;
-    int kneg = klim;
-    for (int k = 1 + tid; k < klim; k += blockDim.x) {
-        const float pair_sum = (z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var;
-        if (pair_sum < 0.0f) {
-            kneg = k;
-            break;
+        // One pass over the series: raw values into the packed complex layout
+        // z_j = x_2j + i x_2j+1 (zero padded), the FP64 sum alongside
+        // (mcmc.cpp:25-27), and the split-R-hat moments of its two halves
+        // (sum and sum of squares of samples [0, half) and [half, 2 half) in FP64,
+        // mcmc.cpp:150-165), written for the R-hat pass by the first ESS pass.
+        // A thread owns at most kPerThread points (H <= 16 * blockDim); their
+        // loads are all issued before any of them is used.
+        double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // total, half-0 sum, sq, half-1 sum, sq
+        constexpr int kPerThread = 16;
+        const int nq = (H + (int)blockDim.x - 1) / (int)blockDim.x;  // <= kPerThread, uniform
+        float2 xv[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            xv[q] = make_float2(0.0f, 0.0f);
+            if (q < nq) {
+                const int s0 = 2 * (tid + q * (int)blockDim.x);
+                if (s0 + 1 < n)
+                    xv[q] = *reinterpret_cast<const float2*>(x + s0);  // rows are 256-byte aligned, s0 even
+                else if (s0 < n)
+                    xv[q].x = x[s0];
+            }
         }
-    }
-    for (int off = 32; off > 0; off >>= 1) kneg = min(kneg, __shfl_xor(kneg, off));
-    if ((tid & 63) == 0) kfirst[tid >> 6] = kneg;
-    __syncthreads();
-    int K = klim;
-    for (int w = 0; w < (int)(blockDim.x / 64); ++w) K = min(K, kfirst[w]);
-    if (K == klim && klim < kend) {
-        if (tid == 0) *out = -1.0f;  // truncation beyond the exact lags: redo at full size
-        return;
-    }
-    double acc = 0.0;
-    for (int k = 1 + tid; k < K; k += blockDim.x)
-        acc += (double)((z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if ((tid & 63) == 0) psum[tid >> 6] = acc;
-    __syncthreads();
-    if (tid == 0) {
-        double sum = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += psum[w];
-        const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
-        *out = (float)n / tau;
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int j = tid + q * (int)blockDim.x;
+            if (q >= nq || j >= H) continue;
+            const int s0 = 2 * j, s1 = 2 * j + 1;
+            const float x0 = xv[q].x, x1 = xv[q].y;
+            const double d0 = (double)x0, d1 = (double)x1;
+            part[0] += d0 + d1;
+            if (s0 < 2 * half) {
+                const int h = s0 >= half;
+                part[1 + 2 * h] += d0;
+                part[2 + 2 * h] += d0 * d0;
+            }
+            if (s1 < 2 * half) {
+                const int h = s1 >= half;
+                part[1 + 2 * h] += d1;
+                part[2 + 2 * h] += d1 * d1;
+            }
+            z[j] = make_float2(x0, x1);
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            for (int off = 32; off > 0; off >>= 1) part[q] += __shfl_xor(part[q], off);
+            if ((tid & 63) == 0) red[q][tid >> 6] = part[q];
+        }
+        __syncthreads();
+        if (tid < 5) {
+            double sum = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += red[tid][w];
+            if (tid == 0) meanf_s = (float)(sum / n);
+            else if (mom && mode != 2) {
+                // mom[v][2c + h][d][moment]
+                const int h = (tid - 1) >> 1, q = (tid - 1) & 1;
+                mom[(((int64_t)v * 2 * nchains + 2 * c + h) * dim + d) * 2 + q] = sum;
+            }
+        }
+        __syncthreads();
+        const float meanf = meanf_s;
+        // Centre in float; the padding stays zero.
+        for (int j = tid; j < H; j += blockDim.x) {
+            const int s0 = 2 * j, s1 = 2 * j + 1;
+            float2 p = z[j];
+            if (s0 < n) p.x -= meanf;
+            if (s1 < n) p.y -= meanf;
+            z[j] = p;
+        }
+        __syncthreads();
+        fft_dif(z, H, logH, tw);  // Z_k now sits at z[bitrev(k)]
+
+        // Split the real spectrum, form P = |X|^2 and re-pack it for the
+        // half-size inverse: A_k = P_k + P_{H-k}, B_k = (P_k - P_{H-k}) e^{+2 pi i k/N},
+        // Z'_k = A_k + i B_k, written back to z[bitrev(k)] (the slot it came
+        // from, which is where the DIT inverse wants it). Each thread owns the
+        // pair {k, H-k}, so no other thread touches those two slots. Threads
+        // walk the slots rather than k, so that the z[pk] accesses of a
+        // wavefront are adjacent: the even slots hold every k < H/2 (the top bit
+        // of k is the low bit of its slot) and slot 1 holds k = H/2.
+        for (int i = tid; i <= H / 2; i += blockDim.x) {
+            const int pk = (i < H / 2) ? 2 * i : 1;
+            const int k = bitrev(pk, logH);
+            const int kb = (H - k) & (H - 1);
+            const int pb = bitrev(kb, logH);
+            const float2 zk = z[pk];
+            const float2 zb = z[pb];
+            // E = (Z_k + conj Z_{H-k})/2, O = (Z_k - conj Z_{H-k})/(2i)
+            const float2 Ek = make_float2(0.5f * (zk.x + zb.x), 0.5f * (zk.y - zb.y));
+            const float2 Ok = make_float2(0.5f * (zk.y + zb.y), -0.5f * (zk.x - zb.x));
+            const float2 Eb = make_float2(0.5f * (zb.x + zk.x), 0.5f * (zb.y - zk.y));
+            const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
+            // X_k = E_k + W^k O_k, W = e^{-2 pi i/N}; the partner of k = 0 is
+            // X_H = E_0 + W^H O_0 with W^H = -1.
+            const float2 wk = tw.at(k);
+            const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw.at(H - k);
+            const float2 Xk = make_float2(Ek.x + (wk.x * Ok.x - wk.y * Ok.y), Ek.y + (wk.x * Ok.y + wk.y * Ok.x));
+            const float2 Xb = make_float2(Eb.x + (wb.x * Ob.x - wb.y * Ob.y), Eb.y + (wb.x * Ob.y + wb.y * Ob.x));
+            const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
+            const float Pb = Xb.x * Xb.x + Xb.y * Xb.y;
+            {
+                const float Ak = Pk + Pb;
+                const float Dk = Pk - Pb;
+                // e^{+2 pi i k/N} = conj(tw[k])
+                z[pk] = make_float2(Ak + Dk * wk.y, Dk * wk.x);
+            }
+            if (k != 0 && kb != k) {
+                const float Ab = Pb + Pk;
+                const float Db = Pb - Pk;
+                z[pb] = make_float2(Ab + Db * wb.y, Db * wb.x);
+            }
+        }
+        __syncthreads();
+        ifft_dit(z, H, logH, tw);
+
+        // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
+        const float norm = 1.0f / ((float)N * (float)n);
+        const float var = z.real(0) * norm;
+        const int kend = n / 2;  // pairs k = 1 .. kend-1
+        if (var < 1e-30f) {
+            if (tid == 0) *out = 1.0f;
+            return;  // uniform across the workgroup: no barrier follows for anyone
+        }
+        // Pairs whose lags are exact in this transform: 2k <= N - n.
+        const int klim = (mode == 1) ? min(kend, (N - n) / 2 + 1) : kend;
+        int kneg = klim;
+        for (int k = 1 + tid; k < klim; k += blockDim.x) {
+            const float pair_sum = (z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var;
+            if (pair_sum < 0.0f) {
+                kneg = k;
+                break;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) kneg = min(kneg, __shfl_xor(kneg, off));
+        if ((tid & 63) == 0) kfirst[tid >> 6] = kneg;
+        __syncthreads();
+        int K = klim;
+        for (int w = 0; w < (int)(blockDim.x / 64); ++w) K = min(K, kfirst[w]);
+        if (K == klim && klim < kend) {
+            // Truncation beyond the exact lags: redo at full size.
+            if (tid == 0) *out = -1.0f;
+            return;
+        }
+        double acc = 0.0;
+        for (int k = 1 + tid; k < K; k += blockDim.x)
+            acc += (double)((z.real(2 * k - 1) * norm) / var + (z.real(2 * k) * norm) / var);
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if ((tid & 63) == 0) psum[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double sum = 0.0;
+            for (int w = 0; w < (int)(blockDim.x / 64); ++w) sum += psum[w];
+            const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
+            *out = (float)n / tau;
+        }
+    };
+
+    for (int64_t sid = blockIdx.x; sid < n_series; sid += gridDim.x) {
+        one(sid);
+        __syncthreads();  // LDS is reused by the next series
     }
 }
 
@@ -662,12 +767,16 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     // truncation point lies beyond them.
     const bool two_pass = N / 2 - spc >= spc / 8;
     if (two_pass) {
-        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)), padded_lds_bytes(H / 2),
-                           stream, ser, dim, nchains, spc, spcp, logH - 1, tw, tb, 1, 1, ec, half, mom);
+        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)),
+                           padded_lds_bytes(H / 2), stream,
+                           ser, dim, nchains, spc, spcp, logH - 1, tw, tb, 1, 1, ec, half, mom, n_series);
         NAJA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(ess_series_kernel, dim3(ess_grid(H, n_series)), dim3(ess_threads(H)), padded_lds_bytes(H),
+                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 2, ec, half, mom, n_series);
+    } else {
+        hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), padded_lds_bytes(H),
+                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 0, ec, half, mom, n_series);
     }
-    hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), padded_lds_bytes(H),
-                       stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, two_pass ? 2 : 0, ec, half, mom);
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
     NAJA_HIP(hipGetLastError());

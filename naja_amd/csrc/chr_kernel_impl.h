@@ -311,19 +311,30 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // one column each, ranged to the variant's real rows: padding rows read 0
     // without touching memory (their slack is +inf, so their chord ratio is
     // inf * 0 or inf / 0 and never bounds the chord).
-    const float* Acol0 = a.A + V.a_off;
-    const float* Icol0 = a.Ainv + V.a_off;
     // Whole runs in range or out of range: m rounded up to VEC rows (the rows
-    // past m in memory are padding).
+    // past m in memory are padding). One descriptor per column, its base the
+    // block base plus a 32-bit byte offset (the range check covers the
+    // instruction's SGPR offset too, so the column cannot go there).
     const uint32_t col_bytes = 4u * (uint32_t)((V.m + VEC - 1) / VEC * VEC);
+    const uint64_t a_base = (uint64_t)(a.A + V.a_off);
+    const uint64_t inv_base = (uint64_t)(a.Ainv + V.a_off);
+    const uint32_t col_stride = 4u * (uint32_t)lda;  // bytes from one column to the next
+    auto col_rsrc = [&](uint64_t base, uint32_t off) {
+        const uint64_t addr = base + off;
+        chr_i32x4 r;
+        r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
+        r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
+        r[2] = (int)col_bytes;
+        r[3] = 0x00020000;
+        return r;
+    };
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
         const int mt = CO ? kMoveCoord : m.type;
         const bool real = real_move && mt != kMoveKsparse;
         const bool pair = real && mt == kMovePair;
-        const float* pi = Acol0 + (int64_t)(real ? m.i : 0) * lda;
-        const float* pj = pair ? Acol0 + (int64_t)m.j * lda : Icol0 + (int64_t)(real ? m.i : 0) * lda;
-        const chr_i32x4 ri = chr_make_rsrc(pi, col_bytes);
-        const chr_i32x4 rj = chr_make_rsrc(pj, col_bytes);
+        const uint32_t oi = (uint32_t)(real ? m.i : 0) * col_stride;
+        const chr_i32x4 ri = col_rsrc(a_base, oi);
+        const chr_i32x4 rj = pair ? col_rsrc(a_base, (uint32_t)m.j * col_stride) : col_rsrc(inv_base, oi);
 #pragma unroll
         for (int q = 0; q < R / VEC; ++q) {
             const int off = 4 * (q * VEC * T + tid * VEC);

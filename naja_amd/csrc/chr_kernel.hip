@@ -265,6 +265,9 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R)
             fn = coord_only && t.e[q].fn_coord ? t.e[q].fn_coord : t.e[q].fn;
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d", sh.W, sh.G, sh.R);
+    NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),
+                 NAJA_ERR_UNSUPPORTED, "%lld steps in one launch (at most 2^31 - 1; the streamed API splits longer runs)",
+                 (long long)args.nsteps);
     const size_t lds = sh.lds_bytes(args.d);
     NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
                  args.d, sh.G);

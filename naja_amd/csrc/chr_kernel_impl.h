@@ -426,7 +426,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : -1;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t - t_begin < a.nlog && tid == 0) {
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
             const int64_t tl = t - t_begin;
             a.move_log[tl * 3 + 0] = mtype;
             a.move_log[tl * 3 + 1] = mv.i;
@@ -442,7 +442,9 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 #pragma unroll
         for (int g = 0; g < G; ++g) U[g] = Ucur[g];
         Move nf = mv;  // the move of step t + PF
-        const bool has_pf = t + PF < t_end;
+        // Launches run fewer than 2^31 steps: 32-bit differences compare on
+        // the scalar unit.
+        const bool has_pf = (int)(t_end - t) > PF;
         if (PF == 1) {
             if (has_pf) {
                 if (L == 63) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
@@ -734,7 +736,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : -1;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && t - t_begin < a.nlog && tid == 0) {
+        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
             const int64_t tl = t - t_begin;
             a.move_log[tl * 3 + 0] = mv.type;
             a.move_log[tl * 3 + 1] = mv.i;

@@ -106,11 +106,22 @@ __device__ __forceinline__ uint32_t dpp_mov(uint32_t v)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
-// Unsigned and signed minimum over the 64 lanes of a wavefront, left in every
-// lane: DPP within each row of 16 (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
-// row_half_mirror, row_mirror), then rows 0/1 and 2/3 through
-// v_permlane16_swap and the two halves through v_permlane32_swap (gfx950),
-// all in VALU with no trip through scalar registers.
+// DPP row broadcast (row_bcast:15 = 0x142, row_bcast:31 = 0x143, gfx9):
+// the rows in ROW_MASK take lane 15 (31) of the row below; the other rows get
+// IDENT, the identity of the consuming min, so the move folds into it.
+template <int CTRL, int ROW_MASK, uint32_t IDENT>
+__device__ __forceinline__ uint32_t dpp_bcast(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)IDENT, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Unsigned and signed minimum over the 64 lanes of a wavefront, returned
+// wave-uniform: DPP within each row of 16 (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then across rows with
+// row_bcast:15 (rows 1 and 3 take the row below) and row_bcast:31 (rows 2
+// and 3 take lane 31), which leaves the wave minimum in lane 63; one
+// v_readlane each brings it to a scalar register. Every step is a DPP
+// operand of a min: three VALU per value across rows.
 __device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
 {
     uint32_t l = (uint32_t)lb;
@@ -122,14 +133,12 @@ __device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
     l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0x141>(l));
     hb = umin32(hb, dpp_mov<0x140>(hb));
     l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_mov<0x140>(l));
-    const auto h16 = __builtin_amdgcn_permlane16_swap(hb, hb, false, false);
-    const auto l16 = __builtin_amdgcn_permlane16_swap(l, l, false, false);
-    hb = umin32(h16[0], h16[1]);
-    l = (uint32_t)smin32((int32_t)l16[0], (int32_t)l16[1]);
-    const auto h32 = __builtin_amdgcn_permlane32_swap(hb, hb, false, false);
-    const auto l32 = __builtin_amdgcn_permlane32_swap(l, l, false, false);
-    hb = umin32(h32[0], h32[1]);
-    lb = smin32((int32_t)l32[0], (int32_t)l32[1]);
+    hb = umin32(hb, dpp_bcast<0x142, 0xA, 0xFFFFFFFFu>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_bcast<0x142, 0xA, 0x7FFFFFFFu>(l));
+    hb = umin32(hb, dpp_bcast<0x143, 0xC, 0xFFFFFFFFu>(hb));
+    l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_bcast<0x143, 0xC, 0x7FFFFFFFu>(l));
+    hb = (uint32_t)__builtin_amdgcn_readlane((int)hb, 63);
+    lb = __builtin_amdgcn_readlane((int)l, 63);
 }
 
 // Chord ends in the direct form q = s / ae:

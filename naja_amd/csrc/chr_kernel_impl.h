@@ -432,7 +432,8 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
             prev_i = -1;
             prev_j = -1;
-            until_resync = a.resync > 0 ? a.resync : -1;
+            // No resync: a count the launch never reaches (fewer than 2^31 steps).
+            until_resync = a.resync > 0 ? a.resync : 0x7FFFFFFF;
         }
 
         if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             until_save = a.thin;
         }
         --until_save;
-        if (until_resync > 0) --until_resync;
+        --until_resync;
 
     if (PF == 1) {
         mv = nf;
@@ -742,7 +743,8 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             }
             prev_i = -1;
             prev_j = -1;
-            until_resync = a.resync > 0 ? a.resync : -1;
+            // No resync: a count the launch never reaches (fewer than 2^31 steps).
+            until_resync = a.resync > 0 ? a.resync : 0x7FFFFFFF;
         }
 
         if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
@@ -866,7 +868,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             until_save = a.thin;
         }
         --until_save;
-        if (until_resync > 0) --until_resync;
+        --until_resync;
         mv = nx;
         buf ^= 1;
     };

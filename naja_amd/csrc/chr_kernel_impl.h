@@ -272,7 +272,7 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 template <int W, int G, int R, bool CO = false, int PF = 1>
 __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 {
-    static_assert(PF == 1 || PF == 2, "prefetch distance 1 or 2");
+    static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
     constexpr int T = 64 * W;
     extern __shared__ float xs[];  // G x d current states
     __shared__ uint2 part[2][G][W];
@@ -307,14 +307,13 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 
     // Slot k of lane tid holds row row_of(k).
     float s[G][R];
-    float ca[R], cb[R];
+    float cs[PF + 1][2][R];  // rotating column sets: step t uses set (t - t_begin) % (PF + 1)
     // Steps t0 .. t0 + nsteps - 1 (t0 > 0 continues an earlier launch).
     const int64_t t_begin = a.t0;
     const int64_t t_end = a.t0 + a.nsteps;
     MoveBatch<G> mb;
     gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
     const int L0 = (int)(t_begin & 63);
-    Move mv = read_move<G, CO>(mb, L0);
     // (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j]) for a pair
     // move; a k-sparse move (or no move) loads a dummy pair. Buffer loads over
     // one column each, ranged to the variant's real rows: padding rows read 0
@@ -368,20 +367,28 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
         }
     };
-    load_cols(mv, true, ca, cb);
-    float Ucur[G];  // alpha uniforms of the current step (PF = 2 reads them a step early)
+    // Draw FIFO: the move and alpha uniforms of steps t .. t + PF - 1 (entry
+    // 0 is the current step), each read from the batch holding it at the
+    // step that requests its columns, PF steps ahead.
+    Move fm[PF];
+    float fu[PF][G];
+    fm[0] = read_move<G, CO>(mb, L0);
 #pragma unroll
-    for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], L0);
-    // Columns of set 1 for PF = 2: the move of t_begin + 1, and from here on
-    // mb holds the batch of the step after the current one.
-    float c1a[R], c1b[R];
-    Move nx = mv;
-    if (PF == 2) {
-        if (t_begin + 1 < t_end) {
-            if (L0 == 63) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + 1), group, item.c0, mb);
-            nx = read_move<G, CO>(mb, (L0 + 1) & 63);
+    for (int g = 0; g < G; ++g) fu[0][g] = readlane_f32(mb.u[g], L0);
+    load_cols(fm[0], true, cs[0][0], cs[0][1]);
+#pragma unroll
+    for (int k = 1; k < PF; ++k) {
+        fm[k] = fm[0];
+#pragma unroll
+        for (int g = 0; g < G; ++g) fu[k][g] = 0.0f;
+        if (t_begin + k < t_end) {
+            const int Lk = (L0 + k) & 63;
+            if (Lk == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + k), group, item.c0, mb);
+            fm[k] = read_move<G, CO>(mb, Lk);
+#pragma unroll
+            for (int g = 0; g < G; ++g) fu[k][g] = readlane_f32(mb.u[g], Lk);
         }
-        load_cols(nx, t_begin + 1 < t_end, c1a, c1b);
+        load_cols(fm[k], t_begin + k < t_end, cs[k][0], cs[k][1]);
     }
 
     float last_i[G], last_j[G];
@@ -401,6 +408,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // steps (the loop below is unrolled by PF + 1), so the prefetched columns
     // are never copied and their loads stay in flight across PF steps.
     auto step = [&](int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+        const Move mv = fm[0];
         const int mtype = CO ? kMoveCoord : mv.type;
         if (until_resync == 0) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
@@ -443,32 +451,25 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
         }
 
-        // Alpha uniforms of step t and the draw of step t + PF (a new batch of
-        // 64 steps when t + PF crosses a batch boundary; with PF = 2 the
-        // uniforms of step t + 1 are read first, before that batch replaces
-        // the one holding them).
-        const int L = (int)(t & 63);
+        // Alpha uniforms of step t, and the draw of step t + PF (a new batch
+        // of 64 steps when t + PF starts one; the FIFO already holds the
+        // draws of the steps before it).
         float U[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) U[g] = Ucur[g];
+        for (int g = 0; g < G; ++g) U[g] = fu[0][g];
         Move nf = mv;  // the move of step t + PF
+        float nu[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) nu[g] = 0.0f;
         // Launches run fewer than 2^31 steps: 32-bit differences compare on
         // the scalar unit.
         const bool has_pf = (int)(t_end - t) > PF;
-        if (PF == 1) {
-            if (has_pf) {
-                if (L == 63) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + 1), group, item.c0, mb);
-                nf = read_move<G, CO>(mb, (L + 1) & 63);
+        if (has_pf) {
+            const int Lp = (int)((t + PF) & 63);
+            if (Lp == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + PF), group, item.c0, mb);
+            nf = read_move<G, CO>(mb, Lp);
 #pragma unroll
-                for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], (L + 1) & 63);
-            }
-        } else {
-#pragma unroll
-            for (int g = 0; g < G; ++g) Ucur[g] = readlane_f32(mb.u[g], (L + 1) & 63);
-            if (has_pf) {
-                if (L == 62) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + 2), group, item.c0, mb);
-                nf = read_move<G, CO>(mb, (L + 2) & 63);
-            }
+            for (int g = 0; g < G; ++g) nu[g] = readlane_f32(mb.u[g], Lp);
         }
 
         // Prefetch the columns of step t + PF. Every step issues exactly 2R
@@ -634,31 +635,28 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         --until_save;
         --until_resync;
 
-    if (PF == 1) {
-        mv = nf;
-    } else {
-        mv = nx;
-        nx = nf;
-    }
-    buf ^= 1;
+#pragma unroll
+        for (int k = 0; k + 1 < PF; ++k) {
+            fm[k] = fm[k + 1];
+#pragma unroll
+            for (int g = 0; g < G; ++g) fu[k][g] = fu[k + 1][g];
+        }
+        fm[PF - 1] = nf;
+#pragma unroll
+        for (int g = 0; g < G; ++g) fu[PF - 1][g] = nu[g];
+        buf ^= 1;
     };
+    // Step t + j of a round computes on set j and prefetches into set
+    // (j + PF) % (PF + 1), the one the step before it has finished with.
     int64_t t = t_begin;
-    if (PF == 1) {
-        for (; t + 1 < t_end; t += 2) {
-            step(t, ca, cb, c1a, c1b);
-            step(t + 1, c1a, c1b, ca, cb);
-        }
-        if (t < t_end) step(t, ca, cb, c1a, c1b);
-    } else {
-        float c2a[R], c2b[R];
-        for (; t + 2 < t_end; t += 3) {
-            step(t, ca, cb, c2a, c2b);
-            step(t + 1, c1a, c1b, ca, cb);
-            step(t + 2, c2a, c2b, c1a, c1b);
-        }
-        if (t < t_end) step(t, ca, cb, c2a, c2b);
-        if (t + 1 < t_end) step(t + 1, c1a, c1b, ca, cb);
+    for (; t + PF < t_end; t += PF + 1) {
+#pragma unroll
+        for (int j = 0; j <= PF; ++j)
+            step(t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0], cs[(j + PF) % (PF + 1)][1]);
     }
+#pragma unroll
+    for (int j = 0; j < PF; ++j)
+        if (t + j < t_end) step(t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0], cs[(j + PF) % (PF + 1)][1]);
 
     __syncthreads();
     if (a.y_final) {

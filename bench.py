@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
 # Bytes the main learned-rounded launch moves between L2 and the memory fabric
 # (256 variants x 4 chains x 200000 steps): rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes, scaled by what the same counters read for a 1 GiB copy.
-MAIN_LAUNCH_TRAFFIC_BYTES = 365.9e9 + 5.88e9
+MAIN_LAUNCH_TRAFFIC_BYTES = 358.1e9 + 5.82e9
 METRIC = "min-ESS/sec at split-R-hat<1.05, learned-rounded vs plain CHR, 1 MI355X"
 
 

@@ -132,18 +132,21 @@ __global__ __launch_bounds__(256) void backmap_mfma_kernel(const float* __restri
 // ---------------------------------------------------------------------------
 // Small operands (n <= 128, d <= 128, d % 4 == 0): the rounding backmap of
 // batched variants, y = c + P z with n = d = 100 for the learned-rounded
-// batch. A persistent workgroup takes a run of 128-row sample tiles of one
-// batch item and keeps T in LDS for all of them. Each wave owns 32 rows x
-// 16 NB columns as 2 x NB blocks of v_mfma_f32_16x16x4f32; K runs in groups
+// batch. A persistent workgroup takes a run of 64-row sample tiles of one
+// batch item and keeps T in LDS for all of them (about 70 KB of LDS for
+// n = d = 100, so two workgroups share a CU and one's tile loads and
+// epilogue overlap the other's MFMAs). Each wave owns 16 rows x 16 NB
+// columns as NB blocks of v_mfma_f32_16x16x4f32; K runs in groups
 // of 16 where lane group g = lane / 16 takes k = kk + 4 g .. kk + 4 g + 3
 // over four consecutive MFMAs, so each operand is one 16-byte LDS read
 // (A and B use the same k order, so the sums are unchanged). The LDS row
 // stride S >= d has S % 8 == 4: the 8 lanes of a 16-byte read phase then
 // hit distinct bank groups. The next tile's rows are loaded into registers
 // while the current tile's MFMAs run; results go back through LDS so every
-// tile's output block V[s0 .. s0 + 128) is written as one contiguous run.
+// tile's output block V[s0 .. s0 + 64) is written as one contiguous run.
 // ---------------------------------------------------------------------------
-constexpr int kSmRows = 128;
+constexpr int kSmRows = 64;
+constexpr int kSmIB = kSmRows / 64;  // 16-row blocks per wave (four waves per tile)
 constexpr int kSmMaxN = 128;
 constexpr int kSmMaxD = 128;
 constexpr int kSmPer = kSmRows * kSmMaxD / 4 / 256;  // float4 loads per thread per tile (<= 16)
@@ -220,19 +223,19 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
         __syncthreads();
         if (t + 1 < t_end) load_tile(t + 1, next);  // in flight during the MFMAs
 
-        floatx4 acc[2][NB];
+        floatx4 acc[kSmIB][NB];
 #pragma unroll
-        for (int ib = 0; ib < 2; ++ib)
+        for (int ib = 0; ib < kSmIB; ++ib)
 #pragma unroll
             for (int jb = 0; jb < NB; ++jb) acc[ib][jb] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
         const int g = lane >> 4, r16 = lane & 15;
-        const float* ya = Ys + (32 * wave + r16) * S + 4 * g;
+        const float* ya = Ys + (16 * kSmIB * wave + r16) * S + 4 * g;
         const float* tb = Ts + r16 * S + 4 * g;
         for (int kk = 0; kk < d; kk += 16) {
             const bool full = kk + 16 <= d;
-            float4 a[2], b[NB];
+            float4 a[kSmIB], b[NB];
 #pragma unroll
-            for (int ib = 0; ib < 2; ++ib) a[ib] = *reinterpret_cast<const float4*>(ya + ib * 16 * S + kk);
+            for (int ib = 0; ib < kSmIB; ++ib) a[ib] = *reinterpret_cast<const float4*>(ya + ib * 16 * S + kk);
 #pragma unroll
             for (int jb = 0; jb < NB; ++jb) b[jb] = *reinterpret_cast<const float4*>(tb + jb * 16 * S + kk);
             if (!full) {
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
                 // LDS holds there).
                 const int k0 = kk + 4 * g;
 #pragma unroll
-                for (int ib = 0; ib < 2; ++ib) {
+                for (int ib = 0; ib < kSmIB; ++ib) {
                     if (k0 + 0 >= d) a[ib].x = 0.0f;
                     if (k0 + 1 >= d) a[ib].y = 0.0f;
                     if (k0 + 2 >= d) a[ib].z = 0.0f;
@@ -255,7 +258,7 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
                 }
             }
 #pragma unroll
-            for (int ib = 0; ib < 2; ++ib)
+            for (int ib = 0; ib < kSmIB; ++ib)
 #pragma unroll
                 for (int jb = 0; jb < NB; ++jb) {
                     acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].x, b[jb].x, acc[ib][jb], 0, 0, 0);
@@ -269,14 +272,14 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
         // e holds row 4 (lane / 16) + e. Output tile kSmRows x n, row-major.
         float* Os = Ys;
 #pragma unroll
-        for (int ib = 0; ib < 2; ++ib)
+        for (int ib = 0; ib < kSmIB; ++ib)
 #pragma unroll
             for (int jb = 0; jb < NB; ++jb) {
                 const int col = 16 * jb + r16;
                 if (col < n) {
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int row = 32 * wave + 16 * ib + 4 * g + e;
+                        const int row = 16 * kSmIB * wave + 16 * ib + 4 * g + e;
                         Os[row * n + col] = acc[ib][jb][e] + sh[jb];
                     }
                 }
@@ -310,8 +313,8 @@ void launch_small(const float* T, const float* shift, const float* Y, float* V, 
                   int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
 {
     const int64_t n_tiles = (ns + kSmRows - 1) / kSmRows;
-    // About two workgroups per CU in all (one fits at a time at this LDS
-    // size); each takes a contiguous run of tiles of one batch item.
+    // About 512 workgroups in all (two per CU at this LDS size), each
+    // taking a contiguous run of tiles of one batch item.
     const int64_t wpb = std::max<int64_t>(1, std::min<int64_t>(n_tiles, (512 + batch - 1) / batch));
     const int tpw = (int)((n_tiles + wpb - 1) / wpb);
     const int64_t gx = (n_tiles + tpw - 1) / tpw;

@@ -9,6 +9,10 @@ namespace naja {
 // Batch item z computes V_z = Y_z T_z^T + shift_z (strides in floats).
 void backmap_launch_batched(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
                             int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream);
+// True when backmap_launch_batched with V == Y (n == d, strides sY) runs the
+// small-operand kernel, which may work in place: each workgroup reads a whole
+// tile of rows before writing the same rows, and no two share a tile.
+bool backmap_inplace_ok(int d, const float* Y, int64_t sY, int batch);
 void backmap_launch(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
                     hipStream_t stream);
 

@@ -159,10 +159,11 @@ __host__ __device__ inline int small_stride(int d)
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Y and V may be the same array (in place, n == d): no __restrict__ on them.
 template <int NB>
 __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restrict__ T,
                                                            const float* __restrict__ shift,
-                                                           const float* __restrict__ Y, float* __restrict__ V, int n,
+                                                           const float* Y, float* V, int n,
                                                            int d, int64_t ns, int tiles_per_wg, int64_t sT,
                                                            int64_t sShift, int64_t sY, int64_t sV)
 {
@@ -325,15 +326,21 @@ void launch_small(const float* T, const float* shift, const float* Y, float* V, 
 
 }  // namespace
 
+// The small-operand kernel needs 16-byte rows and bases for its float4 tile
+// loads and stores.
+bool small_ok(int n, int d, const float* Y, const float* V, int64_t sY, int64_t sV, int batch)
+{
+    return n <= kSmMaxN && d <= kSmMaxD && d % 4 == 0 && n >= 1 && reinterpret_cast<uintptr_t>(Y) % 16 == 0 &&
+           reinterpret_cast<uintptr_t>(V) % 16 == 0 && sY % 4 == 0 && sV % 4 == 0 && batch <= 65535;
+}
+
+bool backmap_inplace_ok(int d, const float* Y, int64_t sY, int batch) { return small_ok(d, d, Y, Y, sY, sY, batch); }
+
 void backmap_launch_batched(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
                             int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
 {
     if (ns == 0 || batch == 0) return;
-    // The small-operand kernel needs 16-byte rows and bases for its float4
-    // tile loads and stores.
-    const bool small = n <= kSmMaxN && d <= kSmMaxD && d % 4 == 0 && n >= 1 &&
-                       reinterpret_cast<uintptr_t>(Y) % 16 == 0 && reinterpret_cast<uintptr_t>(V) % 16 == 0 &&
-                       sY % 4 == 0 && sV % 4 == 0 && batch <= 65535;
+    const bool small = small_ok(n, d, Y, V, sY, sV, batch);
     if (small) {
         switch ((n + 15) / 16) {
         case 1: launch_small<1>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;

@@ -52,6 +52,7 @@ struct naja_chr_plan {
     float* A_raw = nullptr;   // untransformed pack (transform plans only)
     double* b_raw = nullptr;
     float* zbuf = nullptr;    // z-space samples (transform plans only)
+    float* inplace = nullptr; // transform plan sampling z into this output and backmapping it in place
     int64_t pack_floats = 0;  // capacity of the pack buffers (floats)
     int64_t pack_rows = 0;    // capacity of b buffers (rows)
 };
@@ -219,13 +220,17 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
 void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s,
             int64_t t0 = 0)
 {
-    float* out = pl.P ? pl.zbuf : samples;
+    if (pl.inplace)
+        NAJA_REQUIRE(samples == pl.inplace, NAJA_ERR_INVALID_ARGUMENT,
+                     "this plan samples in place into the output it was created for");
+    float* zs = pl.inplace ? samples : pl.zbuf;
+    float* out = pl.P ? zs : samples;
     const ChrArgs a = make_args(pl, y0, out, y_final, nsteps, t0);
     chr_launch(a, (int)pl.items.size(), pl.shape, s);
     if (pl.P) {
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
         const int64_t per = (int64_t)pl.p.nchains * pl.p.nspc;
-        backmap_launch_batched(pl.P, pl.c, pl.zbuf, samples, pl.d, pl.d, per, pl.n_variants, (int64_t)pl.d * pl.d,
+        backmap_launch_batched(pl.P, pl.c, zs, samples, pl.d, pl.d, per, pl.n_variants, (int64_t)pl.d * pl.d,
                                pl.d, per * pl.d, per * pl.d, s);
     }
 }
@@ -234,11 +239,14 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
 
 }  // namespace naja
 
-extern "C" {
+namespace naja {
+namespace {
 
-int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_variants, const int* extra_rows,
-                         const float* A_extra, const double* b_extra, const float* P, const float* c,
-                         const naja_chr_params* p, void* stream, naja_chr_plan** out)
+// Plan creation behind naja_chr_plan_create; inplace_out (may be null) is the
+// output a one-shot rounded call will launch into.
+int plan_create(const float* A, const double* b, int m0, int d, int n_variants, const int* extra_rows,
+                const float* A_extra, const double* b_extra, const float* P, const float* c, const naja_chr_params* p,
+                void* stream, naja_chr_plan** out, float* inplace_out)
 {
     using namespace naja;
     if (out) *out = nullptr;
@@ -292,7 +300,14 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_v
         NAJA_HIP(hipMemcpyAsync(pl->eoff_d, pl->eoff.data(), sizeof(int64_t) * pl->eoff.size(),
                                 hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(pl->ek_d, pl->ek.data(), sizeof(int) * pl->ek.size(), hipMemcpyHostToDevice, s));
-        if (P) alloc_async(&pl->zbuf, (size_t)n_variants * p->nchains * p->nspc * d, s);
+        // A one-shot call may sample z straight into its output and backmap it
+        // there (the small-operand backmap reads each tile whole before
+        // writing it back); otherwise z goes to a buffer of the plan's own.
+        const int64_t per = (int64_t)p->nchains * p->nspc;
+        if (P && inplace_out && backmap_inplace_ok(d, inplace_out, per * d, n_variants))
+            pl->inplace = inplace_out;
+        else if (P)
+            alloc_async(&pl->zbuf, (size_t)n_variants * per * d, s);
         ChrShape sh = chr_choose_shape(pl->m_max, pl->G, p->tpb, (int)pl->items.size());
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -311,6 +326,19 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_v
     *out = pl;
     return NAJA_OK;
 }
+
+}  // namespace
+}  // namespace naja
+
+extern "C" {
+
+int naja_chr_plan_create(const float* A, const double* b, int m0, int d, int n_variants, const int* extra_rows,
+                         const float* A_extra, const double* b_extra, const float* P, const float* c,
+                         const naja_chr_params* p, void* stream, naja_chr_plan** out)
+{
+    return naja::plan_create(A, b, m0, d, n_variants, extra_rows, A_extra, b_extra, P, c, p, stream, out, nullptr);
+}
+
 
 int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, float* y_final, void* stream)
 {
@@ -348,7 +376,7 @@ int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, i
         float best_ms = 1e30f;
         for (const ChrShape& sh : cand) {
             repack(pl, sh);
-            const ChrArgs a = make_args(pl, y0, pl.P ? pl.zbuf : probe, nullptr, nsteps);
+            const ChrArgs a = make_args(pl, y0, (pl.P && pl.zbuf) ? pl.zbuf : probe, nullptr, nsteps);
             chr_launch(a, (int)pl.items.size(), sh, s);  // warm-up
             NAJA_HIP(hipEventRecord(e0, s));
             chr_launch(a, (int)pl.items.size(), sh, s);
@@ -414,7 +442,7 @@ int naja_chr_sample_batched_rounded(const float* A, const double* b, int m0, int
         NAJA_REQUIRE(y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL y0 or samples");
     });
     if (rc) return rc;
-    rc = naja_chr_plan_create(A, b, m0, d, n_variants, extra_rows, A_extra, b_extra, P, c, p, stream, &pl);
+    rc = plan_create(A, b, m0, d, n_variants, extra_rows, A_extra, b_extra, P, c, p, stream, &pl, samples);
     if (rc) return rc;
     rc = naja_chr_plan_launch(pl, y0, samples, y_final, stream);
     const int rc2 = naja_chr_plan_destroy(pl);

@@ -40,10 +40,11 @@ namespace naja {
 namespace {
 
 // Threads per ESS workgroup: one per 16-point group of a radix-16 pass
-// (H >= 1024) or per 4-point group of a radix-4 pass (shorter transforms),
-// between 64 and 1024.
+// (H >= kRadix16MinH) or per 4-point group of a radix-4 pass (shorter
+// transforms), between 64 and 1024.
 constexpr int kDiagThreads = 1024;
-int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H >= 1024 ? H / 16 : H / 4)); }
+constexpr int kRadix16MinH = 512;  // radix-16 passes from here (3 LDS passes at 512 points instead of 5)
+int ess_threads(int H) { return std::max(64, std::min(kDiagThreads, H >= kRadix16MinH ? H / 16 : H / 4)); }
 constexpr int kMaxHalf = 16384;  // N/2 complex points held in LDS (136 KB with the padding)
 
 __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int nchains, int spc)
@@ -227,7 +228,7 @@ __device__ void dit16_pass(const PaddedZ& z, int H, int logq, int logN, const Tw
     }
 }
 
-// Radix-4 passes of the same scheme for short transforms (H < 1024, where
+// Radix-4 passes of the same scheme for short transforms (H < 512, where
 // H / 16 groups would leave most of a 64-thread workgroup idle): per block
 // of L = 4 q points, DFT4 of p0 + m q, output r scaled by W_L^(p0 r) and
 // stored at p0 + bitrev2(r) q (bitrev2 swaps outputs 1 and 2).
@@ -266,7 +267,6 @@ __device__ void dit4_pass(const PaddedZ& z, int H, int logq, int logN, const Twi
     }
 }
 
-constexpr int kRadix16MinH = 1024;  // radix-16 passes from here (H / 16 >= 64 groups)
 
 // The last DIF pass (first DIT pass) when logH is not a multiple of 4:
 // DFTs of 2^rem contiguous points, no twiddles; DIF stores output r at

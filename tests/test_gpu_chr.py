@@ -218,6 +218,38 @@ def test_rounded_batched_matches_oracle(gpu):
         assert viol <= 1e-4
 
 
+@pytest.mark.parametrize("d", [12, 10])
+def test_rounded_plan_equals_one_shot_call(gpu, d):
+    """A rounding-transform plan (z in the plan's own buffer, then the
+    backmap) and the one-shot rounded call (z sampled into the output and
+    backmapped in place when d % 4 == 0) give identical samples, on repeated
+    launches of the plan too."""
+    import naja_amd
+
+    t = gpu
+    A, b, _ = cube(d)
+    rng = np.random.default_rng(d)
+    ks = [0, 5, 9, 3]
+    EA = [rng.standard_normal((k, d)) for k in ks]
+    Eb = [np.ones(k) * 0.4 for k in ks]
+    nv, nch, nspc, thin = len(ks), 4, 700, 3
+    P = np.stack([np.eye(d) + 0.1 * rng.standard_normal((d, d)) for _ in ks]).astype(np.float32)
+    c = (0.02 * rng.standard_normal((nv, d))).astype(np.float32)
+    Ad, bd = t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda()
+    EAd = [t.from_numpy(e.astype(np.float32)).cuda() for e in EA]
+    Ebd = [t.from_numpy(f).cuda() for f in Eb]
+    Pd, cdv = t.from_numpy(P).cuda(), t.from_numpy(c).cuda()
+    Z0 = t.zeros((nv, nch, d), device="cuda")
+    one = naja_amd.coordinate_hit_and_run_batched(Ad, bd, EAd, Ebd, Z0, nspc, thin, nch, seed=3, pair_prob=0.2,
+                                                  P=Pd, c=cdv)
+    plan = naja_amd.ChrPlan(Ad, bd, nspc, thin, nch, extra_A=EAd, extra_b=Ebd, P=Pd, c=cdv, seed=3, pair_prob=0.2)
+    for _ in range(2):
+        out = plan.launch(Z0, plan.samples_like())
+        t.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), one.cpu().numpy())
+    plan.close()
+
+
 def test_plan_autotune_relaunch_and_graph_replay(gpu):
     """A plan launches without allocating: relaunches and hipGraph replays
     (torch.cuda.CUDAGraph) reproduce the eager samples exactly, and the

@@ -108,10 +108,11 @@ def main():
 
     batch = VariantBatch(A0, b0, EAh, Ebh)
     batch2 = batch.repeat(2)
+    Fh_dev = torch.as_tensor(Fh, dtype=torch.float32, device="cuda")  # condition features, resident like the rows
 
     def learned_step(evs=None):
         P0 = cd.base_rounding(batch, None, None, None)
-        Pl = cd.learned_maps(lp, Fh, P0)
+        Pl = cd.learned_maps(lp, Fh_dev, P0)
         Yp = cd.sample_rounded(batch2, None, None, None, torch.cat([Pl, P0]), nch, 500, thin, seed + 1, resync)
         ep, rp = cd.variant_diagnostics(Yp, nch)
         del Yp

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -55,6 +56,10 @@ struct naja_chr_plan {
     float* inplace = nullptr; // transform plan sampling z into this output and backmapping it in place
     int64_t pack_floats = 0;  // capacity of the pack buffers (floats)
     int64_t pack_rows = 0;    // capacity of b buffers (rows)
+    // The tables and the pack buffers are each carved out of one scratch
+    // allocation (every pool call costs tens of microseconds of host time).
+    char* tables_mem = nullptr;
+    char* pack_mem = nullptr;
 };
 
 namespace naja {
@@ -71,6 +76,19 @@ TuneKey tune_key(const naja_chr_plan& pl)
 }
 
 void free_async(void* ptr, hipStream_t s) { scratch_free(ptr, s); }
+
+// Byte offsets of consecutive sub-buffers of one allocation, each 256-byte
+// aligned; returns the total size.
+size_t carve(std::initializer_list<size_t> bytes, size_t* off)
+{
+    size_t total = 0;
+    int q = 0;
+    for (size_t b : bytes) {
+        off[q++] = total;
+        total += (b + 255) / 256 * 256;
+    }
+    return total;
+}
 
 template <typename T>
 void alloc_async(T** ptr, size_t count, hipStream_t s)
@@ -143,20 +161,17 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         b_off += lda;
     }
     if (a_off > pl.pack_floats || b_off > pl.pack_rows) {
-        free_async(pl.A_pack, s);
-        free_async(pl.Ainv_pack, s);
-        free_async(pl.b_pack, s);
-        free_async(pl.A_raw, s);
-        free_async(pl.b_raw, s);
-        pl.A_pack = pl.Ainv_pack = pl.A_raw = nullptr;
-        pl.b_pack = pl.b_raw = nullptr;
-        alloc_async(&pl.A_pack, (size_t)a_off, s);
-        alloc_async(&pl.Ainv_pack, (size_t)a_off, s);
-        alloc_async(&pl.b_pack, (size_t)b_off, s);
-        if (pl.P) {
-            alloc_async(&pl.A_raw, (size_t)a_off, s);
-            alloc_async(&pl.b_raw, (size_t)b_off, s);
-        }
+        free_async(pl.pack_mem, s);
+        pl.pack_mem = nullptr;
+        const size_t af = sizeof(float) * (size_t)a_off, bd = sizeof(double) * (size_t)b_off;
+        size_t off[5];
+        const size_t total = carve({af, af, bd, pl.P ? af : 0, pl.P ? bd : 0}, off);
+        alloc_async(&pl.pack_mem, total, s);
+        pl.A_pack = reinterpret_cast<float*>(pl.pack_mem + off[0]);
+        pl.Ainv_pack = reinterpret_cast<float*>(pl.pack_mem + off[1]);
+        pl.b_pack = reinterpret_cast<double*>(pl.pack_mem + off[2]);
+        pl.A_raw = pl.P ? reinterpret_cast<float*>(pl.pack_mem + off[3]) : nullptr;
+        pl.b_raw = pl.P ? reinterpret_cast<double*>(pl.pack_mem + off[4]) : nullptr;
         pl.pack_floats = a_off;
         pl.pack_rows = b_off;
     }
@@ -291,10 +306,15 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
             for (int c0 = 0; c0 < p->nchains; c0 += pl->G) logical.push_back(ChrItem{v, c0});
         pl->items = xcd_order(logical);
         const hipStream_t s = pl->stream;
-        alloc_async(&pl->var_d, pl->var.size(), s);
-        alloc_async(&pl->items_d, pl->items.size(), s);
-        alloc_async(&pl->eoff_d, pl->eoff.size(), s);
-        alloc_async(&pl->ek_d, pl->ek.size(), s);
+        size_t off[4];
+        const size_t total = carve({sizeof(ChrVariant) * pl->var.size(), sizeof(ChrItem) * pl->items.size(),
+                                    sizeof(int64_t) * pl->eoff.size(), sizeof(int) * pl->ek.size()},
+                                   off);
+        alloc_async(&pl->tables_mem, total, s);
+        pl->var_d = reinterpret_cast<ChrVariant*>(pl->tables_mem + off[0]);
+        pl->items_d = reinterpret_cast<ChrItem*>(pl->tables_mem + off[1]);
+        pl->eoff_d = reinterpret_cast<int64_t*>(pl->tables_mem + off[2]);
+        pl->ek_d = reinterpret_cast<int*>(pl->tables_mem + off[3]);
         NAJA_HIP(hipMemcpyAsync(pl->items_d, pl->items.data(), sizeof(ChrItem) * pl->items.size(),
                                 hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(pl->eoff_d, pl->eoff.data(), sizeof(int64_t) * pl->eoff.size(),
@@ -423,8 +443,7 @@ int naja_chr_plan_destroy(naja_chr_plan* plan)
     if (!plan) return NAJA_OK;
     const int rc = guarded([&] {
         const hipStream_t s = plan->stream;
-        void* bufs[] = {plan->var_d,     plan->items_d, plan->eoff_d, plan->ek_d,  plan->A_pack,
-                        plan->Ainv_pack, plan->b_pack,  plan->A_raw,  plan->b_raw, plan->zbuf};
+        void* bufs[] = {plan->tables_mem, plan->pack_mem, plan->zbuf};
         for (void* ptr : bufs) free_async(ptr, s);
     });
     delete plan;

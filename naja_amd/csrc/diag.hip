@@ -753,11 +753,20 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     const int64_t t_blocks = (int64_t)((s_tiles + 7) / 8) * 8 * nchains * d_tiles * nv;
     NAJA_REQUIRE(t_blocks < (int64_t)1 << 31 && n_series < (int64_t)1 << 31, NAJA_ERR_UNSUPPORTED,
                  "too many series for one diagnostics launch");
-    scratch_alloc((void**)&ser, sizeof(float) * (size_t)n_series * spcp, stream);
-    scratch_alloc((void**)&tw, sizeof(float2) * tw_h.size(), stream);
+    // One scratch allocation for the series, the twiddle tables, the moments
+    // and (when the caller passes none) the per-chain ESS: each pool call
+    // costs tens of microseconds of host time.
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t b_ser = up(sizeof(float) * (size_t)n_series * spcp), b_tw = up(sizeof(float2) * tw_h.size());
+    const size_t b_mom = up(sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv);
+    const size_t b_ec = ec ? 0 : up(sizeof(float) * (size_t)dim * nchains * nv);
+    char* mem = nullptr;
+    scratch_alloc((void**)&mem, b_ser + b_tw + b_mom + b_ec, stream);
+    ser = reinterpret_cast<float*>(mem);
+    tw = reinterpret_cast<float2*>(mem + b_ser);
+    mom = reinterpret_cast<double*>(mem + b_ser + b_tw);
+    if (!ec) ec = reinterpret_cast<float*>(mem + b_ser + b_tw + b_mom);
     NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
-    if (!ec) scratch_alloc((void**)&ec, sizeof(float) * (size_t)dim * nchains * nv, stream);
-    scratch_alloc((void**)&mom, sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv, stream);
 
     hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)t_blocks), dim3(256), 0, stream, samples, dim, nchains,
                        spc, spcp, layout, vstride, s_tiles, d_tiles, ser);
@@ -788,10 +797,7 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
         NAJA_HIP(hipGetLastError());
     }
 
-    scratch_free(ser, stream);
-    scratch_free(tw, stream);
-    scratch_free(mom, stream);
-    if (ec != ess_chain) scratch_free(ec, stream);
+    scratch_free(mem, stream);
 }
 
 }  // namespace naja

@@ -851,12 +851,14 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
                      "naja_precond_ellipsoid_device: d=%d with %d rows exceeds the LDS budget (use the host routine)",
                      d, m0 + kmax);
         hipStream_t s = (hipStream_t)stream;
-        int64_t* eoff = nullptr;
-        int* ek = nullptr;
-        int* status = nullptr;
-        scratch_alloc((void**)&eoff, sizeof(int64_t) * n_variants, s);
-        scratch_alloc((void**)&ek, sizeof(int) * n_variants, s);
-        scratch_alloc((void**)&status, sizeof(int) * n_variants, s);
+        // Offsets, row counts and per-variant status in one scratch allocation.
+        char* mem = nullptr;
+        const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
+        const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
+        scratch_alloc((void**)&mem, b_off + 2 * b_k, s);
+        int64_t* eoff = reinterpret_cast<int64_t*>(mem);
+        int* ek = reinterpret_cast<int*>(mem + b_off);
+        int* status = reinterpret_cast<int*>(mem + b_off + b_k);
         NAJA_HIP(hipMemcpyAsync(eoff, off.data(), sizeof(int64_t) * n_variants, hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(ek, extra_rows, sizeof(int) * n_variants, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(mlp::ellipsoid_kernel, dim3(n_variants), dim3(256), lds, s, A0, b0, m0, d, extra_A, extra_b,
@@ -865,9 +867,7 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
-        scratch_free(eoff, s);
-        scratch_free(ek, s);
-        scratch_free(status, s);
+        scratch_free(mem, s);
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_INVALID_ARGUMENT,
                          "Dikin ellipsoid of variant %d: the centre is not interior or the barrier Hessian is singular",

@@ -52,9 +52,11 @@ __device__ __forceinline__ int64_t sample_row(int layout, int s, int c, int ncha
     return layout == 0 ? (int64_t)s * nchains + c : (int64_t)c * spc + s;
 }
 
+// Complex product with each component one multiply and one fused
+// multiply-add (the library is built without FMA contraction elsewhere).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 
 // Twiddles W^k = exp(-2 pi i k / N), N = 2H, from two small LDS tables:
@@ -192,6 +194,18 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2 w4, float2 (&w)
 // points (q = 2^logq): per block offset p0 < q, the DFT16 of the points
 // p0 + m q, output r scaled by W_L^(p0 r) and stored at p0 + bitrev4(r) q.
 // Passes at L = H, H/16, ... leave the spectrum in bit-reversed order.
+// The group of block offset p0 (points base + m q, m < 16, already in x):
+// DFT16, twiddles W_L^(p0 r) (W_L^j = W_N^(j << sh)), bit-reversed stores.
+__device__ __forceinline__ void dif16_group(const PaddedZ& z, float2 (&x)[16], int base, int q, int p0, int sh,
+                                            const Twiddles& tw)
+{
+    float2 w[16];
+    dft16<false>(x);
+    twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) z[base + bitrev4(r) * q] = cmul(x[4 * (r & 3) + (r >> 2)], w[r]);
+}
+
 __device__ void dif16_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
 {
     const int q = 1 << logq;
@@ -199,13 +213,10 @@ __device__ void dif16_pass(const PaddedZ& z, int H, int logq, int logN, const Tw
     for (int g = threadIdx.x; g < H / 16; g += blockDim.x) {
         const int p0 = g & (q - 1);
         const int base = ((g >> logq) << (logq + 4)) + p0;
-        float2 x[16], w[16];
+        float2 x[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) x[m] = z[base + m * q];
-        dft16<false>(x);
-        twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) z[base + bitrev4(r) * q] = cmul(x[4 * (r & 3) + (r >> 2)], w[r]);
+        dif16_group(z, x, base, q, p0, sh, tw);
     }
 }
 
@@ -315,11 +326,17 @@ __device__ void tail_pass(const PaddedZ& z, int H, int rem)
     }
 }
 
+// Points of the last DIF pass (first DIT pass) that the radix-16 or radix-4
+// passes leave over: 2^tail_bits contiguous points.
+__device__ __forceinline__ int tail_bits(int H, int logH) { return H >= kRadix16MinH ? (logH & 3) : (logH & 1); }
+
 // In-place FFT over H points, decimation in frequency (natural in,
 // bit-reversed out): radix-16 passes at block sizes H, H/16, ... while
 // >= 16, then one radix-2/4/8 pass. Each pass holds 16 (or 2^rem) points
 // per thread in registers, so an 8192-point transform takes 4 LDS passes.
-__device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
+// first_done: the caller ran the first radix-16 pass; skip_tail: the caller
+// folds a radix-2 last pass into the split that follows.
+__device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw, bool first_done, bool skip_tail)
 {
     const int logN = logH + 1;
     int logL = logH;
@@ -328,30 +345,32 @@ __device__ void fft_dif(const PaddedZ& z, int H, int logH, const Twiddles& tw)
             dif4_pass(z, H, logL - 2, logN, tw);
             __syncthreads();
         }
-        if (logL == 1) {
+        if (logL == 1 && !skip_tail) {
             tail_pass<false>(z, H, 1);
             __syncthreads();
         }
         return;
     }
+    if (first_done) logL -= 4;
     for (; logL >= 4; logL -= 4) {
         dif16_pass(z, H, logL - 4, logN, tw);
         __syncthreads();
     }
-    if (logL > 0) {
+    if (logL > 0 && !skip_tail) {
         tail_pass<false>(z, H, logL);
         __syncthreads();
     }
 }
 
 // Unnormalised inverse of fft_dif's output order (bit-reversed in, natural
-// out): the passes of fft_dif undone in reverse order.
-__device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw)
+// out): the passes of fft_dif undone in reverse order (skip_head: the
+// caller ran the radix-2 first pass).
+__device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw, bool skip_head)
 {
     const int logN = logH + 1;
     if (H < kRadix16MinH) {
         const int rem2 = logH & 1;
-        if (rem2) {
+        if (rem2 && !skip_head) {
             tail_pass<true>(z, H, 1);
             __syncthreads();
         }
@@ -362,7 +381,7 @@ __device__ void ifft_dit(const PaddedZ& z, int H, int logH, const Twiddles& tw)
         return;
     }
     const int rem = logH & 3;
-    if (rem > 0) {
+    if (rem > 0 && !skip_head) {
         tail_pass<true>(z, H, rem);
         __syncthreads();
     }
@@ -431,6 +450,27 @@ __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __re
         out[(((int64_t)v * nchains + c) * dim + d0 + d) * spcp + ss] = tile[s * (kTd + 1) + d];
 }
 
+// Split of the packed spectrum for the pair {k, H-k}: Z_k = zk, Z_{H-k} = zb,
+// wk = W^k, wb = W^(H-k) (W^H = -1 for k = 0). X_k = E_k + W^k O_k with
+// E = (Z_k + conj Z_{H-k})/2, O = (Z_k - conj Z_{H-k})/(2i), P = |X|^2, and
+// the re-packed Z'_k = A_k + i B_k, A_k = P_k + P_{H-k},
+// B_k = (P_k - P_{H-k}) e^{+2 pi i k/N} = (P_k - P_{H-k}) conj(W^k); likewise
+// Z'_{H-k}.
+__device__ __forceinline__ void split_pair(float2 zk, float2 zb, float2 wk, float2 wb, float2& ok, float2& ob)
+{
+    const float2 Ek = make_float2(0.5f * (zk.x + zb.x), 0.5f * (zk.y - zb.y));
+    const float2 Ok = make_float2(0.5f * (zk.y + zb.y), -0.5f * (zk.x - zb.x));
+    const float2 Eb = make_float2(0.5f * (zb.x + zk.x), 0.5f * (zb.y - zk.y));
+    const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
+    const float2 Xk = cadd(Ek, cmul(wk, Ok));
+    const float2 Xb = cadd(Eb, cmul(wb, Ob));
+    const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
+    const float Pb = Xb.x * Xb.x + Xb.y * Xb.y;
+    const float Dk = Pk - Pb, Db = Pb - Pk;
+    ok = make_float2((Pk + Pb) + Dk * wk.y, Dk * wk.x);
+    ob = make_float2((Pb + Pk) + Db * wb.y, Db * wb.x);
+}
+
 // ESS of one contiguous series per workgroup (blockIdx.x = (v * nchains + c) * dim + d).
 // mode 0: the full transform (N >= 2n, every lag). mode 1: a shorter
 // transform, N < 2n: zero padding to N keeps the correlation exact for lags
@@ -487,7 +527,12 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         // (sum and sum of squares of samples [0, half) and [half, 2 half) in FP64,
         // mcmc.cpp:150-165), written for the R-hat pass by the first ESS pass.
         // A thread owns at most kPerThread points (H <= 16 * blockDim); their
-        // loads are all issued before any of them is used.
+        // loads are all issued before any of them is used. With exactly 16
+        // (radix-16 passes, H = 16 * blockDim) thread tid holds the points
+        // tid + m H/16, which is its group of the first radix-16 pass: the
+        // points stay in registers, are centred there and go through that pass
+        // before anything is written to LDS.
+        const bool regs_first = H >= kRadix16MinH && 16 * (int)blockDim.x == H;
         double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // total, half-0 sum, sq, half-1 sum, sq
         constexpr int kPerThread = 16;
         const int nq = (H + (int)blockDim.x - 1) / (int)blockDim.x;  // <= kPerThread, uniform
@@ -521,7 +566,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
                 part[1 + 2 * h] += d1;
                 part[2 + 2 * h] += d1 * d1;
             }
-            z[j] = make_float2(x0, x1);
+            if (!regs_first) z[j] = make_float2(x0, x1);
         }
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
@@ -542,58 +587,86 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         __syncthreads();
         const float meanf = meanf_s;
         // Centre in float; the padding stays zero.
-        for (int j = tid; j < H; j += blockDim.x) {
-            const int s0 = 2 * j, s1 = 2 * j + 1;
-            float2 p = z[j];
-            if (s0 < n) p.x -= meanf;
-            if (s1 < n) p.y -= meanf;
-            z[j] = p;
+        if (regs_first) {
+            float2 xr[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int j = tid + m * (int)blockDim.x;
+                xr[m] = xv[m];
+                if (2 * j < n) xr[m].x -= meanf;
+                if (2 * j + 1 < n) xr[m].y -= meanf;
+            }
+            // first pass: L = H, q = H/16, p0 = base = tid, W_L^j = W_N^(2j)
+            dif16_group(z, xr, tid, H / 16, tid, 1, tw);
+        } else {
+            for (int j = tid; j < H; j += blockDim.x) {
+                const int s0 = 2 * j, s1 = 2 * j + 1;
+                float2 p = z[j];
+                if (s0 < n) p.x -= meanf;
+                if (s1 < n) p.y -= meanf;
+                z[j] = p;
+            }
         }
         __syncthreads();
-        fft_dif(z, H, logH, tw);  // Z_k now sits at z[bitrev(k)]
+        // A radix-2 last pass is folded into the split below.
+        const bool fuse2 = tail_bits(H, logH) == 1;
+        fft_dif(z, H, logH, tw, regs_first, fuse2);  // Z_k now sits at z[bitrev(k)] (after the radix-2 stage)
 
         // Split the real spectrum, form P = |X|^2 and re-pack it for the
-        // half-size inverse: A_k = P_k + P_{H-k}, B_k = (P_k - P_{H-k}) e^{+2 pi i k/N},
-        // Z'_k = A_k + i B_k, written back to z[bitrev(k)] (the slot it came
-        // from, which is where the DIT inverse wants it). Each thread owns the
-        // pair {k, H-k}, so no other thread touches those two slots. Threads
-        // walk the slots rather than k, so that the z[pk] accesses of a
-        // wavefront are adjacent: the even slots hold every k < H/2 (the top bit
-        // of k is the low bit of its slot) and slot 1 holds k = H/2.
-        for (int i = tid; i <= H / 2; i += blockDim.x) {
-            const int pk = (i < H / 2) ? 2 * i : 1;
-            const int k = bitrev(pk, logH);
-            const int kb = (H - k) & (H - 1);
-            const int pb = bitrev(kb, logH);
-            const float2 zk = z[pk];
-            const float2 zb = z[pb];
-            // E = (Z_k + conj Z_{H-k})/2, O = (Z_k - conj Z_{H-k})/(2i)
-            const float2 Ek = make_float2(0.5f * (zk.x + zb.x), 0.5f * (zk.y - zb.y));
-            const float2 Ok = make_float2(0.5f * (zk.y + zb.y), -0.5f * (zk.x - zb.x));
-            const float2 Eb = make_float2(0.5f * (zb.x + zk.x), 0.5f * (zb.y - zk.y));
-            const float2 Ob = make_float2(0.5f * (zb.y + zk.y), -0.5f * (zb.x - zk.x));
-            // X_k = E_k + W^k O_k, W = e^{-2 pi i/N}; the partner of k = 0 is
-            // X_H = E_0 + W^H O_0 with W^H = -1.
-            const float2 wk = tw.at(k);
-            const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw.at(H - k);
-            const float2 Xk = make_float2(Ek.x + (wk.x * Ok.x - wk.y * Ok.y), Ek.y + (wk.x * Ok.y + wk.y * Ok.x));
-            const float2 Xb = make_float2(Eb.x + (wb.x * Ob.x - wb.y * Ob.y), Eb.y + (wb.x * Ob.y + wb.y * Ob.x));
-            const float Pk = Xk.x * Xk.x + Xk.y * Xk.y;
-            const float Pb = Xb.x * Xb.x + Xb.y * Xb.y;
-            {
-                const float Ak = Pk + Pb;
-                const float Dk = Pk - Pb;
-                // e^{+2 pi i k/N} = conj(tw[k])
-                z[pk] = make_float2(Ak + Dk * wk.y, Dk * wk.x);
+        // half-size inverse (split_pair), written back to z[bitrev(k)] (the
+        // slot it came from, which is where the DIT inverse wants it).
+        if (fuse2) {
+            // The DIF radix-2 last pass, the split and the DIT radix-2 first
+            // pass in one sweep. Item k (0 <= k <= H/4) owns the slot pairs
+            // a = bitrev(k)/2 and b = bitrev(H/2 - k)/2: after the radix-2
+            // stage on them, slots 2a, 2a+1, 2b, 2b+1 hold Z_k, Z_{k+H/2},
+            // Z_{H/2-k} and Z_{H-k}, two split pairs {k, H-k} and
+            // {H/2+k, H/2-k} (k = 0: b = a with pairs {0} and {H/2};
+            // k = H/4: b = a with the one pair {H/4, 3H/4}). The twiddles
+            // come from W^k: W^(H-k) = -conj W^k, W^(H/2+k) = -i W^k,
+            // W^(H/2-k) = -i conj W^k.
+            for (int k = tid; k <= H / 4; k += blockDim.x) {
+                const int a = bitrev(k, logH) >> 1;
+                const int b = bitrev((H / 2 - k) & (H - 1), logH) >> 1;
+                const float2 ua = z[2 * a], va = z[2 * a + 1], ub = z[2 * b], vb = z[2 * b + 1];
+                const float2 Zk = cadd(ua, va), Zkh = csub(ua, va), Zm = cadd(ub, vb), Zmh = csub(ub, vb);
+                const float2 w = tw.at(k);
+                const float2 wmc = make_float2(-w.x, w.y);   // W^(H-k)
+                const float2 wh = make_float2(w.y, -w.x);    // W^(H/2+k)
+                const float2 whc = make_float2(-w.y, -w.x);  // W^(H/2-k)
+                float2 o_k, o_mh, o_kh, o_m;  // Z' of k, H-k, H/2+k, H/2-k
+                split_pair(Zk, k == 0 ? Zk : Zmh, w, wmc, o_k, o_mh);
+                split_pair(Zkh, k == 0 ? Zkh : Zm, wh, whc, o_kh, o_m);
+                // DIT radix-2 first pass on each slot pair.
+                z[2 * a] = cadd(o_k, o_kh);
+                z[2 * a + 1] = csub(o_k, o_kh);
+                if (b != a) {
+                    z[2 * b] = cadd(o_m, o_mh);
+                    z[2 * b + 1] = csub(o_m, o_mh);
+                }
             }
-            if (k != 0 && kb != k) {
-                const float Ab = Pb + Pk;
-                const float Db = Pb - Pk;
-                z[pb] = make_float2(Ab + Db * wb.y, Db * wb.x);
+        } else {
+            // Each thread owns the pair {k, H-k}, so no other thread touches
+            // those two slots. Threads walk the slots rather than k, so that
+            // the z[pk] accesses of a wavefront are adjacent: the even slots
+            // hold every k < H/2 (the top bit of k is the low bit of its slot)
+            // and slot 1 holds k = H/2.
+            for (int i = tid; i <= H / 2; i += blockDim.x) {
+                const int pk = (i < H / 2) ? 2 * i : 1;
+                const int k = bitrev(pk, logH);
+                const int kb = (H - k) & (H - 1);
+                const int pb = bitrev(kb, logH);
+                // The partner of k = 0 is X_H, with W^H = -1.
+                const float2 wk = tw.at(k);
+                const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : tw.at(H - k);
+                float2 ok, ob;
+                split_pair(z[pk], z[pb], wk, wb, ok, ob);
+                z[pk] = ok;
+                if (k != 0 && kb != k) z[pb] = ob;
             }
         }
         __syncthreads();
-        ifft_dit(z, H, logH, tw);
+        ifft_dit(z, H, logH, tw, fuse2);
 
         // acov[k] = raw[k] / (N n); raw is the interleaved real view of z.
         const float norm = 1.0f / ((float)N * (float)n);

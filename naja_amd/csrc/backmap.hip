@@ -143,7 +143,8 @@ __global__ __launch_bounds__(256) void backmap_mfma_kernel(const float* __restri
 // stride S >= d has S % 8 == 4: the 8 lanes of a 16-byte read phase then
 // hit distinct bank groups. The next tile's rows are loaded into registers
 // while the current tile's MFMAs run; results go back through LDS so every
-// tile's output block V[s0 .. s0 + 64) is written as one contiguous run.
+// tile's output block V[s0 .. s0 + 64) is written as one contiguous run. A K
+// tail (d % 16 != 0) takes one MFMA per 4 k instead of padding K to 16.
 // ---------------------------------------------------------------------------
 constexpr int kSmRows = 64;
 constexpr int kSmIB = kSmRows / 64;  // 16-row blocks per wave (four waves per tile)
@@ -232,32 +233,13 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
         const int g = lane >> 4, r16 = lane & 15;
         const float* ya = Ys + (16 * kSmIB * wave + r16) * S + 4 * g;
         const float* tb = Ts + r16 * S + 4 * g;
-        for (int kk = 0; kk < d; kk += 16) {
-            const bool full = kk + 16 <= d;
+        int kk = 0;
+        for (; kk + 16 <= d; kk += 16) {
             float4 a[kSmIB], b[NB];
 #pragma unroll
             for (int ib = 0; ib < kSmIB; ++ib) a[ib] = *reinterpret_cast<const float4*>(ya + ib * 16 * S + kk);
 #pragma unroll
             for (int jb = 0; jb < NB; ++jb) b[jb] = *reinterpret_cast<const float4*>(tb + jb * 16 * S + kk);
-            if (!full) {
-                // k = kk + 4 g + st past d: both operands zero (whatever the
-                // LDS holds there).
-                const int k0 = kk + 4 * g;
-#pragma unroll
-                for (int ib = 0; ib < kSmIB; ++ib) {
-                    if (k0 + 0 >= d) a[ib].x = 0.0f;
-                    if (k0 + 1 >= d) a[ib].y = 0.0f;
-                    if (k0 + 2 >= d) a[ib].z = 0.0f;
-                    if (k0 + 3 >= d) a[ib].w = 0.0f;
-                }
-#pragma unroll
-                for (int jb = 0; jb < NB; ++jb) {
-                    if (k0 + 0 >= d) b[jb].x = 0.0f;
-                    if (k0 + 1 >= d) b[jb].y = 0.0f;
-                    if (k0 + 2 >= d) b[jb].z = 0.0f;
-                    if (k0 + 3 >= d) b[jb].w = 0.0f;
-                }
-            }
 #pragma unroll
             for (int ib = 0; ib < kSmIB; ++ib)
 #pragma unroll
@@ -267,6 +249,20 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
                     acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].z, b[jb].z, acc[ib][jb], 0, 0, 0);
                     acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib].w, b[jb].w, acc[ib][jb], 0, 0, 0);
                 }
+        }
+        // K tail (d % 16 = 4, 8 or 12; d % 4 = 0): one MFMA per 4 k, lane
+        // group g taking k = kk + g, so no MFMA multiplies padding.
+        for (; kk < d; kk += 4) {
+            float a[kSmIB], b[NB];
+#pragma unroll
+            for (int ib = 0; ib < kSmIB; ++ib) a[ib] = ya[ib * 16 * S + kk - 3 * g];
+#pragma unroll
+            for (int jb = 0; jb < NB; ++jb) b[jb] = tb[jb * 16 * S + kk - 3 * g];
+#pragma unroll
+            for (int ib = 0; ib < kSmIB; ++ib)
+#pragma unroll
+                for (int jb = 0; jb < NB; ++jb)
+                    acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[ib], b[jb], acc[ib][jb], 0, 0, 0);
         }
         __syncthreads();  // every wave is done reading Ys
         // D layout of a 16 x 16 block: lane holds column lane % 16, register

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "chr_kernel.h"
 #include "philox.h"
@@ -407,7 +408,13 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // t + PF into (na, nb). The register sets rotate between consecutive
     // steps (the loop below is unrolled by PF + 1), so the prefetched columns
     // are never copied and their loads stay in flight across PF steps.
-    auto step = [&](int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+    // Move logging (a test hook) is decided once per workgroup; the per-step
+    // test below is then one scalar branch.
+    const bool log_on = a.move_log != nullptr && item.variant == 0 && item.c0 == 0;
+    // PF_ALL: every step of the caller's round has a step PF ahead to prefetch
+    // for (the main loop), so the per-step bound test is dropped there.
+    auto step = [&](auto pf_all, int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+        constexpr bool kPfAll = decltype(pf_all)::value;
         const Move mv = fm[0];
         const int mtype = CO ? kMoveCoord : mv.type;
         if (until_resync == 0) {
@@ -444,11 +451,13 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : 0x7FFFFFFF;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
-            const int64_t tl = t - t_begin;
-            a.move_log[tl * 3 + 0] = mtype;
-            a.move_log[tl * 3 + 1] = mv.i;
-            a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
+        if (log_on) {
+            if ((int)(t - t_begin) < (int)a.nlog && tid == 0) {
+                const int64_t tl = t - t_begin;
+                a.move_log[tl * 3 + 0] = mtype;
+                a.move_log[tl * 3 + 1] = mv.i;
+                a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
+            }
         }
 
         // Alpha uniforms of step t, and the draw of step t + PF (a new batch
@@ -463,7 +472,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) nu[g] = 0.0f;
         // Launches run fewer than 2^31 steps: 32-bit differences compare on
         // the scalar unit.
-        const bool has_pf = (int)(t_end - t) > PF;
+        const bool has_pf = kPfAll || (int)(t_end - t) > PF;
         if (has_pf) {
             const int Lp = (int)((t + PF) & 63);
             if (Lp == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + PF), group, item.c0, mb);
@@ -648,15 +657,27 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     };
     // Step t + j of a round computes on set j and prefetches into set
     // (j + PF) % (PF + 1), the one the step before it has finished with.
+    // Rounds whose every step t + j (j <= PF) has t + j + PF < t_end, then
+    // at most one checked round, then the last steps.
     int64_t t = t_begin;
-    for (; t + PF < t_end; t += PF + 1) {
+    for (; t + 2 * PF < t_end; t += PF + 1) {
 #pragma unroll
         for (int j = 0; j <= PF; ++j)
-            step(t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0], cs[(j + PF) % (PF + 1)][1]);
+            step(std::true_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+                 cs[(j + PF) % (PF + 1)][1]);
+    }
+    if (t + PF < t_end) {
+#pragma unroll
+        for (int j = 0; j <= PF; ++j)
+            step(std::false_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+                 cs[(j + PF) % (PF + 1)][1]);
+        t += PF + 1;
     }
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        if (t + j < t_end) step(t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0], cs[(j + PF) % (PF + 1)][1]);
+        if (t + j < t_end)
+            step(std::false_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+                 cs[(j + PF) % (PF + 1)][1]);
 
     __syncthreads();
     if (a.y_final) {
@@ -730,6 +751,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
     int until_resync = 0;
     int64_t sample_idx = 0;
 
+    const bool log_on = a.move_log != nullptr && item.variant == 0 && item.c0 == 0;
     auto step = [&](int64_t t, float (&X)[R], float (&Y)[R]) {
         if (until_resync == 0) {
             __syncthreads();
@@ -745,11 +767,13 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             until_resync = a.resync > 0 ? a.resync : 0x7FFFFFFF;
         }
 
-        if (a.move_log != nullptr && item.variant == 0 && item.c0 == 0 && (int)(t - t_begin) < (int)a.nlog && tid == 0) {
-            const int64_t tl = t - t_begin;
-            a.move_log[tl * 3 + 0] = mv.type;
-            a.move_log[tl * 3 + 1] = mv.i;
-            a.move_log[tl * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+        if (log_on) {
+            if ((int)(t - t_begin) < (int)a.nlog && tid == 0) {
+                const int64_t tl = t - t_begin;
+                a.move_log[tl * 3 + 0] = mv.type;
+                a.move_log[tl * 3 + 1] = mv.i;
+                a.move_log[tl * 3 + 2] = (mv.type == kMovePair) ? mv.j : -1;
+            }
         }
 
         const int L = (int)(t & 63);

@@ -413,7 +413,10 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const bool log_on = a.move_log != nullptr && item.variant == 0 && item.c0 == 0;
     // PF_ALL: every step of the caller's round has a step PF ahead to prefetch
     // for (the main loop), so the per-step bound test is dropped there.
-    auto step = [&](auto pf_all, int64_t t, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
+    // Steps are counted from t_begin in 32 bits (a launch runs fewer than
+    // 2^31); the 64-bit step number is formed only for a new draw batch.
+    const int nst = (int)(t_end - t_begin);
+    auto step = [&](auto pf_all, int u, float (&ca)[R], float (&cb)[R], float (&na)[R], float (&nb)[R]) {
         constexpr bool kPfAll = decltype(pf_all)::value;
         const Move mv = fm[0];
         const int mtype = CO ? kMoveCoord : mv.type;
@@ -452,8 +455,8 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         }
 
         if (log_on) {
-            if ((int)(t - t_begin) < (int)a.nlog && tid == 0) {
-                const int64_t tl = t - t_begin;
+            if (u < (int)a.nlog && tid == 0) {
+                const int64_t tl = u;
                 a.move_log[tl * 3 + 0] = mtype;
                 a.move_log[tl * 3 + 1] = mv.i;
                 a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
@@ -472,10 +475,10 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) nu[g] = 0.0f;
         // Launches run fewer than 2^31 steps: 32-bit differences compare on
         // the scalar unit.
-        const bool has_pf = kPfAll || (int)(t_end - t) > PF;
+        const bool has_pf = kPfAll || nst - u > PF;
         if (has_pf) {
-            const int Lp = (int)((t + PF) & 63);
-            if (Lp == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t + PF), group, item.c0, mb);
+            const int Lp = (L0 + u + PF) & 63;
+            if (Lp == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
             nf = read_move<G, CO>(mb, Lp);
 #pragma unroll
             for (int g = 0; g < G; ++g) nu[g] = readlane_f32(mb.u[g], Lp);
@@ -499,7 +502,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             // Rare escape move: entry q's index and coefficient go through LDS.
             __syncthreads();
             if (tid < kq) {
-                ks_idx[tid] = ksparse_index(a, k0, k1, (uint64_t)t, group, tid);
+                ks_idx[tid] = ksparse_index(a, k0, k1, (uint64_t)(t_begin + u), group, tid);
                 ks_coef[tid] = ((mv.w3 >> tid) & 1u) ? -a.ks_mag : a.ks_mag;
             }
             __syncthreads();
@@ -657,26 +660,26 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     };
     // Step t + j of a round computes on set j and prefetches into set
     // (j + PF) % (PF + 1), the one the step before it has finished with.
-    // Rounds whose every step t + j (j <= PF) has t + j + PF < t_end, then
+    // Rounds whose every step u + j (j <= PF) has u + j + PF < nst, then
     // at most one checked round, then the last steps.
-    int64_t t = t_begin;
-    for (; t + 2 * PF < t_end; t += PF + 1) {
+    int u = 0;
+    for (; u + 2 * PF < nst; u += PF + 1) {
 #pragma unroll
         for (int j = 0; j <= PF; ++j)
-            step(std::true_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+            step(std::true_type{}, u + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
                  cs[(j + PF) % (PF + 1)][1]);
     }
-    if (t + PF < t_end) {
+    if (u + PF < nst) {
 #pragma unroll
         for (int j = 0; j <= PF; ++j)
-            step(std::false_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+            step(std::false_type{}, u + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
                  cs[(j + PF) % (PF + 1)][1]);
-        t += PF + 1;
+        u += PF + 1;
     }
 #pragma unroll
     for (int j = 0; j < PF; ++j)
-        if (t + j < t_end)
-            step(std::false_type{}, t + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
+        if (u + j < nst)
+            step(std::false_type{}, u + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
                  cs[(j + PF) % (PF + 1)][1]);
 
     __syncthreads();

@@ -400,9 +400,35 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     }
     int prev_i = -1, prev_j = -1;
     int buf = 0;
-    int until_save = 0;    // steps until the next saved sample (t % thin == 0)
-    int until_resync = 0;  // steps until the next FP64 slack computation
+    // Events by step offset u from t_begin: the sample of the state after
+    // step next_save (u % thin == 0) is stored at the start of step
+    // next_save + 1 (or after the last step), the FP64 slacks are recomputed
+    // at the start of step next_resync; each step tests only next_event, the
+    // earlier of the two starts.
+    int64_t next_save = 0, next_resync = 0;
+    int next_event = 0;
     int64_t sample_idx = 0;
+
+    // Stores the current states (the state after the last step) as sample
+    // sample_idx.
+    auto save_sample = [&]() {
+        const int64_t col = sample_idx * a.nchains + item.c0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (g < geff) {
+                float* dst = a.samples + V.sample_off + (col + g) * (int64_t)d;
+                // Streaming stores: the samples are not read again by this
+                // launch, so they should not displace the columns of A in L2.
+                for (int c = tid; c < d; c += T) {
+                    float v = xs[g * d + c];
+                    if (c == prev_i) v = last_i[g];
+                    if (c == prev_j) v = last_j[g];
+                    __builtin_nontemporal_store(v, dst + c);
+                }
+            }
+        }
+        ++sample_idx;
+    };
 
     // One CHR step on the columns (ca, cb), prefetching the columns of step
     // t + PF into (na, nb). The register sets rotate between consecutive
@@ -420,7 +446,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         constexpr bool kPfAll = decltype(pf_all)::value;
         const Move mv = fm[0];
         const int mtype = CO ? kMoveCoord : mv.type;
-        if (until_resync == 0) {
+        const bool event = u == next_event;
+        if (event && u == next_save + 1) {
+            save_sample();
+            next_save += a.thin;
+        }
+        if (event && u == next_resync) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
             __syncthreads();
             double acc[G][R];
@@ -450,8 +481,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
             prev_i = -1;
             prev_j = -1;
-            // No resync: a count the launch never reaches (fewer than 2^31 steps).
-            until_resync = a.resync > 0 ? a.resync : 0x7FFFFFFF;
+            // No resync: a step the launch never reaches (fewer than 2^31 steps).
+            next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
+        }
+        if (event) {
+            const int64_t e = next_resync < next_save + 1 ? next_resync : next_save + 1;
+            next_event = e < 0x7FFFFFFF ? (int)e : 0x7FFFFFFF;
         }
 
         if (log_on) {
@@ -624,28 +659,6 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             prev_j = -1;
         }
 
-        if (until_save == 0) {
-            const int64_t col = sample_idx * a.nchains + item.c0;
-#pragma unroll
-            for (int g = 0; g < G; ++g) {
-                if (g < geff) {
-                    float* dst = a.samples + V.sample_off + (col + g) * (int64_t)d;
-                    // Streaming stores: the samples are not read again by
-                    // this launch, so they should not displace the columns
-                    // of A in L2.
-                    for (int c = tid; c < d; c += T) {
-                        float v = xs[g * d + c];
-                        if (c == prev_i) v = last_i[g];
-                        if (c == prev_j) v = last_j[g];
-                        __builtin_nontemporal_store(v, dst + c);
-                    }
-                }
-            }
-            ++sample_idx;
-            until_save = a.thin;
-        }
-        --until_save;
-        --until_resync;
 
 #pragma unroll
         for (int k = 0; k + 1 < PF; ++k) {
@@ -681,6 +694,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         if (u + j < nst)
             step(std::false_type{}, u + j, cs[j][0], cs[j][1], cs[(j + PF) % (PF + 1)][0],
                  cs[(j + PF) % (PF + 1)][1]);
+    if ((int64_t)nst == next_save + 1) save_sample();  // the last step's sample
 
     __syncthreads();
     if (a.y_final) {

@@ -328,13 +328,19 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const uint64_t a_base = (uint64_t)(a.A + V.a_off);
     const uint64_t inv_base = (uint64_t)(a.Ainv + V.a_off);
     const uint32_t col_stride = 4u * (uint32_t)lda;  // bytes from one column to the next
-    auto col_rsrc = [&](uint64_t base, uint32_t off) {
+    // Each descriptor gets its own copy of the constant words (range and
+    // format), opaque to the compiler, so that it keeps one SGPR quad per
+    // descriptor and writes the per-step base straight into it instead of
+    // copying one shared pair of constant words into a second quad every step.
+    int w2a = (int)col_bytes, w3a = 0x00020000, w2b = (int)col_bytes, w3b = 0x00020000;
+    asm volatile("" : "+s"(w2a), "+s"(w3a), "+s"(w2b), "+s"(w3b));
+    auto col_rsrc = [&](uint64_t base, uint32_t off, int w2, int w3) {
         const uint64_t addr = base + off;
         chr_i32x4 r;
         r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
         r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
-        r[2] = (int)col_bytes;
-        r[3] = 0x00020000;
+        r[2] = w2;
+        r[3] = w3;
         return r;
     };
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
@@ -342,8 +348,9 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         const bool real = real_move && mt != kMoveKsparse;
         const bool pair = real && mt == kMovePair;
         const uint32_t oi = (uint32_t)(real ? m.i : 0) * col_stride;
-        const chr_i32x4 ri = col_rsrc(a_base, oi);
-        const chr_i32x4 rj = pair ? col_rsrc(a_base, (uint32_t)m.j * col_stride) : col_rsrc(inv_base, oi);
+        const chr_i32x4 ri = col_rsrc(a_base, oi, w2a, w3a);
+        const chr_i32x4 rj = pair ? col_rsrc(a_base, (uint32_t)m.j * col_stride, w2b, w3b)
+                                  : col_rsrc(inv_base, oi, w2b, w3b);
 #pragma unroll
         for (int q = 0; q < R / VEC; ++q) {
             const int off = 4 * (q * VEC * T + tid * VEC);

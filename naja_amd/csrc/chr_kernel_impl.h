@@ -441,8 +441,8 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // t + PF into (na, nb). The register sets rotate between consecutive
     // steps (the loop below is unrolled by PF + 1), so the prefetched columns
     // are never copied and their loads stay in flight across PF steps.
-    // Move logging (a test hook) is decided once per workgroup; the per-step
-    // test below is then one scalar branch.
+    // Move logging (a test hook) is decided once per workgroup and runs in
+    // the step's event path.
     const bool log_on = a.move_log != nullptr && item.variant == 0 && item.c0 == 0;
     // PF_ALL: every step of the caller's round has a step PF ahead to prefetch
     // for (the main loop), so the per-step bound test is dropped there.
@@ -492,17 +492,16 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
         }
         if (event) {
-            const int64_t e = next_resync < next_save + 1 ? next_resync : next_save + 1;
-            next_event = e < 0x7FFFFFFF ? (int)e : 0x7FFFFFFF;
-        }
-
-        if (log_on) {
-            if (u < (int)a.nlog && tid == 0) {
+            // Move logging is an event of every step while it lasts.
+            if (log_on && u < (int)a.nlog && tid == 0) {
                 const int64_t tl = u;
                 a.move_log[tl * 3 + 0] = mtype;
                 a.move_log[tl * 3 + 1] = mv.i;
                 a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
             }
+            int64_t e = next_resync < next_save + 1 ? next_resync : next_save + 1;
+            if (log_on && u + 1 < (int)a.nlog && u + 1 < e) e = u + 1;
+            next_event = e < 0x7FFFFFFF ? (int)e : 0x7FFFFFFF;
         }
 
         // Alpha uniforms of step t, and the draw of step t + PF (a new batch

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/naja_amd.h"
@@ -824,6 +825,41 @@ __global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict_
     if (tid == 0) status[v] = NAJA_OK;
 }
 
+// Small device buffer per device for the status words and row tables of the
+// synchronous preconditioner calls, kept between calls (a stream-ordered
+// free on the idle stream after the status read-back cost about 0.4 ms of
+// host time per call). Each call holds the device's lock until its stream
+// has finished with the buffer, so the buffer is idle whenever it changes
+// hands or grows.
+constexpr int kMaxDevices = 64;
+
+struct SmallBuf {
+    std::mutex mu;
+    char* p = nullptr;
+    size_t cap = 0;
+};
+
+SmallBuf& small_buf()
+{
+    static SmallBuf bufs[kMaxDevices];
+    int dev = 0;
+    NAJA_HIP(hipGetDevice(&dev));
+    NAJA_REQUIRE(dev >= 0 && dev < kMaxDevices, NAJA_ERR_UNSUPPORTED, "device %d", dev);
+    return bufs[dev];
+}
+
+char* small_buf_reserve(SmallBuf& b, size_t bytes)
+{
+    if (b.cap < bytes) {
+        if (b.p) NAJA_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+        NAJA_HIP(hipMalloc((void**)&b.p, bytes));
+        b.cap = bytes;
+    }
+    return b.p;
+}
+
 }  // namespace mlp
 }  // namespace naja
 
@@ -851,11 +887,13 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
                      "naja_precond_ellipsoid_device: d=%d with %d rows exceeds the LDS budget (use the host routine)",
                      d, m0 + kmax);
         hipStream_t s = (hipStream_t)stream;
-        // Offsets, row counts and per-variant status in one scratch allocation.
-        char* mem = nullptr;
+        // Offsets, row counts and per-variant status in the device's small
+        // buffer, held until the status has been read back.
         const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
         const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
-        scratch_alloc((void**)&mem, b_off + 2 * b_k, s);
+        mlp::SmallBuf& sb = mlp::small_buf();
+        std::lock_guard<std::mutex> hold(sb.mu);
+        char* mem = mlp::small_buf_reserve(sb, b_off + 2 * b_k);
         int64_t* eoff = reinterpret_cast<int64_t*>(mem);
         int* ek = reinterpret_cast<int*>(mem + b_off);
         int* status = reinterpret_cast<int*>(mem + b_off + b_k);
@@ -867,7 +905,6 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
-        scratch_free(mem, s);
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_INVALID_ARGUMENT,
                          "Dikin ellipsoid of variant %d: the centre is not interior or the barrier Hessian is singular",
@@ -1087,15 +1124,15 @@ extern "C" int naja_lowrank_maps_device(const float* P0, const float* V, int n_v
         const size_t rr = (size_t)rank * rank;
         const size_t lds = sizeof(double) * ((size_t)d * rank + rr + std::max(5 * rr, 2 * (size_t)d * rank));
         hipStream_t s = (hipStream_t)stream;
-        int* status = nullptr;
-        scratch_alloc((void**)&status, sizeof(int) * n_variants, s);
+        mlp::SmallBuf& sb = mlp::small_buf();
+        std::lock_guard<std::mutex> hold(sb.mu);
+        int* status = reinterpret_cast<int*>(mlp::small_buf_reserve(sb, sizeof(int) * n_variants));
         hipLaunchKernelGGL(mlp::lowrank_map_kernel, dim3(n_variants), dim3(mlp::kLrThreads), lds, s, P0, V, d, rank,
                            expand, P, status);
         NAJA_HIP(hipGetLastError());
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
-        scratch_free(status, s);
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_RUNTIME,
                          "low-rank map of variant %d: the Newton-Schulz square root did not converge", v);

@@ -35,7 +35,7 @@ MOVE_CASES = {
 
 
 @pytest.mark.parametrize("case", list(MOVE_CASES))
-@pytest.mark.parametrize("tpb", [64, 256, 1024])
+@pytest.mark.parametrize("tpb", [64, 256, 512, 1024])
 def test_move_log_bit_exact(gpu, case, tpb):
     A, b, y0 = random_polytope(9, 40, seed=3)
     n = 4096
@@ -156,6 +156,43 @@ def test_cube_d100_moments(gpu):
     assert (np.abs(zm) < 3).mean() >= 0.97 and (np.abs(zv) < 3).mean() >= 0.97
     assert np.abs(zm).max() < 4.5 and np.abs(zv).max() < 4.5
     assert np.abs(x).max() <= 1 + 1e-5
+
+
+def test_genome_scale_shape_matches_oracle(gpu):
+    """The genome-scale reduced polytope (n = 2583 reactions, d = 512,
+    A = [T; -T] with m = 5166 rows, pair-prob 0.3, thinning 85, Philox seed 0)
+    through the one-shot call, at the launch shape it picks for 4 chains
+    (8 wavefronts, 1 chain per group, 11 rows per lane; odd R takes the
+    one-row-run column loads): samples, final states and chain 0's move log
+    equal the oracle's bit for bit."""
+    import naja_amd
+    from naja_amd import workloads
+
+    t = gpu
+    g = workloads.genome_scale(n=2583, d=512, seed=0)
+    A32 = g["A"].astype(np.float32)
+    m, d = A32.shape
+    nch, nspc, thin, pp = 4, 200, 85, 0.3
+    At, bt = t.from_numpy(A32).cuda(), t.from_numpy(g["b"]).cuda()
+    X0 = t.zeros((nch, d), dtype=t.float32, device="cuda")
+    plan = naja_amd.ChrPlan(At, bt, nspc, thin, nch, seed=0, pair_prob=pp)
+    shape = plan.shape()
+    plan.close()
+    print(f"[genome scale] m={m} d={d} shape (W, G, R) = {shape}", flush=True)
+    assert shape == (8, 1, 11)
+    nlog = nspc * thin
+    S, fin, mlog = naja_amd.coordinate_hit_and_run(At, bt, X0, nspc, thin, nch, seed=0, pair_prob=pp,
+                                                   return_final=True, move_log_steps=nlog)
+    t.cuda.synchronize()
+    ref = oracle.chr_run_f32(A32, g["b"], np.zeros((nch, d), np.float32), nspc, thin, seed=0, pair_prob=pp,
+                             nlog=nlog, nthreads=nch)
+    S = S.cpu().numpy()
+    report("genome-scale", S, ref["samples"])
+    np.testing.assert_array_equal(mlog.cpu().numpy(), ref["move_log"])
+    np.testing.assert_array_equal(S, ref["samples"])
+    np.testing.assert_array_equal(fin.cpu().numpy(), ref["y_final"])
+    viol = (S.astype(np.float64) @ A32.T.astype(np.float64) - g["b"][None, :]).max()
+    assert viol <= 1e-5
 
 
 def test_genome_like_feasible(gpu):

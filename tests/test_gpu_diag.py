@@ -16,12 +16,45 @@ def gpu_diag(t, S, nch, layout=0):
     return e.cpu().numpy(), r.cpu().numpy(), ec.cpu().numpy()
 
 
-@pytest.mark.parametrize("spc,nch,dim,phi", [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99), (2000, 4, 8, 0.6),
-                                             (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7),
-                                             (200, 37, 300, 0.5), (77, 4, 100, 0.8)])
+def rounding_sensitive(S, nch, eps=1e-5):
+    """Per (dim, chain): True where the oracle's truncation could move under
+    float rounding of the autocovariance, i.e. where a pair sum
+    rho_{2k-1} + rho_{2k} up to and including the first negative one lies
+    within eps of zero (mcmc.cpp:60-76). Elsewhere the ESS is a smooth
+    function of the autocovariance and must agree to float rounding."""
+    n_total, dim = S.shape
+    spc = n_total // nch
+    out = np.zeros((dim, nch), bool)
+    if spc < 4:
+        return out
+    for d in range(dim):
+        for c in range(nch):
+            acov = oracle.autocov(S[c::nch, d]).astype(np.float64)
+            if acov[0] < 1e-30:
+                continue
+            k = np.arange(1, spc // 2)
+            ps = acov[2 * k - 1] / acov[0] + acov[2 * k] / acov[0]
+            neg = np.flatnonzero(ps < 0)
+            last = neg[0] + 1 if neg.size else ps.size
+            out[d, c] = bool(np.abs(ps[:last]).min(initial=np.inf) < eps)
+    return out
+
+
+DIAG_CASES = [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99), (2000, 4, 8, 0.6),
+              (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7), (200, 37, 300, 0.5), (77, 4, 100, 0.8),
+              # truncation beyond the exact lags of the half-size pass: the full-size redo pass
+              (8000, 2, 3, 0.999), (16000, 2, 2, 0.9995), (4000, 3, 2, 0.998), (1000, 4, 3, 0.9999),
+              (2000, 3, 2, 1.0), (3001, 2, 3, 0.998)]
+
+
+@pytest.mark.parametrize("spc,nch,dim,phi", DIAG_CASES)
 def test_diag_matches_oracle(gpu, spc, nch, dim, phi):
     rng = np.random.default_rng(spc + nch)
-    series = ar1(spc, phi, rng, nseries=nch * dim).reshape(nch, dim, spc) + rng.standard_normal((nch, dim, 1))
+    if phi < 1.0:
+        series = ar1(spc, phi, rng, nseries=nch * dim).reshape(nch, dim, spc)
+    else:  # random walk: the truncation runs out to about n/2
+        series = np.cumsum(rng.standard_normal((nch, dim, spc)), axis=2).astype(np.float32)
+    series = series + rng.standard_normal((nch, dim, 1)).astype(np.float32)
     S = np.zeros((spc * nch, dim), np.float32)
     for c in range(nch):
         S[c::nch] = series[c].T
@@ -31,11 +64,13 @@ def test_diag_matches_oracle(gpu, spc, nch, dim, phi):
         r_ref, _, _ = oracle.split_rhat(S, nch)
         assert np.abs(r - r_ref).max() <= 1e-5 * np.abs(r_ref).max()
     rel = np.abs(ec - ec_ref) / ec_ref
-    print(f"ess spc={spc}: max rel {rel.max():.3e}", flush=True)
-    # float FFT rounding differs from the double oracle; the truncation point
-    # may move by one pair only when a pair sum sits within rounding of zero
-    assert np.median(rel) <= 1e-4
-    assert (rel <= 1e-3).mean() >= 0.9
+    sens = rounding_sensitive(S, nch)
+    tight = rel[~sens]
+    print(f"ess spc={spc} phi={phi}: max rel {rel.max():.3e}, off the rounding-sensitive series "
+          f"{tight.max(initial=0.0):.3e} ({int(sens.sum())} of {sens.size} sensitive)", flush=True)
+    # The GPU transforms in float and sums the truncated pairs in FP64; the
+    # oracle autocovariance is FP64 rounded to float and summed in float.
+    assert tight.max(initial=0.0) <= 1e-5
     np.testing.assert_array_equal(e, ec.min(1))
 
 

@@ -178,6 +178,23 @@ int naja_diag(const float* samples, int dim, int n_total, int nchains, int layou
               float* ess, float* rhat, float* ess_chain, void* stream);
 
 /*
+ * naja_diag without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as naja_diag. Its scratch (the transposed series, the half-chain
+ * moments and, when ess_chain is NULL, the per-chain ESS: about
+ * 4 * dim * nchains * (spc rounded up to 64) bytes plus 32 * dim * nchains)
+ * lives in a static device buffer of 64 MiB per GPU, held until the process
+ * ends; larger calls fail with NAJA_ERR_UNSUPPORTED. Every call on one GPU
+ * shares that buffer, so calls that overlap on one GPU (different streams or
+ * host threads) race. The first call on a device sets it up with a
+ * synchronous 7 KB upload of the FFT twiddle tables, so it must not be made
+ * under stream capture; later calls only launch kernels on `stream`.
+ */
+int naja_diag_noalloc(const float* samples, int dim, int n_total, int nchains, int layout,
+                      float* ess, float* rhat, float* ess_chain, void* stream);
+
+/*
  * naja_diag over n_variants sample sets stored back to back (variant v at
  * samples + v * n_total * dim), in one set of launches. ess, rhat:
  * n_variants x dim; ess_min, rhat_max (nullable): per variant, the minimum

@@ -47,6 +47,7 @@ EXPORTED_SYMBOLS = (
     "naja_backmap",
     "naja_diag",
     "naja_diag_batched",
+    "naja_diag_noalloc",
     "naja_chr_timing",
     "naja_chr_kernel_times",
     "naja_kernel_basis",
@@ -153,6 +154,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_diag.restype = i32
     lib.naja_diag_batched.argtypes = [vp, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp]
     lib.naja_diag_batched.restype = i32
+    lib.naja_diag_noalloc.argtypes = [vp, i32, i32, i32, i32, vp, vp, vp, vp]
+    lib.naja_diag_noalloc.restype = i32
     f64 = ctypes.c_double
     lib.naja_kernel_basis.argtypes = [vp, i32, i32, f64, vp, vp]
     lib.naja_chr_timing.argtypes = [i32]

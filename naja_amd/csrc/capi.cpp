@@ -112,6 +112,23 @@ int naja_diag(const float* samples, int dim, int n_total, int nchains, int layou
     });
 }
 
+int naja_diag_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
+                      float* ess_chain, void* stream)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(samples && ess && rhat, NAJA_ERR_INVALID_ARGUMENT, "NULL samples, ess or rhat");
+        NAJA_REQUIRE(dim > 0 && nchains > 0 && n_total > 0, NAJA_ERR_INVALID_ARGUMENT, "invalid diagnostics shape");
+        NAJA_REQUIRE(layout == NAJA_LAYOUT_INTERLEAVED || layout == NAJA_LAYOUT_BLOCKS, NAJA_ERR_INVALID_ARGUMENT,
+                     "invalid layout %d", layout);
+        NAJA_REQUIRE(n_total % nchains == 0, NAJA_ERR_INVALID_ARGUMENT, "n_total (%d) not divisible by n_chains (%d)",
+                     n_total, nchains);
+        NAJA_REQUIRE(n_total / nchains >= 4, NAJA_ERR_INVALID_ARGUMENT, "compute_ess: fewer than 4 samples per chain");
+        diag_launch_noalloc(samples, dim, n_total, nchains, layout, ess, rhat, ess_chain,
+                            reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
 int naja_diag_batched(const float* samples, int n_variants, int dim, int n_total, int nchains, int layout, float* ess,
                       float* rhat, float* ess_min, float* rhat_max, void* stream)
 {

@@ -11,5 +11,11 @@ int diag_max_chain_length();
 void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
                  float* ess_chain, hipStream_t stream, int n_variants = 1, float* ess_min = nullptr,
                  float* rhat_max = nullptr);
+// Scratch held by naja_diag_noalloc's static device buffer (per GPU).
+constexpr size_t kDiagNoAllocBytes = (size_t)64 << 20;
+// diag_launch for one sample set with all scratch in that static buffer
+// (no allocation; NAJA_ERR_UNSUPPORTED beyond kDiagNoAllocBytes).
+void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
+                         float* ess_chain, hipStream_t stream);
 
 }  // namespace naja

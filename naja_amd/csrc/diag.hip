@@ -30,10 +30,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
 #include "diag.h"
+
+// Device memory of naja_diag_noalloc (see diag_launch_noalloc below): the
+// per-call scratch, and the FFT twiddle tables of every transform size.
+constexpr int kTwiddleEntries = 1024;
+static __device__ __align__(256) unsigned char scratch[naja::kDiagNoAllocBytes];
+static __device__ __align__(256) float2 scratch_twiddles[kTwiddleEntries];
 
 namespace naja {
 
@@ -788,61 +795,86 @@ __global__ __launch_bounds__(256) void variant_summary_kernel(const float* __res
 
 int diag_max_chain_length() { return kMaxHalf; }
 
-void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
-                 float* ess_chain, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
+namespace {
+
+// Sizes and launch geometry of one diagnostics call.
+struct DiagPlan {
+    int spc, logH, H, N, tb, half, spcp, s_tiles, d_tiles;
+    int64_t n_series, t_blocks, vstride;
+    unsigned nv;
+    size_t b_ser, b_mom, b_ec;  // 256-byte rounded scratch regions
+};
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+DiagPlan diag_plan(int dim, int n_total, int nchains, int n_variants, bool need_ec)
 {
-    const int spc = n_total / nchains;
-    int logH = 1;
-    while ((1 << (logH + 1)) < 2 * spc) ++logH;  // N = 2H >= 2 spc
-    const int H = 1 << logH;
-    NAJA_REQUIRE(H <= kMaxHalf, NAJA_ERR_UNSUPPORTED, "chains of %d samples exceed the LDS FFT limit (%d)", spc,
+    DiagPlan p{};
+    p.spc = n_total / nchains;
+    p.logH = 1;
+    while ((1 << (p.logH + 1)) < 2 * p.spc) ++p.logH;  // N = 2H >= 2 spc
+    p.H = 1 << p.logH;
+    NAJA_REQUIRE(p.H <= kMaxHalf, NAJA_ERR_UNSUPPORTED, "chains of %d samples exceed the LDS FFT limit (%d)", p.spc,
                  kMaxHalf);
     NAJA_REQUIRE(n_variants >= 1 && n_variants <= 65535, NAJA_ERR_UNSUPPORTED, "%d variants (1..65535)", n_variants);
-    const int N = 2 * H;
-    const int64_t vstride = (int64_t)n_total * dim;
-    const unsigned nv = (unsigned)n_variants;
+    p.N = 2 * p.H;
+    p.tb = (p.logH + 1) / 2;
+    p.half = p.spc / 2;
+    p.nv = (unsigned)n_variants;
+    p.vstride = (int64_t)n_total * dim;
+    p.n_series = (int64_t)dim * nchains * p.nv;
+    p.spcp = (p.spc + kTs - 1) / kTs * kTs;  // series row stride (256-byte aligned rows)
+    p.s_tiles = (p.spc + kTs - 1) / kTs;
+    p.d_tiles = (dim + kTd - 1) / kTd;
+    p.t_blocks = (int64_t)((p.s_tiles + 7) / 8) * 8 * nchains * p.d_tiles * p.nv;
+    NAJA_REQUIRE(p.t_blocks < (int64_t)1 << 31 && p.n_series < (int64_t)1 << 31, NAJA_ERR_UNSUPPORTED,
+                 "too many series for one diagnostics launch");
+    p.b_ser = round256(sizeof(float) * (size_t)p.n_series * p.spcp);
+    p.b_mom = round256(sizeof(double) * 2 * (size_t)dim * 2 * nchains * p.nv);
+    p.b_ec = need_ec ? round256(sizeof(float) * (size_t)dim * nchains * p.nv) : 0;
+    return p;
+}
 
-    // Twiddle tables: tlo[l] = W^l (l < 2^tb), thi[h] = W^(h 2^tb) (h < H / 2^tb).
+// Twiddle tables of the transform with H = 2^logH complex points, N = 2H:
+// tlo[l] = W^l (l < 2^tb), then thi[h] = W^(h 2^tb) (h < H / 2^tb), each
+// rounded once from FP64.
+int twiddle_count(int logH)
+{
     const int tb = (logH + 1) / 2;
-    const int nlo = 1 << tb, nhi = H >> tb;
-    std::vector<float2> tw_h((size_t)(nlo + nhi));
+    return (1 << tb) + ((1 << logH) >> tb);
+}
+
+void twiddle_tables(int logH, float2* out)
+{
+    const int tb = (logH + 1) / 2;
+    const int nlo = 1 << tb, nhi = (1 << logH) >> tb;
+    const double N = (double)(2 << logH);
     for (int l = 0; l < nlo; ++l) {
-        const double ang = -2.0 * M_PI * (double)l / (double)N;
-        tw_h[(size_t)l] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        const double ang = -2.0 * M_PI * (double)l / N;
+        out[l] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     for (int h = 0; h < nhi; ++h) {
-        const double ang = -2.0 * M_PI * (double)h * (double)nlo / (double)N;
-        tw_h[(size_t)(nlo + h)] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        const double ang = -2.0 * M_PI * (double)h * (double)nlo / N;
+        out[nlo + h] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
-    float2* tw = nullptr;
-    double* mom = nullptr;
-    float* ser = nullptr;
-    float* ec = ess_chain;
-    const int half = spc / 2;
-    const int64_t n_series = (int64_t)dim * nchains * nv;
-    const int spcp = (spc + kTs - 1) / kTs * kTs;  // series row stride (256-byte aligned rows)
-    const int s_tiles = (spc + kTs - 1) / kTs;
-    const int d_tiles = (dim + kTd - 1) / kTd;
-    const int64_t t_blocks = (int64_t)((s_tiles + 7) / 8) * 8 * nchains * d_tiles * nv;
-    NAJA_REQUIRE(t_blocks < (int64_t)1 << 31 && n_series < (int64_t)1 << 31, NAJA_ERR_UNSUPPORTED,
-                 "too many series for one diagnostics launch");
-    // One scratch allocation for the series, the twiddle tables, the moments
-    // and (when the caller passes none) the per-chain ESS: each pool call
-    // costs tens of microseconds of host time.
-    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t b_ser = up(sizeof(float) * (size_t)n_series * spcp), b_tw = up(sizeof(float2) * tw_h.size());
-    const size_t b_mom = up(sizeof(double) * 2 * (size_t)dim * 2 * nchains * nv);
-    const size_t b_ec = ec ? 0 : up(sizeof(float) * (size_t)dim * nchains * nv);
-    char* mem = nullptr;
-    scratch_alloc((void**)&mem, b_ser + b_tw + b_mom + b_ec, stream);
-    ser = reinterpret_cast<float*>(mem);
-    tw = reinterpret_cast<float2*>(mem + b_ser);
-    mom = reinterpret_cast<double*>(mem + b_ser + b_tw);
-    if (!ec) ec = reinterpret_cast<float*>(mem + b_ser + b_tw + b_mom);
-    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
+}
 
-    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)t_blocks), dim3(256), 0, stream, samples, dim, nchains,
-                       spc, spcp, layout, vstride, s_tiles, d_tiles, ser);
+// The launches of one diagnostics call. mem holds the series, the moments
+// and, when ess_chain is NULL, the per-chain ESS (b_ser + b_mom + b_ec
+// bytes); tw the twiddle tables of logH. Nothing in mem needs to start
+// zeroed: every kernel writes what it later reads.
+void diag_run(const DiagPlan& p, const float* samples, int dim, int nchains, int layout, float* ess, float* rhat,
+              float* ess_chain, char* mem, const float2* tw, hipStream_t stream, float* ess_min, float* rhat_max)
+{
+    float* ser = reinterpret_cast<float*>(mem);
+    double* mom = reinterpret_cast<double*>(mem + p.b_ser);
+    float* ec = ess_chain ? ess_chain : reinterpret_cast<float*>(mem + p.b_ser + p.b_mom);
+    const int spc = p.spc, H = p.H, logH = p.logH, N = p.N, tb = p.tb, half = p.half, spcp = p.spcp;
+    const int64_t n_series = p.n_series;
+    const unsigned nv = p.nv;
+
+    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
+                       nchains, spc, spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser);
     NAJA_HIP(hipGetLastError());
     // A half-size pass first when it leaves at least 1/8 of the chain as exact
     // lags (N/2 - n >= n/8); the full pass then redoes only the series whose
@@ -869,8 +901,88 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
         hipLaunchKernelGGL(variant_summary_kernel, dim3(nv), dim3(256), 0, stream, ess, rhat, dim, ess_min, rhat_max);
         NAJA_HIP(hipGetLastError());
     }
+}
 
+// Allocation-free diagnostics: the series, moments and per-chain ESS live in
+// a file-scope device array (one copy per GPU, held until the process ends),
+// the twiddle tables of every transform size in a second one. Host code
+// never uses these names as addresses: each device's addresses come from
+// hipGetSymbolAddress at set-up.
+constexpr int kMinLogH = 2, kMaxLogH = 14;  // H = 4 (4-sample chains) .. kMaxHalf
+static_assert((1 << kMaxLogH) == kMaxHalf, "twiddle tables cover every transform size");
+static_assert(kTwiddleEntries >= 886, "sum of twiddle_count(logH) over kMinLogH..kMaxLogH");
+
+struct NoAllocDevice {
+    bool ready = false;
+    char* scratch = nullptr;       // this device's address of `scratch`
+    float2* twiddles = nullptr;    // this device's address of `scratch_twiddles`
+    int offset[kMaxLogH + 1] = {}; // first entry of each transform size's tables
+};
+std::mutex g_noalloc_mu;
+std::vector<NoAllocDevice> g_noalloc;
+
+const NoAllocDevice& noalloc_device()
+{
+    int dev = 0;
+    NAJA_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc.size() <= dev) g_noalloc.resize((size_t)dev + 1);
+    NoAllocDevice& st = g_noalloc[(size_t)dev];
+    if (!st.ready) {
+        void* ptr = nullptr;
+        NAJA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        void* tptr = nullptr;
+        NAJA_HIP(hipGetSymbolAddress(&tptr, HIP_SYMBOL(scratch_twiddles)));
+        std::vector<float2> tables;
+        for (int lh = kMinLogH; lh <= kMaxLogH; ++lh) {
+            st.offset[lh] = (int)tables.size();
+            tables.resize(tables.size() + (size_t)twiddle_count(lh));
+            twiddle_tables(lh, tables.data() + st.offset[lh]);
+        }
+        NAJA_REQUIRE((int)tables.size() <= kTwiddleEntries, NAJA_ERR_RUNTIME, "twiddle tables exceed their array");
+        // Synchronous upload, once per device (the first call on a device
+        // must not be under stream capture).
+        NAJA_HIP(hipMemcpy(tptr, tables.data(), sizeof(float2) * tables.size(), hipMemcpyHostToDevice));
+        st.scratch = static_cast<char*>(ptr);
+        st.twiddles = static_cast<float2*>(tptr);
+        st.ready = true;
+    }
+    return st;
+}
+
+}  // namespace
+
+void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
+                 float* ess_chain, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
+{
+    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, ess_chain == nullptr);
+    std::vector<float2> tw_h((size_t)twiddle_count(p.logH));
+    twiddle_tables(p.logH, tw_h.data());
+    // One scratch allocation for the series, the moments, the per-chain ESS
+    // (when the caller passes none) and the twiddle tables: each pool call
+    // costs tens of microseconds of host time.
+    const size_t b_tw = round256(sizeof(float2) * tw_h.size());
+    const size_t b_main = p.b_ser + p.b_mom + p.b_ec;
+    char* mem = nullptr;
+    scratch_alloc((void**)&mem, b_main + b_tw, stream);
+    float2* tw = reinterpret_cast<float2*>(mem + b_main);
+    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
+    diag_run(p, samples, dim, nchains, layout, ess, rhat, ess_chain, mem, tw, stream, ess_min, rhat_max);
     scratch_free(mem, stream);
+}
+
+void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
+                         float* ess_chain, hipStream_t stream)
+{
+    const DiagPlan p = diag_plan(dim, n_total, nchains, 1, ess_chain == nullptr);
+    const size_t need = p.b_ser + p.b_mom + p.b_ec;
+    NAJA_REQUIRE(need <= kDiagNoAllocBytes, NAJA_ERR_UNSUPPORTED,
+                 "naja_diag_noalloc: %d dims x %d chains x %d samples need %zu bytes of scratch; the static buffer "
+                 "holds %zu (about %d x 4 bytes per (dim, chain) series, rounded up to 64 samples)",
+                 dim, nchains, p.spc, need, (size_t)kDiagNoAllocBytes, p.spcp);
+    const NoAllocDevice& st = noalloc_device();
+    diag_run(p, samples, dim, nchains, layout, ess, rhat, ess_chain, st.scratch, st.twiddles + st.offset[p.logH],
+             stream, nullptr, nullptr);
 }
 
 }  // namespace naja

@@ -120,8 +120,14 @@ int naja_chr_sample_batched_rounded(const float* A, const double* b, int m0, int
  * Sampling plans: pack (and transform) the polytope blocks once, then launch
  * any number of times without allocating, so a launch can be captured in a
  * hipGraph. A plan keeps pointers to A, b, A_extra, b_extra, P and c (the
- * caller keeps them alive) and allocates and frees its own buffers on the
- * creation stream. naja_chr_plan_autotune times every compiled launch shape
+ * caller keeps them alive) and allocates, packs and frees its own buffers on
+ * the creation stream. Launches and autotuning may use another stream: that
+ * stream first waits (once per repack) for the packing, and the creation
+ * stream waits for such launches before it repacks or frees. A captured
+ * launch records neither wait: capture a plan only after its creation (or
+ * autotuning) stream has finished packing (e.g. after a synchronisation),
+ * and order graph replays before naja_chr_plan_destroy or a later autotune
+ * yourself. naja_chr_plan_autotune times every compiled launch shape
  * on about 1024 steps of the real run, keeps the fastest, and remembers it
  * for later plans of the same size (results do not depend on the shape).
  * shape_out receives {wavefronts per workgroup, chains per group, rows per lane}.

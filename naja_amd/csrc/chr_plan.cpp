@@ -60,6 +60,18 @@ struct naja_chr_plan {
     // allocation (every pool call costs tens of microseconds of host time).
     char* tables_mem = nullptr;
     char* pack_mem = nullptr;
+
+    // Ordering between the creation stream (packs, allocations, frees) and
+    // launches on other streams: `packed` is recorded on the creation stream
+    // after every repack (pack_seq counts them), and a launch stream waits on
+    // it once per repack; `launched` is recorded after a launch on another
+    // stream, and the creation stream waits on it before it rewrites or
+    // frees the packs.
+    hipEvent_t packed = nullptr;
+    hipEvent_t launched = nullptr;
+    uint64_t pack_seq = 0;
+    std::vector<std::pair<hipStream_t, uint64_t>> waited;  // launch stream -> pack_seq it has waited for
+    bool launch_pending = false;
 };
 
 namespace naja {
@@ -146,6 +158,10 @@ void check_params(const naja_chr_params* p, int d)
 void repack(naja_chr_plan& pl, const ChrShape& sh)
 {
     const hipStream_t s = pl.stream;
+    if (pl.launch_pending) {  // a launch on another stream may still read the packs
+        NAJA_HIP(hipStreamWaitEvent(s, pl.launched, 0));
+        pl.launch_pending = false;
+    }
     const int lda = sh.rows();
     int64_t a_off = 0, b_off = 0;
     const int64_t per_variant_samples = (int64_t)pl.p.nchains * pl.p.nspc * pl.d;
@@ -186,6 +202,40 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
                  pl.A_pack, pl.Ainv_pack, pl.b_pack, lda, s);
     }
     pl.shape = sh;
+    NAJA_HIP(hipEventRecord(pl.packed, s));
+    ++pl.pack_seq;
+}
+
+// Before a launch on stream s: s waits for the latest repack on the
+// creation stream (once per repack and stream). A captured launch records no
+// wait (an event recorded outside the capture cannot be waited on inside
+// it): the packing must be complete before capture begins.
+void order_after_pack(naja_chr_plan& pl, hipStream_t s)
+{
+    if (s == pl.stream) return;
+    auto it = std::find_if(pl.waited.begin(), pl.waited.end(), [&](const auto& e) { return e.first == s; });
+    if (it != pl.waited.end() && it->second == pl.pack_seq) return;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NAJA_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return;
+    NAJA_HIP(hipStreamWaitEvent(s, pl.packed, 0));
+    if (it != pl.waited.end())
+        it->second = pl.pack_seq;
+    else
+        pl.waited.emplace_back(s, pl.pack_seq);
+}
+
+// After a launch on stream s: the creation stream will wait for it before
+// repacking or freeing (not for captured launches: replays are ordered by
+// the caller).
+void note_launch(naja_chr_plan& pl, hipStream_t s)
+{
+    if (s == pl.stream) return;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NAJA_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return;
+    NAJA_HIP(hipEventRecord(pl.launched, s));
+    pl.launch_pending = true;
 }
 
 ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps,
@@ -241,6 +291,7 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
     float* zs = pl.inplace ? samples : pl.zbuf;
     float* out = pl.P ? zs : samples;
     const ChrArgs a = make_args(pl, y0, out, y_final, nsteps, t0);
+    order_after_pack(pl, s);
     chr_launch(a, (int)pl.items.size(), pl.shape, s);
     if (pl.P) {
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
@@ -248,6 +299,7 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
         backmap_launch_batched(pl.P, pl.c, zs, samples, pl.d, pl.d, per, pl.n_variants, (int64_t)pl.d * pl.d,
                                pl.d, per * pl.d, per * pl.d, s);
     }
+    note_launch(pl, s);
 }
 
 }  // namespace
@@ -306,6 +358,8 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
             for (int c0 = 0; c0 < p->nchains; c0 += pl->G) logical.push_back(ChrItem{v, c0});
         pl->items = xcd_order(logical);
         const hipStream_t s = pl->stream;
+        NAJA_HIP(hipEventCreateWithFlags(&pl->packed, hipEventDisableTiming));
+        NAJA_HIP(hipEventCreateWithFlags(&pl->launched, hipEventDisableTiming));
         size_t off[4];
         const size_t total = carve({sizeof(ChrVariant) * pl->var.size(), sizeof(ChrItem) * pl->items.size(),
                                     sizeof(int64_t) * pl->eoff.size(), sizeof(int) * pl->ek.size()},
@@ -397,10 +451,12 @@ int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, i
         for (const ChrShape& sh : cand) {
             repack(pl, sh);
             const ChrArgs a = make_args(pl, y0, (pl.P && pl.zbuf) ? pl.zbuf : probe, nullptr, nsteps);
+            order_after_pack(pl, s);
             chr_launch(a, (int)pl.items.size(), sh, s);  // warm-up
             NAJA_HIP(hipEventRecord(e0, s));
             chr_launch(a, (int)pl.items.size(), sh, s);
             NAJA_HIP(hipEventRecord(e1, s));
+            note_launch(pl, s);
             NAJA_HIP(hipEventSynchronize(e1));
             float ms = 0.0f;
             NAJA_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -443,9 +499,12 @@ int naja_chr_plan_destroy(naja_chr_plan* plan)
     if (!plan) return NAJA_OK;
     const int rc = guarded([&] {
         const hipStream_t s = plan->stream;
+        if (plan->launch_pending) NAJA_HIP(hipStreamWaitEvent(s, plan->launched, 0));
         void* bufs[] = {plan->tables_mem, plan->pack_mem, plan->zbuf};
         for (void* ptr : bufs) free_async(ptr, s);
     });
+    if (plan->packed) (void)hipEventDestroy(plan->packed);
+    if (plan->launched) (void)hipEventDestroy(plan->launched);
     delete plan;
     return rc;
 }

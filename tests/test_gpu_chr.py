@@ -331,6 +331,35 @@ def test_plan_autotune_relaunch_and_graph_replay(gpu):
     plan.close()
 
 
+def test_plan_launches_on_other_streams(gpu):
+    """A plan created on one stream and launched and autotuned on another:
+    the launch stream waits for the packing and the creation stream for the
+    launches before it repacks or frees, so every launch equals the oracle."""
+    import naja_amd
+
+    t = gpu
+    A, b, _ = random_polytope(16, 900, seed=43)
+    nch, nspc, thin = 4, 200, 3
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, np.zeros((nch, 16), np.float32), nspc, thin, seed=2,
+                             pair_prob=0.3)["samples"]
+    make, use = t.cuda.Stream(), t.cuda.Stream()
+    with t.cuda.stream(make):
+        At, bt = t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda()
+        X0 = t.zeros((1, nch, 16), device="cuda")
+        plan = naja_amd.ChrPlan(At, bt, nspc, thin, nch, seed=2, pair_prob=0.3, stream=make)
+    use.wait_stream(make)
+    with t.cuda.stream(use):
+        out = plan.samples_like()
+        outs = [plan.launch(X0, out, stream=use).clone()]
+        plan.autotune(X0, stream=use)  # repacks on `make` between timed launches on `use`
+        outs.append(plan.launch(X0, plan.samples_like(), stream=use))
+    use.synchronize()
+    for o in outs:
+        np.testing.assert_array_equal(o.cpu().numpy()[0], ref)
+    plan.close()
+    make.synchronize()
+
+
 def test_streamed_chunks_reproduce_one_run(gpu):
     """CoordinateHitAndRunStreamed semantics: chunks continue the Philox
     stream and the chain state; with chunk steps a multiple of the resync

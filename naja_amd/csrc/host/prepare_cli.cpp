@@ -1,35 +1,42 @@
-// `naja prepare`: produces the conditioned model directories the sampler
-// reads (cmd_prepare.cpp:213-457), with the start-point LP run by the
-// library's own interior-point solver instead of Gurobi.
+// `naja prepare`: turns a list of model directories into directories the
+// sampler can read, with conditioning expressed as extra rows over a shared
+// base model's rounded variables.
 //
-// Per model of --model-list under --models-root:
-//   * with --conditioner-cmd: run it once per model (unless gem/ is present
-//     with a matching conditioning.signature), check its reaction ids against
-//     the base model's, validate both bound sets, and write the tightened
-//     bounds as extra rows over the base model's rounded variables
-//     (naja_extra_from_bounds; extra_A/extra_b CSV, or no files when nothing
-//     tightened), then the signature;
-//   * drop an empty extra pair (inherit.cpp:14-29); inherit the base model's
-//     rounding files by symlink or copy when its own are missing
-//     (inherit.cpp:31-71, with INHERITED_FROM.txt);
-//   * after all models: where extra rows exist and the current start violates
-//     them (1e-9), replace the start by the cube centre of the augmented
-//     polytope (feasible_start_files.cpp:41-90), models in parallel.
-// Bulk-mode config generation (--out-bulk-config) belongs to the reference's
-// multi-model job runner, which this build does not have; the flag is
-// rejected with that reason.
-#include <sys/stat.h>
-#include <unistd.h>
-
+// The observable contract is the reference's (cmd_prepare.cpp:213-457,
+// inherit.cpp:14-71, feasible_start_files.cpp:41-90): its flags, the
+// conditioner's command line, the gem/ and rounding/ file names, the
+// conditioning.signature text and the INHERITED_FROM.txt keys, the summary
+// lines. The driver itself is organised differently:
+//
+//   * every model is a PrepareJob (its paths, whether gem/ is complete, what
+//     happened to it); the base model is loaded once, lazily, as a BaseModel
+//     (reaction ids, bounds, T and shift);
+//   * stage 1 runs per job in list order: decide whether the conditioner
+//     must run (signature check), run it, and re-derive the extra rows from
+//     the conditioned bounds through the library (naja_extra_from_bounds);
+//   * stage 2 completes each job's rounding file set: an empty extra pair is
+//     dropped, missing core files are materialised from the base (symlink or
+//     copy), and the set is checked;
+//   * stage 3 repairs, on a pool of host threads, the start of every job
+//     whose extra rows cut off its current start (cube centre of the
+//     augmented polytope by the library's own interior-point LP, in place of
+//     the reference's Gurobi model).
+// `--out-bulk-config` configures the reference's multi-model job runner,
+// which this build does not have, and is rejected with that reason.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <exception>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <iomanip>
 #include <iostream>
+#include <map>
+#include <memory>
 #include <mutex>
+#include <optional>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -39,441 +46,542 @@
 #include "../../../include/naja_amd.h"
 #include "host_io.h"
 
-using namespace naja::host;
+namespace fs = std::filesystem;
+using naja::host::Matrix;
 
 namespace {
 
-struct Contract {
-    std::string model_dir, rounding_dir, gem_dir, model_name;
-    std::string prefix() const { return rounding_dir + "/" + model_name + "_rounding"; }
+// ---------------------------------------------------------------------------
+// Files of one model directory.
+
+enum class Role { A, b, start, T, shift, extra_A, extra_b };
+constexpr Role kCoreRoles[] = {Role::A, Role::b, Role::T, Role::shift, Role::start};
+
+const char* role_suffix(Role r)
+{
+    switch (r) {
+    case Role::A: return "A.csv";
+    case Role::b: return "b.csv";
+    case Role::start: return "start.csv";
+    case Role::T: return "T.csv";
+    case Role::shift: return "shift.csv";
+    case Role::extra_A: return "extra_A.csv";
+    case Role::extra_b: return "extra_b.csv";
+    }
+    return "";
+}
+
+struct ModelPaths {
+    std::string dir;   // as given, without a trailing slash
+    std::string name;  // last path component (model_contract.cpp:53-67)
+
+    static ModelPaths of(std::string d)
+    {
+        while (d.size() > 1 && d.back() == '/') d.pop_back();
+        if (!naja::host::is_directory(d)) throw std::runtime_error("model-dir is not a directory: " + d);
+        const size_t cut = d.find_last_of('/');
+        return ModelPaths{d, cut == std::string::npos ? d : d.substr(cut + 1)};
+    }
+    std::string rounding_dir() const { return dir + "/rounding"; }
+    std::string gem_dir() const { return dir + "/gem"; }
+    std::string file(Role r) const { return rounding_dir() + "/" + name + "_rounding_" + role_suffix(r); }
+    std::string gem(const char* leaf) const { return gem_dir() + "/" + leaf; }
 };
 
-[[noreturn]] void usage(const std::string& msg)
+bool exists_or_dangling(const std::string& p)
 {
-    std::cerr << "error: " << msg << "\n\nusage:\n  naja prepare --models-root <dir> --model-list <file> "
-                 "[--base-model-dir <dir>] [--mode symlink|copy] [--out-model-list <file>] [--dry-run] "
-                 "[--conditioner-cmd <cmd>] [--conditioner-arg <arg>]... [--skip-conditioning-if-gem-present] "
-                 "[--force-conditioning] [--verbose]\n";
+    std::error_code ec;
+    return fs::symlink_status(p, ec).type() != fs::file_type::not_found;
+}
+
+bool has_file(const std::string& p)
+{
+    std::error_code ec;
+    return fs::exists(p, ec);
+}
+
+bool zero_bytes(const std::string& p)
+{
+    std::error_code ec;
+    const auto n = fs::file_size(p, ec);
+    if (ec) throw std::runtime_error("cannot stat: " + p);
+    return n == 0;
+}
+
+void unlink_path(const std::string& p)
+{
+    std::error_code ec;
+    fs::remove(p, ec);  // removes a symlink itself, never its target
+    if (ec) throw std::runtime_error("cannot remove: " + p);
+}
+
+std::string slurp(const std::string& p)
+{
+    std::ifstream in(p, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot read: " + p);
+    return std::string(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
+}
+
+void spit(const std::string& p, const std::string& text, const char* what)
+{
+    std::ofstream out(p, std::ios::binary | std::ios::trunc);
+    if (!out) throw std::runtime_error(std::string("cannot write ") + what + ": " + p);
+    out << text;
+}
+
+// Non-empty lines with surrounding blanks stripped; '#' lines dropped when
+// `comments` (the model list format).
+std::vector<std::string> text_lines(const std::string& p, bool comments)
+{
+    std::vector<std::string> out;
+    std::istringstream in(slurp(p));
+    for (std::string line; std::getline(in, line);) {
+        const size_t a = line.find_first_not_of(" \t\r\n");
+        if (a == std::string::npos) continue;
+        const size_t z = line.find_last_not_of(" \t\r\n");
+        std::string t = line.substr(a, z - a + 1);
+        if (comments && t[0] == '#') continue;
+        out.push_back(std::move(t));
+    }
+    return out;
+}
+
+// The CSV layout the reference writes for extra rows: 12 significant digits.
+std::string csv_text(const double* v, int rows, int cols, int digits)
+{
+    std::ostringstream o;
+    o << std::setprecision(digits);
+    for (int r = 0; r < rows; ++r) {
+        for (int c = 0; c < cols; ++c) o << (c ? "," : "") << v[(size_t)r * cols + c];
+        o << "\n";
+    }
+    return o.str();
+}
+
+// The extra pair must be both present or both absent, each non-empty
+// (model_contract.cpp:30-40).
+void check_extra_pair(const ModelPaths& m, bool require_nonempty)
+{
+    const bool ea = has_file(m.file(Role::extra_A)), eb = has_file(m.file(Role::extra_b));
+    if (ea != eb)
+        throw std::runtime_error("extra constraints must be both-present or both-absent: " + m.file(Role::extra_A) +
+                                 " / " + m.file(Role::extra_b));
+    if (ea && require_nonempty) {
+        naja::host::require_nonempty_file(m.file(Role::extra_A), "extra_A");
+        naja::host::require_nonempty_file(m.file(Role::extra_b), "extra_b");
+    }
+}
+
+// The sampler's contract with T and shift required (model_contract.cpp:69-84).
+void check_contract(const ModelPaths& m)
+{
+    if (!naja::host::is_directory(m.rounding_dir()))
+        throw std::runtime_error("missing rounding/ dir: " + m.rounding_dir());
+    for (Role r : {Role::A, Role::b, Role::start, Role::T, Role::shift})
+        naja::host::require_nonempty_file(m.file(r), std::string("rounding_") + role_suffix(r));
+    check_extra_pair(m, true);
+}
+
+// ---------------------------------------------------------------------------
+// Options.
+
+struct Options {
+    std::string models_root, model_list, base_dir, mode = "symlink", out_list, cond_cmd;
+    std::vector<std::string> cond_args;
+    bool dry_run = false, force = false;
+};
+
+[[noreturn]] void die_usage(const std::string& why)
+{
+    std::cerr << "error: " << why
+              << "\n\nusage:\n  naja prepare --models-root <dir> --model-list <file> [--base-model-dir <dir>] "
+                 "[--mode symlink|copy] [--out-model-list <file>] [--dry-run] [--conditioner-cmd <cmd>] "
+                 "[--conditioner-arg <arg>]... [--skip-conditioning-if-gem-present] [--force-conditioning] "
+                 "[--verbose]\n";
     std::exit(2);
 }
 
-std::string arg_value(int& i, int argc, char** argv, const std::string& flag)
+Options parse_options(int argc, char** argv)
 {
-    if (i + 1 >= argc) usage("missing value for " + flag);
-    return std::string(argv[++i]);
-}
-
-// model_contract.cpp:53-67
-Contract parse_model_dir(const std::string& dir)
-{
-    std::string md = dir;
-    if (!md.empty() && md.back() == '/') md.pop_back();
-    if (!is_directory(md)) throw std::runtime_error("model-dir is not a directory: " + md);
-    const size_t slash = md.find_last_of('/');
-    Contract c;
-    c.model_dir = md;
-    c.rounding_dir = md + "/rounding";
-    c.gem_dir = md + "/gem";
-    c.model_name = slash == std::string::npos ? md : md.substr(slash + 1);
-    return c;
-}
-
-// model_contract.cpp:69-84 (backmap = true: T and shift required)
-void validate_contract(const Contract& c)
-{
-    if (!is_directory(c.rounding_dir)) throw std::runtime_error("missing rounding/ dir: " + c.rounding_dir);
-    const std::string p = c.prefix() + "_";
-    require_nonempty_file(p + "A.csv", "rounding_A");
-    require_nonempty_file(p + "b.csv", "rounding_b");
-    require_nonempty_file(p + "start.csv", "rounding_start");
-    require_nonempty_file(p + "T.csv", "rounding_T");
-    require_nonempty_file(p + "shift.csv", "rounding_shift");
-    const bool ha = path_exists(p + "extra_A.csv"), hb = path_exists(p + "extra_b.csv");
-    if (ha != hb)
-        throw std::runtime_error("extra constraints must be both-present or both-absent: " + p + "extra_A.csv / " + p +
-                                 "extra_b.csv");
-    if (ha) {
-        require_nonempty_file(p + "extra_A.csv", "extra_A");
-        require_nonempty_file(p + "extra_b.csv", "extra_b");
-    }
-}
-
-std::vector<std::string> load_model_list(const std::string& path)
-{
-    require_nonempty_file(path, "model list");
-    std::ifstream f(path);
-    if (!f.is_open()) throw std::runtime_error("cannot open model list: " + path);
-    std::vector<std::string> out;
-    std::string line;
-    auto ws = [](unsigned char ch) { return ch == ' ' || ch == '\t' || ch == '\r' || ch == '\n'; };
-    while (std::getline(f, line)) {
-        while (!line.empty() && ws((unsigned char)line.front())) line.erase(line.begin());
-        while (!line.empty() && ws((unsigned char)line.back())) line.pop_back();
-        if (!line.empty() && line[0] != '#') out.push_back(line);
-    }
-    if (out.empty()) throw std::runtime_error("no models in list: " + path);
-    return out;
-}
-
-std::vector<std::string> trimmed_lines(const std::string& path)
-{
-    std::ifstream f(path);
-    if (!f.is_open()) throw std::runtime_error("cannot read: " + path);
-    std::vector<std::string> out;
-    std::string line;
-    auto ws = [](unsigned char ch) { return ch == ' ' || ch == '\t' || ch == '\r' || ch == '\n'; };
-    while (std::getline(f, line)) {
-        while (!line.empty() && ws((unsigned char)line.front())) line.erase(line.begin());
-        while (!line.empty() && ws((unsigned char)line.back())) line.pop_back();
-        if (!line.empty()) out.push_back(line);
-    }
-    return out;
-}
-
-std::string read_all(const std::string& path)
-{
-    std::ifstream f(path);
-    if (!f.is_open()) throw std::runtime_error("cannot read: " + path);
-    std::ostringstream s;
-    s << f.rdbuf();
-    return s.str();
-}
-
-bool file_is_empty(const std::string& path)
-{
-    struct stat st;
-    if (stat(path.c_str(), &st) != 0) throw std::runtime_error("cannot stat: " + path);
-    return st.st_size == 0;
-}
-
-void remove_if_exists(const std::string& path)
-{
-    if (!path_exists(path)) {
-        struct stat st;  // a dangling symlink still has to go
-        if (lstat(path.c_str(), &st) != 0) return;
-    }
-    if (std::remove(path.c_str()) != 0) throw std::runtime_error("cannot remove: " + path);
-}
-
-void write_matrix_csv(const std::string& path, const std::vector<double>& M, int rows, int cols, int prec)
-{
-    std::ofstream f(path);
-    if (!f.is_open()) throw std::runtime_error("cannot write: " + path);
-    f << std::setprecision(prec);
-    for (int i = 0; i < rows; ++i) {
-        for (int j = 0; j < cols; ++j) {
-            if (j) f << ",";
-            f << M[(size_t)i * cols + j];
+    Options o;
+    const std::map<std::string, std::function<void(const std::string&)>> valued = {
+        {"--models-root", [&](const std::string& v) { o.models_root = v; }},
+        {"--model-list", [&](const std::string& v) { o.model_list = v; }},
+        {"--base-model-dir", [&](const std::string& v) { o.base_dir = v; }},
+        {"--mode", [&](const std::string& v) { o.mode = v; }},
+        {"--out-model-list", [&](const std::string& v) { o.out_list = v; }},
+        {"--conditioner-cmd", [&](const std::string& v) { o.cond_cmd = v; }},
+        {"--conditioner-arg", [&](const std::string& v) { o.cond_args.push_back(v); }},
+    };
+    const std::map<std::string, std::function<void()>> switches = {
+        {"--dry-run", [&] { o.dry_run = true; }},
+        {"--force-conditioning", [&] { o.force = true; }},
+        {"--skip-conditioning-if-gem-present", [] {}},  // the default; kept for the flag set
+        {"--verbose", [] {}},
+    };
+    for (int i = 0; i < argc; ++i) {
+        const std::string flag = argv[i];
+        if (auto s = switches.find(flag); s != switches.end()) {
+            s->second();
+        } else if (auto v = valued.find(flag); v != valued.end()) {
+            if (i + 1 >= argc) die_usage("missing value for " + flag);
+            v->second(argv[++i]);
+        } else if (flag == "--out-bulk-config") {
+            die_usage("--out-bulk-config configures the reference's bulk job runner, which this build does not have");
+        } else {
+            die_usage("unknown flag: " + flag);
         }
-        f << "\n";
     }
+    if (o.models_root.empty()) die_usage("missing --models-root");
+    if (o.model_list.empty()) die_usage("missing --model-list");
+    if (!o.base_dir.empty() && o.mode != "symlink" && o.mode != "copy") die_usage("invalid --mode: " + o.mode);
+    return o;
 }
 
-void write_vector_csv(const std::string& path, const std::vector<double>& v, int prec)
-{
-    std::ofstream f(path);
-    if (!f.is_open()) throw std::runtime_error("cannot write: " + path);
-    f << std::setprecision(prec);
-    for (double x : v) f << x << "\n";
-}
+// ---------------------------------------------------------------------------
+// The base model and the conditioner.
 
-void napi(int rc, const char* what)
-{
-    if (rc != NAJA_OK) throw std::runtime_error(std::string(what) + ": " + naja_last_error());
-}
+struct BaseModel {
+    ModelPaths paths;
+    // Loaded on first use (only conditioning needs them).
+    std::optional<std::vector<std::string>> rxn;
+    std::optional<std::vector<double>> lb, ub;
+    std::optional<Matrix> T;
+    std::vector<double> shift;
 
-// cmd_prepare.cpp:43-57 (std::hash of the base reaction-id text)
-std::string conditioning_signature(const std::string& cmd, const std::vector<std::string>& args,
-                                   const std::string& base_rxn_path, const std::string& row_id)
-{
-    std::string sig = cmd;
-    for (const auto& a : args) sig.append("\nARG ").append(a);
-    sig.append("\nROW_ID ").append(row_id);
-    sig.append("\nBASE_RXN_SHA ").append(std::to_string(std::hash<std::string>{}(read_all(base_rxn_path))));
-    return sig;
-}
+    const std::vector<std::string>& reaction_ids()
+    {
+        if (!rxn) rxn = text_lines(paths.gem("reaction_ids.txt"), false);
+        return *rxn;
+    }
+    void load_bounds()
+    {
+        if (!lb) {
+            lb = naja::host::load_csv_vector(paths.gem("l_bounds.csv"));
+            ub = naja::host::load_csv_vector(paths.gem("u_bounds.csv"));
+        }
+    }
+    const Matrix& rounding_T()
+    {
+        if (!T) {
+            T = naja::host::load_csv_matrix(paths.file(Role::T));
+            shift = naja::host::load_csv_vector(paths.file(Role::shift));
+        }
+        return *T;
+    }
+};
 
-// cmd_prepare.cpp:80-93
-void validate_bounds(const std::vector<double>& lb, const std::vector<double>& ub, size_t expected)
+// Bounds of one model: equal lengths, finite, lb <= ub (cmd_prepare.cpp:80-93).
+void check_bounds(const std::vector<double>& lo, const std::vector<double>& hi, size_t n)
 {
-    if (lb.size() != ub.size()) throw std::runtime_error("l_bounds size != u_bounds size");
-    if (lb.size() != expected)
-        throw std::runtime_error("bounds length mismatch: got " + std::to_string(lb.size()) + " expected " +
-                                 std::to_string(expected));
-    for (size_t i = 0; i < lb.size(); ++i) {
-        if (!std::isfinite(lb[i]) || !std::isfinite(ub[i]))
+    if (lo.size() != hi.size()) throw std::runtime_error("l_bounds size != u_bounds size");
+    if (lo.size() != n)
+        throw std::runtime_error("bounds length mismatch: got " + std::to_string(lo.size()) + " expected " +
+                                 std::to_string(n));
+    for (size_t i = 0; i < n; ++i) {
+        if (!(std::isfinite(lo[i]) && std::isfinite(hi[i])))
             throw std::runtime_error("non-finite bound at index " + std::to_string(i));
-        if (lb[i] > ub[i]) throw std::runtime_error("l_bounds > u_bounds at index " + std::to_string(i));
+        if (lo[i] > hi[i]) throw std::runtime_error("l_bounds > u_bounds at index " + std::to_string(i));
     }
 }
 
-// cmd_prepare.cpp:111-156 over the C ABI
-void build_extra_from_bounds(const Contract& c, const std::string& base_prefix, const std::vector<double>& lb_base,
-                             const std::vector<double>& ub_base, const std::vector<double>& lb_new,
-                             const std::vector<double>& ub_new)
+struct Conditioner {
+    std::string cmd;
+    std::vector<std::string> args;
+
+    // The text stored in gem/conditioning.signature: the command, its extra
+    // arguments, the row id and the std::hash of the base reaction-id file
+    // (the format cmd_prepare.cpp:43-57 writes, so directories prepared by
+    // either tool are recognised by the other).
+    std::string signature(const std::string& row, const std::string& base_rxn_file) const
+    {
+        std::ostringstream s;
+        s << cmd;
+        for (const auto& a : args) s << "\nARG " << a;
+        s << "\nROW_ID " << row << "\nBASE_RXN_SHA " << std::hash<std::string>{}(slurp(base_rxn_file));
+        return s.str();
+    }
+
+    // `<cmd> --base-model-dir <base> --out-model-dir <dir> --row-id <row> <args...>`
+    void run(const std::string& base_dir, const ModelPaths& m) const
+    {
+        std::string line = cmd + " --base-model-dir " + base_dir + " --out-model-dir " + m.dir + " --row-id " + m.name;
+        for (const auto& a : args) line += " " + a;
+        const int rc = std::system(line.c_str());
+        if (rc != 0) throw std::runtime_error("conditioner failed with exit code " + std::to_string(rc));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Jobs.
+
+struct PrepareJob {
+    std::string row;  // the model-list entry
+    ModelPaths paths;
+    bool gem_complete = false;
+
+    bool gem_files_present() const
+    {
+        return has_file(paths.gem("reaction_ids.txt")) && has_file(paths.gem("l_bounds.csv")) &&
+               has_file(paths.gem("u_bounds.csv"));
+    }
+};
+
+// Whether the conditioner must (re)run: always when forced or when gem/ is
+// incomplete; otherwise only when no signature proves which run produced the
+// gem files. A signature from other settings is an error unless forced.
+bool conditioner_needed(const PrepareJob& job, const std::string& expected, bool force)
 {
-    const Matrix T = load_csv_matrix(base_prefix + "_T.csv");
-    const std::vector<double> shift = load_csv_vector(base_prefix + "_shift.csv");
-    if ((size_t)T.rows != lb_base.size()) throw std::runtime_error("T rows != base bounds length");
-    if (shift.size() != (size_t)T.rows) throw std::runtime_error("shift length != T rows");
+    if (force || !job.gem_complete) return true;
+    const std::string sig = job.paths.gem("conditioning.signature");
+    if (!has_file(sig)) return true;
+    if (slurp(sig) != expected)
+        throw std::runtime_error("conditioning signature mismatch; use --force-conditioning to overwrite");
+    return false;
+}
+
+// Extra rows of the conditioned bounds over the base's rounded variables,
+// written as the model's extra pair (or removed when nothing tightened).
+void derive_extra_rows(const PrepareJob& job, BaseModel& base)
+{
+    const ModelPaths& m = job.paths;
+    if (text_lines(m.gem("reaction_ids.txt"), false) != base.reaction_ids())
+        throw std::runtime_error("reaction_ids mismatch vs base for " + m.dir);
+    const std::vector<double> lo = naja::host::load_csv_vector(m.gem("l_bounds.csv"));
+    const std::vector<double> hi = naja::host::load_csv_vector(m.gem("u_bounds.csv"));
+    check_bounds(lo, hi, base.reaction_ids().size());
+    base.load_bounds();
+    check_bounds(*base.lb, *base.ub, lo.size());
+    const Matrix& T = base.rounding_T();
+    if ((size_t)T.rows != base.lb->size()) throw std::runtime_error("T rows != base bounds length");
+    if (base.shift.size() != (size_t)T.rows) throw std::runtime_error("shift length != T rows");
+
     std::vector<double> EA((size_t)2 * T.rows * T.cols), Eb((size_t)2 * T.rows);
     int k = 0;
-    napi(naja_extra_from_bounds(T.v.data(), shift.data(), T.rows, T.cols, lb_base.data(), ub_base.data(),
-                                lb_new.data(), ub_new.data(), 1e-9, EA.data(), Eb.data(), &k),
-         "build_extra_from_bounds");
-    const std::string ea = c.prefix() + "_extra_A.csv", eb = c.prefix() + "_extra_b.csv";
+    if (naja_extra_from_bounds(T.v.data(), base.shift.data(), T.rows, T.cols, base.lb->data(), base.ub->data(),
+                               lo.data(), hi.data(), 1e-9, EA.data(), Eb.data(), &k) != NAJA_OK)
+        throw std::runtime_error(std::string("build_extra_from_bounds: ") + naja_last_error());
+    naja::host::ensure_dir(m.rounding_dir());
     if (k == 0) {
-        remove_if_exists(ea);
-        remove_if_exists(eb);
+        unlink_path(m.file(Role::extra_A));
+        unlink_path(m.file(Role::extra_b));
         return;
     }
-    Eb.resize((size_t)k);
-    write_matrix_csv(ea, EA, k, T.cols, 12);
-    write_vector_csv(eb, Eb, 12);
+    spit(m.file(Role::extra_A), csv_text(EA.data(), k, T.cols, 12), "extra_A");
+    spit(m.file(Role::extra_b), csv_text(Eb.data(), k, 1, 12), "extra_b");
 }
 
-// inherit.cpp:14-29
-void normalize_extra_constraints(const Contract& c)
+// An extra pair where either file is empty means "no extra rows".
+void drop_empty_extra_pair(const ModelPaths& m)
 {
-    const std::string ea = c.prefix() + "_extra_A.csv", eb = c.prefix() + "_extra_b.csv";
-    const bool ha = path_exists(ea), hb = path_exists(eb);
-    if (ha != hb) throw std::runtime_error("extra constraints must be both-present or both-absent: " + ea + " / " + eb);
-    if (!ha) return;
-    if (file_is_empty(ea) || file_is_empty(eb)) {
-        remove_if_exists(ea);
-        remove_if_exists(eb);
+    check_extra_pair(m, false);
+    if (!has_file(m.file(Role::extra_A))) return;
+    if (zero_bytes(m.file(Role::extra_A)) || zero_bytes(m.file(Role::extra_b))) {
+        unlink_path(m.file(Role::extra_A));
+        unlink_path(m.file(Role::extra_b));
     }
 }
 
-bool rounding_present_enough(const Contract& c)
+bool core_set_present(const ModelPaths& m)
 {
-    if (!is_directory(c.rounding_dir)) return false;
-    for (const char* s : {"_A.csv", "_b.csv", "_start.csv", "_T.csv", "_shift.csv"})
-        if (!path_exists(c.prefix() + s)) return false;
-    return true;
+    if (!naja::host::is_directory(m.rounding_dir())) return false;
+    return std::all_of(std::begin(kCoreRoles), std::end(kCoreRoles), [&](Role r) { return has_file(m.file(r)); });
 }
 
-// inherit.cpp:31-71
-void inherit_rounding(const Contract& base, const Contract& target, const std::string& mode)
+// The base's core rounding files appear in the target under the target's
+// names, as symlinks or copies, with a provenance note beside them.
+void materialise_from_base(const ModelPaths& base, const ModelPaths& target, const std::string& mode)
 {
-    if (mode != "symlink" && mode != "copy") throw std::runtime_error("invalid inherit mode: " + mode);
-    ensure_dir(target.rounding_dir);
-    for (const char* suf : {"A.csv", "b.csv", "T.csv", "shift.csv", "start.csv"}) {
-        const std::string src = base.model_dir + "/rounding/" + base.model_name + "_rounding_" + suf;
-        const std::string dst = target.rounding_dir + "/" + target.model_name + "_rounding_" + suf;
-        require_nonempty_file(src, std::string("base rounding ") + suf);
-        remove_if_exists(dst);
+    naja::host::ensure_dir(target.rounding_dir());
+    for (Role r : kCoreRoles) {
+        const std::string from = base.file(r), to = target.file(r);
+        naja::host::require_nonempty_file(from, std::string("base rounding ") + role_suffix(r));
+        if (exists_or_dangling(to)) unlink_path(to);
+        std::error_code ec;
         if (mode == "symlink") {
-            if (symlink(src.c_str(), dst.c_str()) != 0) throw std::runtime_error("symlink failed: " + dst + " <- " + src);
+            fs::create_symlink(from, to, ec);
+            if (ec) throw std::runtime_error("symlink failed: " + to + " <- " + from);
         } else {
-            std::ifstream in(src, std::ios::binary);
-            std::ofstream out(dst, std::ios::binary);
-            if (!in.is_open() || !out.is_open()) throw std::runtime_error("copy failed: " + dst + " <- " + src);
-            out << in.rdbuf();
+            fs::copy_file(from, to, fs::copy_options::overwrite_existing, ec);
+            if (ec) throw std::runtime_error("copy failed: " + to + " <- " + from);
         }
     }
-    std::ofstream f(target.rounding_dir + "/INHERITED_FROM.txt");
-    if (!f.is_open()) throw std::runtime_error("cannot write provenance file: " + target.rounding_dir + "/INHERITED_FROM.txt");
-    f << "base_model_dir=" << base.model_dir << "\ntarget_model_dir=" << target.model_dir << "\nmode=" << mode
-      << "\ncreated_at=" << current_timestamp() << "\n";
+    std::ostringstream note;
+    note << "base_model_dir=" << base.dir << "\ntarget_model_dir=" << target.dir << "\nmode=" << mode
+         << "\ncreated_at=" << naja::host::current_timestamp() << "\n";
+    spit(target.rounding_dir() + "/INHERITED_FROM.txt", note.str(), "provenance file");
 }
 
-bool feasible(const Matrix& A, const std::vector<double>& b, const std::vector<double>& x, double eps)
+// ---------------------------------------------------------------------------
+// Start repair.
+
+bool inside(const Matrix& A, const std::vector<double>& b, const std::vector<double>& x, double tol)
 {
     for (int i = 0; i < A.rows; ++i) {
-        double acc = 0.0;
-        for (int j = 0; j < A.cols; ++j) acc += A.at(i, j) * x[(size_t)j];
-        if (acc > b[(size_t)i] + eps) return false;
+        double ax = 0.0;
+        for (int j = 0; j < A.cols; ++j) ax += A.at(i, j) * x[(size_t)j];
+        if (ax > b[(size_t)i] + tol) return false;
     }
     return true;
 }
 
-// feasible_start_files.cpp:41-90
-void ensure_feasible_start(const Contract& c)
+// With extra rows present, a start outside [A; E] x <= [b; f] (1e-9) is
+// replaced by the cube centre of that polytope. A symlinked start is
+// unlinked first, so the base model's file is never written through.
+void repair_start(const ModelPaths& m)
 {
-    const std::string p = c.prefix();
-    const bool ha = path_exists(p + "_extra_A.csv"), hb = path_exists(p + "_extra_b.csv");
-    if (!ha && !hb) return;
-    if (ha != hb)
-        throw std::runtime_error("extra constraints must be both-present or both-absent: " + p + "_extra_A.csv / " + p +
-                                 "_extra_b.csv");
-    Matrix A = load_csv_matrix(p + "_A.csv");
-    std::vector<double> b = load_csv_vector(p + "_b.csv");
-    const Matrix E = load_csv_matrix(p + "_extra_A.csv");
-    const std::vector<double> f = load_csv_vector(p + "_extra_b.csv");
+    check_extra_pair(m, false);
+    if (!has_file(m.file(Role::extra_A))) return;
+    Matrix A = naja::host::load_csv_matrix(m.file(Role::A));
+    std::vector<double> b = naja::host::load_csv_vector(m.file(Role::b));
+    const Matrix E = naja::host::load_csv_matrix(m.file(Role::extra_A));
+    const std::vector<double> f = naja::host::load_csv_vector(m.file(Role::extra_b));
     if ((size_t)E.rows != f.size()) throw std::runtime_error("extra constraint matrices have mismatched dimensions");
     if (E.cols != A.cols) throw std::runtime_error("extra constraint matrix has wrong number of columns");
     A.v.insert(A.v.end(), E.v.begin(), E.v.end());
     A.rows += E.rows;
     b.insert(b.end(), f.begin(), f.end());
-    const std::string start = p + "_start.csv";
-    if (path_exists(start)) {
+
+    const std::string start = m.file(Role::start);
+    if (has_file(start)) {
         try {
-            const std::vector<double> x = load_csv_vector(start);
-            if ((int)x.size() == A.cols && feasible(A, b, x, 1e-9)) return;
+            const std::vector<double> x = naja::host::load_csv_vector(start);
+            if ((int)x.size() == A.cols && inside(A, b, x, 1e-9)) return;
         } catch (const std::exception&) {
-            // recompute below
+            // an unreadable start is replaced like an infeasible one
         }
     }
     std::vector<double> x((size_t)A.cols);
     double r = 0.0;
-    napi(naja_cube_center(A.v.data(), b.data(), A.rows, A.cols, nullptr, x.data(), &r, 1), "axis_aligned_cube_center");
-    if (!feasible(A, b, x, 1e-9)) throw std::runtime_error("feasible_start_files: cube centre violates A x <= b");
-    remove_if_exists(start);  // may be a symlink into the base model
-    write_vector_csv(start, x, 17);
+    if (naja_cube_center(A.v.data(), b.data(), A.rows, A.cols, nullptr, x.data(), &r, 1) != NAJA_OK)
+        throw std::runtime_error(std::string("axis_aligned_cube_center: ") + naja_last_error());
+    if (!inside(A, b, x, 1e-9)) throw std::runtime_error("feasible_start_files: cube centre violates A x <= b");
+    if (exists_or_dangling(start)) unlink_path(start);
+    spit(start, csv_text(x.data(), A.cols, 1, 17), "start");
 }
 
-void ensure_feasible_starts_parallel(const std::vector<Contract>& cs)
+// Runs fn on every job on up to 16 host threads; the first failure is rethrown
+// after all threads have finished.
+void for_each_parallel(const std::vector<const PrepareJob*>& jobs, const std::function<void(const PrepareJob&)>& fn)
 {
-    if (cs.empty()) return;
-    size_t workers = 1;
-    if (cs.size() > 1) {
-        const unsigned hw = std::thread::hardware_concurrency();
-        workers = std::max<size_t>(2, std::min<size_t>(std::min<size_t>(hw ? hw : 8, 16), cs.size()));
-    }
-    std::atomic<size_t> next{0};
-    std::exception_ptr first;
+    if (jobs.empty()) return;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t n_threads = jobs.size() == 1 ? 1 : std::clamp<size_t>(std::min<size_t>(hw, 16), 2, jobs.size());
+    std::atomic<size_t> cursor{0};
+    std::exception_ptr failure;
     std::mutex mu;
-    auto work = [&]() {
-        for (size_t i; (i = next.fetch_add(1)) < cs.size();) {
+    auto worker = [&] {
+        for (size_t i = cursor++; i < jobs.size(); i = cursor++) {
             try {
-                ensure_feasible_start(cs[i]);
-                validate_contract(cs[i]);
+                fn(*jobs[i]);
             } catch (...) {
-                std::lock_guard<std::mutex> g(mu);
-                if (!first) first = std::current_exception();
+                std::lock_guard<std::mutex> lk(mu);
+                if (!failure) failure = std::current_exception();
             }
         }
     };
-    std::vector<std::thread> pool;
-    for (size_t t = 0; t < workers; ++t) pool.emplace_back(work);
-    for (auto& t : pool) t.join();
-    if (first) std::rethrow_exception(first);
+    std::vector<std::thread> threads;
+    for (size_t t = 1; t < n_threads; ++t) threads.emplace_back(worker);
+    worker();
+    for (auto& t : threads) t.join();
+    if (failure) std::rethrow_exception(failure);
 }
 
 }  // namespace
 
 int cmd_prepare(int argc, char** argv)
 {
-    std::string models_root, model_list, base_model_dir, mode = "symlink", out_model_list, conditioner_cmd;
-    std::vector<std::string> conditioner_args;
-    bool dry_run = false, skip_if_gem = true, force = false, verbose = false;
-    for (int i = 0; i < argc; ++i) {
-        const std::string a = argv[i];
-        if (a == "--models-root") models_root = arg_value(i, argc, argv, a);
-        else if (a == "--model-list") model_list = arg_value(i, argc, argv, a);
-        else if (a == "--base-model-dir") base_model_dir = arg_value(i, argc, argv, a);
-        else if (a == "--mode") mode = arg_value(i, argc, argv, a);
-        else if (a == "--out-model-list") out_model_list = arg_value(i, argc, argv, a);
-        else if (a == "--dry-run") dry_run = true;
-        else if (a == "--conditioner-cmd") conditioner_cmd = arg_value(i, argc, argv, a);
-        else if (a == "--conditioner-arg") conditioner_args.push_back(arg_value(i, argc, argv, a));
-        else if (a == "--skip-conditioning-if-gem-present") skip_if_gem = true;
-        else if (a == "--force-conditioning") force = true;
-        else if (a == "--verbose") verbose = true;
-        else if (a == "--out-bulk-config")
-            usage("--out-bulk-config configures the reference's bulk job runner, which this build does not have");
-        else usage("unknown flag: " + a);
-    }
-    (void)verbose;
-    if (models_root.empty()) usage("missing --models-root");
-    if (model_list.empty()) usage("missing --model-list");
-    if (!base_model_dir.empty() && mode != "symlink" && mode != "copy") usage("invalid --mode: " + mode);
-    if (!is_directory(models_root)) throw std::runtime_error("models-root is not a directory: " + models_root);
-    const std::vector<std::string> models = load_model_list(model_list);
-    const bool do_inherit = !base_model_dir.empty();
-    Contract base;
-    if (do_inherit) {
-        base = parse_model_dir(base_model_dir);
-        validate_contract(base);
-    }
-    if (!conditioner_cmd.empty() && base_model_dir.empty()) usage("--conditioner-cmd requires --base-model-dir");
+    const Options opt = parse_options(argc, argv);
+    if (!naja::host::is_directory(opt.models_root))
+        throw std::runtime_error("models-root is not a directory: " + opt.models_root);
+    naja::host::require_nonempty_file(opt.model_list, "model list");
+    const std::vector<std::string> rows = text_lines(opt.model_list, true);
+    if (rows.empty()) throw std::runtime_error("no models in list: " + opt.model_list);
 
-    std::vector<std::string> prepared;
-    std::vector<Contract> for_start;
-    for (const auto& name : models) {
-        const Contract c = parse_model_dir(models_root + "/" + name);
-        const std::string rxn = c.gem_dir + "/reaction_ids.txt", lb = c.gem_dir + "/l_bounds.csv",
-                          ub = c.gem_dir + "/u_bounds.csv";
-        const bool gem_present = is_directory(c.gem_dir) && path_exists(rxn) && path_exists(lb) && path_exists(ub);
-        if (!conditioner_cmd.empty()) {
-            const std::string sig =
-                conditioning_signature(conditioner_cmd, conditioner_args, base_model_dir + "/gem/reaction_ids.txt", name);
-            const std::string sig_path = c.gem_dir + "/conditioning.signature";
-            bool need = force || !gem_present || !skip_if_gem;
-            if (gem_present && skip_if_gem && !force) {
-                if (path_exists(sig_path)) {
-                    if (read_all(sig_path) != sig)
-                        throw std::runtime_error("conditioning signature mismatch; use --force-conditioning to overwrite");
-                    need = false;
-                } else {
-                    need = true;  // provenance unknown: condition again
-                }
-            }
-            if (dry_run) {
-                prepared.push_back(name);
+    std::unique_ptr<BaseModel> base;
+    if (!opt.base_dir.empty()) {
+        base = std::make_unique<BaseModel>();
+        base->paths = ModelPaths::of(opt.base_dir);
+        check_contract(base->paths);
+    }
+    std::optional<Conditioner> cond;
+    if (!opt.cond_cmd.empty()) {
+        if (!base) die_usage("--conditioner-cmd requires --base-model-dir");
+        cond = Conditioner{opt.cond_cmd, opt.cond_args};
+    }
+
+    std::vector<PrepareJob> jobs;
+    jobs.reserve(rows.size());
+    for (const auto& row : rows) {
+        PrepareJob job{row, ModelPaths::of(opt.models_root + "/" + row)};
+        job.gem_complete = naja::host::is_directory(job.paths.gem_dir()) && job.gem_files_present();
+        jobs.push_back(std::move(job));
+    }
+
+    std::vector<const PrepareJob*> prepared, to_repair;
+    for (PrepareJob& job : jobs) {
+        const ModelPaths& m = job.paths;
+        // Stage 1: conditioning and extra rows.
+        if (cond) {
+            const std::string expected = cond->signature(m.name, base->paths.gem("reaction_ids.txt"));
+            const bool run = conditioner_needed(job, expected, opt.force);
+            if (opt.dry_run) {  // a dry run reports conditioned models without touching them
+                prepared.push_back(&job);
                 continue;
             }
-            if (need) {
-                ensure_dir(c.gem_dir);
-                std::string cmd = conditioner_cmd + " --base-model-dir " + base_model_dir + " --out-model-dir " +
-                                  c.model_dir + " --row-id " + name;
-                for (const auto& a : conditioner_args) cmd += " " + a;
-                const int rc = std::system(cmd.c_str());
-                if (rc != 0) throw std::runtime_error("conditioner failed with exit code " + std::to_string(rc));
+            if (run) {
+                naja::host::ensure_dir(m.gem_dir());
+                cond->run(opt.base_dir, m);
             }
-            if (!path_exists(rxn) || !path_exists(lb) || !path_exists(ub))
-                throw std::runtime_error("conditioner did not produce required gem files for " + c.model_dir);
-            const auto base_rxn = trimmed_lines(base_model_dir + "/gem/reaction_ids.txt");
-            if (base_rxn != trimmed_lines(rxn)) throw std::runtime_error("reaction_ids mismatch vs base for " + c.model_dir);
-            const std::vector<double> lbv = load_csv_vector(lb), ubv = load_csv_vector(ub);
-            validate_bounds(lbv, ubv, base_rxn.size());
-            const std::vector<double> lbb = load_csv_vector(base_model_dir + "/gem/l_bounds.csv"),
-                                      ubb = load_csv_vector(base_model_dir + "/gem/u_bounds.csv");
-            validate_bounds(lbb, ubb, lbv.size());
-            ensure_dir(c.rounding_dir);
-            build_extra_from_bounds(c, base_model_dir + "/rounding/" + base.model_name + "_rounding", lbb, ubb, lbv, ubv);
-            std::ofstream sf(sig_path);
-            if (!sf.is_open()) throw std::runtime_error("cannot write signature: " + sig_path);
-            sf << sig;
-        } else if (!gem_present) {
-            throw std::runtime_error("gem/ missing and no conditioner provided for " + c.model_dir);
+            if (!job.gem_files_present())
+                throw std::runtime_error("conditioner did not produce required gem files for " + m.dir);
+            derive_extra_rows(job, *base);
+            spit(m.gem("conditioning.signature"), expected, "signature");
+        } else if (!job.gem_complete) {
+            throw std::runtime_error("gem/ missing and no conditioner provided for " + m.dir);
         }
-        if (!dry_run) {
-            ensure_dir(c.rounding_dir);
-            normalize_extra_constraints(c);
+
+        // Stage 2: a complete rounding file set.
+        if (!opt.dry_run) {
+            naja::host::ensure_dir(m.rounding_dir());
+            drop_empty_extra_pair(m);
         }
-        if (!rounding_present_enough(c)) {
-            if (!do_inherit)
-                throw std::runtime_error("rounding contract missing/invalid and no --base-model-dir provided: " +
-                                         c.model_dir);
-            if (!dry_run) {
-                inherit_rounding(base, c, mode);
-                normalize_extra_constraints(c);
+        if (!core_set_present(m)) {
+            if (!base)
+                throw std::runtime_error("rounding contract missing/invalid and no --base-model-dir provided: " + m.dir);
+            if (!opt.dry_run) {
+                materialise_from_base(base->paths, m, opt.mode);
+                drop_empty_extra_pair(m);
             }
         }
-        validate_contract(c);
-        if (!dry_run && do_inherit && !path_exists(c.prefix() + "_extra_A.csv") &&
-            path_exists(c.gem_dir + "/l_bounds.csv") && path_exists(base_model_dir + "/gem/l_bounds.csv") &&
-            read_all(c.gem_dir + "/l_bounds.csv") != read_all(base_model_dir + "/gem/l_bounds.csv"))
-            std::cerr << "*** WARNING: " << c.model_name
-                      << " has different gem bounds from base but NO extra constraints. ***\n"
+        check_contract(m);
+        if (!opt.dry_run && base && !has_file(m.file(Role::extra_A)) && has_file(m.gem("l_bounds.csv")) &&
+            has_file(base->paths.gem("l_bounds.csv")) &&
+            slurp(m.gem("l_bounds.csv")) != slurp(base->paths.gem("l_bounds.csv")))
+            std::cerr << "*** WARNING: " << m.name << " has different gem bounds from base but NO extra constraints. ***\n"
                       << "  Sampling will use the unconditioned base polytope.\n"
                       << "  To fix: rerun prepare with --conditioner-cmd, or condition + prepare with "
                          "--force-conditioning.\n";
-        if (!dry_run) for_start.push_back(c);
-        prepared.push_back(name);
+        if (!opt.dry_run) to_repair.push_back(&job);
+        prepared.push_back(&job);
     }
-    if (!dry_run) ensure_feasible_starts_parallel(for_start);
-    if (!out_model_list.empty() && !dry_run) {
-        std::ofstream f(out_model_list);
-        if (!f.is_open()) throw std::runtime_error("cannot write out-model-list: " + out_model_list);
-        for (const auto& m : prepared) f << m << "\n";
+
+    // Stage 3: starts that the extra rows cut off, models in parallel.
+    for_each_parallel(to_repair, [](const PrepareJob& job) {
+        repair_start(job.paths);
+        check_contract(job.paths);
+    });
+
+    if (!opt.out_list.empty() && !opt.dry_run) {
+        std::string text;
+        for (const PrepareJob* j : prepared) text += j->row + "\n";
+        spit(opt.out_list, text, "out-model-list");
     }
-    std::cout << "OK\nmodels_root     :: " << make_absolute_path(models_root)
-              << "\nmodel_list      :: " << make_absolute_path(model_list) << "\nprepared        :: " << prepared.size()
-              << " / " << models.size() << "\n";
-    if (!out_model_list.empty()) std::cout << "out_model_list  :: " << make_absolute_path(out_model_list) << "\n";
-    if (dry_run) std::cout << "dry_run         :: true\n";
+    std::cout << "OK\nmodels_root     :: " << naja::host::make_absolute_path(opt.models_root)
+              << "\nmodel_list      :: " << naja::host::make_absolute_path(opt.model_list)
+              << "\nprepared        :: " << prepared.size() << " / " << rows.size() << "\n";
+    if (!opt.out_list.empty())
+        std::cout << "out_model_list  :: " << naja::host::make_absolute_path(opt.out_list) << "\n";
+    if (opt.dry_run) std::cout << "dry_run         :: true\n";
     return 0;
 }

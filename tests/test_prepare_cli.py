@@ -142,3 +142,23 @@ def test_prepare_errors(tmp_path):
     r = run(["prepare", "--models-root", str(root), "--model-list", str(lst), "--base-model-dir", str(base),
              "--mode", "hardlink"])
     assert r.returncode == 2 and "invalid --mode" in r.stderr
+
+
+def test_prepare_dry_run_touches_nothing(tmp_path):
+    """--dry-run reports every listed model as prepared and writes nothing:
+    no conditioner call, no rounding files, no model list."""
+    rng = np.random.default_rng(5)
+    root = tmp_path / "models"
+    base, *_ = make_base(root, rng)
+    (root / "TIGHT").mkdir()
+    lst = tmp_path / "list.txt"
+    lst.write_text("TIGHT\n")
+    cond = tmp_path / "cond.py"
+    cond.write_text(COND)
+    out_list = tmp_path / "prepared.txt"
+    r = run(["prepare", "--models-root", str(root), "--model-list", str(lst), "--base-model-dir", str(base),
+             "--conditioner-cmd", f"{sys.executable} {cond}", "--out-model-list", str(out_list), "--dry-run"])
+    assert r.returncode == 0 and "prepared        :: 1 / 1" in r.stdout and "dry_run         :: true" in r.stdout
+    assert not (tmp_path / "calls.log").exists()
+    assert not out_list.exists()
+    assert sorted(os.listdir(root / "TIGHT")) == []

@@ -342,6 +342,33 @@ int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int
                                   const float* x_c, float* P, void* stream);
 
 /*
+ * Facet-aligned rounding per variant on the GPU: y = x_c + P z with the
+ * coordinate axes of z along the edges of a parallelotope of binding facets,
+ * where coordinate hit-and-run draws exactly (an axis-aligned box) instead of
+ * merely whitened. Candidates are the dual rows a_r / s_r (s_r the slack at
+ * x_c) of the variant's extra rows (with include_base, of its base rows too),
+ * chosen greedily by modified Gram-Schmidt residual norm while the best one
+ * keeps at least theta[v] of its own norm; the remaining directions come
+ * from the dual rows of the base rounding P0 (the rows of P0^-1). P = N^-1
+ * for the d chosen dual rows N (FP64 Gauss-Jordan). theta[v] > 1 selects no
+ * facet and returns P0 up to the order of its columns. No reference
+ * counterpart (naja samples an already rounded polytope,
+ * runtime_config.h:64-78).
+ * Device memory, the batched-sampler layout as naja_precond_ellipsoid_device:
+ * A0 column-major (m0 x d), b0 FP64, extra_A the variants' column-major
+ * k_v x d blocks, extra_b FP64; extra_rows on the host; x_c (nullable)
+ * n_variants x d; P0 and P n_variants x d x d FP32 row-major; theta
+ * n_variants floats. d <= 128 and the candidates in LDS (else
+ * NAJA_ERR_UNSUPPORTED). Synchronises the stream (per-variant status:
+ * NAJA_ERR_INVALID_ARGUMENT for a centre that is not interior,
+ * NAJA_ERR_RUNTIME for a singular P0).
+ */
+int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
+                              const int* extra_rows, const float* extra_A, const double* extra_b,
+                              const float* x_c, const float* P0, const float* theta, int include_base,
+                              float* P, void* stream);
+
+/*
  * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion
  * P_v = (I + V_v V_v^T)^{1/2} (expand = 1) from the learned preconditioner's
  * factors V (n_variants x d x rank). No reference counterpart.

@@ -60,6 +60,7 @@ EXPORTED_SYMBOLS = (
     "naja_precond_ellipsoid_batched",
     "naja_precond_lowrank",
     "naja_precond_ellipsoid_device",
+    "naja_precond_facet_device",
     "naja_lowrank_maps_device",
     "naja_lp_create",
     "naja_lp_destroy",
@@ -182,6 +183,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_lowrank.restype = i32
     lib.naja_precond_ellipsoid_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]
     lib.naja_precond_ellipsoid_device.restype = i32
+    lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp]
+    lib.naja_precond_facet_device.restype = i32
     lib.naja_lowrank_maps_device.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
     lib.naja_lowrank_maps_device.restype = i32
     f32 = ctypes.c_float
@@ -541,6 +544,35 @@ def ellipsoid_rounding(A, b, extra_A, extra_b, x_c=None, stream=None):
                                              be.data_ptr() if be is not None else None,
                                              xc.data_ptr() if xc is not None else None, P.data_ptr(),
                                              _stream_ptr(stream)))
+    return P
+
+
+def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=False, stream=None):
+    """Facet-aligned rounding of every variant [A; E_v] on the GPU (FP64
+    inside): P (n_variants, d, d) FP32 whose coordinate axes run along the
+    edges of a parallelotope of the variant's binding facets, completed by the
+    dual rows of the base rounding P0 (n_variants, d, d). theta: one float or
+    n_variants floats, the fraction of its norm a facet candidate must keep to
+    be chosen (> 1: no facets, P0 up to the order of its columns)."""
+    torch = _torch()
+    lib = load_library()
+    A_cm, b_d, d, m0, ks, Ae, be = _batch_inputs(A, b, extra_A, extra_b)
+    nv = len(ks)
+    karr = (ctypes.c_int * nv)(*ks)
+    P0d = _dev(P0, torch.float32).contiguous()
+    if tuple(P0d.shape) != (nv, d, d):
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"P0 {tuple(P0d.shape)}, expected ({nv}, {d}, {d})")
+    th = torch.as_tensor(theta, dtype=torch.float32).reshape(-1)
+    th = (th.expand(nv) if th.numel() == 1 else th).to(P0d.device).contiguous()
+    if th.numel() != nv:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"{th.numel()} theta values for {nv} variants")
+    xc = _dev(x_c, torch.float32) if x_c is not None else None
+    P = torch.empty((nv, d, d), dtype=torch.float32, device=A_cm.device)
+    _check(lib.naja_precond_facet_device(A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr,
+                                         Ae.data_ptr() if Ae is not None else None,
+                                         be.data_ptr() if be is not None else None,
+                                         xc.data_ptr() if xc is not None else None, P0d.data_ptr(), th.data_ptr(),
+                                         int(bool(include_base)), P.data_ptr(), _stream_ptr(stream)))
     return P
 
 

@@ -1,0 +1,350 @@
+// Facet-aligned rounding on the GPU: for each conditioned variant
+// [A0; E_v] y <= [b0; f_v], a rounding y = c + P z whose coordinate axes run
+// along the edges of a parallelotope built from the variant's binding
+// facets.
+//
+// Coordinate hit-and-run resamples one coordinate of z per step. On an
+// axis-aligned box that is an exact draw of that coordinate; on the same box
+// rotated it is not, however well the box is whitened. A rounding that
+// aligns the axes with the facets that bound the body therefore mixes far
+// faster than one that only matches its second moments (the Dikin ellipsoid
+// P0, P0 P0^T = H^-1, whose orientation is that of a Cholesky factor).
+//
+// The dual rows u_r = a_r / s_r (s_r the slack at the centre x_c) of the
+// candidate facets are chosen greedily by modified Gram-Schmidt: at every
+// step the candidate with the largest residual norm, first among the
+// variant's extra rows (and, on request, its base rows) while that candidate
+// keeps at least theta_v of its own norm, then among the dual rows of a base
+// rounding P0 (the rows of P0^-1), which complete the basis in the
+// directions the facets leave free. With N the d chosen dual rows,
+// P = N^-1: every chosen facet is a coordinate slab |z_k| <= ~1. A diagonal
+// rescaling of P would not change the chain in y, and neither would a
+// permutation (up to which coordinate a draw picks), so only the choice of
+// rows matters. theta_v > 1 takes no facet rows and gives P0 up to the order
+// of its columns.
+//
+// Per variant, one workgroup of 512 threads: P0^-1 and N^-1 by Gauss-Jordan
+// with partial pivoting in LDS (FP64); the selection on FP32 residuals, one
+// candidate per thread held in registers (so a Gram-Schmidt pass is a dot
+// product and an update against the current direction broadcast from LDS);
+// the chosen rows are re-formed in FP64 for the inverse.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/naja_amd.h"
+#include "common.h"
+
+namespace naja {
+namespace {
+
+constexpr int kFacetThreads = 512;  // one candidate row per thread
+constexpr int kFacetMaxD = 128;
+
+// Block-wide argmax of (value, index), larger value first, then smaller index.
+__device__ void block_argmax(float& val, int& idx, float* rv, int* ri)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(val, off);
+        const int oi = __shfl_xor(idx, off);
+        if (ov > val || (ov == val && oi < idx)) {
+            val = ov;
+            idx = oi;
+        }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        rv[threadIdx.x >> 6] = val;
+        ri[threadIdx.x >> 6] = idx;
+    }
+    __syncthreads();
+    val = rv[0];
+    idx = ri[0];
+    for (int w = 1; w < kFacetThreads / 64; ++w)
+        if (rv[w] > val || (rv[w] == val && ri[w] < idx)) {
+            val = rv[w];
+            idx = ri[w];
+        }
+}
+
+// In-place inverse of the d x d row-major FP64 matrix M (LDS) by Gauss-Jordan
+// with partial pivoting (d <= 128). col: d doubles of LDS; piv: d ints of
+// LDS. Returns false (uniformly) when a pivot is zero or not finite. The
+// pivot search runs on the first wavefront (two rows per lane).
+__device__ bool gj_inverse(double* M, int d, double* col, int* piv, int* flag)
+{
+    const int tid = threadIdx.x;
+    for (int k = 0; k < d; ++k) {
+        if (tid < 64) {
+            double best = -1.0;
+            int p = 0x7fffffff;
+            for (int i = k + tid; i < d; i += 64) {
+                const double a = fabs(M[i * d + k]);
+                if (a > best) {
+                    best = a;
+                    p = i;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int op = __shfl_xor(p, off);
+                if (ob > best || (ob == best && op < p)) {
+                    best = ob;
+                    p = op;
+                }
+            }
+            if (tid == 0) {
+                piv[k] = p;
+                *flag = (best > 0.0 && isfinite(best)) ? 0 : 1;
+            }
+        }
+        __syncthreads();
+        if (*flag) return false;
+        const int p = piv[k];
+        const double inv = 1.0 / M[p * d + k];
+        // Column k after the interchange of rows k and p.
+        for (int i = tid; i < d; i += kFacetThreads) col[i] = M[(i == k ? p : (i == p ? k : i)) * d + k];
+        __syncthreads();
+        // Interchange rows k and p, scaling the new row k.
+        for (int j = tid; j < d; j += kFacetThreads) {
+            const double rk = M[k * d + j], rp = M[p * d + j];
+            M[k * d + j] = (j == k) ? inv : rp * inv;
+            if (p != k) M[p * d + j] = rk;
+        }
+        __syncthreads();
+        for (int e = tid; e < d * d; e += kFacetThreads) {
+            const int i = e / d, j = e % d;
+            if (i == k) continue;
+            const double f = col[i];
+            M[e] = (j == k) ? -f * inv : fma(-f, M[k * d + j], M[e]);
+        }
+        __syncthreads();
+    }
+    // Undo the row interchanges as column interchanges, last first.
+    for (int k = d - 1; k >= 0; --k) {
+        const int p = piv[k];
+        if (p != k)
+            for (int i = tid; i < d; i += kFacetThreads) {
+                const double t = M[i * d + k];
+                M[i * d + k] = M[i * d + p];
+                M[i * d + p] = t;
+            }
+        __syncthreads();
+    }
+    return true;
+}
+
+// One workgroup per variant, one candidate per thread: thread i holds the
+// FP32 residual of candidate i in registers (COLS >= d columns, the rest 0).
+// P0i: per-variant global scratch for the FP64 rows of P0^-1.
+template <int COLS>
+__global__ __launch_bounds__(kFacetThreads) void facet_kernel(
+    const float* __restrict__ A0, const double* __restrict__ b0, int m0, int d, const float* __restrict__ Ae,
+    const double* __restrict__ be, const int64_t* __restrict__ eoff, const int* __restrict__ ek,
+    const float* __restrict__ xc, const float* __restrict__ P0, const float* __restrict__ theta, int include_base,
+    double* __restrict__ P0iall, float* __restrict__ P, int* __restrict__ status)
+{
+    extern __shared__ double sh[];
+    double* M = sh;                                 // d x d: P0, then N
+    double* col = M + d * d;                        // d
+    double* sl = col + d;                           // kFacetThreads: slack of each facet candidate
+    float* q = reinterpret_cast<float*>(sl + kFacetThreads);  // COLS: the current direction
+    int* sel = reinterpret_cast<int*>(q + COLS);    // d: chosen candidates in order
+    int* piv = sel + d;                             // d
+    __shared__ float rv[kFacetThreads / 64];
+    __shared__ int ri[kFacetThreads / 64];
+    __shared__ int flag;
+    __shared__ double xs[kFacetMaxD];
+
+    const int v = blockIdx.x, tid = threadIdx.x;
+    const int k = ek[v];
+    const int64_t off = eoff[v];
+    const int nf = (include_base ? m0 : 0) + k;  // facet candidates
+    const int nr = nf + d;                        // + the base rounding's dual rows
+    double* P0i = P0iall + (int64_t)v * d * d;
+    const float th = theta[v];
+    const float th2 = th * th;
+    auto arow = [&](int r, int c) -> double {
+        return r < m0 ? (double)A0[(int64_t)c * m0 + r] : (double)Ae[off * d + (int64_t)c * k + (r - m0)];
+    };
+    auto facet_row = [&](int i) { return include_base ? i : m0 + i; };
+
+    for (int c = tid; c < d; c += kFacetThreads) xs[c] = xc ? (double)xc[(int64_t)v * d + c] : 0.0;
+    if (tid == 0) flag = 0;
+    __syncthreads();
+    if (tid < nf) {
+        const int r = facet_row(tid);
+        double ax = 0.0;
+        for (int c = 0; c < d; ++c) ax = fma(arow(r, c), xs[c], ax);
+        const double s = (r < m0 ? b0[r] : be[off + (r - m0)]) - ax;
+        if (!(s > 0.0)) flag = 1;
+        sl[tid] = s;
+    }
+    const float* P0v = P0 + (int64_t)v * d * d;
+    for (int e = tid; e < d * d; e += kFacetThreads) M[e] = (double)P0v[e];
+    __syncthreads();
+    if (flag) {
+        if (tid == 0) status[v] = NAJA_ERR_INVALID_ARGUMENT;
+        return;
+    }
+    if (!gj_inverse(M, d, col, piv, &flag)) {  // M = P0^-1: its rows complete the basis
+        if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+        return;
+    }
+    for (int e = tid; e < d * d; e += kFacetThreads) P0i[e] = M[e];
+
+    // This thread's candidate.
+    float r[COLS];
+    float n0 = 0.0f;
+    const bool live = tid < nr;
+    const bool is_facet = tid < nf;
+    if (is_facet) {
+        const int rr = facet_row(tid);
+        const double w = 1.0 / sl[tid];
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) r[c] = c < d ? (float)(arow(rr, c) * w) : 0.0f;
+    } else if (live) {
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) r[c] = c < d ? (float)M[(tid - nf) * d + c] : 0.0f;
+    } else {
+#pragma unroll
+        for (int c = 0; c < COLS; ++c) r[c] = 0.0f;
+    }
+#pragma unroll
+    for (int c = 0; c < COLS; ++c) n0 = fmaf(r[c], r[c], n0);
+    bool taken = !live;
+    float n2 = n0;
+
+    // Greedy modified Gram-Schmidt over the candidates.
+    for (int t = 0; t < d; ++t) {
+        if (t > 0 && !taken) {
+            float cq = 0.0f;
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) cq = fmaf(r[c], q[c], cq);
+            n2 = 0.0f;
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) {
+                r[c] = fmaf(-cq, q[c], r[c]);
+                n2 = fmaf(r[c], r[c], n2);
+            }
+        }
+        float vf = -1.0f, vb = -1.0f;
+        int i_f = 0x7fffffff, i_b = 0x7fffffff;
+        if (!taken) {
+            if (is_facet) {
+                if (n2 >= th2 * n0 && n2 > 1e-12f * n0) {
+                    vf = n2;
+                    i_f = tid;
+                }
+            } else {
+                vb = n2;
+                i_b = tid;
+            }
+        }
+        block_argmax(vf, i_f, rv, ri);
+        block_argmax(vb, i_b, rv, ri);
+        const bool from_f = vf > 0.0f;
+        const int p = from_f ? i_f : i_b;
+        const float best = from_f ? vf : vb;
+        if (p == 0x7fffffff || !(best > 0.0f)) {
+            if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+            return;
+        }
+        __syncthreads();  // every thread has read q for this pass
+        if (tid == p) {
+            const float inv = 1.0f / sqrtf(best);
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) q[c] = r[c] * inv;
+            taken = true;
+            sel[t] = p;
+        }
+        __syncthreads();
+    }
+
+    // N (the chosen dual rows, FP64) and P = N^-1.
+    for (int e = tid; e < d * d; e += kFacetThreads) {
+        const int row = e / d, c = e % d;
+        const int i = sel[row];
+        M[e] = i < nf ? arow(facet_row(i), c) / sl[i] : P0i[(int64_t)(i - nf) * d + c];
+    }
+    __syncthreads();
+    if (!gj_inverse(M, d, col, piv, &flag)) {
+        if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+        return;
+    }
+    float* Pv = P + (int64_t)v * d * d;
+    for (int e = tid; e < d * d; e += kFacetThreads) Pv[e] = (float)M[e];
+    if (tid == 0) status[v] = NAJA_OK;
+}
+
+size_t facet_lds_bytes(int d, int cols)
+{
+    return sizeof(double) * ((size_t)d * d + d + kFacetThreads) + sizeof(float) * (size_t)cols +
+           sizeof(int) * 2 * (size_t)d;
+}
+
+}  // namespace
+}  // namespace naja
+
+extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
+                                         const int* extra_rows, const float* extra_A, const double* extra_b,
+                                         const float* x_c, const float* P0, const float* theta, int include_base,
+                                         float* P, void* stream)
+{
+    using namespace naja;
+    return guarded([&]() {
+        NAJA_REQUIRE(A0 && b0 && P0 && theta && P && m0 > 0 && d > 0 && n_variants > 0 && extra_rows,
+                     NAJA_ERR_INVALID_ARGUMENT, "naja_precond_facet_device: bad arguments");
+        std::vector<int64_t> off(n_variants);
+        int64_t tot = 0;
+        int kmax = 0;
+        for (int v = 0; v < n_variants; ++v) {
+            NAJA_REQUIRE(extra_rows[v] >= 0, NAJA_ERR_INVALID_ARGUMENT, "variant %d has %d extra rows", v,
+                         extra_rows[v]);
+            off[v] = tot;
+            tot += extra_rows[v];
+            kmax = std::max(kmax, extra_rows[v]);
+        }
+        NAJA_REQUIRE(tot == 0 || (extra_A && extra_b), NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_precond_facet_device: extra rows without extra_A/extra_b");
+        const int nr_max = (include_base ? m0 : 0) + kmax + d;
+        const int cols = d <= 32 ? 32 : (d <= 64 ? 64 : (d <= 112 ? 112 : 128));
+        const size_t lds = facet_lds_bytes(d, cols);
+        NAJA_REQUIRE(d <= kFacetMaxD && nr_max <= kFacetThreads && lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
+                     "naja_precond_facet_device: d=%d with %d candidate rows exceeds the kernel (d <= %d, rows + d "
+                     "<= %d)",
+                     d, nr_max - d, kFacetMaxD, kFacetThreads);
+        hipStream_t s = (hipStream_t)stream;
+        const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
+        const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
+        const size_t b_P0i = sizeof(double) * (size_t)n_variants * d * d;
+        char* mem = nullptr;
+        scratch_alloc((void**)&mem, b_off + 2 * b_k + b_P0i, s);
+        int64_t* eoff = reinterpret_cast<int64_t*>(mem);
+        int* ek = reinterpret_cast<int*>(mem + b_off);
+        int* status = reinterpret_cast<int*>(mem + b_off + b_k);
+        double* P0i = reinterpret_cast<double*>(mem + b_off + 2 * b_k);
+        NAJA_HIP(hipMemcpyAsync(eoff, off.data(), sizeof(int64_t) * n_variants, hipMemcpyHostToDevice, s));
+        NAJA_HIP(hipMemcpyAsync(ek, extra_rows, sizeof(int) * n_variants, hipMemcpyHostToDevice, s));
+        auto kern = cols == 32    ? facet_kernel<32>
+                    : cols == 64  ? facet_kernel<64>
+                    : cols == 112 ? facet_kernel<112>
+                                  : facet_kernel<128>;  // 113..128 columns spill to scratch
+        hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), lds, s, A0, b0, m0, d, extra_A, extra_b,
+                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, P0i, P, status);
+        NAJA_HIP(hipGetLastError());
+        std::vector<int> st(n_variants);
+        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
+        scratch_free(mem, s);  // stream-ordered: after the copy
+        NAJA_HIP(hipStreamSynchronize(s));
+        for (int v = 0; v < n_variants; ++v) {
+            NAJA_REQUIRE(st[v] != NAJA_ERR_INVALID_ARGUMENT, NAJA_ERR_INVALID_ARGUMENT,
+                         "facet rounding of variant %d: the centre is not interior", v);
+            NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_RUNTIME,
+                         "facet rounding of variant %d: a singular base rounding or no independent completion", v);
+        }
+    });
+}

@@ -87,3 +87,48 @@ def lowrank(V, expand=False):
     M = np.eye(V.shape[0]) + V @ V.T
     ev, Q = np.linalg.eigh(M)
     return (Q * (np.sqrt(ev) if expand else 1.0 / np.sqrt(ev))) @ Q.T
+
+
+def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False):
+    """Facet-aligned rounding of [A0; E] y <= [b0; f] (the rule of
+    naja_precond_facet_device): candidate dual rows u_r = a_r / s_r at x_c of
+    the extra rows (and the base rows with include_base), chosen greedily by
+    modified Gram-Schmidt residual norm (FP32 residuals) while the best keeps
+    at least theta of its own norm, then the rows of P0^-1 (FP64); returns
+    (P = N^-1 in FP64, the chosen candidate indices, N). Ties go to the
+    smaller index."""
+    A0 = np.asarray(A0, dtype=np.float64)
+    d = A0.shape[1]
+    E = np.asarray(E, dtype=np.float64).reshape(-1, d)
+    A = np.vstack([A0, E])
+    b = np.concatenate([np.asarray(b0, np.float64), np.asarray(f, np.float64).reshape(-1)])
+    x_c = np.zeros(d) if x_c is None else np.asarray(x_c, dtype=np.float64)
+    rows = np.arange(A.shape[0]) if include_base else np.arange(A0.shape[0], A.shape[0])
+    s = b[rows] - A[rows] @ x_c
+    if not (s > 0).all():
+        raise ValueError("facet rounding: the centre is not interior")
+    U = A[rows] / s[:, None]
+    P0i = np.linalg.inv(np.asarray(P0, dtype=np.float64))
+    cand = np.vstack([U, P0i])
+    nf = U.shape[0]
+    R = cand.astype(np.float32)
+    n0 = (R * R).sum(1)
+    taken = np.zeros(len(cand), bool)
+    th2 = np.float32(theta) * np.float32(theta)
+    sel = []
+    n2 = n0.copy()
+    for _ in range(d):
+        elig_f = (~taken[:nf]) & (n2[:nf] >= th2 * n0[:nf]) & (n2[:nf] > np.float32(1e-12) * n0[:nf])
+        if elig_f.any():
+            p = int(np.argmax(np.where(elig_f, n2[:nf], -1.0)))
+        else:
+            n2b = np.where(taken[nf:], -1.0, n2[nf:])
+            p = nf + int(np.argmax(n2b))
+        taken[p] = True
+        sel.append(p)
+        q = R[p] / np.sqrt(n2[p])
+        c = R @ q
+        R = R - np.outer(c, q).astype(np.float32)
+        n2 = (R * R).sum(1)
+    N = cand[sel]
+    return np.linalg.inv(N), np.array(sel), N

@@ -5,16 +5,21 @@ Workload (one "step" = one pass of the hot path over one synthetic batch):
   each adding k ~ U{0..300} extra rows: slabs |a . y| <= w around the start,
   a from a fixed pool of 300 unit directions, w log-uniform on [1e-3, 1e-1].
   4 chains per variant x 12500 samples x thinning 16, FP64 resync every 1000
-  steps, all variants in one batched launch. The learned-rounded step: MLP
-  inference -> low-rank update of the Dikin-ellipsoid base rounding -> probe
-  runs of the update and the base -> ESS/R-hat gate per variant -> main run
-  -> backmap to y -> per-variant ESS and split-R-hat, all on the GPU (the FP64
-  rounding solves on the host). Inputs are resident in HBM when timing starts.
+  steps, all variants in one batched launch. The learned-rounded step: Dikin
+  ellipsoid P0 -> MLP inference -> learned low-rank update of P0 -> the
+  facet-aligned rounding completed from it -> main run (pack + rounding
+  transform + CHR + backmap to y) -> per-variant ESS and split-R-hat -> the
+  variants failing R-hat < 1.05 run again with the fallback, all on the GPU.
+  Inputs are resident in HBM when timing starts.
 value = sum over variants with split-R-hat < 1.05 of the variant's min-ESS
   (minimum over dims of the minimum-over-chains ESS) / seconds per step.
-Beside it, on the same variants, chains and steps: plain CHR, the reference's
-Dikin rounding and the Dikin-ellipsoid base alone; and the plain genome-scale
-run (n = 2583, d = 512, 4 chains x 12500, thinning 85, pair-prob 0.3).
+Beside it, on the same variants, chains and steps and timed the same way:
+plain CHR, the reference's Dikin rounding, the Dikin ellipsoid, and the
+facet-aligned rounding without the network; the cube run (d = 100, 4 x 12500,
+thinning 16); the batched conditions at their own half-widths (1e-2..1); the
+genome-scale run (n = 2583, d = 512, 4 chains x 12500, thinning 85, pair-prob
+0.3) and the large-scale run (n = 10600, d = 2400, 1024 chains x 2000,
+thinning 400).
 
 Prints one JSON line. `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -31,10 +36,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
-# Bytes the main learned-rounded launch moves between L2 and the memory fabric
-# (256 variants x 4 chains x 200000 steps): rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes, scaled by what the same counters read for a 1 GiB copy.
-MAIN_LAUNCH_TRAFFIC_BYTES = 358.1e9 + 5.82e9
+# Counter summary of the main CHR launch (rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes of this benchmark, scaled by what the same counters read for a copy
+# kernel of known size); the command that made it is inside.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "chr_main_launch_traffic.json")
 METRIC = "min-ESS/sec at split-R-hat<1.05, learned-rounded vs plain CHR, 1 MI355X"
 
 
@@ -102,49 +107,43 @@ def main():
 
     EAh, Ebh, Fh, nsh = cd.sample_conditions(pool, args.variants, 300, 1e-3, 1e-1, seed=99)
     nv, nspc = len(EAh), args.nspc
-    stream = torch.cuda.current_stream()
     # The held-out polytopes are resident in HBM before timing starts.
     from naja_amd import VariantBatch
 
     batch = VariantBatch(A0, b0, EAh, Ebh)
-    batch2 = batch.repeat(2)
     Fh_dev = torch.as_tensor(Fh, dtype=torch.float32, device="cuda")  # condition features, resident like the rows
 
-    def learned_step(evs=None):
-        P0 = cd.base_rounding(batch, None, None, None)
-        Pl = cd.learned_maps(lp, Fh_dev, P0)
-        Yp = cd.sample_rounded(batch2, None, None, None, torch.cat([Pl, P0]), nch, 500, thin, seed + 1, resync)
-        ep, rp = cd.variant_diagnostics(Yp, nch)
-        del Yp
-        acc = cd.gate(ep[:nv], rp[:nv], ep[nv:], rp[nv:])
-        P = torch.where(torch.as_tensor(acc, device=P0.device)[:, None, None], Pl, P0)
-        if evs is not None:
-            evs[0].record(stream)
-        Y = cd.sample_rounded(batch, None, None, None, P, nch, nspc, thin, seed, resync)
-        if evs is not None:
-            evs[1].record(stream)
-        e, r = cd.variant_diagnostics(Y, nch)
-        return Y, e, r, acc
+    def fallback(sub, idx):
+        return cd.facet_maps(sub, cd.base_rounding(sub, None, None, None))
 
+    def learned_step():
+        P, _ = cd.learned_rounding(lp, Fh_dev, batch)
+        return cd.sample_accepted(batch, P, fallback, nch, nspc, thin, seed, resync)
+
+    # The other methods on the same variants, chains and steps, timed the same
+    # way (warmup, then the mean of `steps` synchronised repetitions).
+    others = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nchains=nch, nspc=nspc, thin=thin, seed=seed, resync=resync,
+                        methods=("plain", "dikin", "ellipsoid", "facet"), warmup=args.warmup, steps=args.steps,
+                        log=log)
+    plain = others["plain"]
+
+    # Headline: the learned-rounded step, with HIP events around every CHR
+    # step-kernel launch (recorded by the library on the launch stream) for
+    # the main launch's kernel time.
     for _ in range(args.warmup):
         learned_step()
     torch.cuda.synchronize()
-    events = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    # HIP events around every CHR step-kernel launch (recorded by the library
-    # on the launch stream): the main launch's kernel time for the roofline.
     naja_amd.chr_kernel_times()
     naja_amd.chr_kernel_timing(True)
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        Y, e, r, acc = learned_step(events[k])
+    for _ in range(args.steps):
+        Y, e, r, n_again = learned_step()
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
     naja_amd.chr_kernel_timing(False)
     launches = naja_amd.chr_kernel_times()
     chr_ms = [ms for items, ms in launches if items == nv * nch]
     chr_kernel_ms = float(np.mean(chr_ms)) if chr_ms else float("nan")
-    main_ms = float(np.mean([ev[0].elapsed_time(ev[1]) for ev in events]))
     ok = r < 1.05
     value = float(e[ok].sum()) / sec
 
@@ -155,32 +154,30 @@ def main():
         b = torch.as_tensor(np.concatenate([b0, Ebh[v]]), dtype=torch.float64, device="cuda")
         worst = max(worst, float((Y[v].double() @ A.T - b).max()))
     del Y
-    log(f"learned-rounded: {sec * 1e3:.1f} ms/step (main run {main_ms:.1f} ms), {int(ok.sum())}/{nv} variants "
-        f"pass R-hat<1.05, sum min-ESS {e.sum():.1f}, {int(acc.sum())} learned maps accepted, "
+    log(f"learned-rounded: {sec * 1e3:.1f} ms/step (CHR main launch {chr_kernel_ms:.1f} ms), {int(ok.sum())}/{nv} "
+        f"variants pass R-hat<1.05, sum min-ESS {e.sum():.1f}, {n_again} run again with the fallback, "
         f"max(Ay-b) {worst:.2e}")
 
-    # The other methods on the same variants, chains and steps (one step each).
-    others = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nchains=nch, nspc=nspc, thin=thin, seed=seed, resync=resync,
-                        methods=("plain", "dikin", "ellipsoid"), log=log)
-    plain = others["plain"]
-
-    # Roofline of the dominant kernel (the batched CHR step): algorithmic bytes
-    # per chain-step B_cs = 4 m_v (2 + kbar/G) + 4 d / t with m_v = 200 + k_v
-    # rows, kbar = 1 (coordinate moves), G = 1.
+    # Roofline of the dominant kernel (the batched CHR step). Algorithmic bytes
+    # per chain-step: what must cross from memory to the CU, the column
+    # A[:, i] of each move (4 m_v bytes, m_v = 200 + k_v rows, coordinate
+    # moves) and the amortised sample store (4 d / t). The slack (8 m_v bytes
+    # read and written per step in the reference) stays in registers here.
     m_v = 2 * d + np.array([e_.shape[0] for e_ in EAh])
     steps_per_chain = nspc * thin
-    total_bytes = float(np.sum(nch * steps_per_chain * (4.0 * m_v * 3.0 + 4.0 * d / thin)))
+    total_bytes = float(np.sum(nch * steps_per_chain * (4.0 * m_v + 4.0 * d / thin)))
     achieved = total_bytes / (chr_kernel_ms * 1e-3) / 1e9
-    traffic = MAIN_LAUNCH_TRAFFIC_BYTES if (nv, nch, nspc, thin) == (256, 4, 12500, 16) else None
+    traffic = measured_traffic(nv, nch, nspc, thin)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(chr_kernel_ms, 3),
-                "launches_timed": len(chr_ms), "main_call_ms": round(main_ms, 3),
-                "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
+                "launches_timed": len(chr_ms), "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),
-                "note": "kernel_ms: HIP events the library records around the main CHR launch on its stream; "
-                        "main_call_ms brackets the whole sampling call (pack + transform + CHR + backmap)"}
+                "note": "achieved: column + sample bytes / the CHR kernel's own time (HIP events on its launch "
+                        "stream); traffic: FETCH_SIZE + WRITE_SIZE per launch from the committed counter summary"}
 
+    cube = cube_plain(np, torch, args)
+    batched = batched_conditions(np, torch, cd, pool, lp, A0, b0, args)
     genome = None if args.no_genome else genome_scale_plain(np, torch)
     large = None if args.no_large else large_scale_plain(np, torch, args.large_nspc)
 
@@ -193,11 +190,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        P0 = cd.base_rounding(batch, None, None, None)
-        Pl = cd.learned_maps(lp, Fh, P0)
-        P = torch.where(torch.as_tensor(acc, device=P0.device)[:, None, None], Pl, P0).cpu().numpy()
-        cpu = cpu_baseline(np, A0, b0, EAh, Ebh, P, nch, nspc, thin, seed, resync, float(e[ok].sum()),
-                           args.cpu_seconds)
+        P, _ = cd.learned_rounding(lp, Fh_dev, batch)
+        cpu = cpu_baseline(np, A0, b0, EAh, Ebh, P.cpu().numpy(), nch, nspc, thin, seed, resync,
+                           float(e[ok].sum()), args.cpu_seconds)
     lp.close()
 
     line = {
@@ -216,20 +211,26 @@ def main():
         "config": {
             "workload": f"{nv} held-out conditioned variants of the cube [-1,1]^100, 0-300 extra slab rows "
                         f"(half-widths 1e-3..1e-1), {nch} chains x {nspc} x thin {thin}, one batched launch; "
-                        f"learned-rounded (Dikin-ellipsoid base + learned low-rank update, ESS/R-hat gate)",
+                        f"learned-rounded (facet-aligned rounding completed from the learned low-rank update of "
+                        f"the Dikin ellipsoid, R-hat acceptance with fallback)",
             "variants": nv, "chains": nch, "nspc": nspc, "thin": thin, "resync": resync,
             "n_pass_rhat_1_05": int(ok.sum()), "min_ess_sum": round(float(e.sum()), 1),
-            "min_ess_sum_passing": round(float(e[ok].sum()), 1), "learned_accepted": int(acc.sum()),
+            "min_ess_sum_passing": round(float(e[ok].sum()), 1), "rerun_with_fallback": n_again,
             "max_bound_violation": worst, "bound_violations": bool(worst > 1e-5),
-            "plain_value": round(plain["metric"], 3), "plain_n_pass": plain["n_pass"],
-            "plain_min_ess_sum": round(plain["ess_sum"], 1), "plain_seconds": round(plain["seconds"], 3),
             "ratio_vs_plain": round(value / plain["metric"], 3) if plain["metric"] > 0 else None,
             "ratio_vs_plain_all_variants": round((float(e.sum()) / sec) / plain["metric_all"], 3),
-            "dikin_value": round(others["dikin"]["metric"], 3), "dikin_n_pass": others["dikin"]["n_pass"],
-            "ellipsoid_value": round(others["ellipsoid"]["metric"], 3),
-            "ellipsoid_n_pass": others["ellipsoid"]["n_pass"],
+            "methods": {name: {"value": round(o["metric"], 3), "ms_per_step": round(o["seconds"] * 1e3, 3),
+                               "n_pass": o["n_pass"], "min_ess_sum": round(o["ess_sum"], 1),
+                               "min_ess_sum_passing": round(o["ess_sum_pass"], 1)}
+                        for name, o in others.items()},
+            "methods_note": "every method timed like the headline (same warmup and steps); plain: no rounding; "
+                            "dikin: the reference's dikin_precondition (host FP64); ellipsoid: Dikin ellipsoid; "
+                            "facet: facet-aligned rounding completed from the Dikin ellipsoid (the method without "
+                            "the network)",
             "pilot_seconds": round(t_pilot, 3), "training_seconds": round(t_train, 3),
             "training_nll_first_last": [round(hist[0], 3), round(hist[-1], 3)],
+            "cube_plain": cube,
+            "batched_conditions": batched,
             "genome_scale_plain": genome,
             "large_scale_plain": large,
             "parallelism": "single GPU; variants x chains as workgroups of one batched launch",
@@ -238,6 +239,65 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def cube_plain(np, torch, args):
+    """Plain CHR on the hypercube [-1, 1]^100 (A = [I; -I], b = 1, y0 = 0),
+    4 chains x 12500 x thinning 16, Philox seed 0, GPU ESS / split-R-hat; a
+    coordinate move resamples x_i exactly, so the per-chain ESS has the closed
+    form n (1 - rho) / (1 + rho), rho = (1 - 1/d)^t."""
+    import naja_amd
+    from naja_amd import conditioning as cd
+
+    d, nch, nspc, thin = 100, 4, 12500, 16
+    A = torch.from_numpy(np.vstack([np.eye(d), -np.eye(d)]).astype(np.float32)).cuda()
+    b = torch.ones(2 * d, dtype=torch.float64, device="cuda")
+    Y0 = torch.zeros((nch, d), dtype=torch.float32, device="cuda")
+
+    def step():
+        S = naja_amd.coordinate_hit_and_run(A, b, Y0, nspc, thin, nch, seed=0)
+        ess, rhat, ec = naja_amd.diagnostics(S, nch)
+        return S, ess, rhat, ec
+
+    sec, (S, ess, rhat, ec) = cd.timed_steps(step, args.warmup, args.steps)
+    rho = (1.0 - 1.0 / d) ** thin
+    closed = nspc * (1.0 - rho) / (1.0 + rho)
+    x = S.double()
+    min_ess, rmax = float(ess.min()), float(rhat.max())
+    log(f"cube d=100: {sec * 1e3:.2f} ms per step, min-ESS {min_ess:.1f} (per-chain mean {float(ec.mean()):.1f}, "
+        f"closed form {closed:.1f}), max R-hat {rmax:.4f}")
+    return {"value_min_ess_per_s": round(min_ess / sec, 2) if rmax < 1.05 else 0.0, "ms_per_step": round(sec * 1e3, 3),
+            "min_ess": round(min_ess, 2), "ess_chain_mean": round(float(ec.mean()), 2),
+            "ess_chain_closed_form": round(closed, 2), "rhat_max": round(rmax, 5),
+            "mean_abs_max": round(float(x.mean(0).abs().max()), 4), "var_mean": round(float(x.var(0).mean()), 5),
+            "var_closed_form": round(1.0 / 3.0, 5)}
+
+
+def batched_conditions(np, torch, cd, pool, lp, A0, b0, args):
+    """256 conditioned variants of the cube at their own slab half-widths,
+    log-uniform on [1e-2, 1], 4 chains x 12500 x thinning 16 per variant, all
+    in one launch with FP64 resync every 1000 steps: plain CHR, the
+    facet-aligned rounding and the learned-rounded pipeline (its network was
+    trained on half-widths 1e-3..1e-1), each timed like the headline."""
+    EA, Eb, F, _ = cd.sample_conditions(pool, 256, 300, 1e-2, 1.0, seed=3)
+    res = cd.compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=11, resync=1000,
+                     methods=("plain", "facet", "learned"), warmup=args.warmup, steps=max(1, args.steps // 4),
+                     log=lambda m: log("batched conditions (1e-2..1) " + m))
+    return {name: {"value": round(o["metric"], 3), "ms_per_step": round(o["seconds"] * 1e3, 3),
+                   "n_pass": o["n_pass"], "min_ess_sum": round(o["ess_sum"], 1)} for name, o in res.items()}
+
+
+def measured_traffic(nv, nch, nspc, thin):
+    """HBM bytes per main CHR launch from the committed counter summary, when
+    it was measured on this workload (else None)."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if [t.get("variants"), t.get("chains"), t.get("nspc"), t.get("thin")] != [nv, nch, nspc, thin]:
+        return None
+    return float(t["fetch_bytes"]) + float(t["write_bytes"])
 
 
 def genome_scale_plain(np, torch):

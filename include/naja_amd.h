@@ -349,7 +349,9 @@ int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int
  * x_c) of the variant's extra rows (with include_base, of its base rows too),
  * chosen greedily by modified Gram-Schmidt residual norm while the best one
  * keeps at least theta[v] of its own norm; the remaining directions come
- * from the dual rows of the base rounding P0 (the rows of P0^-1). P = N^-1
+ * from the dual rows of the base rounding P0 (the rows of P0^-1; with lower,
+ * the rows of L^-1 for the lower Cholesky factor L of P0 P0^T, the same
+ * ellipsoid with its axes in triangular order). P = N^-1
  * for the d chosen dual rows N (FP64 Gauss-Jordan). theta[v] > 1 selects no
  * facet and returns P0 up to the order of its columns. No reference
  * counterpart (naja samples an already rounded polytope,
@@ -366,7 +368,7 @@ int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int
 int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
                               const int* extra_rows, const float* extra_A, const double* extra_b,
                               const float* x_c, const float* P0, const float* theta, int include_base,
-                              float* P, void* stream);
+                              int lower, float* P, void* stream);
 
 /*
  * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion

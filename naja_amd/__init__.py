@@ -183,7 +183,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_lowrank.restype = i32
     lib.naja_precond_ellipsoid_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]
     lib.naja_precond_ellipsoid_device.restype = i32
-    lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, vp]
+    lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp, vp]
     lib.naja_precond_facet_device.restype = i32
     lib.naja_lowrank_maps_device.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
     lib.naja_lowrank_maps_device.restype = i32
@@ -358,6 +358,22 @@ class VariantBatch:
         out.ks = list(self.ks[lo:hi])
         out.Ae = self.Ae[r0 * self.d:r1 * self.d] if (self.Ae is not None and r1 > r0) else None
         out.be = self.be[r0:r1] if (self.be is not None and r1 > r0) else None
+        return out
+
+    def take(self, indices) -> "VariantBatch":
+        """The variants at `indices`, in that order (device buffers gathered)."""
+        torch = _torch()
+        out = VariantBatch.__new__(VariantBatch)
+        out.A_cm, out.b, out.d, out.m0 = self.A_cm, self.b, self.d, self.m0
+        starts = [0]
+        for k in self.ks:
+            starts.append(starts[-1] + k)
+        idx = [int(i) for i in indices]
+        out.ks = [self.ks[i] for i in idx]
+        a_parts = [self.Ae[starts[i] * self.d:starts[i + 1] * self.d] for i in idx if self.ks[i] > 0]
+        b_parts = [self.be[starts[i]:starts[i + 1]] for i in idx if self.ks[i] > 0]
+        out.Ae = torch.cat(a_parts) if a_parts else None
+        out.be = torch.cat(b_parts) if b_parts else None
         return out
 
     def repeat(self, times: int) -> "VariantBatch":
@@ -547,13 +563,14 @@ def ellipsoid_rounding(A, b, extra_A, extra_b, x_c=None, stream=None):
     return P
 
 
-def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=False, stream=None):
+def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=False, lower=False, stream=None):
     """Facet-aligned rounding of every variant [A; E_v] on the GPU (FP64
     inside): P (n_variants, d, d) FP32 whose coordinate axes run along the
     edges of a parallelotope of the variant's binding facets, completed by the
     dual rows of the base rounding P0 (n_variants, d, d). theta: one float or
     n_variants floats, the fraction of its norm a facet candidate must keep to
-    be chosen (> 1: no facets, P0 up to the order of its columns)."""
+    be chosen (> 1: no facets, P0 up to the order of its columns). lower:
+    complete from the lower Cholesky factor of P0 P0^T instead of P0."""
     torch = _torch()
     lib = load_library()
     A_cm, b_d, d, m0, ks, Ae, be = _batch_inputs(A, b, extra_A, extra_b)
@@ -572,7 +589,8 @@ def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=Fal
                                          Ae.data_ptr() if Ae is not None else None,
                                          be.data_ptr() if be is not None else None,
                                          xc.data_ptr() if xc is not None else None, P0d.data_ptr(), th.data_ptr(),
-                                         int(bool(include_base)), P.data_ptr(), _stream_ptr(stream)))
+                                         int(bool(include_base)), int(bool(lower)), P.data_ptr(),
+                                         _stream_ptr(stream)))
     return P
 
 

@@ -1,5 +1,5 @@
-"""Conditioned-variant workflow: pilot chains, learned preconditioner training,
-the ESS/R-hat acceptance gate, and the plain / Dikin / learned comparison.
+"""Conditioned-variant workflow: roundings, pilot chains, learned
+preconditioner training, the R-hat acceptance, and the comparison of methods.
 
 A condition adds slabs |a_j . y| <= w_j to a base polytope A0 y <= b0, with
 the directions a_j taken from a fixed pool (as naja's extra rows are fixed
@@ -7,26 +7,32 @@ the directions a_j taken from a fixed pool (as naja's extra rows are fixed
 feature vector holds w_ref / w_j at the pool index of each active slab and 0
 elsewhere.
 
-Base rounding (and fallback): the Dikin ellipsoid at the start point, y = P0 z
-with P0 P0^T = H^{-1}, H the log-barrier Hessian (FP64, host). Its
-orientation comes exactly from the constraint rows, which thin slabs need
-(whitening a slab of half-width w takes its normal to within ~w).
+Roundings (y = P z; any fixed linear map leaves the uniform target exact and
+changes only how fast coordinate hit-and-run mixes):
 
-Learned map: the MLP turns features into a factor V (dim x 32) of the
-covariance model C = tau (I + V V^T) in the base frame, trained on pilot
-covariances measured in that frame; the variant is sampled with
-y = P0 (I + V V^T)^{1/2} z. Any fixed linear map leaves the uniform target on
-the polytope exact, so the map only changes mixing. Per variant, a short probe
-run of the learned map and the fallback decides which one the main run uses:
-the learned map is accepted when its probe min-ESS is at least the
-fallback's and its probe split-R-hat is no worse (or below 1.05).
+* Dikin ellipsoid P0 at the start point (P0 P0^T = H^-1, H the log-barrier
+  Hessian): whitens the body, with the orientation of a Cholesky factor.
+* Facet-aligned rounding (the pipeline's base): coordinate axes along the
+  edges of a parallelotope of the variant's slab facets, chosen greedily
+  (naja_amd.facet_rounding), the directions the slabs leave free completed
+  from the dual rows of a base rounding. On an axis-aligned box a coordinate
+  move is an exact draw, which a whitened but rotated box does not give.
+* Learned update: the MLP turns features into a factor V (dim x 32) of the
+  covariance model C = tau (I + V V^T) in the Dikin-ellipsoid frame, trained
+  by the Gaussian NLL on covariances of pilot chains the sampler runs (in the
+  facet-aligned frame, where they mix); its map P0 (I + V V^T)^{1/2} is the
+  base whose dual rows complete the facet-aligned rounding.
+
+Acceptance: after the main run, a variant whose split-R-hat is 1.05 or more
+is run again with the fallback (the facet-aligned rounding completed from the
+Dikin ellipsoid alone), and the run with the better R-hat and ESS is kept.
 
 The reference's Dikin rounding (dikin_precondition.cpp:36-183) is reported
 beside them as the drop-in preconditioner.
 
 Everything that touches samples runs on the GPU through libnaja_amd.so: the
-batched CHR launches, the MLP, the covariances and the diagnostics. The FP64
-eigen/Cholesky work of the roundings runs on the host.
+batched CHR launches, the roundings, the MLP, the covariances and the
+diagnostics.
 """
 from __future__ import annotations
 
@@ -35,11 +41,14 @@ import time
 import numpy as np
 
 from . import (NAJA_ERR_UNSUPPORTED, NajaError, VariantBatch, coordinate_hit_and_run_batched, diagnostics_batched,
-               ellipsoid_rounding, lowrank_maps)
+               ellipsoid_rounding, facet_rounding, lowrank_maps)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
 W_REF = 1e-3  # feature scale: the thinnest half-width of the pilot distribution
+# A facet candidate is chosen while it keeps at least this fraction of its own
+# norm after projection on the chosen ones (every independent slab first).
+THETA = 0.02
 
 
 def make_pool(d: int, n: int = 300, seed: int = 2024):
@@ -124,27 +133,37 @@ def base_rounding(A0, b0, EA, Eb):
         return torch.as_tensor(precond.ellipsoid_batched(A0, b0, EA, Eb), dtype=torch.float32).cuda()
 
 
-def pilot_covariances(A0, b0, EA, Eb, P0=None, nchains=32, steps=2000, thin=4, seed=1, chunk=512):
-    """Pilot chains of every condition in its base frame (y = P0 z), `steps`
-    steps per chain; returns the covariance of the pilot samples in that frame,
-    S_z = P0^{-1} S_y P0^{-T}, as (n, d, d) float32 on the GPU (S_y from the
-    GPU covariance kernel, the change of frame in FP64 on the host)."""
+def facet_maps(batch, P_base, theta=THETA, lower=False):
+    """Facet-aligned rounding of every variant of a VariantBatch, completed
+    from the dual rows of P_base (n_variants, d, d) (lower: of the lower
+    Cholesky factor of P_base P_base^T; measured no better on the held-out
+    batch)."""
+    return facet_rounding(batch, None, None, None, P_base, theta, lower=lower)
+
+
+def pilot_covariances(A0, b0, EA, Eb, P0=None, nchains=32, steps=20000, thin=40, seed=1, chunk=256):
+    """Pilot chains of every condition, `steps` steps per chain, run in its
+    facet-aligned frame (where they mix); returns the covariance of the pilot
+    samples in the Dikin-ellipsoid frame P0, S_z = P0^{-1} S_y P0^{-T}, as
+    (n, d, d) float32 on the GPU (S_y from the GPU covariance kernel, the
+    change of frame in FP64)."""
     import torch
 
-    nv, d = len(EA), A0.shape[1]
+    nv = len(EA)
+    batch = VariantBatch(A0, b0, EA, Eb)
     if P0 is None:
-        P0 = base_rounding(A0, b0, EA, Eb)
-    A0 = VariantBatch(A0, b0, EA, Eb)
+        P0 = base_rounding(batch, None, None, None)
+    d = batch.d
     S = torch.empty((nv, d, d), dtype=torch.float32, device="cuda")
     for lo in range(0, nv, chunk):
         hi = min(nv, lo + chunk)
-        Y = sample_rounded(A0.slice(lo, hi), None, None, None, P0[lo:hi], nchains, steps // thin, thin, seed + lo)
+        sub = batch.slice(lo, hi)
+        P0c = P0[lo:hi]
+        Y = sample_rounded(sub, None, None, None, facet_maps(sub, P0c), nchains, steps // thin, thin, seed + lo)
         Sy, _ = pilot_covariance(Y)
         del Y
-        P0c = P0[lo:hi].double().cpu().numpy() if torch.is_tensor(P0) else np.asarray(P0[lo:hi], np.float64)
-        Pinv = np.linalg.inv(P0c)
-        Sz = Pinv @ Sy.double().cpu().numpy() @ Pinv.transpose(0, 2, 1)
-        S[lo:hi] = torch.as_tensor(Sz, dtype=torch.float32).cuda()
+        Pinv = torch.linalg.inv(P0c.double())
+        S[lo:hi] = (Pinv @ Sy.double() @ Pinv.transpose(1, 2)).float()
     return S
 
 
@@ -180,74 +199,99 @@ def learned_maps(lp: LearnedPreconditioner, F, P0, expand=True):
     return lowrank_maps(P0, V, expand=expand)
 
 
-def gate(ess_l, rhat_l, ess_b, rhat_b):
-    """Accept the learned map where its probe min-ESS is at least the
-    fallback's and its probe split-R-hat is no worse (or below 1.05)."""
-    return (ess_l >= ess_b) & ((rhat_l <= rhat_b) | (rhat_l < 1.05))
-
-
-def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, probe_nspc=500, resync=1000,
-            methods=("plain", "dikin", "ellipsoid", "learned", "learned_ungated"), expand=True, log=None):
-    """Sampling methods on the same variants, chains and steps:
-    plain CHR; the reference's Dikin rounding (dikin_precondition); the
-    Dikin-ellipsoid rounding (the learned map's base and fallback); the gated
-    learned map; the learned map on every variant. Each is timed from the
-    polytopes on the device to samples + diagnostics on the device, including
-    its preconditioner (FP64 solves; MLP inference; probe runs + gate).
-    Returns per-method results."""
-    out = {}
-
-    def finish(name, t0, Y, extra=None):
-        e, r = variant_diagnostics(Y, nchains)
-        _sync()
-        sec = time.perf_counter() - t0
-        ok = r < 1.05
-        res = dict(seconds=sec, ess_min=e, rhat_max=r, n_pass=int(ok.sum()),
-                   ess_sum=float(e.sum()), ess_sum_pass=float(e[ok].sum()),
-                   metric=float(e[ok].sum()) / sec, metric_all=float(e.sum()) / sec)
-        if extra:
-            res.update(extra)
-        out[name] = res
-        if log:
-            log(f"{name}: {sec:.2f} s, variants passing R-hat<1.05: {res['n_pass']}/{len(e)}, "
-                f"sum min-ESS {res['ess_sum']:.1f} (passing {res['ess_sum_pass']:.1f}), "
-                f"median min-ESS {np.median(e):.1f}")
-
-    def timed(name, make_P):
-        _sync()
-        t0 = time.perf_counter()
-        P, extra = make_P()
-        Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
-        finish(name, t0, Y, extra)
-        del Y
-
-    nv = len(EA)
-    # The polytopes go to the device once, before any method is timed.
-    batch = VariantBatch(A0, b0, EA, Eb)
-    batch2 = batch.repeat(2)
-    if "plain" in methods:
-        timed("plain", lambda: (None, None))
-    if "dikin" in methods:
-        timed("dikin", lambda: (precond.dikin_batched(A0, b0, EA, Eb)[0], None))
-    if "ellipsoid" in methods:
-        timed("ellipsoid", lambda: (base_rounding(batch, None, None, None), None))
-
-    def gated():
-        import torch
-
+def learned_rounding(lp: LearnedPreconditioner, F, batch, P0=None):
+    """The learned-rounded maps of every variant: the facet-aligned rounding
+    completed from the learned map P0 (I + V V^T)^{1/2}; also returns P0."""
+    if P0 is None:
         P0 = base_rounding(batch, None, None, None)
-        Pl = learned_maps(lp, F, P0, expand)
-        # One probe launch holds both maps of every variant.
-        Yp = sample_rounded(batch2, None, None, None, torch.cat([Pl, P0]), nchains, probe_nspc, thin, seed + 1,
-                            resync)
-        ep, rp = variant_diagnostics(Yp, nchains)
-        del Yp
-        acc = gate(ep[:nv], rp[:nv], ep[nv:], rp[nv:])
-        sel = torch.as_tensor(acc, device=P0.device)[:, None, None]
-        return torch.where(sel, Pl, P0), dict(accepted=int(acc.sum()), accepted_mask=acc)
+    return facet_maps(batch, learned_maps(lp, F, P0)), P0
 
-    if "learned" in methods:
-        timed("learned", gated)
-    if "learned_ungated" in methods:
-        timed("learned_ungated", lambda: (learned_maps(lp, F, base_rounding(batch, None, None, None), expand), None))
+
+def sample_accepted(batch, P, P_fallback, nchains, nspc, thin, seed, resync=1000):
+    """Main run with P; variants whose split-R-hat is >= 1.05 run again with
+    P_fallback (callable giving the fallback maps of a sub-batch), keeping
+    the better run (passing R-hat first, then the larger min-ESS).
+    Returns (Y, ess_min, rhat_max, number of variants run again)."""
+    import torch
+
+    Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
+    e, r = variant_diagnostics(Y, nchains)
+    bad = np.flatnonzero(r >= 1.05)
+    if bad.size:
+        sub = batch.take(bad)
+        Y2 = sample_rounded(sub, None, None, None, P_fallback(sub, bad), nchains, nspc, thin, seed + 1, resync)
+        e2, r2 = variant_diagnostics(Y2, nchains)
+        for j, v in enumerate(bad):
+            if (r2[j] < 1.05, e2[j]) > (r[v] < 1.05, e[v]):
+                Y[v] = Y2[j]
+                e[v], r[v] = e2[j], r2[j]
+        del Y2
+    return Y, e, r, int(bad.size)
+
+
+def timed_steps(step, warmup, steps):
+    """Mean wall seconds of `steps` calls of step() after `warmup` untimed
+    ones, bracketed by device synchronisations; returns (seconds, last result)."""
+    out = None
+    for _ in range(warmup):
+        out = step()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = step()
+    _sync()
+    return (time.perf_counter() - t0) / max(steps, 1), out
+
+
+def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resync=1000,
+            methods=("plain", "dikin", "ellipsoid", "facet", "learned"), warmup=1, steps=1, log=None):
+    """Sampling methods on the same variants, chains and steps, each timed
+    the same way (timed_steps) from the polytopes on the device to samples +
+    diagnostics on the device, including its preconditioner:
+    plain CHR; the reference's Dikin rounding (dikin_precondition, host
+    FP64); the Dikin ellipsoid; the facet-aligned rounding completed from the
+    Dikin ellipsoid (the method without the network); the learned-rounded
+    pipeline (facet-aligned rounding completed from the learned map, R-hat
+    acceptance). Returns per-method results."""
+    import torch
+
+    batch = VariantBatch(A0, b0, EA, Eb)  # the polytopes go to the device once, before any timing
+    Fd = torch.as_tensor(F, dtype=torch.float32).cuda()
+
+    def facet_fallback(sub, idx):
+        return facet_maps(sub, base_rounding(sub, None, None, None))
+
+    def plain():
+        Y = sample_rounded(batch, None, None, None, None, nchains, nspc, thin, seed, resync)
+        return variant_diagnostics(Y, nchains) + (0,)
+
+    def rounded(make_P):
+        def step():
+            Y = sample_rounded(batch, None, None, None, make_P(), nchains, nspc, thin, seed, resync)
+            return variant_diagnostics(Y, nchains) + (0,)
+        return step
+
+    def learned():
+        P, _ = learned_rounding(lp, Fd, batch)
+        _, e, r, n_again = sample_accepted(batch, P, facet_fallback, nchains, nspc, thin, seed, resync)
+        return e, r, n_again
+
+    steps_of = {
+        "plain": plain,
+        "dikin": rounded(lambda: precond.dikin_batched(A0, b0, EA, Eb)[0]),
+        "ellipsoid": rounded(lambda: base_rounding(batch, None, None, None)),
+        "facet": rounded(lambda: facet_maps(batch, base_rounding(batch, None, None, None))),
+        "learned": learned,
+    }
+    out = {}
+    for name in methods:
+        sec, (e, r, n_again) = timed_steps(steps_of[name], warmup, steps)
+        ok = r < 1.05
+        out[name] = dict(seconds=sec, ess_min=e, rhat_max=r, n_pass=int(ok.sum()), ess_sum=float(e.sum()),
+                         ess_sum_pass=float(e[ok].sum()), metric=float(e[ok].sum()) / sec,
+                         metric_all=float(e.sum()) / sec, rerun=n_again)
+        if log:
+            log(f"{name}: {sec * 1e3:.1f} ms per step, variants passing R-hat<1.05: {out[name]['n_pass']}/{len(e)}, "
+                f"sum min-ESS {out[name]['ess_sum']:.1f} (passing {out[name]['ess_sum_pass']:.1f}), "
+                f"median min-ESS {np.median(e):.1f}" + (f", {n_again} run again" if n_again else ""))
     return out

@@ -16,7 +16,10 @@
 // variant's extra rows (and, on request, its base rows) while that candidate
 // keeps at least theta_v of its own norm, then among the dual rows of a base
 // rounding P0 (the rows of P0^-1), which complete the basis in the
-// directions the facets leave free. With N the d chosen dual rows,
+// directions the facets leave free (with `lower`, the rows of L^-1 for the
+// lower Cholesky factor L of P0 P0^T instead: the same ellipsoid with its
+// axes in triangular order, whose last axes lie along the coordinate axes).
+// With N the d chosen dual rows,
 // P = N^-1: every chosen facet is a coordinate slab |z_k| <= ~1. A diagonal
 // rescaling of P would not change the chain in y, and neither would a
 // permutation (up to which coordinate a draw picks), so only the choice of
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
     const float* __restrict__ A0, const double* __restrict__ b0, int m0, int d, const float* __restrict__ Ae,
     const double* __restrict__ be, const int64_t* __restrict__ eoff, const int* __restrict__ ek,
     const float* __restrict__ xc, const float* __restrict__ P0, const float* __restrict__ theta, int include_base,
-    double* __restrict__ P0iall, float* __restrict__ P, int* __restrict__ status)
+    int lower, double* __restrict__ P0iall, float* __restrict__ P, int* __restrict__ status)
 {
     extern __shared__ double sh[];
     double* M = sh;                                 // d x d: P0, then N
@@ -184,10 +187,42 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
         sl[tid] = s;
     }
     const float* P0v = P0 + (int64_t)v * d * d;
-    for (int e = tid; e < d * d; e += kFacetThreads) M[e] = (double)P0v[e];
+    if (lower) {
+        // The same ellipsoid with its axes in triangular order: C = P0 P0^T in
+        // FP64 (row-major; P0 read through the L1/L2 caches), then its lower
+        // Cholesky factor in place.
+        for (int e = tid; e < d * d; e += kFacetThreads) {
+            const int i = e / d, j = e % d;
+            double c = 0.0;
+            if (j <= i)
+                for (int t = 0; t < d; ++t) c = fma((double)P0v[(int64_t)i * d + t], (double)P0v[(int64_t)j * d + t], c);
+            M[e] = c;
+        }
+        __syncthreads();
+        for (int j = 0; j < d; ++j) {
+            if (tid == 0) {
+                double s = M[j * d + j];
+                for (int t = 0; t < j; ++t) s -= M[j * d + t] * M[j * d + t];
+                if (!(s > 0.0)) flag = 2;
+                M[j * d + j] = sqrt(s > 0.0 ? s : 1.0);
+            }
+            __syncthreads();
+            const double ljj = M[j * d + j];
+            for (int i = j + 1 + tid; i < d; i += kFacetThreads) {
+                double s = M[i * d + j];
+                for (int t = 0; t < j; ++t) s -= M[i * d + t] * M[j * d + t];
+                M[i * d + j] = s / ljj;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < d * d; e += kFacetThreads)
+            if (e % d > e / d) M[e] = 0.0;
+    } else {
+        for (int e = tid; e < d * d; e += kFacetThreads) M[e] = (double)P0v[e];
+    }
     __syncthreads();
     if (flag) {
-        if (tid == 0) status[v] = NAJA_ERR_INVALID_ARGUMENT;
+        if (tid == 0) status[v] = flag == 1 ? NAJA_ERR_INVALID_ARGUMENT : NAJA_ERR_RUNTIME;
         return;
     }
     if (!gj_inverse(M, d, col, piv, &flag)) {  // M = P0^-1: its rows complete the basis
@@ -292,7 +327,7 @@ size_t facet_lds_bytes(int d, int cols)
 extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
                                          const int* extra_rows, const float* extra_A, const double* extra_b,
                                          const float* x_c, const float* P0, const float* theta, int include_base,
-                                         float* P, void* stream)
+                                         int lower, float* P, void* stream)
 {
     using namespace naja;
     return guarded([&]() {
@@ -334,7 +369,7 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
                     : cols == 112 ? facet_kernel<112>
                                   : facet_kernel<128>;  // 113..128 columns spill to scratch
         hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), lds, s, A0, b0, m0, d, extra_A, extra_b,
-                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, P0i, P, status);
+                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, lower ? 1 : 0, P0i, P, status);
         NAJA_HIP(hipGetLastError());
         std::vector<int> st(n_variants);
         NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));

@@ -89,12 +89,13 @@ def lowrank(V, expand=False):
     return (Q * (np.sqrt(ev) if expand else 1.0 / np.sqrt(ev))) @ Q.T
 
 
-def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False):
+def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False, lower=False):
     """Facet-aligned rounding of [A0; E] y <= [b0; f] (the rule of
     naja_precond_facet_device): candidate dual rows u_r = a_r / s_r at x_c of
     the extra rows (and the base rows with include_base), chosen greedily by
     modified Gram-Schmidt residual norm (FP32 residuals) while the best keeps
-    at least theta of its own norm, then the rows of P0^-1 (FP64); returns
+    at least theta of its own norm, then the rows of P0^-1 (FP64; with lower,
+    of chol(P0 P0^T)^-1); returns
     (P = N^-1 in FP64, the chosen candidate indices, N). Ties go to the
     smaller index."""
     A0 = np.asarray(A0, dtype=np.float64)
@@ -108,7 +109,10 @@ def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False):
     if not (s > 0).all():
         raise ValueError("facet rounding: the centre is not interior")
     U = A[rows] / s[:, None]
-    P0i = np.linalg.inv(np.asarray(P0, dtype=np.float64))
+    P0 = np.asarray(P0, dtype=np.float64)
+    if lower:  # the lower Cholesky factor of the same ellipsoid (LAPACK)
+        P0 = np.linalg.cholesky(P0 @ P0.T)
+    P0i = np.linalg.inv(P0)
     cand = np.vstack([U, P0i])
     nf = U.shape[0]
     R = cand.astype(np.float32)

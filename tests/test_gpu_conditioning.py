@@ -1,7 +1,8 @@
 """The conditioned-variant workflow end to end on the GPU: pilots in the
-Dikin-ellipsoid frame, training the learned update on their covariances,
-the ESS/R-hat gate, and the comparison with plain CHR, the reference's Dikin
-rounding and the base rounding on held-out conditions."""
+facet-aligned frame, training the learned update on their covariances, the
+learned-rounded pipeline with its R-hat acceptance, and the comparison with
+plain CHR, the reference's Dikin rounding, the Dikin ellipsoid and the
+facet-aligned rounding without the network on held-out conditions."""
 import numpy as np
 import pytest
 
@@ -22,22 +23,19 @@ def test_learned_rounded_vs_plain(gpu):
     hist = cd.train(lp, F, S, epochs=40)
     assert hist[-1] < hist[0]
     EAh, Ebh, Fh, nsh = cd.sample_conditions(pool, 48, 300, 1e-3, 1e-1, seed=99)
-    res = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nspc=3000, thin=16, probe_nspc=250,
-                     log=lambda s: print(s, flush=True))
+    res = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nspc=3000, thin=16, log=lambda s: print(s, flush=True))
     lp.close()
-    plain, dikin, base = res["plain"], res["dikin"], res["ellipsoid"]
-    learned, ungated = res["learned"], res["learned_ungated"]
-    print(f"sum min-ESS: plain {plain['ess_sum']:.0f}, Dikin {dikin['ess_sum']:.0f}, base {base['ess_sum']:.0f}, "
-          f"learned gated {learned['ess_sum']:.0f} ({learned['accepted']} accepted), "
-          f"learned ungated {ungated['ess_sum']:.0f}", flush=True)
-    # The learned-rounded sampler (base + gated update) against plain CHR on
-    # the same variants, chains and steps. At this short length plain CHR sits
-    # at the estimator's floor (~2.7 per variant); the full-length benchmark
-    # reports the ratio at 12500 samples per chain.
+    plain, dikin, ell, facet, learned = (res[k] for k in ("plain", "dikin", "ellipsoid", "facet", "learned"))
+    print(f"sum min-ESS: plain {plain['ess_sum']:.0f}, Dikin {dikin['ess_sum']:.0f}, ellipsoid {ell['ess_sum']:.0f}, "
+          f"facet {facet['ess_sum']:.0f}, learned {learned['ess_sum']:.0f} ({learned['rerun']} run again)",
+          flush=True)
+    # At this short length plain CHR sits at the estimator's floor (~2.7 per
+    # variant); the full-length benchmark reports the ratio at 12500 samples
+    # per chain.
     assert learned["ess_sum"] >= 3.0 * plain["ess_sum"]
-    # The gate keeps the fallback where the update loses.
-    assert learned["ess_sum"] >= 0.8 * base["ess_sum"]
-    assert learned["n_pass"] >= plain["n_pass"]
+    # The learned-rounded pipeline against the Dikin-ellipsoid base rounding.
+    assert learned["ess_sum"] >= ell["ess_sum"]
+    assert learned["n_pass"] >= max(plain["n_pass"], ell["n_pass"])
 
 
 def test_rounded_samples_stay_feasible(gpu):
@@ -51,8 +49,12 @@ def test_rounded_samples_stay_feasible(gpu):
     A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
     pool = cd.make_pool(d)
     EA, Eb, _, _ = cd.sample_conditions(pool, 16, 300, 1e-3, 1e-1, seed=5)
+    import naja_amd
+
+    batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
+    P0 = cd.base_rounding(A0, b0, EA, Eb)
     for name, P in [("plain", None), ("dikin", precond.dikin_batched(A0, b0, EA, Eb)[0]),
-                    ("ellipsoid", cd.base_rounding(A0, b0, EA, Eb))]:
+                    ("ellipsoid", P0), ("facet", cd.facet_maps(batch, P0))]:
         Y = cd.sample_rounded(A0, b0, EA, Eb, P, 4, 1500, 16, seed=3)
         worst = -np.inf
         for v in range(len(EA)):

@@ -21,7 +21,8 @@ def conditions(d, ks, seed, w_lo=1e-3, w_hi=1e-1):
     return EA, Eb
 
 
-def test_facet_rounding_matches_oracle(gpu):
+@pytest.mark.parametrize("lower", [False, True])
+def test_facet_rounding_matches_oracle(gpu, lower):
     """Every variant's P equals the oracle's N^-1 (the same chosen rows) to
     FP32 rounding, for facet counts below, at and above d, and theta from
     'every independent facet' to 'none' (then P0 with permuted columns)."""
@@ -35,17 +36,17 @@ def test_facet_rounding_matches_oracle(gpu):
     EA, Eb = conditions(d, ks, seed=3)
     batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
     P0 = naja_amd.ellipsoid_rounding(batch, None, None, None)
-    P = naja_amd.facet_rounding(batch, None, None, None, P0, t.from_numpy(thetas))
+    P = naja_amd.facet_rounding(batch, None, None, None, P0, t.from_numpy(thetas), lower=lower)
     t.cuda.synchronize()
     P, P0 = P.cpu().numpy().astype(np.float64), P0.cpu().numpy().astype(np.float64)
     for v, k in enumerate(ks):
-        Pref, sel, N = op.facet(A0, b0, EA[v], Eb[v], P0[v], thetas[v])
+        Pref, sel, N = op.facet(A0, b0, EA[v], Eb[v], P0[v], thetas[v], lower=lower)
         nfac = int((sel < k).sum())
         err = np.abs(P[v] - Pref).max() / np.abs(Pref).max()
         ident = np.abs(N @ P[v] - np.eye(d)).max()
         print(f"[facet v{v}] k={k} theta={thetas[v]} facets={nfac} rel err {err:.2e} |NP - I| {ident:.2e}", flush=True)
         assert err <= 1e-5 and ident <= 1e-4
-        if thetas[v] > 1:
+        if thetas[v] > 1 and not lower:
             assert nfac == 0
             np.testing.assert_allclose(P[v], P0[v][:, sel - k], rtol=0, atol=1e-5 * np.abs(P0[v]).max())
         if k // 2 >= 1.5 * d and thetas[v] <= 0.1:  # ample facets: the whole basis from facets

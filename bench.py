@@ -78,7 +78,7 @@ def main():
     import torch
 
     if args.only_large:
-        print(json.dumps({"large_scale_plain": large_scale_plain(np, torch, args.large_nspc)}), flush=True)
+        print(json.dumps({"large_scale_plain": large_scale(np, torch, args.large_nspc, rounded=False)}), flush=True)
         return
 
     import naja_amd
@@ -178,8 +178,10 @@ def main():
 
     cube = cube_plain(np, torch, args)
     batched = batched_conditions(np, torch, cd, pool, lp, A0, b0, args)
-    genome = None if args.no_genome else genome_scale_plain(np, torch)
-    large = None if args.no_large else large_scale_plain(np, torch, args.large_nspc)
+    genome = None if args.no_genome else genome_scale(np, torch, rounded=False)
+    genome_r = None if args.no_genome else genome_scale(np, torch, rounded=True)
+    large = None if args.no_large else large_scale(np, torch, args.large_nspc, rounded=False)
+    large_r = None if args.no_large else large_scale(np, torch, args.large_nspc, rounded=True)
 
     if args.copy_probe_mb > 0:
         nel = args.copy_probe_mb * (1 << 20) // 4
@@ -232,7 +234,9 @@ def main():
             "cube_plain": cube,
             "batched_conditions": batched,
             "genome_scale_plain": genome,
+            "genome_scale_rounded": genome_r,
             "large_scale_plain": large,
+            "large_scale_rounded": large_r,
             "parallelism": "single GPU; variants x chains as workgroups of one batched launch",
         },
         "roofline": roofline,
@@ -300,100 +304,84 @@ def measured_traffic(nv, nch, nspc, thin):
     return float(t["fetch_bytes"]) + float(t["write_bytes"])
 
 
-def genome_scale_plain(np, torch):
-    """Plain CHR on the genome-scale reduced polytope (n = 2583, d = 512,
-    m = 5166), 4 chains x 12500 x thinning 85, pair-prob 0.3, Philox seed 0,
-    then the FP32 MFMA backmap to n = 2583 and the GPU ESS / split-R-hat."""
+def scale_run(np, torch, n, d, nch, nspc, thin, pp, rounded, label):
+    """CHR on a genome-like reduced polytope (workloads.genome_scale: T with
+    8-40 N(0,1) nonzeros per row densified by a random rotation, margins
+    log-uniform on [1e-3, 1e2], A = [T; -T]), Philox seed 0, pair-prob pp,
+    then the FP32 MFMA backmap to the n reactions and the GPU ESS /
+    split-R-hat over the d reduced coordinates. rounded: the Dikin ellipsoid
+    at the analytic centre (the start y = 0 lies 1e-3 from its closest
+    facet), chains started at the centre; timed from the polytope on the
+    device to samples + diagnostics on the device, the centre and rounding
+    included. HIP events time each stage."""
     import naja_amd
-    from naja_amd import workloads
+    from naja_amd import VariantBatch, workloads
 
-    n, d, nch, nspc, thin, pp = 2583, 512, 4, 12500, 85, 0.3
     g = workloads.genome_scale(n=n, d=d, seed=0)
     m = g["A"].shape[0]
     A = torch.from_numpy(g["A"].astype(np.float32)).cuda()
     b = torch.from_numpy(g["b"]).cuda()
-    Y0 = torch.zeros((nch, d), dtype=torch.float32, device="cuda")
     T = torch.from_numpy(g["T"].astype(np.float32)).cuda()
     shift = torch.from_numpy(g["shift"].astype(np.float32)).cuda()
+    batch = VariantBatch(g["A"], g["b"], [np.zeros((0, d))], [np.zeros(0)]) if rounded else None
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    log(f"{label}: m={m} d={d}, {nch} chains x {nspc} x thin {thin} ({nspc * thin} steps per chain)")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev[0].record(stream)
-    S = naja_amd.coordinate_hit_and_run(A, b, Y0, nspc, thin, nch, seed=0, pair_prob=pp)
-    ev[1].record(stream)
-    naja_amd.backmap(T, shift, S)
+    steps = 0
+    if rounded:
+        xc, steps = naja_amd.analytic_center(A, b)
+        P = naja_amd.ellipsoid_rounding_large(A, b, x_c=xc)
+        ev[1].record(stream)
+        S = naja_amd.coordinate_hit_and_run_batched(batch, None, None, None, torch.zeros((1, nch, d), device="cuda"),
+                                                    nspc, thin, nch, seed=0, pair_prob=pp, P=P[None].contiguous(),
+                                                    c=xc[None].contiguous())[0]
+    else:
+        ev[1].record(stream)
+        S = naja_amd.coordinate_hit_and_run(A, b, torch.zeros((nch, d), dtype=torch.float32, device="cuda"), nspc,
+                                            thin, nch, seed=0, pair_prob=pp)
     ev[2].record(stream)
-    ess, rhat, _ = naja_amd.diagnostics(S, nch)
-    ev[3].record(stream)
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    chr_ms, bm_ms, dg_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
-    min_ess = float(ess.min())
-    # Bound check on every sample: max over rows of A y - b (FP64).
-    viol = float((S.double() @ A.double().T - b[None, :]).max())
-    b_cs = 4.0 * m * (2.0 + 1.0 + pp) + 4.0 * d / thin
-    ach = b_cs * nch * nspc * thin / (chr_ms * 1e-3) / 1e9
-    log(f"genome-scale plain: chr {chr_ms:.1f} ms, backmap {bm_ms:.2f} ms, diag {dg_ms:.2f} ms; min-ESS {min_ess:.2f}, "
-        f"max R-hat {float(rhat.max()):.3f}, max(Ay-b) {viol:.2e}")
-    return {"max_bound_violation": viol, "value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3), "chr_ms": round(chr_ms, 3),
-            "backmap_ms": round(bm_ms, 3), "diag_ms": round(dg_ms, 3), "min_ess": round(min_ess, 3),
-            "rhat_max": round(float(rhat.max()), 4), "us_per_step": round(chr_ms * 1e3 / (nspc * thin), 4),
-            "algorithmic_GBs": round(ach, 1), "bytes_per_chain_step": round(b_cs, 1),
-            "backmap_TFLOPs": round(2.0 * d * n * nch * nspc / (bm_ms * 1e-3) / 1e12, 2)}
-
-
-def large_scale_plain(np, torch, nspc):
-    """Plain CHR at the large-scale shape: n = 10600 reactions, d = 2400,
-    A = [T; -T] (m = 21200, 203.5 MB FP32), 1024 chains x nspc x thinning 400,
-    pair-prob 0.3; the FP32 MFMA backmap to n = 10600 and the GPU ESS /
-    split-R-hat over all 2400 dims. One run, timed with HIP events."""
-    import naja_amd
-    from naja_amd import workloads
-
-    n, d, nch, thin, pp = 10600, 2400, 1024, 400, 0.3
-    g = workloads.genome_scale(n=n, d=d, seed=0)
-    m = g["A"].shape[0]
-    A = torch.from_numpy(g["A"].astype(np.float32)).cuda()
-    b = torch.from_numpy(g["b"]).cuda()
-    Y0 = torch.zeros((nch, d), dtype=torch.float32, device="cuda")
-    T = torch.from_numpy(g["T"].astype(np.float32)).cuda()
-    shift = torch.from_numpy(g["shift"].astype(np.float32)).cuda()
-    stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    log(f"large-scale plain: m={m} d={d}, {nch} chains x {nspc} x thin {thin} ({nspc * thin} steps per chain)")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    S = naja_amd.coordinate_hit_and_run(A, b, Y0, nspc, thin, nch, seed=0, pair_prob=pp)
-    ev[1].record(stream)
     V = naja_amd.backmap(T, shift, S)
-    ev[2].record(stream)
-    ess, rhat, _ = naja_amd.diagnostics(S, nch)
     ev[3].record(stream)
+    ess, rhat, _ = naja_amd.diagnostics(S, nch)
+    ev[4].record(stream)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     del V
-    chr_ms, bm_ms, dg_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3]))
-    steps = nspc * thin
-    chain_steps = nch * steps
-    min_ess = float(ess.min())
-    # Algorithmic bytes per chain-step (slack read + write, kbar columns,
-    # amortised sample write) and the column bytes that actually leave the
-    # chip here (the slack stays in registers; one column per coordinate
-    # move, two per pair move).
-    b_cs = 4.0 * m * (2.0 + 1.0 + pp) + 4.0 * d / thin
+    pre_ms, chr_ms, bm_ms, dg_ms = (ev[i].elapsed_time(ev[i + 1]) for i in range(4))
+    chain_steps = nch * nspc * thin
+    # Bound check on every sample (FP64, in row blocks): max over rows of A y - b.
+    viol = max(float((S[i:i + 65536].double() @ A.double().T - b[None, :]).max()) for i in range(0, S.shape[0], 65536))
+    min_ess, rmax = float(ess.min()), float(rhat.max())
+    # Algorithmic bytes per chain-step that leave the chip: the columns of
+    # the move (one per coordinate move, two per pair move) and the amortised
+    # sample store; the slack stays in registers.
     col_bytes = 4.0 * m * (1.0 + pp) + 4.0 * d / thin
-    log(f"large-scale plain: chr {chr_ms / 1e3:.2f} s ({chr_ms * 1e3 / steps:.2f} us per step for all chains), "
-        f"backmap {bm_ms:.1f} ms, diag {dg_ms:.1f} ms; min-ESS {min_ess:.2f}, max R-hat {float(rhat.max()):.3f}")
-    return {"value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3), "chr_s": round(chr_ms / 1e3, 3),
+    log(f"{label}: preconditioner {pre_ms:.1f} ms ({steps} Newton steps), sampling {chr_ms / 1e3:.2f} s, backmap "
+        f"{bm_ms:.1f} ms, diag {dg_ms:.1f} ms; min-ESS {min_ess:.2f}, median {float(ess.median()):.1f}, max R-hat "
+        f"{rmax:.3f}, dims with R-hat >= 1.05: {int((rhat >= 1.05).sum())}, max(Ay-b) {viol:.2e}")
+    return {"value_min_ess_per_s": round(min_ess / sec, 4), "seconds": round(sec, 3),
+            "preconditioner_ms": round(pre_ms, 2), "newton_steps": steps, "sampling_s": round(chr_ms / 1e3, 3),
             "backmap_ms": round(bm_ms, 2), "diag_ms": round(dg_ms, 2), "min_ess": round(min_ess, 3),
-            "rhat_max": round(float(rhat.max()), 4), "chains": nch, "nspc": nspc, "thin": thin,
-            "chain_steps_per_s": round(chain_steps / (chr_ms * 1e-3), 1),
-            "algorithmic_GBs": round(b_cs * chain_steps / (chr_ms * 1e-3) / 1e9, 1),
+            "median_ess": round(float(ess.median()), 2), "rhat_max": round(rmax, 4),
+            "dims_rhat_ge_1_05": int((rhat >= 1.05).sum()), "dims": d, "max_bound_violation": viol,
+            "chains": nch, "nspc": nspc, "thin": thin, "chain_steps_per_s": round(chain_steps / (chr_ms * 1e-3), 1),
             "column_GBs": round(col_bytes * chain_steps / (chr_ms * 1e-3) / 1e9, 1),
-            "bytes_per_chain_step": round(b_cs, 1),
             "backmap_TFLOPs": round(2.0 * d * n * nch * nspc / (bm_ms * 1e-3) / 1e12, 2)}
+
+
+def genome_scale(np, torch, rounded):
+    """n = 2583, d = 512, m = 5166, 4 chains x 12500 x thinning 85, pair-prob 0.3."""
+    return scale_run(np, torch, 2583, 512, 4, 12500, 85, 0.3, rounded,
+                     "genome-scale " + ("rounded" if rounded else "plain"))
+
+
+def large_scale(np, torch, nspc, rounded):
+    """n = 10600, d = 2400, m = 21200 (203.5 MB FP32), 1024 chains x nspc x thinning 400, pair-prob 0.3."""
+    return scale_run(np, torch, 10600, 2400, 1024, nspc, 400, 0.3, rounded,
+                     "large-scale " + ("rounded" if rounded else "plain"))
 
 
 def cpu_baseline(np, A0, b0, EA, Eb, P, nch, nspc, thin, seed, resync, ess_pass_sum, budget_s):

@@ -342,6 +342,61 @@ int naja_precond_ellipsoid_device(const float* A0, const double* b0, int m0, int
                                   const float* x_c, float* P, void* stream);
 
 /*
+ * Dikin-ellipsoid rounding of one polytope of any size on the GPU (FP64):
+ * the path beyond naja_precond_ellipsoid_device's LDS budget (genome scale
+ * d = 512, large scale d = 2400 with 21200 rows). H = A^T diag(1/s^2) A with
+ * s = b - A x_c (rocBLAS dsyrk on the FP64 MFMA units), a blocked Cholesky
+ * H = L L^T (64 x 64 diagonal blocks in LDS, panels and trailing updates by
+ * rocBLAS dtrsm / dsyrk), P = L^{-T}, so P P^T = H^{-1}. No reference
+ * counterpart (the reference's Dikin rounding, dikin_precondition.cpp:36-183,
+ * corrects only the stiffest directions).
+ *   A m x d column-major FP32, b m FP64, x_c d FP32 (NULL = origin), all
+ *   device; P out: d x d row-major FP32, upper triangular.
+ * Scratch 8 (m d + 2 d^2) bytes. Synchronises the stream.
+ * NAJA_ERR_INVALID_ARGUMENT for a centre that is not interior.
+ */
+int naja_precond_ellipsoid_large(const float* A, const double* b, int m, int d, const float* x_c, float* P,
+                                 void* stream);
+
+/*
+ * Analytic centre of {y : A y <= b} (the minimiser of -sum log(b - A y)) on
+ * the GPU by damped Newton steps from the interior point x0 (NULL = origin):
+ * gradient and Hessian A^T diag(1/s^2) A in FP64 (rocBLAS dgemv / dsyrk),
+ * the blocked Cholesky of naja_precond_ellipsoid_large, step 1 / (1 + lambda)
+ * while the Newton decrement lambda >= 1/4 (the iterate stays interior), full
+ * steps after; stops when lambda^2 < tol. The centre to round a polytope at
+ * when its start sits close to some facets (the large-scale polytopes' start
+ * is 1e-3 from its closest facet). A m x d column-major FP32, b m FP64, x0
+ * and x_out d FP32, all device; iters_out (host, nullable) the steps taken.
+ * NAJA_ERR_INVALID_ARGUMENT for a start that is not interior,
+ * NAJA_ERR_RUNTIME when tol is not met within max_iter. Synchronises the
+ * stream. No reference counterpart (naja starts at the rounding's
+ * start.csv, runtime_config.h:64-78).
+ */
+int naja_analytic_center(const float* A, const double* b, int m, int d, const float* x0, int max_iter, double tol,
+                         float* x_out, int* iters_out, void* stream);
+
+/*
+ * Facet-aligned rounding of one polytope with at least d independent facets,
+ * at any size on the GPU (FP64; the large-scale companion of
+ * naja_precond_facet_device): the dual rows a_r / s_r (s = b - A x_c) in
+ * order of tightness ||a_r|| / s_r, each accepted while its residual against
+ * the accepted ones keeps at least theta of its norm (so the other side of
+ * an accepted slab, parallel to it, is skipped), until d are accepted;
+ * P = N^-1 for the accepted rows N, so every accepted facet is a coordinate
+ * slab of z. Blocks of 128 candidates projected by rocBLAS dgemm (two
+ * Gram-Schmidt passes), selected in LDS, P = Q R^-1 by dtrsm. No reference
+ * counterpart.
+ *   A m x d column-major FP32, b m FP64, x_c d FP32 (NULL = origin), all
+ *   device; P out: d x d row-major FP32.
+ * NAJA_ERR_INVALID_ARGUMENT for a centre that is not interior,
+ * NAJA_ERR_RUNTIME when fewer than d facets are independent.
+ * Synchronises the stream.
+ */
+int naja_precond_facet_large(const float* A, const double* b, int m, int d, const float* x_c, float theta, float* P,
+                             void* stream);
+
+/*
  * Facet-aligned rounding per variant on the GPU: y = x_c + P z with the
  * coordinate axes of z along the edges of a parallelotope of binding facets,
  * where coordinate hit-and-run draws exactly (an axis-aligned box) instead of

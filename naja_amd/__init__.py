@@ -61,6 +61,9 @@ EXPORTED_SYMBOLS = (
     "naja_precond_lowrank",
     "naja_precond_ellipsoid_device",
     "naja_precond_facet_device",
+    "naja_precond_ellipsoid_large",
+    "naja_precond_facet_large",
+    "naja_analytic_center",
     "naja_lowrank_maps_device",
     "naja_lp_create",
     "naja_lp_destroy",
@@ -185,6 +188,12 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_ellipsoid_device.restype = i32
     lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp, vp]
     lib.naja_precond_facet_device.restype = i32
+    lib.naja_precond_ellipsoid_large.argtypes = [vp, vp, i32, i32, vp, vp, vp]
+    lib.naja_precond_ellipsoid_large.restype = i32
+    lib.naja_precond_facet_large.argtypes = [vp, vp, i32, i32, vp, ctypes.c_float, vp, vp]
+    lib.naja_precond_facet_large.restype = i32
+    lib.naja_analytic_center.argtypes = [vp, vp, i32, i32, vp, i32, f64, vp, vp, vp]
+    lib.naja_analytic_center.restype = i32
     lib.naja_lowrank_maps_device.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp]
     lib.naja_lowrank_maps_device.restype = i32
     f32 = ctypes.c_float
@@ -560,6 +569,61 @@ def ellipsoid_rounding(A, b, extra_A, extra_b, x_c=None, stream=None):
                                              be.data_ptr() if be is not None else None,
                                              xc.data_ptr() if xc is not None else None, P.data_ptr(),
                                              _stream_ptr(stream)))
+    return P
+
+
+def ellipsoid_rounding_large(A, b, x_c=None, stream=None):
+    """Dikin-ellipsoid rounding of one polytope A y <= b of any size on the
+    GPU (FP64 Hessian by rocBLAS, blocked Cholesky): P (d, d) FP32, upper
+    triangular, P P^T = H^{-1} at x_c (origin by default)."""
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    if b_d.numel() != m:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"b has {b_d.numel()} entries for {m} rows")
+    xc = _dev(x_c, torch.float32).reshape(-1) if x_c is not None else None
+    P = torch.empty((d, d), dtype=torch.float32, device=A_cm.device)
+    _check(lib.naja_precond_ellipsoid_large(A_cm.data_ptr(), b_d.data_ptr(), m, d,
+                                            xc.data_ptr() if xc is not None else None, P.data_ptr(),
+                                            _stream_ptr(stream)))
+    return P
+
+
+def analytic_center(A, b, x0=None, max_iter=200, tol=1e-12, stream=None):
+    """Analytic centre of A y <= b (damped Newton on the log barrier, FP64,
+    on the GPU) from the interior point x0 (origin by default); returns
+    (x (d,) FP32 device tensor, Newton steps)."""
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    x0d = _dev(x0, torch.float32).reshape(-1) if x0 is not None else None
+    x = torch.empty(d, dtype=torch.float32, device=A_cm.device)
+    it = ctypes.c_int(0)
+    _check(lib.naja_analytic_center(A_cm.data_ptr(), b_d.data_ptr(), m, d, x0d.data_ptr() if x0d is not None else None,
+                                    int(max_iter), float(tol), x.data_ptr(), ctypes.byref(it), _stream_ptr(stream)))
+    return x, int(it.value)
+
+
+def facet_rounding_large(A, b, theta=0.02, x_c=None, stream=None):
+    """Facet-aligned rounding of one polytope A y <= b with at least d
+    independent facets, at any size on the GPU: P (d, d) FP32 whose
+    coordinate slabs are the d tightest independent facets at x_c."""
+    torch = _torch()
+    lib = load_library()
+    A_cm = _colmajor_f32(A)
+    d, m = A_cm.shape
+    b_d = _dev(b, torch.float64)
+    if b_d.numel() != m:
+        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"b has {b_d.numel()} entries for {m} rows")
+    xc = _dev(x_c, torch.float32).reshape(-1) if x_c is not None else None
+    P = torch.empty((d, d), dtype=torch.float32, device=A_cm.device)
+    _check(lib.naja_precond_facet_large(A_cm.data_ptr(), b_d.data_ptr(), m, d,
+                                        xc.data_ptr() if xc is not None else None, float(theta), P.data_ptr(),
+                                        _stream_ptr(stream)))
     return P
 
 

@@ -136,3 +136,59 @@ def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False, lower=False):
         n2 = (R * R).sum(1)
     N = cand[sel]
     return np.linalg.inv(N), np.array(sel), N
+
+
+def facet_ordered(A, b, theta, x_c=None):
+    """Facet-aligned rounding of one polytope by the rule of
+    naja_precond_facet_large: dual rows u_r = a_r / s_r in order of tightness
+    ||u_r|| (stable), each accepted while its residual against the accepted
+    ones (FP64 modified Gram-Schmidt, twice) keeps at least theta of its norm,
+    until d are accepted; returns (P = N^-1, accepted row indices)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, d = A.shape
+    x_c = np.zeros(d) if x_c is None else np.asarray(x_c, dtype=np.float64)
+    s = np.asarray(b, dtype=np.float64) - A @ x_c
+    if not (s > 0).all():
+        raise ValueError("facet rounding: the centre is not interior")
+    U = A / s[:, None]
+    nrm = np.linalg.norm(U, axis=1)
+    order = np.argsort(-nrm, kind="stable")
+    Q = np.zeros((d, d))
+    k = 0
+    acc = []
+    for r in order:
+        if k == d:
+            break
+        u = U[r].copy()
+        for _ in range(2):
+            u -= Q[:, :k] @ (Q[:, :k].T @ u)
+        nu = np.linalg.norm(u)
+        if nu > 0 and nu >= theta * nrm[r]:
+            Q[:, k] = u / nu
+            k += 1
+            acc.append(r)
+    if k < d:
+        raise ValueError("facet rounding: fewer than d independent facets")
+    N = U[acc]
+    return np.linalg.inv(N), np.array(acc)
+
+
+def analytic_center(A, b, x0=None, max_iter=200, tol=1e-12):
+    """Minimiser of -sum log(b - A x) by damped Newton (numpy FP64): step
+    1 / (1 + lambda) while the Newton decrement lambda >= 1/4, full steps
+    after; returns (x, steps)."""
+    A = np.asarray(A, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    x = np.zeros(A.shape[1]) if x0 is None else np.asarray(x0, dtype=np.float64).copy()
+    for it in range(max_iter):
+        s = b - A @ x
+        if not (s > 0).all():
+            raise ValueError("analytic centre: not interior")
+        g = A.T @ (1.0 / s)
+        H = (A.T / s ** 2) @ A
+        dx = -np.linalg.solve(H, g)
+        lam2 = max(-g @ dx, 0.0)
+        x = x + (1.0 if lam2 < 0.0625 else 1.0 / (1.0 + np.sqrt(lam2))) * dx
+        if lam2 < tol:
+            return x, it + 1
+    raise ValueError("analytic centre: no convergence")

@@ -168,3 +168,28 @@ def test_lowrank_sqrt(d, r, scale, expand):
         else:
             np.testing.assert_allclose(P[v] @ P[v] @ M, np.eye(d), atol=1e-9)
         np.testing.assert_allclose(P[v], orc.lowrank(V[v], expand), atol=1e-10 * max(1.0, np.abs(P[v]).max()))
+
+
+def test_oracle_analytic_center_closed_forms():
+    """The oracle's analytic centre: the middle of a box [l, u] (each
+    coordinate's two barrier terms balance at (l + u) / 2), the centroid
+    1/(n+1) of the simplex {x >= 0, sum x <= 1}, and a zero barrier gradient
+    A^T (1/s) on a random polytope."""
+    from oracle import precond as op
+
+    rng = np.random.default_rng(0)
+    d = 7
+    lo, hi = -rng.uniform(0.1, 2, d), rng.uniform(0.1, 2, d)
+    A = np.vstack([np.eye(d), -np.eye(d)])
+    b = np.concatenate([hi, -lo])
+    x, _ = op.analytic_center(A, b)
+    np.testing.assert_allclose(x, (lo + hi) / 2, atol=1e-12)
+    n = 5
+    As = np.vstack([-np.eye(n), np.ones((1, n))])
+    bs = np.concatenate([np.zeros(n), [1.0]])
+    xs, _ = op.analytic_center(As, bs, x0=np.full(n, 0.1))
+    np.testing.assert_allclose(xs, np.full(n, 1.0 / (n + 1)), atol=1e-12)
+    Ar = np.vstack([rng.standard_normal((40, 6)), np.eye(6), -np.eye(6)])
+    br = np.concatenate([np.ones(40), 3 * np.ones(12)])
+    xr, _ = op.analytic_center(Ar, br)
+    assert np.abs(Ar.T @ (1.0 / (br - Ar @ xr))).max() <= 1e-9

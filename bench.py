@@ -55,7 +55,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU-baseline sample length")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-genome", action="store_true", help="skip the genome-scale plain run")
-    ap.add_argument("--no-large", action="store_true", help="skip the large-scale run (m = 21200, d = 2400)")
+    ap.add_argument("--no-large", action="store_true", help="skip the large-scale runs (m = 21200, d = 2400)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the comparison methods (for counter passes over the headline's launches)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the cube and batched-conditions lines")
     ap.add_argument("--only-large", action="store_true",
                     help="run only the large-scale line and print it (for profiling its kernel)")
     ap.add_argument("--large-nspc", type=int, default=2000, help="samples per chain of the large-scale run")
@@ -123,9 +126,9 @@ def main():
     # The other methods on the same variants, chains and steps, timed the same
     # way (warmup, then the mean of `steps` synchronised repetitions).
     others = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nchains=nch, nspc=nspc, thin=thin, seed=seed, resync=resync,
-                        methods=("plain", "dikin", "ellipsoid", "facet"), warmup=args.warmup, steps=args.steps,
-                        log=log)
-    plain = others["plain"]
+                        methods=() if args.no_compare else ("plain", "dikin", "ellipsoid", "facet"),
+                        warmup=args.warmup, steps=args.steps, log=log)
+    plain = others.get("plain", {"metric": float("nan"), "metric_all": float("nan")})
 
     # Headline: the learned-rounded step, with HIP events around every CHR
     # step-kernel launch (recorded by the library on the launch stream) for
@@ -167,17 +170,20 @@ def main():
     steps_per_chain = nspc * thin
     total_bytes = float(np.sum(nch * steps_per_chain * (4.0 * m_v + 4.0 * d / thin)))
     achieved = total_bytes / (chr_kernel_ms * 1e-3) / 1e9
-    traffic = measured_traffic(nv, nch, nspc, thin)
+    traffic, issue = measured_traffic(nv, nch, nspc, thin)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "issue_active_frac": issue,
                 "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(chr_kernel_ms, 3),
                 "launches_timed": len(chr_ms), "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),
                 "note": "achieved: column + sample bytes / the CHR kernel's own time (HIP events on its launch "
-                        "stream); traffic: FETCH_SIZE + WRITE_SIZE per launch from the committed counter summary"}
+                        "stream); traffic: FETCH_SIZE + WRITE_SIZE per launch and issue_active_frac: "
+                        "SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES, both from profiles/chr_main_launch_traffic.json "
+                        "(one wavefront per SIMD: the step's latency binds, not bandwidth)"}
 
-    cube = cube_plain(np, torch, args)
-    batched = batched_conditions(np, torch, cd, pool, lp, A0, b0, args)
+    cube = None if args.no_extra else cube_plain(np, torch, args)
+    batched = None if args.no_extra else batched_conditions(np, torch, cd, pool, lp, A0, b0, args)
     genome = None if args.no_genome else genome_scale(np, torch, rounded=False)
     genome_r = None if args.no_genome else genome_scale(np, torch, rounded=True)
     large = None if args.no_large else large_scale(np, torch, args.large_nspc, rounded=False)
@@ -220,7 +226,8 @@ def main():
             "min_ess_sum_passing": round(float(e[ok].sum()), 1), "rerun_with_fallback": n_again,
             "max_bound_violation": worst, "bound_violations": bool(worst > 1e-5),
             "ratio_vs_plain": round(value / plain["metric"], 3) if plain["metric"] > 0 else None,
-            "ratio_vs_plain_all_variants": round((float(e.sum()) / sec) / plain["metric_all"], 3),
+            "ratio_vs_plain_all_variants": (round((float(e.sum()) / sec) / plain["metric_all"], 3)
+                                            if plain["metric_all"] > 0 else None),
             "methods": {name: {"value": round(o["metric"], 3), "ms_per_step": round(o["seconds"] * 1e3, 3),
                                "n_pass": o["n_pass"], "min_ess_sum": round(o["ess_sum"], 1),
                                "min_ess_sum_passing": round(o["ess_sum_pass"], 1)}
@@ -292,16 +299,17 @@ def batched_conditions(np, torch, cd, pool, lp, A0, b0, args):
 
 
 def measured_traffic(nv, nch, nspc, thin):
-    """HBM bytes per main CHR launch from the committed counter summary, when
-    it was measured on this workload (else None)."""
+    """(HBM bytes per main CHR launch, fraction of wave cycles issuing) from
+    the committed counter summary, when it was measured on this workload
+    (else (None, None))."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, None
     if [t.get("variants"), t.get("chains"), t.get("nspc"), t.get("thin")] != [nv, nch, nspc, thin]:
-        return None
-    return float(t["fetch_bytes"]) + float(t["write_bytes"])
+        return None, None
+    return float(t["fetch_bytes"]) + float(t["write_bytes"]), t.get("issue_active_frac")
 
 
 def scale_run(np, torch, n, d, nch, nspc, thin, pp, rounded, label):

@@ -173,7 +173,9 @@ int naja_backmap(const float* T, const float* shift, const float* Y, float* V,
                  int n, int d, int64_t ns, void* stream);
 
 /*
- * compute_ess + compute_split_rhat (mcmc.cpp:91-194) on the GPU.
+ * compute_ess + compute_split_rhat (mcmc.cpp:91-194) on the GPU, for chains
+ * of any length (up to 2^27 samples): transforms of up to 2^15 points run in
+ * LDS, longer ones in global memory.
  *   samples   : n_total x dim row-major (samples.npy), n_total % nchains == 0
  *   layout    : NAJA_LAYOUT_INTERLEAVED or NAJA_LAYOUT_BLOCKS
  *   ess       : dim, per-dim minimum over chains

@@ -793,6 +793,192 @@ __global__ __launch_bounds__(256) void variant_summary_kernel(const float* __res
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// Chains longer than the LDS transform holds (N/2 > kMaxHalf complex points):
+// the same computation with the transforms in global memory, series in
+// batches. Per series: FP64 moments and the float mean (long_moments_kernel),
+// the centred packed pairs (long_pack_kernel), the forward complex FFT of
+// length H by Stockham passes of radix 4 (one radix-2 pass when log2 H is
+// odd), natural order in and out (stockham_kernel; Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC 2008),
+// the real-spectrum split and power-spectrum re-pack of split_pair
+// (long_split_kernel), the inverse passes, and the truncated pair sum
+// (long_trunc_kernel) with the short kernel's arithmetic.
+// ---------------------------------------------------------------------------
+constexpr int kLongThreads = 256;
+
+__global__ __launch_bounds__(kLongThreads) void long_moments_kernel(const float* __restrict__ series, int dim,
+                                                                   int nchains, int n, int spcp, int half,
+                                                                   int64_t s0, float* __restrict__ meanf,
+                                                                   double* __restrict__ mom)
+{
+    __shared__ double red[5][kLongThreads / 64];
+    const int64_t sid = s0 + blockIdx.x;
+    const float* x = series + sid * spcp;
+    double part[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < n; t += kLongThreads) {
+        const double v = (double)x[t];
+        part[0] += v;
+        if (t < 2 * half) {
+            const int h = t >= half;
+            part[1 + 2 * h] += v;
+            part[2 + 2 * h] += v * v;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        for (int off = 32; off > 0; off >>= 1) part[q] += __shfl_xor(part[q], off);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = part[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double sum = 0.0;
+        for (int w = 0; w < kLongThreads / 64; ++w) sum += red[threadIdx.x][w];
+        const int d = (int)(sid % dim);
+        const int vc = (int)(sid / dim);
+        const int c = vc % nchains, v = vc / nchains;
+        if (threadIdx.x == 0) {
+            meanf[blockIdx.x] = (float)(sum / n);
+        } else {
+            const int h = (threadIdx.x - 1) >> 1, q = (threadIdx.x - 1) & 1;
+            mom[(((int64_t)v * 2 * nchains + 2 * c + h) * dim + d) * 2 + q] = sum;
+        }
+    }
+}
+
+// z_j = (x_2j - mean) + i (x_2j+1 - mean), zero beyond the series, j < H.
+__global__ void long_pack_kernel(const float* __restrict__ series, int n, int spcp, int H, int64_t s0,
+                                 const float* __restrict__ meanf, float2* __restrict__ z)
+{
+    const int b = blockIdx.y;
+    const float* x = series + (s0 + b) * spcp;
+    const float mu = meanf[b];
+    float2* zb = z + (int64_t)b * H;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < H; j += gridDim.x * blockDim.x) {
+        const int t0 = 2 * j, t1 = 2 * j + 1;
+        zb[j] = make_float2(t0 < n ? x[t0] - mu : 0.0f, t1 < n ? x[t1] - mu : 0.0f);
+    }
+}
+
+// One Stockham pass of radix R over a batch of length-H transforms: butterfly
+// j (< H/R) reads src[j + r H/R], multiplies by W_{Ns R}^{r (j mod Ns)}
+// (from tw[t] = W_H^t, conjugated for the inverse), transforms, and writes
+// dst[(j / Ns) Ns R + (j mod Ns) + r Ns].
+template <int R>
+__global__ void stockham_kernel(const float2* __restrict__ src, float2* __restrict__ dst, int H, int Ns,
+                                const float2* __restrict__ tw, int inverse)
+{
+    const int b = blockIdx.y;
+    const float2* sb = src + (int64_t)b * H;
+    float2* db = dst + (int64_t)b * H;
+    const int nbf = H / R;
+    const int step = H / (Ns * R);  // W_{Ns R}^e = W_H^(e step)
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nbf; j += gridDim.x * blockDim.x) {
+        const int jm = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 a = sb[j + r * nbf];
+            if (r > 0) {
+                float2 w = tw[r * jm * step];
+                if (inverse) w.y = -w.y;
+                a = cmul(a, w);
+            }
+            v[r] = a;
+        }
+        float2 o[R];
+        if (R == 2) {
+            o[0] = cadd(v[0], v[1]);
+            o[1] = csub(v[0], v[1]);
+        } else {
+            // 4-point DFT; the forward kernel is W_4 = -i, the inverse +i.
+            const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+            const float2 a2 = cadd(v[1], v[3]), d3 = csub(v[1], v[3]);
+            const float2 b3 = inverse ? make_float2(-d3.y, d3.x) : make_float2(d3.y, -d3.x);  // (+/-i)(v1 - v3)
+            o[0] = cadd(a0, a2);
+            o[1 % R] = cadd(a1, b3);
+            o[2 % R] = csub(a0, a2);
+            o[3 % R] = csub(a1, b3);
+        }
+        const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) db[base + r * Ns] = o[r];
+    }
+}
+
+// Y_k and Y_{H-k} from Z_k, Z_{H-k} (split_pair), k in [0, H/2]; wN[k] = W_N^k.
+__global__ void long_split_kernel(const float2* __restrict__ Z, float2* __restrict__ Y, int H,
+                                  const float2* __restrict__ wN)
+{
+    const int b = blockIdx.y;
+    const float2* zb = Z + (int64_t)b * H;
+    float2* yb = Y + (int64_t)b * H;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= H / 2; k += gridDim.x * blockDim.x) {
+        const int kb = (H - k) & (H - 1);
+        const float2 wk = wN[k];
+        const float2 wb = (k == 0) ? make_float2(-1.0f, 0.0f) : wN[H - k];  // the partner of 0 is X_H
+        float2 ok, ob;
+        split_pair(zb[k], zb[kb], wk, wb, ok, ob);
+        yb[k] = ok;
+        if (k != 0 && kb != k) yb[kb] = ob;
+    }
+}
+
+// The truncated pair sum of one series (one workgroup) from the inverse
+// transform R (interleaved real view: lag m at R[m/2].x or .y), with the
+// short kernel's arithmetic.
+__global__ __launch_bounds__(kLongThreads) void long_trunc_kernel(const float2* __restrict__ Rz, int H, int n,
+                                                                 int dim, int nchains, int64_t s0,
+                                                                 float* __restrict__ ess_chain)
+{
+    __shared__ int kfirst[kLongThreads / 64];
+    __shared__ double psum[kLongThreads / 64];
+    const int b = blockIdx.x;
+    const int64_t sid = s0 + b;
+    const int d = (int)(sid % dim);
+    const int vc = (int)(sid / dim);
+    const int c = vc % nchains, v = vc / nchains;
+    float* out = ess_chain + (int64_t)v * dim * nchains + (int64_t)d * nchains + c;
+    const float* r = reinterpret_cast<const float*>(Rz + (int64_t)b * H);
+    const int N = 2 * H;
+    const float norm = 1.0f / ((float)N * (float)n);
+    const float var = r[0] * norm;
+    if (var < 1e-30f) {
+        if (threadIdx.x == 0) *out = 1.0f;
+        return;
+    }
+    const int kend = n / 2;
+    int kneg = kend;
+    for (int k = 1 + threadIdx.x; k < kend; k += kLongThreads) {
+        const float pair_sum = (r[2 * k - 1] * norm) / var + (r[2 * k] * norm) / var;
+        if (pair_sum < 0.0f) {
+            kneg = k;
+            break;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) kneg = min(kneg, __shfl_xor(kneg, off));
+    if ((threadIdx.x & 63) == 0) kfirst[threadIdx.x >> 6] = kneg;
+    __syncthreads();
+    int K = kend;
+    for (int w = 0; w < kLongThreads / 64; ++w) K = min(K, kfirst[w]);
+    double acc = 0.0;
+    for (int k = 1 + threadIdx.x; k < K; k += kLongThreads)
+        acc += (double)((r[2 * k - 1] * norm) / var + (r[2 * k] * norm) / var);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) psum[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kLongThreads / 64; ++w) sum += psum[w];
+        const float tau = fmaxf(1.0f + 2.0f * (float)sum, 1.0f);
+        *out = (float)n / tau;
+    }
+}
+
+}  // namespace
+
 int diag_max_chain_length() { return kMaxHalf; }
 
 namespace {
@@ -807,15 +993,16 @@ struct DiagPlan {
 
 size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
-DiagPlan diag_plan(int dim, int n_total, int nchains, int n_variants, bool need_ec)
+DiagPlan diag_plan(int dim, int n_total, int nchains, int n_variants, bool need_ec, bool allow_long = false)
 {
     DiagPlan p{};
     p.spc = n_total / nchains;
     p.logH = 1;
     while ((1 << (p.logH + 1)) < 2 * p.spc) ++p.logH;  // N = 2H >= 2 spc
     p.H = 1 << p.logH;
-    NAJA_REQUIRE(p.H <= kMaxHalf, NAJA_ERR_UNSUPPORTED, "chains of %d samples exceed the LDS FFT limit (%d)", p.spc,
-                 kMaxHalf);
+    NAJA_REQUIRE(allow_long || p.H <= kMaxHalf, NAJA_ERR_UNSUPPORTED,
+                 "chains of %d samples exceed the LDS FFT limit (%d)", p.spc, kMaxHalf);
+    NAJA_REQUIRE(p.logH <= 28, NAJA_ERR_UNSUPPORTED, "chains of %d samples are too long", p.spc);
     NAJA_REQUIRE(n_variants >= 1 && n_variants <= 65535, NAJA_ERR_UNSUPPORTED, "%d variants (1..65535)", n_variants);
     p.N = 2 * p.H;
     p.tb = (p.logH + 1) / 2;
@@ -952,10 +1139,89 @@ const NoAllocDevice& noalloc_device()
 
 }  // namespace
 
+// Chains beyond the LDS transform: the transforms in global memory (the
+// long_* kernels), series in batches whose two complex buffers stay within
+// about 2 GB.
+void diag_launch_long(const DiagPlan& p, const float* samples, int dim, int nchains, int layout, float* ess,
+                      float* rhat, float* ess_chain, hipStream_t stream, float* ess_min, float* rhat_max)
+{
+    const int H = p.H, N = p.N, spc = p.spc;
+    std::vector<float2> tw_h((size_t)2 * H);  // W_H^t, then W_N^k, t, k < H
+    for (int t = 0; t < H; ++t) {
+        const double a = -2.0 * M_PI * (double)t / (double)H;
+        tw_h[(size_t)t] = make_float2((float)std::cos(a), (float)std::sin(a));
+        const double c = -2.0 * M_PI * (double)t / (double)N;
+        tw_h[(size_t)H + t] = make_float2((float)std::cos(c), (float)std::sin(c));
+    }
+    const int64_t nb_max = std::max<int64_t>(1, std::min<int64_t>(p.n_series, ((int64_t)1 << 30) / ((int64_t)H * 8)));
+    const size_t b_tw = round256(sizeof(float2) * tw_h.size());
+    const size_t b_buf = round256(sizeof(float2) * (size_t)nb_max * H);
+    const size_t b_mean = round256(sizeof(float) * (size_t)nb_max);
+    const size_t b_main = p.b_ser + p.b_mom + p.b_ec;
+    char* mem = nullptr;
+    scratch_alloc((void**)&mem, b_main + b_tw + 2 * b_buf + b_mean, stream);
+    float* ser = reinterpret_cast<float*>(mem);
+    double* mom = reinterpret_cast<double*>(mem + p.b_ser);
+    float* ec = ess_chain ? ess_chain : reinterpret_cast<float*>(mem + p.b_ser + p.b_mom);
+    float2* tw = reinterpret_cast<float2*>(mem + b_main);
+    float2* bufA = reinterpret_cast<float2*>(mem + b_main + b_tw);
+    float2* bufB = reinterpret_cast<float2*>(mem + b_main + b_tw + b_buf);
+    float* meanf = reinterpret_cast<float*>(mem + b_main + b_tw + 2 * b_buf);
+    NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
+                       nchains, spc, p.spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser);
+    NAJA_HIP(hipGetLastError());
+    const float2* twH = tw;
+    const float2* twN = tw + H;
+    for (int64_t s0 = 0; s0 < p.n_series; s0 += nb_max) {
+        const int bn = (int)std::min<int64_t>(nb_max, p.n_series - s0);
+        const dim3 g2((unsigned)std::min(64, std::max(1, H / 4 / 256)), (unsigned)bn);
+        hipLaunchKernelGGL(long_moments_kernel, dim3(bn), dim3(kLongThreads), 0, stream, ser, dim, nchains, spc,
+                           p.spcp, p.half, s0, meanf, mom);
+        hipLaunchKernelGGL(long_pack_kernel, g2, dim3(256), 0, stream, ser, spc, p.spcp, H, s0, meanf, bufA);
+        NAJA_HIP(hipGetLastError());
+        float2* cur = bufA;
+        float2* oth = bufB;
+        auto passes = [&](int inverse) {
+            int Ns = 1;
+            if (p.logH & 1) {
+                hipLaunchKernelGGL(stockham_kernel<2>, g2, dim3(256), 0, stream, cur, oth, H, Ns, twH, inverse);
+                std::swap(cur, oth);
+                Ns *= 2;
+            }
+            for (; Ns < H; Ns *= 4) {
+                hipLaunchKernelGGL(stockham_kernel<4>, g2, dim3(256), 0, stream, cur, oth, H, Ns, twH, inverse);
+                std::swap(cur, oth);
+            }
+            NAJA_HIP(hipGetLastError());
+        };
+        passes(0);
+        hipLaunchKernelGGL(long_split_kernel, g2, dim3(256), 0, stream, cur, oth, H, twN);
+        std::swap(cur, oth);
+        passes(1);
+        hipLaunchKernelGGL(long_trunc_kernel, dim3(bn), dim3(kLongThreads), 0, stream, cur, H, spc, dim, nchains, s0,
+                           ec);
+        NAJA_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, p.nv), dim3(256), 0, stream, ec, dim, nchains, ess);
+    hipLaunchKernelGGL(rhat_final_kernel, dim3((dim + 255) / 256, p.nv), dim3(256), 0, stream, mom, dim, nchains,
+                       p.half, rhat);
+    NAJA_HIP(hipGetLastError());
+    if (ess_min || rhat_max) {
+        hipLaunchKernelGGL(variant_summary_kernel, dim3(p.nv), dim3(256), 0, stream, ess, rhat, dim, ess_min, rhat_max);
+        NAJA_HIP(hipGetLastError());
+    }
+    scratch_free(mem, stream);
+}
+
 void diag_launch(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
                  float* ess_chain, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
 {
-    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, ess_chain == nullptr);
+    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, ess_chain == nullptr, true);
+    if (p.H > kMaxHalf) {
+        diag_launch_long(p, samples, dim, nchains, layout, ess, rhat, ess_chain, stream, ess_min, rhat_max);
+        return;
+    }
     std::vector<float2> tw_h((size_t)twiddle_count(p.logH));
     twiddle_tables(p.logH, tw_h.data());
     // One scratch allocation for the series, the moments, the per-chain ESS

@@ -187,3 +187,27 @@ def test_sample_run_affine_hull(gpu, tmp_path, backmap):
         assert V[:, 0].std() > 0.2  # the chain moves along the face
     Vref = oracle.backmap(Tc.astype(np.float32).astype(np.float64), sc.astype(np.float32).astype(np.float64), Z)
     assert np.abs(V - Vref).max() <= 1e-5 * max(1.0, np.abs(Vref).max())
+
+
+def test_validate_long_chains(gpu, tmp_path):
+    """`naja validate` on chains longer than the LDS transform (2 chains x
+    20000 samples, interleaved rows): per-dim ESS and R-hat counts as the
+    oracle's (validate_cli.cpp:140-146 takes any length)."""
+    from tests.test_oracle_diag import ar1
+
+    rng = np.random.default_rng(4)
+    nch, spc, dim = 2, 20000, 3
+    series = ar1(spc, 0.97, rng, nseries=nch * dim).reshape(nch, dim, spc)
+    S = np.zeros((spc * nch, dim), np.float32)
+    for c in range(nch):
+        S[c::nch] = series[c].T
+    f = tmp_path / "long.npy"
+    np.save(f, S)
+    v = subprocess.run([NAJA, "validate", "--samples", str(f), "--n-chains", str(nch)], capture_output=True, text=True,
+                       timeout=120)
+    assert v.returncode == 0, v.stderr
+    js = json.loads(v.stdout)
+    e_ref, _ = oracle.ess(S, nch)
+    _, a11, _ = oracle.split_rhat(S, nch)
+    np.testing.assert_allclose(js["per_dim_ess"], np.round(e_ref, 1), rtol=2e-3, atol=0.6)
+    assert js["rhat"]["above_1_1"] == a11

@@ -44,7 +44,9 @@ DIAG_CASES = [(12500, 4, 6, 0.9), (2000, 8, 5, 0.5), (333, 3, 4, 0.99), (2000, 4
               (4, 2, 3, 0.1), (5, 1, 2, 0.0), (16384, 2, 2, 0.7), (200, 37, 300, 0.5), (77, 4, 100, 0.8),
               # truncation beyond the exact lags of the half-size pass: the full-size redo pass
               (8000, 2, 3, 0.999), (16000, 2, 2, 0.9995), (4000, 3, 2, 0.998), (1000, 4, 3, 0.9999),
-              (2000, 3, 2, 1.0), (3001, 2, 3, 0.998)]
+              (2000, 3, 2, 1.0), (3001, 2, 3, 0.998),
+              # beyond the LDS transform (N / 2 > 16384 points): the global-memory path
+              (16385, 2, 2, 0.9), (20001, 3, 2, 0.95), (50000, 2, 3, 0.9), (40000, 2, 2, 1.0)]
 
 
 @pytest.mark.parametrize("spc,nch,dim,phi", DIAG_CASES)
@@ -96,6 +98,24 @@ def test_diag_errors(gpu):
         naja_amd.diagnostics(t.zeros((10, 3)), 3)  # not divisible
     with pytest.raises(naja_amd.NajaInvalidArgument):
         naja_amd.diagnostics(t.zeros((6, 3)), 2)  # fewer than 4 per chain
+
+
+def test_long_chains_batched_and_blocks_layout(gpu):
+    """Chains longer than the LDS transform through naja_diag_batched and the
+    chain-major layout: equal to per-variant calls and to the oracle."""
+    import naja_amd
+
+    t = gpu
+    rng = np.random.default_rng(21)
+    nv, nch, spc, dim = 2, 2, 30000, 3
+    X = np.stack([ar1(spc * nch, 0.8, rng, nseries=dim).T for _ in range(nv)]).astype(np.float32)
+    Xd = t.from_numpy(X).cuda()
+    ess, rhat, emin, rmax = naja_amd.diagnostics_batched(Xd, nch, layout=1)
+    for v in range(nv):
+        e_ref, _ = oracle.ess(X[v], nch, layout=1)
+        r_ref, _, _ = oracle.split_rhat(X[v], nch, layout=1)
+        assert (np.abs(ess[v].cpu().numpy() - e_ref) / e_ref).max() <= 1e-5
+        assert np.abs(rhat[v].cpu().numpy() - r_ref).max() <= 1e-5
 
 
 def test_batched_diagnostics_equal_per_variant_calls(gpu):

@@ -107,6 +107,14 @@ def main():
     t_train = time.perf_counter() - t0
     del S
     log(f"pilots {t_pilot:.2f} s, training {t_train:.2f} s (NLL {hist[0]:.2f} -> {hist[-1]:.2f})")
+    # Model-level acceptance on fresh training-distribution conditions.
+    t0 = time.perf_counter()
+    val = cd.validate_learned(lp, A0, b0, pool)
+    t_val = time.perf_counter() - t0
+    use_net = val["accept"]
+    log(f"validation ({val['conditions']} conditions, {t_val:.2f} s): learned completion sum min-ESS "
+        f"{val['learned_min_ess_sum']} ({val['learned_n_pass']} pass) vs base {val['base_min_ess_sum']} "
+        f"({val['base_n_pass']} pass): learned map {'accepted' if use_net else 'rejected'}")
 
     EAh, Ebh, Fh, nsh = cd.sample_conditions(pool, args.variants, 300, 1e-3, 1e-1, seed=99)
     nv, nspc = len(EAh), args.nspc
@@ -120,7 +128,10 @@ def main():
         return cd.facet_maps(sub, cd.base_rounding(sub, None, None, None))
 
     def learned_step():
-        P, _ = cd.learned_rounding(lp, Fh_dev, batch)
+        if use_net:
+            P, _ = cd.learned_rounding(lp, Fh_dev, batch)
+        else:  # rejected by validation: the base rounding, no network work
+            P = cd.facet_maps(batch, cd.base_rounding(batch, None, None, None))
         return cd.sample_accepted(batch, P, fallback, nch, nspc, thin, seed, resync)
 
     # The other methods on the same variants, chains and steps, timed the same
@@ -183,7 +194,7 @@ def main():
                         "(one wavefront per SIMD: the step's latency binds, not bandwidth)"}
 
     cube = None if args.no_extra else cube_plain(np, torch, args)
-    batched = None if args.no_extra else batched_conditions(np, torch, cd, pool, lp, A0, b0, args)
+    batched = None if args.no_extra else batched_conditions(np, torch, cd, pool, lp, A0, b0, args, use_net)
     genome = None if args.no_genome else genome_scale(np, torch, rounded=False)
     genome_r = None if args.no_genome else genome_scale(np, torch, rounded=True)
     large = None if args.no_large else large_scale(np, torch, args.large_nspc, rounded=False)
@@ -198,7 +209,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        P, _ = cd.learned_rounding(lp, Fh_dev, batch)
+        P = (cd.learned_rounding(lp, Fh_dev, batch)[0] if use_net
+             else cd.facet_maps(batch, cd.base_rounding(batch, None, None, None)))
         cpu = cpu_baseline(np, A0, b0, EAh, Ebh, P.cpu().numpy(), nch, nspc, thin, seed, resync,
                            float(e[ok].sum()), args.cpu_seconds)
     lp.close()
@@ -220,7 +232,8 @@ def main():
             "workload": f"{nv} held-out conditioned variants of the cube [-1,1]^100, 0-300 extra slab rows "
                         f"(half-widths 1e-3..1e-1), {nch} chains x {nspc} x thin {thin}, one batched launch; "
                         f"learned-rounded (facet-aligned rounding completed from the learned low-rank update of "
-                        f"the Dikin ellipsoid, R-hat acceptance with fallback)",
+                        f"the Dikin ellipsoid when validation accepts the network, else from the ellipsoid; "
+                        f"R-hat acceptance with fallback)",
             "variants": nv, "chains": nch, "nspc": nspc, "thin": thin, "resync": resync,
             "n_pass_rhat_1_05": int(ok.sum()), "min_ess_sum": round(float(e.sum()), 1),
             "min_ess_sum_passing": round(float(e[ok].sum()), 1), "rerun_with_fallback": n_again,
@@ -237,6 +250,7 @@ def main():
                             "facet: facet-aligned rounding completed from the Dikin ellipsoid (the method without "
                             "the network)",
             "pilot_seconds": round(t_pilot, 3), "training_seconds": round(t_train, 3),
+            "learned_validation": dict(val, seconds=round(t_val, 3)),
             "training_nll_first_last": [round(hist[0], 3), round(hist[-1], 3)],
             "cube_plain": cube,
             "batched_conditions": batched,
@@ -284,7 +298,7 @@ def cube_plain(np, torch, args):
             "var_closed_form": round(1.0 / 3.0, 5)}
 
 
-def batched_conditions(np, torch, cd, pool, lp, A0, b0, args):
+def batched_conditions(np, torch, cd, pool, lp, A0, b0, args, use_net):
     """256 conditioned variants of the cube at their own slab half-widths,
     log-uniform on [1e-2, 1], 4 chains x 12500 x thinning 16 per variant, all
     in one launch with FP64 resync every 1000 steps: plain CHR, the
@@ -293,7 +307,7 @@ def batched_conditions(np, torch, cd, pool, lp, A0, b0, args):
     EA, Eb, F, _ = cd.sample_conditions(pool, 256, 300, 1e-2, 1.0, seed=3)
     res = cd.compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=11, resync=1000,
                      methods=("plain", "facet", "learned"), warmup=args.warmup, steps=max(1, args.steps // 4),
-                     log=lambda m: log("batched conditions (1e-2..1) " + m))
+                     log=lambda m: log("batched conditions (1e-2..1) " + m), use_network=use_net)
     return {name: {"value": round(o["metric"], 3), "ms_per_step": round(o["seconds"] * 1e3, 3),
                    "n_pass": o["n_pass"], "min_ess_sum": round(o["ess_sum"], 1)} for name, o in res.items()}
 

@@ -23,9 +23,13 @@ changes only how fast coordinate hit-and-run mixes):
   facet-aligned frame, where they mix); its map P0 (I + V V^T)^{1/2} is the
   base whose dual rows complete the facet-aligned rounding.
 
-Acceptance: after the main run, a variant whose split-R-hat is 1.05 or more
-is run again with the fallback (the facet-aligned rounding completed from the
-Dikin ellipsoid alone), and the run with the better R-hat and ESS is kept.
+Acceptance, at two levels. After training, validate_learned runs both
+completions on fresh conditions of the training distribution, and the learned
+map is used only if it loses neither summed min-ESS nor R-hat passes there
+(else the pipeline runs its base and the network costs nothing). After each
+main run, a variant whose split-R-hat is 1.05 or more is run again with the
+fallback (the facet-aligned rounding completed from the Dikin ellipsoid
+alone), and the run with the better R-hat and ESS is kept.
 
 The reference's Dikin rounding (dikin_precondition.cpp:36-183) is reported
 beside them as the drop-in preconditioner.
@@ -199,6 +203,31 @@ def learned_maps(lp: LearnedPreconditioner, F, P0, expand=True):
     return lowrank_maps(P0, V, expand=expand)
 
 
+def validate_learned(lp: LearnedPreconditioner, A0, b0, pool, n=256, w_lo=1e-3, w_hi=1e-1, seed=12345,
+                     nchains=4, nspc=12500, thin=16, resync=1000):
+    """Model-level acceptance of the trained network, on n fresh conditions of
+    the training distribution (not the evaluation variants): full-length runs
+    of the facet-aligned rounding completed from the learned map and from the
+    Dikin ellipsoid alone. The learned map is accepted when it loses neither
+    summed min-ESS nor variants passing split-R-hat < 1.05."""
+    import torch
+
+    EA, Eb, F, _ = sample_conditions(pool, n, 300, w_lo, w_hi, seed=seed)
+    batch = VariantBatch(A0, b0, EA, Eb)
+    P0 = base_rounding(batch, None, None, None)
+    PL, _ = learned_rounding(lp, torch.as_tensor(F, dtype=torch.float32).cuda(), batch, P0)
+    out = {}
+    for name, P in (("learned", PL), ("base", facet_maps(batch, P0))):
+        Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
+        e, r = variant_diagnostics(Y, nchains)
+        del Y
+        out[name] = (float(e.sum()), int((r < 1.05).sum()))
+    accept = out["learned"][0] >= out["base"][0] and out["learned"][1] >= out["base"][1]
+    return dict(accept=bool(accept), learned_min_ess_sum=round(out["learned"][0], 1),
+                base_min_ess_sum=round(out["base"][0], 1), learned_n_pass=out["learned"][1],
+                base_n_pass=out["base"][1], conditions=n)
+
+
 def learned_rounding(lp: LearnedPreconditioner, F, batch, P0=None):
     """The learned-rounded maps of every variant: the facet-aligned rounding
     completed from the learned map P0 (I + V V^T)^{1/2}; also returns P0."""
@@ -244,15 +273,17 @@ def timed_steps(step, warmup, steps):
 
 
 def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resync=1000,
-            methods=("plain", "dikin", "ellipsoid", "facet", "learned"), warmup=1, steps=1, log=None):
+            methods=("plain", "dikin", "ellipsoid", "facet", "learned"), warmup=1, steps=1, log=None,
+            use_network=True):
     """Sampling methods on the same variants, chains and steps, each timed
     the same way (timed_steps) from the polytopes on the device to samples +
     diagnostics on the device, including its preconditioner:
     plain CHR; the reference's Dikin rounding (dikin_precondition, host
     FP64); the Dikin ellipsoid; the facet-aligned rounding completed from the
     Dikin ellipsoid (the method without the network); the learned-rounded
-    pipeline (facet-aligned rounding completed from the learned map, R-hat
-    acceptance). Returns per-method results."""
+    pipeline (facet-aligned rounding completed from the learned map, or from
+    the Dikin ellipsoid when use_network is False, R-hat acceptance). Returns
+    per-method results."""
     import torch
 
     batch = VariantBatch(A0, b0, EA, Eb)  # the polytopes go to the device once, before any timing
@@ -272,7 +303,10 @@ def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resyn
         return step
 
     def learned():
-        P, _ = learned_rounding(lp, Fd, batch)
+        if use_network:
+            P, _ = learned_rounding(lp, Fd, batch)
+        else:  # the network was rejected by validate_learned: the step runs its base
+            P = facet_maps(batch, base_rounding(batch, None, None, None))
         _, e, r, n_again = sample_accepted(batch, P, facet_fallback, nchains, nspc, thin, seed, resync)
         return e, r, n_again
 

@@ -22,8 +22,11 @@ def test_learned_rounded_vs_plain(gpu):
     lp = LearnedPreconditioner(seed=0, model=1, out_scale=1.0)
     hist = cd.train(lp, F, S, epochs=40)
     assert hist[-1] < hist[0]
+    val = cd.validate_learned(lp, A0, b0, pool, n=48, nspc=3000)
+    print(f"validation: {val}", flush=True)
     EAh, Ebh, Fh, nsh = cd.sample_conditions(pool, 48, 300, 1e-3, 1e-1, seed=99)
-    res = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nspc=3000, thin=16, log=lambda s: print(s, flush=True))
+    res = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nspc=3000, thin=16, log=lambda s: print(s, flush=True),
+                     use_network=val["accept"])
     lp.close()
     plain, dikin, ell, facet, learned = (res[k] for k in ("plain", "dikin", "ellipsoid", "facet", "learned"))
     print(f"sum min-ESS: plain {plain['ess_sum']:.0f}, Dikin {dikin['ess_sum']:.0f}, ellipsoid {ell['ess_sum']:.0f}, "
@@ -36,6 +39,13 @@ def test_learned_rounded_vs_plain(gpu):
     # The learned-rounded pipeline against the Dikin-ellipsoid base rounding.
     assert learned["ess_sum"] >= ell["ess_sum"]
     assert learned["n_pass"] >= max(plain["n_pass"], ell["n_pass"])
+    # A network that validation rejects costs nothing: the pipeline is its
+    # base run for run (the same maps, the same Philox stream) on every
+    # variant that passes R-hat there; the others may be run again.
+    assert learned["n_pass"] >= facet["n_pass"]
+    if not val["accept"]:
+        ok = facet["rhat_max"] < 1.05
+        np.testing.assert_array_equal(learned["ess_min"][ok], facet["ess_min"][ok])
 
 
 def test_rounded_samples_stay_feasible(gpu):

@@ -501,10 +501,15 @@ int naja_pilot_covariance(const float* samples, int n_variants, int64_t rows, in
  * C = alpha * op(A) op(B), op(A) M x K, op(B) K x N, row-major FP32 storage,
  * operands rounded to BF16 and multiplied on MFMA with FP32 accumulation (the
  * products of the learned preconditioner). trans_a: A stored K x M;
- * trans_b: B stored N x K.
+ * trans_b: B stored N x K. rowsum (optional, trans_a only): the M sums over k
+ * of op(A) from the unrounded FP32 operand (how the bias gradients are taken).
+ * workspace (optional, workspace_floats floats of device memory): lets K be
+ * split across workgroups when M x N alone gives too few; the split needs
+ * splits * (M * N + M) floats and is skipped when they do not fit.
  */
 int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, const float* A, int64_t lda,
-                   const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, void* stream);
+                   const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, float* rowsum,
+                   float* workspace, int64_t workspace_floats, void* stream);
 
 #ifdef __cplusplus
 }

@@ -206,7 +206,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_lp_loss_grad.argtypes = [vp, vp, vp, i32, vp, vp, vp]
     lib.naja_lp_train_step.argtypes = [vp, vp, vp, i32, f32, vp, vp]
     lib.naja_pilot_covariance.argtypes = [vp, i32, i64, i32, f64, vp, vp, vp]
-    lib.naja_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, i64, vp, i64, f32, vp]
+    lib.naja_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, i64, vp, i64, f32, vp, vp, i64, vp]
     for name in ("naja_lp_create", "naja_lp_destroy", "naja_lp_param_count", "naja_lp_get_params",
                  "naja_lp_set_params", "naja_lp_forward", "naja_lp_loss_grad", "naja_lp_train_step",
                  "naja_pilot_covariance", "naja_gemm_bf16"):

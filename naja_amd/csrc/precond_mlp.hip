@@ -49,60 +49,151 @@ enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2, EPI_MASK = 3 };
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 4;
 
+// One thread's share (8 elements) of a 64-row x 32-k operand tile, held in
+// registers between its global load and its BF16 store into LDS, so that the
+// next tile's loads are in flight while the current one is multiplied.
+// KC (k contiguous in memory, element (row, k) at P[row*ld + k]): row tid / 4,
+// k (tid % 4) * 8 + j. Otherwise (row contiguous, P[k*ld + row]): k tid / 8,
+// row (tid % 8) * 8 + j. `vec`: the host found P 16-byte aligned and ld a
+// multiple of 16 bytes' worth of elements, so a whole in-bounds segment of 8
+// is loaded as 16-byte vectors; partial segments are loaded element-wise.
+template <typename T, bool KC>
+struct TileRegs {
+    float v[8];
+
+    __device__ __forceinline__ void load(const T* __restrict__ P, int64_t ld, int row0, int nrows, int k0, int kend,
+                                         bool vec, int tid)
+    {
+        const int row = KC ? row0 + tid / 4 : row0 + (tid % 8) * 8;
+        const int k = KC ? k0 + (tid % 4) * 8 : k0 + tid / 8;
+        const T* q = KC ? P + (int64_t)row * ld + k : P + (int64_t)k * ld + row;
+        const bool whole = KC ? (row < nrows && k + 8 <= kend) : (k < kend && row + 8 <= nrows);
+        if (vec && whole) {
+            if (sizeof(T) == 4) {
+                const float4 x0 = *reinterpret_cast<const float4*>(q);
+                const float4 x1 = *reinterpret_cast<const float4*>(q + 4);
+                v[0] = x0.x, v[1] = x0.y, v[2] = x0.z, v[3] = x0.w;
+                v[4] = x1.x, v[5] = x1.y, v[6] = x1.z, v[7] = x1.w;
+            } else {
+                const uint4 x = *reinterpret_cast<const uint4*>(q);
+                const unsigned w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    v[2 * j] = __uint_as_float(w[j] << 16);
+                    v[2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
+                }
+            }
+            return;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const bool in = KC ? (row < nrows && k + j < kend) : (k < kend && row + j < nrows);
+            v[j] = in ? elem(q + j) : 0.0f;
+        }
+    }
+
+    __device__ __forceinline__ static float elem(const float* q) { return *q; }
+    __device__ __forceinline__ static float elem(const bf16_t* q) { return __uint_as_float((unsigned)*q << 16); }
+
+    __device__ __forceinline__ void store(bf16_t (*S)[LDK], int tid) const
+    {
+        if (KC) {
+            const int r = tid / 4, kk = (tid % 4) * 8;
+            unsigned w[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = (unsigned)f2bf(v[2 * j]) | ((unsigned)f2bf(v[2 * j + 1]) << 16);
+            *reinterpret_cast<uint2*>(&S[r][kk]) = make_uint2(w[0], w[1]);
+            *reinterpret_cast<uint2*>(&S[r][kk + 4]) = make_uint2(w[2], w[3]);
+        } else {
+            const int k = tid / 8, r = (tid % 8) * 8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) S[r + j][k] = f2bf(v[j]);
+        }
+    }
+};
+
 // C(m, n) = alpha * (sum_k A(m, k) B(k, n) [+ bias(n)]) [relu] [* (mask(m, n) > 0)].
 // A(m, k) = A[m*lda + k] if AK (k contiguous) else A[k*lda + m];
 // B(k, n) = B[n*ldb + k] if BKC else B[k*ldb + n].
-// 64 x 64 output tile per workgroup, 4 waves of 32 x 32, K staged 32 at a time.
+// 64 x 64 output tile per workgroup, 4 waves of 32 x 32, K staged 32 at a time
+// (the next 32 loaded into registers while the current 32 are multiplied).
+// Split K: workgroup z of grid.z covers k in [z kps, (z + 1) kps) and, with
+// `part`, writes its raw tile to part[z][m][n] for gemm_reduce_kernel (the
+// epilogue then runs there). rowsum (A stored m-contiguous only): the FP32
+// sums over k of A(m, k), taken from the unrounded operand while it is staged
+// by the workgroups of the first column block (a bias gradient folded into the
+// weight-gradient product); per split in part[splits*M*N + z*M + m] when split.
 template <typename TA, typename TB, bool AK, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const TA* __restrict__ A, int64_t lda,
                                                    const TB* __restrict__ B, int64_t ldb, float* __restrict__ C,
                                                    int64_t ldc, float alpha, const float* __restrict__ bias,
-                                                   const float* __restrict__ mask, int64_t ldm)
+                                                   const float* __restrict__ mask, int64_t ldm, int kps,
+                                                   float* __restrict__ part, float* __restrict__ rowsum, int vec_a,
+                                                   int vec_b)
 {
     __shared__ __align__(16) bf16_t As[BM][LDK];
     __shared__ __align__(16) bf16_t Bs[BN][LDK];
+    __shared__ float rsh[BK][BM];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN, z = blockIdx.z;
+    const int kbeg = z * kps, kend = min(K, kbeg + kps);
+    const bool do_rs = !AK && rowsum != nullptr && blockIdx.x == 0;
+    float rs[8];  // row sums of this thread's 8 rows over its k (tid / 8 of every tile)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) rs[j] = 0.0f;
     floatx16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        for (int i = tid; i < BM * BK; i += 256) {
-            const int r = AK ? i / BK : i % BM;
-            const int k = AK ? i % BK : i / BM;
-            const int gm = m0 + r, gk = k0 + k;
-            bf16_t v = 0;
-            if (gm < M && gk < K) v = to_bf(AK ? A[(int64_t)gm * lda + gk] : A[(int64_t)gk * lda + gm]);
-            As[r][k] = v;
+    TileRegs<TA, AK> ra;
+    TileRegs<TB, BKC> rb;
+    if (kbeg < kend) {
+        ra.load(A, lda, m0, M, kbeg, kend, vec_a, tid);
+        rb.load(B, ldb, n0, N, kbeg, kend, vec_b, tid);
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        if (do_rs) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) rs[j] += ra.v[j];
         }
-        for (int i = tid; i < BN * BK; i += 256) {
-            const int c = BKC ? i / BK : i % BN;
-            const int k = BKC ? i % BK : i / BN;
-            const int gn = n0 + c, gk = k0 + k;
-            bf16_t v = 0;
-            if (gn < N && gk < K) v = to_bf(BKC ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn]);
-            Bs[c][k] = v;
-        }
+        ra.store(As, tid);
+        rb.store(Bs, tid);
         __syncthreads();
+        if (k0 + BK < kend) {
+            ra.load(A, lda, m0, M, k0 + BK, kend, vec_a, tid);
+            rb.load(B, ldb, n0, N, k0 + BK, kend, vec_b, tid);
+        }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 8) {
             // Lanes 0-31 carry k = kk..kk+3 of rows/columns 0-31, lanes 32-63 k = kk+4..kk+7.
             const int kb = kk + (lane >> 5) * 4;
-            const uint2 ra = *reinterpret_cast<const uint2*>(&As[wm + (lane & 31)][kb]);
-            const uint2 rb = *reinterpret_cast<const uint2*>(&Bs[wn + (lane & 31)][kb]);
+            const uint2 ua = *reinterpret_cast<const uint2*>(&As[wm + (lane & 31)][kb]);
+            const uint2 ub = *reinterpret_cast<const uint2*>(&Bs[wn + (lane & 31)][kb]);
             bf16x4 a, b;
-            a[0] = (short)(ra.x & 0xffffu);
-            a[1] = (short)(ra.x >> 16);
-            a[2] = (short)(ra.y & 0xffffu);
-            a[3] = (short)(ra.y >> 16);
-            b[0] = (short)(rb.x & 0xffffu);
-            b[1] = (short)(rb.x >> 16);
-            b[2] = (short)(rb.y & 0xffffu);
-            b[3] = (short)(rb.y >> 16);
+            a[0] = (short)(ua.x & 0xffffu);
+            a[1] = (short)(ua.x >> 16);
+            a[2] = (short)(ua.y & 0xffffu);
+            a[3] = (short)(ua.y >> 16);
+            b[0] = (short)(ub.x & 0xffffu);
+            b[1] = (short)(ub.x >> 16);
+            b[2] = (short)(ub.y & 0xffffu);
+            b[3] = (short)(ub.y >> 16);
             acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
+    }
+    if (do_rs) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) rsh[tid / 8][(tid % 8) * 8 + j] = rs[j];
+        __syncthreads();
+        if (tid < BM && m0 + tid < M) {
+            float sum = 0.0f;
+            for (int q = 0; q < BK; ++q) sum += rsh[q][tid];
+            if (part)
+                part[(int64_t)gridDim.z * M * N + (int64_t)z * M + m0 + tid] = sum;
+            else
+                rowsum[m0 + tid] = sum;
+        }
     }
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -110,6 +201,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const TA
         const int c = n0 + wn + (lane & 31);
         if (r < M && c < N) {
             float v = acc[e];
+            if (part) {
+                part[((int64_t)z * M + r) * N + c] = v;
+                continue;
+            }
             if (EPI == EPI_BIAS || EPI == EPI_BIAS_RELU) v += bias[c];
             v *= alpha;
             if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.0f);
@@ -119,24 +214,65 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const TA
     }
 }
 
-template <typename TA, typename TB, bool AK, bool BKC, int EPI>
-void gemm(hipStream_t s, int M, int N, int K, const TA* A, int64_t lda, const TB* B, int64_t ldb, float* C,
-          int64_t ldc, float alpha = 1.0f, const float* bias = nullptr, const float* mask = nullptr, int64_t ldm = 0)
+// The split-K sum in split order, then the epilogue of gemm_kernel; and the
+// row sums' split sum.
+template <int EPI>
+__global__ void gemm_reduce_kernel(const float* __restrict__ part, int splits, int M, int N, float* __restrict__ C,
+                                   int64_t ldc, float alpha, const float* __restrict__ bias,
+                                   const float* __restrict__ mask, int64_t ldm, float* __restrict__ rowsum)
 {
-    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-    hipLaunchKernelGGL((gemm_kernel<TA, TB, AK, BKC, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc,
-                       alpha, bias, mask, ldm);
-    NAJA_HIP(hipGetLastError());
+    const int64_t MN = (int64_t)M * N;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < MN; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / N), c = (int)(e % N);
+        float v = part[e];
+        for (int z = 1; z < splits; ++z) v += part[(int64_t)z * MN + e];
+        if (EPI == EPI_BIAS || EPI == EPI_BIAS_RELU) v += bias[c];
+        v *= alpha;
+        if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.0f);
+        if (EPI == EPI_MASK) v = (mask[(int64_t)r * ldm + c] > 0.0f) ? v : 0.0f;
+        C[(int64_t)r * ldc + c] = v;
+    }
+    if (rowsum)
+        for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M; m += (int64_t)gridDim.x * blockDim.x) {
+            float v = part[splits * MN + m];
+            for (int z = 1; z < splits; ++z) v += part[splits * MN + (int64_t)z * M + m];
+            rowsum[m] = v;
+        }
 }
 
-// db(n) = sum over rows of dZ(r, n).
-__global__ void colsum_kernel(const float* __restrict__ dZ, int rows, int cols, float* __restrict__ db)
+// Launch: with `part` (part_floats floats of scratch), K is split while the
+// launch has fewer than 512 workgroups, every split keeps at least 64 of K,
+// and the partial tiles (and row sums) fit.
+template <typename TA, typename TB, bool AK, bool BKC, int EPI>
+void gemm(hipStream_t s, int M, int N, int K, const TA* A, int64_t lda, const TB* B, int64_t ldb, float* C,
+          int64_t ldc, float alpha = 1.0f, const float* bias = nullptr, const float* mask = nullptr, int64_t ldm = 0,
+          float* rowsum = nullptr, float* part = nullptr, size_t part_floats = 0)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    float s = 0.0f;
-    for (int r = 0; r < rows; ++r) s += dZ[(int64_t)r * cols + c];
-    db[c] = s;
+    const int gx = (N + BN - 1) / BN, gy = (M + BM - 1) / BM;
+    int splits = 1;
+    if (part) {
+        auto fits = [&](int sp) { return (size_t)sp * M * N + (size_t)sp * M <= part_floats; };
+        while (gx * gy * splits < 512 && K / (2 * splits) >= 64 && fits(2 * splits)) splits *= 2;
+    }
+    const int kps = (((K + splits - 1) / splits + BK - 1) / BK) * BK;
+    splits = (K + kps - 1) / kps;
+    auto vec = [](const void* p, int64_t ld, size_t elem) {
+        return (int)(((uintptr_t)p % 16) == 0 && ((size_t)ld * elem) % 16 == 0);
+    };
+    const int va = vec(A, lda, sizeof(TA)), vb = vec(B, ldb, sizeof(TB));
+    if (splits <= 1) {
+        hipLaunchKernelGGL((gemm_kernel<TA, TB, AK, BKC, EPI>), dim3(gx, gy, 1), dim3(256), 0, s, M, N, K, A, lda, B,
+                           ldb, C, ldc, alpha, bias, mask, ldm, K, nullptr, rowsum, va, vb);
+        NAJA_HIP(hipGetLastError());
+        return;
+    }
+    hipLaunchKernelGGL((gemm_kernel<TA, TB, AK, BKC, EPI>), dim3(gx, gy, splits), dim3(256), 0, s, M, N, K, A, lda,
+                       B, ldb, C, ldc, alpha, bias, mask, ldm, kps, part, rowsum, va, vb);
+    NAJA_HIP(hipGetLastError());
+    const int64_t MN = (int64_t)M * N;
+    hipLaunchKernelGGL(gemm_reduce_kernel<EPI>, dim3((unsigned)std::min<int64_t>((MN + 255) / 256, 4096)), dim3(256),
+                       0, s, part, splits, M, N, C, ldc, alpha, bias, mask, ldm, rowsum);
+    NAJA_HIP(hipGetLastError());
 }
 
 // Adam (Kingma & Ba, 2015) on the FP32 masters; refreshes the BF16 copies.
@@ -385,6 +521,9 @@ struct naja_lp {
     // activations for `cap` rows
     int cap = 0;
     float *h1 = nullptr, *h2 = nullptr, *o = nullptr, *d1 = nullptr, *d2 = nullptr, *dO = nullptr, *lossb = nullptr;
+    // split-K partial tiles and row sums of the products, part_floats floats
+    float* part = nullptr;
+    size_t part_floats = 0;
 };
 
 namespace {
@@ -393,11 +532,12 @@ constexpr int kMaxNllDim = 160;  // LDS: (2 * dim + 6 * 32) * 32 doubles <= 128 
 
 void free_acts(naja_lp* h)
 {
-    for (float** q : {&h->h1, &h->h2, &h->o, &h->d1, &h->d2, &h->dO, &h->lossb}) {
+    for (float** q : {&h->h1, &h->h2, &h->o, &h->d1, &h->d2, &h->dO, &h->lossb, &h->part}) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;
     }
     h->cap = 0;
+    h->part_floats = 0;
 }
 
 void ensure_acts(naja_lp* h, int batch)
@@ -413,6 +553,10 @@ void ensure_acts(naja_lp* h, int batch)
     NAJA_HIP(hipMalloc(&h->d2, sizeof(float) * cap * h->H2));
     NAJA_HIP(hipMalloc(&h->dO, sizeof(float) * cap * h->O));
     NAJA_HIP(hipMalloc(&h->lossb, sizeof(float) * cap));
+    // Up to 16 K-splits of the batch-row products and their row sums.
+    const size_t part = 16 * ((size_t)cap * std::max(h->H1, h->H2) + std::max({cap, h->O, h->H1, h->H2}));
+    NAJA_HIP(hipMalloc(&h->part, sizeof(float) * part));
+    h->part_floats = part;
     h->cap = cap;
 }
 
@@ -421,12 +565,14 @@ const bf16_t* wbf(const naja_lp* h, int64_t off) { return h->pbf + off; }
 // Layers 1-2 into h1, h2; layer 3 into out3 = alpha3 * (h2 W3^T + b3).
 void forward(naja_lp* h, const float* X, int B, hipStream_t s, float* out3, float alpha3)
 {
+    float* P = h->part;
+    const size_t PF = h->part_floats;
     gemm<float, bf16_t, true, true, EPI_BIAS_RELU>(s, B, h->H1, h->F, X, h->F, wbf(h, h->off_w1), h->F, h->h1, h->H1,
-                                                   1.0f, h->p + h->off_b1);
+                                                   1.0f, h->p + h->off_b1, nullptr, 0, nullptr, P, PF);
     gemm<float, bf16_t, true, true, EPI_BIAS_RELU>(s, B, h->H2, h->H1, h->h1, h->H1, wbf(h, h->off_w2), h->H1, h->h2,
-                                                   h->H2, 1.0f, h->p + h->off_b2);
+                                                   h->H2, 1.0f, h->p + h->off_b2, nullptr, 0, nullptr, P, PF);
     gemm<float, bf16_t, true, true, EPI_BIAS>(s, B, h->O, h->H2, h->h2, h->H2, wbf(h, h->off_w3), h->H2, out3, h->O,
-                                              alpha3, h->p + h->off_b3);
+                                              alpha3, h->p + h->off_b3, nullptr, 0, nullptr, P, PF);
 }
 
 void loss_and_backward(naja_lp* h, const float* X, const float* S, int B, float* loss, hipStream_t s)
@@ -437,20 +583,21 @@ void loss_and_backward(naja_lp* h, const float* X, const float* S, int B, float*
                        loss ? loss : h->lossb, h->dO);
     NAJA_HIP(hipGetLastError());
     float* g = h->g;
-    // Layer 3: dW3 = dO^T h2, db3 = colsum dO, d2 = (dO W3) * relu'(z2).
-    gemm<float, float, false, false, EPI_NONE>(s, h->O, h->H2, B, h->dO, h->O, h->h2, h->H2, g + h->off_w3, h->H2);
-    hipLaunchKernelGGL(colsum_kernel, dim3((h->O + 255) / 256), dim3(256), 0, s, h->dO, B, h->O, g + h->off_b3);
+    float* P = h->part;
+    const size_t PF = h->part_floats;
+    // Layer 3: dW3 = dO^T h2 with db3 = the row sums of dO^T, d2 = (dO W3) * relu'(z2).
+    gemm<float, float, false, false, EPI_NONE>(s, h->O, h->H2, B, h->dO, h->O, h->h2, h->H2, g + h->off_w3, h->H2,
+                                               1.0f, nullptr, nullptr, 0, g + h->off_b3, P, PF);
     gemm<float, bf16_t, true, false, EPI_MASK>(s, B, h->H2, h->O, h->dO, h->O, wbf(h, h->off_w3), h->H2, h->d2, h->H2,
-                                               1.0f, nullptr, h->h2, h->H2);
+                                               1.0f, nullptr, h->h2, h->H2, nullptr, P, PF);
     // Layer 2.
-    gemm<float, float, false, false, EPI_NONE>(s, h->H2, h->H1, B, h->d2, h->H2, h->h1, h->H1, g + h->off_w2, h->H1);
-    hipLaunchKernelGGL(colsum_kernel, dim3((h->H2 + 255) / 256), dim3(256), 0, s, h->d2, B, h->H2, g + h->off_b2);
+    gemm<float, float, false, false, EPI_NONE>(s, h->H2, h->H1, B, h->d2, h->H2, h->h1, h->H1, g + h->off_w2, h->H1,
+                                               1.0f, nullptr, nullptr, 0, g + h->off_b2, P, PF);
     gemm<float, bf16_t, true, false, EPI_MASK>(s, B, h->H1, h->H2, h->d2, h->H2, wbf(h, h->off_w2), h->H1, h->d1, h->H1,
-                                               1.0f, nullptr, h->h1, h->H1);
+                                               1.0f, nullptr, h->h1, h->H1, nullptr, P, PF);
     // Layer 1.
-    gemm<float, float, false, false, EPI_NONE>(s, h->H1, h->F, B, h->d1, h->H1, X, h->F, g + h->off_w1, h->F);
-    hipLaunchKernelGGL(colsum_kernel, dim3((h->H1 + 255) / 256), dim3(256), 0, s, h->d1, B, h->H1, g + h->off_b1);
-    NAJA_HIP(hipGetLastError());
+    gemm<float, float, false, false, EPI_NONE>(s, h->H1, h->F, B, h->d1, h->H1, X, h->F, g + h->off_w1, h->F, 1.0f,
+                                               nullptr, nullptr, 0, g + h->off_b1, P, PF);
 }
 
 void check_batch(const naja_lp* h, const float* X, int B)
@@ -641,21 +788,30 @@ extern "C" int naja_pilot_covariance(const float* samples, int n_variants, int64
 }
 
 extern "C" int naja_gemm_bf16(int trans_a, int trans_b, int M, int N, int K, const float* A, int64_t lda,
-                              const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, void* stream)
+                              const float* B, int64_t ldb, float* C, int64_t ldc, float alpha, float* rowsum,
+                              float* workspace, int64_t workspace_floats, void* stream)
 {
     return guarded([&]() {
         NAJA_REQUIRE(A && B && C && M > 0 && N > 0 && K > 0, NAJA_ERR_INVALID_ARGUMENT, "naja_gemm_bf16: bad arguments");
+        NAJA_REQUIRE(!rowsum || trans_a, NAJA_ERR_INVALID_ARGUMENT, "naja_gemm_bf16: row sums need A stored K x M");
+        NAJA_REQUIRE(workspace_floats >= 0 && (workspace || workspace_floats == 0), NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_gemm_bf16: bad workspace");
         hipStream_t s = (hipStream_t)stream;
+        const size_t wf = (size_t)workspace_floats;
         // op(A) is M x K: row-major A (trans_a = 0) has k contiguous; op(B) is
         // K x N: row-major B (trans_b = 0) has n contiguous.
         if (!trans_a && !trans_b)
-            gemm<float, float, true, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+            gemm<float, float, true, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha, nullptr, nullptr, 0,
+                                                      nullptr, workspace, wf);
         else if (!trans_a && trans_b)
-            gemm<float, float, true, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+            gemm<float, float, true, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha, nullptr, nullptr, 0,
+                                                     nullptr, workspace, wf);
         else if (trans_a && !trans_b)
-            gemm<float, float, false, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+            gemm<float, float, false, false, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha, nullptr, nullptr, 0,
+                                                       rowsum, workspace, wf);
         else
-            gemm<float, float, false, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha);
+            gemm<float, float, false, true, EPI_NONE>(s, M, N, K, A, lda, B, ldb, C, ldc, alpha, nullptr, nullptr, 0,
+                                                      rowsum, workspace, wf);
     });
 }
 

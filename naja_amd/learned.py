@@ -125,8 +125,10 @@ def pilot_covariance(samples, scale=1.0, stream=None):
     return S, mean
 
 
-def gemm_bf16(A, B, trans_a=False, trans_b=False, alpha=1.0, stream=None):
-    """alpha * op(A) op(B) on BF16 MFMA (FP32 accumulate)."""
+def gemm_bf16(A, B, trans_a=False, trans_b=False, alpha=1.0, stream=None, row_sums=False, split_k=False):
+    """alpha * op(A) op(B) on BF16 MFMA (FP32 accumulate). row_sums (trans_a
+    only) also returns the sums over k of op(A) in FP32; split_k hands the
+    product a workspace so that K may be split across workgroups."""
     torch = _torch()
     A = _dev(A, torch.float32)
     B = _dev(B, torch.float32)
@@ -135,7 +137,11 @@ def gemm_bf16(A, B, trans_a=False, trans_b=False, alpha=1.0, stream=None):
     if K != K2:
         raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"inner dimensions {K} != {K2}")
     C = torch.empty((M, N), dtype=torch.float32, device=A.device)
+    rs = torch.empty(M, dtype=torch.float32, device=A.device) if row_sums else None
+    ws = torch.empty(16 * (M * N + M), dtype=torch.float32, device=A.device) if split_k else None
     _check(load_library().naja_gemm_bf16(int(trans_a), int(trans_b), M, N, K, A.data_ptr(), A.shape[1],
                                          B.data_ptr(), B.shape[1], C.data_ptr(), N, float(alpha),
-                                         _stream_ptr(stream)))
-    return C
+                                         rs.data_ptr() if rs is not None else None,
+                                         ws.data_ptr() if ws is not None else None,
+                                         ws.numel() if ws is not None else 0, _stream_ptr(stream)))
+    return (C, rs) if row_sums else C

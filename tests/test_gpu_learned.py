@@ -30,6 +30,42 @@ def test_gemm_bf16_matches_rounded_product(gpu, ta, tb, M, N, K):
     assert np.all(np.abs(C - ref) <= 2e-6 * bound + 1e-30), f"max err {np.abs(C - ref).max()}"
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(256, 512, 3200), (3200, 512, 256), (70, 45, 1000), (256, 512, 64), (5, 3, 129)])
+def test_gemm_bf16_split_k_and_row_sums(gpu, ta, tb, M, N, K):
+    """K split across workgroups (partial tiles summed in split order) keeps
+    the same bound as the single pass; the row sums of op(A) are the FP32 sums
+    of the unrounded operand."""
+    from naja_amd.learned import gemm_bf16
+
+    rng = np.random.default_rng(M + 3 * N + 11 * K)
+    A = rng.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if tb else (K, N)).astype(np.float32)
+    out = gemm_bf16(gpu.from_numpy(A).cuda(), gpu.from_numpy(B).cuda(), bool(ta), bool(tb), alpha=0.5,
+                    row_sums=bool(ta), split_k=True)
+    C, rs = (out[0].cpu().numpy(), out[1].cpu().numpy()) if ta else (out.cpu().numpy(), None)
+    Aq = orc.bf16(A.T if ta else A)
+    Bq = orc.bf16(B.T if tb else B)
+    ref = 0.5 * Aq.astype(np.float64) @ Bq.astype(np.float64)
+    bound = 0.5 * np.abs(Aq).astype(np.float64) @ np.abs(Bq).astype(np.float64)
+    assert np.all(np.abs(C - ref) <= 2e-6 * bound + 1e-30), f"max err {np.abs(C - ref).max()}"
+    if ta:
+        exact = A.astype(np.float64).sum(axis=0)
+        assert np.all(np.abs(rs - exact) <= 1e-6 * np.abs(A).astype(np.float64).sum(axis=0) + 1e-30)
+        # the split takes nothing from what one pass gives
+        single = gemm_bf16(gpu.from_numpy(A).cuda(), gpu.from_numpy(B).cuda(), bool(ta), bool(tb), alpha=0.5,
+                           row_sums=True)
+        assert np.allclose(single[1].cpu().numpy(), rs, rtol=0, atol=1e-6 * np.abs(A).sum(axis=0).max())
+
+
+def test_gemm_bf16_row_sums_need_transposed_a(gpu):
+    from naja_amd.learned import gemm_bf16
+    from naja_amd import NajaInvalidArgument
+
+    with pytest.raises(NajaInvalidArgument):
+        gemm_bf16(gpu.zeros((4, 3)).cuda(), gpu.zeros((3, 2)).cuda(), row_sums=True)
+
+
 def test_pilot_covariance_matches_numpy(gpu):
     from naja_amd.learned import pilot_covariance
 

@@ -87,7 +87,13 @@ struct ChrKernelEntry {
     // Same shape specialised for launches whose every move is a coordinate
     // move (no pair or k-sparse probability); null when not compiled.
     ChrKernelFn fn_coord = nullptr;
+    // One-wavefront shapes with the chain states in VGPRs (d <= 128), general
+    // and coordinate-only; null when not compiled.
+    ChrKernelFn fn_x2 = nullptr;
+    ChrKernelFn fn_coord_x2 = nullptr;
 };
+// Largest d of the register-resident-state kernels (two VGPRs per lane).
+constexpr int kChrRegStateMaxD = 128;
 struct ChrKernelTable {
     const ChrKernelEntry* e;
     int n;

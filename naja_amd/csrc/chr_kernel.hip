@@ -21,9 +21,14 @@
 //     one barrier per step (double-buffered partials), instead of the
 //     reference's two CUB block reductions, three barriers and two final
 //     reciprocals;
-//   * x lives in LDS; the lane-0 owner applies the 1-2 coordinate update, and
-//     every lane carries the last update in registers so no second barrier is
-//     needed before the next read;
+//   * one-wavefront launches with d <= 128 keep x in VGPRs (coordinate c in
+//     lane c % 64 of slot c / 64) and every lane updates its own coordinates:
+//     a step touches no LDS; otherwise x lives in LDS, the lane-0 owner
+//     applies the 1-2 coordinate update, and every lane carries the last
+//     update in registers so no second barrier is needed before the next read;
+//   * the columns are buffer loads over the variant's whole block, the
+//     column in the instruction's SGPR offset and each lane's run of rows in
+//     a fixed VGPR offset (runs past the real rows read 0 from no memory);
 //   * the slack is FP32 and is recomputed in FP64 as s = b - A y at t = 0 and
 //     every resync_interval steps (the reference's resync, chr.cu:386-395).
 // Arithmetic is the oracle's, operation for operation (explicit fmaf, IEEE
@@ -261,13 +266,22 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
     const ChrKernelTable& t = all_kernels();
     ChrKernelFn fn = nullptr;
     const bool coord_only = args.thr_k == 0 && !args.has_pair;
+    const bool reg_state = args.d <= kChrRegStateMaxD;
     for (int q = 0; q < t.n; ++q)
-        if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R)
-            fn = coord_only && t.e[q].fn_coord ? t.e[q].fn_coord : t.e[q].fn;
+        if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R) {
+            const ChrKernelEntry& e = t.e[q];
+            fn = coord_only && e.fn_coord ? e.fn_coord : e.fn;
+            ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
+            if (reg_state && fx) fn = fx;
+        }
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d", sh.W, sh.G, sh.R);
     NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),
                  NAJA_ERR_UNSUPPORTED, "%lld steps in one launch (at most 2^31 - 1; the streamed API splits longer runs)",
                  (long long)args.nsteps);
+    // The register kernels address a variant's block with 32-bit byte
+    // offsets and mark padding runs with bit 31.
+    NAJA_REQUIRE(sh.big() || 4.0 * sh.rows() * (double)args.d < 2147483648.0, NAJA_ERR_UNSUPPORTED,
+                 "a packed block of %d rows x %d columns exceeds 2 GiB", sh.rows(), args.d);
     const size_t lds = sh.lds_bytes(args.d);
     NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
                  args.d, sh.G);

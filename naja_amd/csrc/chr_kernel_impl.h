@@ -270,12 +270,17 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // PF: prefetch distance in steps. The columns of step t + PF are loaded
 // during step t into one of PF + 1 rotating register sets, so each load has
 // PF steps to arrive (moves do not depend on the state).
-template <int W, int G, int R, bool CO = false, int PF = 1>
+// XR > 0 (one wavefront): the chain states live in VGPRs, coordinate c of
+// chain g in lane c % 64 of xv[g][c / 64] (d <= 64 XR), and every lane
+// applies the update of its own coordinates, so a step reads and writes no
+// LDS and needs no exec-masked store; XR = 0 keeps the states in LDS.
+template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0>
 __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 {
     static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
+    static_assert(XR == 0 || W == 1, "register-resident states need one wavefront");
     constexpr int T = 64 * W;
-    extern __shared__ float xs[];  // G x d current states
+    extern __shared__ float xs[];  // G x d current states (XR = 0)
     __shared__ uint2 part[2][G][W];
     __shared__ int ks_idx[kKmax];
     __shared__ float ks_coef[kKmax];
@@ -293,6 +298,7 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // R/VEC vector loads per lane and a wavefront's load covers VEC*T
     // consecutive rows. Padding rows (A = 0, b = +inf) fill every run past m.
     constexpr int VEC = (R % 4 == 0) ? 4 : ((R % 2 == 0) ? 2 : 1);
+    constexpr int XS = XR > 0 ? XR : 1;
     auto row_of = [&](int k) { return (k / VEC) * (VEC * T) + tid * VEC + (k % VEC); };
     const float* Av = a.A + V.a_off;
     const int geff = min(G, a.nchains - item.c0);
@@ -300,9 +306,20 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const uint32_t k0 = a.seed_lo;
     const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
 
-    for (int g = 0; g < G; ++g) {
-        for (int c = tid; c < d; c += T) {
-            xs[g * d + c] = (g < geff) ? a.y0[(V.y0_col + item.c0 + g) * (int64_t)d + c] : 0.0f;
+    float xv[G][XS];
+    if constexpr (XR > 0) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int q = 0; q < XR; ++q) {
+                const int c = q * 64 + lane;
+                xv[g][q] = (g < geff && c < d) ? a.y0[(V.y0_col + item.c0 + g) * (int64_t)d + c] : 0.0f;
+            }
+    } else {
+        for (int g = 0; g < G; ++g) {
+            for (int c = tid; c < d; c += T) {
+                xs[g * d + c] = (g < geff) ? a.y0[(V.y0_col + item.c0 + g) * (int64_t)d + c] : 0.0f;
+            }
         }
     }
 
@@ -316,62 +333,52 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
     const int L0 = (int)(t_begin & 63);
     // (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j]) for a pair
-    // move; a k-sparse move (or no move) loads a dummy pair. Buffer loads over
-    // one column each, ranged to the variant's real rows: padding rows read 0
-    // without touching memory (their slack is +inf, so their chord ratio is
-    // inf * 0 or inf / 0 and never bounds the chord).
-    // Whole runs in range or out of range: m rounded up to VEC rows (the rows
-    // past m in memory are padding). One descriptor per column, its base the
-    // block base plus a 32-bit byte offset (the range check covers the
-    // instruction's SGPR offset too, so the column cannot go there).
-    const uint32_t col_bytes = 4u * (uint32_t)((V.m + VEC - 1) / VEC * VEC);
-    const uint64_t a_base = (uint64_t)(a.A + V.a_off);
-    const uint64_t inv_base = (uint64_t)(a.Ainv + V.a_off);
-    const uint32_t col_stride = 4u * (uint32_t)lda;  // bytes from one column to the next
-    // Each descriptor gets its own copy of the constant words (range and
-    // format), opaque to the compiler, so that it keeps one SGPR quad per
-    // descriptor and writes the per-step base straight into it instead of
-    // copying one shared pair of constant words into a second quad every step.
-    int w2a = (int)col_bytes, w3a = 0x00020000, w2b = (int)col_bytes, w3b = 0x00020000;
-    asm volatile("" : "+s"(w2a), "+s"(w3a), "+s"(w2b), "+s"(w3b));
-    auto col_rsrc = [&](uint64_t base, uint32_t off, int w2, int w3) {
-        const uint64_t addr = base + off;
-        chr_i32x4 r;
-        r[0] = __builtin_amdgcn_readfirstlane((int)(uint32_t)addr);
-        r[1] = __builtin_amdgcn_readfirstlane((int)(uint32_t)(addr >> 32));
-        r[2] = w2;
-        r[3] = w3;
-        return r;
-    };
+    // move; a k-sparse move (or no move) loads a dummy pair. One buffer
+    // descriptor per packed array over the variant's whole block: the column
+    // is the instruction's SGPR offset, the lane's run of rows its VGPR
+    // offset, fixed for the launch. Runs wholly past the real rows (m rounded
+    // up to VEC; the rows past m in memory are padding) get an offset beyond
+    // any block (bit 31; the launch keeps blocks under 2 GiB), so they read 0
+    // without touching memory: their slack is +inf, so their chord ratio is
+    // inf * 0 or inf / 0 and never bounds the chord.
+    const uint32_t blk_bytes = 4u * (uint32_t)lda * (uint32_t)d;
+    const chr_i32x4 rsA = chr_make_rsrc(a.A + V.a_off, blk_bytes);
+    const chr_i32x4 rsI = chr_make_rsrc(a.Ainv + V.a_off, blk_bytes);
+    const int col_stride = 4 * lda;  // bytes from one column to the next
+    const int mpad = (V.m + VEC - 1) / VEC * VEC;
+    int voff[R / VEC];
+#pragma unroll
+    for (int q = 0; q < R / VEC; ++q) {
+        const int r0 = q * VEC * T + tid * VEC;
+        voff[q] = r0 < mpad ? 4 * r0 : (int)0x80000000u;
+    }
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
         const int mt = CO ? kMoveCoord : m.type;
         const bool real = real_move && mt != kMoveKsparse;
         const bool pair = real && mt == kMovePair;
-        const uint32_t oi = (uint32_t)(real ? m.i : 0) * col_stride;
-        const chr_i32x4 ri = col_rsrc(a_base, oi, w2a, w3a);
-        const chr_i32x4 rj = pair ? col_rsrc(a_base, (uint32_t)m.j * col_stride, w2b, w3b)
-                                  : col_rsrc(inv_base, oi, w2b, w3b);
+        const int si = real ? m.i * col_stride : 0;
+        const int sj = pair ? m.j * col_stride : si;
+        const chr_i32x4 rj = pair ? rsA : rsI;
 #pragma unroll
         for (int q = 0; q < R / VEC; ++q) {
-            const int off = 4 * (q * VEC * T + tid * VEC);
             if (VEC == 4) {
-                const chr_f32x4 u = chr_raw_load_v4f32(ri, off, 0, 0);
-                const chr_f32x4 w = chr_raw_load_v4f32(rj, off, 0, 0);
+                const chr_f32x4 u = chr_raw_load_v4f32(rsA, voff[q], si, 0);
+                const chr_f32x4 w = chr_raw_load_v4f32(rj, voff[q], sj, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     xa[q * 4 + e] = u[e];
                     xb[q * 4 + e] = w[e];
                 }
             } else if (VEC == 2) {
-                const chr_f32x2 u = chr_raw_load_v2f32(ri, off, 0, 0);
-                const chr_f32x2 w = chr_raw_load_v2f32(rj, off, 0, 0);
+                const chr_f32x2 u = chr_raw_load_v2f32(rsA, voff[q], si, 0);
+                const chr_f32x2 w = chr_raw_load_v2f32(rj, voff[q], sj, 0);
                 xa[q * 2] = u[0];
                 xa[q * 2 + 1] = u[1];
                 xb[q * 2] = w[0];
                 xb[q * 2 + 1] = w[1];
             } else {
-                xa[q] = chr_raw_load_f32(ri, off, 0, 0);
-                xb[q] = chr_raw_load_f32(rj, off, 0, 0);
+                xa[q] = chr_raw_load_f32(rsA, voff[q], si, 0);
+                xb[q] = chr_raw_load_f32(rj, voff[q], sj, 0);
             }
         }
     };
@@ -410,9 +417,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     // Events by step offset u from t_begin: the sample of the state after
     // step next_save (u % thin == 0) is stored at the start of step
     // next_save + 1 (or after the last step), the FP64 slacks are recomputed
-    // at the start of step next_resync; each step tests only next_event, the
-    // earlier of the two starts.
+    // at the start of step next_resync, and a new batch of 64 draws is made
+    // at the start of step next_batch, the first whose draw PF steps ahead
+    // opens one; each step tests only next_event, the earliest of these.
     int64_t next_save = 0, next_resync = 0;
+    int next_batch = (64 - ((L0 + PF) & 63)) & 63;
     int next_event = 0;
     int64_t sample_idx = 0;
 
@@ -426,15 +435,30 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                 float* dst = a.samples + V.sample_off + (col + g) * (int64_t)d;
                 // Streaming stores: the samples are not read again by this
                 // launch, so they should not displace the columns of A in L2.
-                for (int c = tid; c < d; c += T) {
-                    float v = xs[g * d + c];
-                    if (c == prev_i) v = last_i[g];
-                    if (c == prev_j) v = last_j[g];
-                    __builtin_nontemporal_store(v, dst + c);
+                if constexpr (XR > 0) {
+#pragma unroll
+                    for (int q = 0; q < XR; ++q)
+                        if (q * 64 + lane < d) __builtin_nontemporal_store(xv[g][q], dst + q * 64 + lane);
+                } else {
+                    for (int c = tid; c < d; c += T) {
+                        float v = xs[g * d + c];
+                        if (c == prev_i) v = last_i[g];
+                        if (c == prev_j) v = last_j[g];
+                        __builtin_nontemporal_store(v, dst + c);
+                    }
                 }
             }
         }
         ++sample_idx;
+    };
+
+    // x_c <- fmaf(alpha, coef, x_c) for chain g, in the lane that holds c.
+    auto xr_update = [&](int g, int c, float alpha_g, float coef) {
+#pragma unroll
+        for (int q = 0; q < XS; ++q) {
+            const float nv = fmaf(alpha_g, coef, xv[g][q]);
+            xv[g][q] = (q * 64 + lane == c) ? nv : xv[g][q];
+        }
     };
 
     // One CHR step on the columns (ca, cb), prefetching the columns of step
@@ -460,24 +484,38 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         }
         if (event && u == next_resync) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
-            __syncthreads();
             double acc[G][R];
 #pragma unroll
             for (int k = 0; k < R; ++k)
 #pragma unroll
                 for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
             const float* pc = Av;
-            for (int c = 0; c < d; ++c) {
-                double xv[G];
-#pragma unroll
-                for (int g = 0; g < G; ++g) xv[g] = (double)xs[g * d + c];
+            auto add_col = [&](const double (&xc)[G]) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     const double av = (double)pc[row_of(k)];
 #pragma unroll
-                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xv[g], acc[g][k]);
+                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xc[g], acc[g][k]);
                 }
                 pc += lda;
+            };
+            if constexpr (XR > 0) {
+#pragma unroll
+                for (int q = 0; q < XR; ++q)
+                    for (int l = 0; l < 64 && q * 64 + l < d; ++l) {
+                        double xc[G];
+#pragma unroll
+                        for (int g = 0; g < G; ++g) xc[g] = (double)readlane_f32(xv[g][q], l);
+                        add_col(xc);
+                    }
+            } else {
+                __syncthreads();
+                for (int c = 0; c < d; ++c) {
+                    double xc[G];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) xc[g] = (double)xs[g * d + c];
+                    add_col(xc);
+                }
             }
             const double* bv = a.b + V.b_off;
 #pragma unroll
@@ -491,6 +529,10 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             // No resync: a step the launch never reaches (fewer than 2^31 steps).
             next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
         }
+        if (event && u == next_batch) {
+            gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
+            next_batch += 64;
+        }
         if (event) {
             // Move logging is an event of every step while it lasts.
             if (log_on && u < (int)a.nlog && tid == 0) {
@@ -500,13 +542,14 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                 a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
             }
             int64_t e = next_resync < next_save + 1 ? next_resync : next_save + 1;
+            if (next_batch < e) e = next_batch;
             if (log_on && u + 1 < (int)a.nlog && u + 1 < e) e = u + 1;
             next_event = e < 0x7FFFFFFF ? (int)e : 0x7FFFFFFF;
         }
 
-        // Alpha uniforms of step t, and the draw of step t + PF (a new batch
-        // of 64 steps when t + PF starts one; the FIFO already holds the
-        // draws of the steps before it).
+        // Alpha uniforms of step t, and the draw of step t + PF (from the
+        // batch that holds it; the FIFO already holds the draws of the steps
+        // before it).
         float U[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) U[g] = fu[0][g];
@@ -519,13 +562,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         const bool has_pf = kPfAll || nst - u > PF;
         if (has_pf) {
             const int Lp = (L0 + u + PF) & 63;
-            if (Lp == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
             nf = read_move<G, CO>(mb, Lp);
 #pragma unroll
             for (int g = 0; g < G; ++g) nu[g] = readlane_f32(mb.u[g], Lp);
         }
 
-        // Prefetch the columns of step t + PF. Every step issues exactly 2R
+        // Prefetch the columns of step t + PF. Every step issues exactly 2R/VEC
         // loads, so the in-order vmcnt wait for this step's columns never
         // waits on the later prefetches.
         load_cols(nf, has_pf, na, nb);
@@ -557,12 +599,13 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             }
         }
 
-        // Pre-update coordinates. With one wavefront its own LDS write of the
-        // previous update is ordered before this read, so the read is issued
-        // here, early; with several, registers carry the previous update
-        // because the owner's LDS write of it is not yet ordered.
+        // Pre-update coordinates (LDS states). With one wavefront its own LDS
+        // write of the previous update is ordered before this read, so the
+        // read is issued here, early; with several, registers carry the
+        // previous update because the owner's LDS write of it is not yet
+        // ordered.
         float oi[G], oj[G];
-        if (mtype != kMoveKsparse) {
+        if (XR == 0 && mtype != kMoveKsparse) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 if (W == 1) {
@@ -633,7 +676,25 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             for (int k = 0; k < R; ++k) s[g][k] = fmaf(nal, ae[k], s[g][k]);
         }
 
-        if (mtype != kMoveKsparse) {
+        if constexpr (XR > 0) {
+            // Same roundings as the LDS form: x_i = fmaf(alpha, ci, x_i), then
+            // x_j = fmaf(alpha, cj, x_j), the second on the first's result
+            // when j == i; k-sparse entries in order.
+            if (mtype != kMoveKsparse) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    xr_update(g, mv.i, alpha[g], mv.ci);
+                    if (mtype == kMovePair) xr_update(g, mv.j, alpha[g], mv.cj);
+                }
+            } else {
+                for (int q = 0; q < kq; ++q) {
+                    const int c = __builtin_amdgcn_readfirstlane(ks_idx[q]);
+                    const float cq = ks_coef[q];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) xr_update(g, c, alpha[g], cq);
+                }
+            }
+        } else if (mtype != kMoveKsparse) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 const float ni = fmaf(alpha[g], mv.ci, oi[g]);
@@ -702,10 +763,22 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                  cs[(j + PF) % (PF + 1)][1]);
     if ((int64_t)nst == next_save + 1) save_sample();  // the last step's sample
 
-    __syncthreads();
-    if (a.y_final) {
-        for (int g = 0; g < geff; ++g)
-            for (int c = tid; c < d; c += T) a.y_final[(V.y0_col + item.c0 + g) * (int64_t)d + c] = xs[g * d + c];
+    if constexpr (XR > 0) {
+        if (a.y_final) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int q = 0; q < XR; ++q)
+                    if (g < geff && q * 64 + lane < d)
+                        a.y_final[(V.y0_col + item.c0 + g) * (int64_t)d + q * 64 + lane] = xv[g][q];
+        }
+    } else {
+        __syncthreads();
+        if (a.y_final) {
+            for (int g = 0; g < geff; ++g)
+                for (int c = tid; c < d; c += T)
+                    a.y_final[(V.y0_col + item.c0 + g) * (int64_t)d + c] = xs[g * d + c];
+        }
     }
 }
 

@@ -96,6 +96,24 @@ def test_trajectory_matches_oracle(gpu, case):
     np.testing.assert_array_equal(fin, ref["y_final"]) if exact == 1.0 else None
 
 
+@pytest.mark.parametrize("d,m", [(100, 300), (128, 200), (64, 100), (150, 200)])
+@pytest.mark.parametrize("moves", ["coord", "pair-ksparse"])
+def test_one_wavefront_state_paths_bit_exact(gpu, d, m, moves):
+    """One-wavefront launches keep the chain states in VGPRs for d <= 128 and
+    in LDS above; both repeat the oracle's operations bit for bit (samples,
+    final states and moves), with every move kind."""
+    A, b, _ = random_polytope(d, m, seed=d + m)
+    kw = {} if moves == "coord" else dict(pair_prob=0.3, ksparse_prob=0.05, ksparse_k=5)
+    nch, n, thin = 3, 700, 3
+    Y0 = np.zeros((nch, d))
+    samples, fin, mlog = run_gpu(gpu, A, b, Y0, n, thin, tpb=64, move_log_steps=n * thin, resync_interval=500, **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), n, thin, nlog=n * thin,
+                             resync_interval=500, **kw)
+    np.testing.assert_array_equal(mlog, ref["move_log"])
+    np.testing.assert_array_equal(samples, ref["samples"])
+    np.testing.assert_array_equal(fin, ref["y_final"])
+
+
 def test_chains_per_group_matches_oracle(gpu):
     A, b, y0 = random_polytope(12, 70, seed=31)
     nch = 6  # one full group of 4 and a partial group of 2

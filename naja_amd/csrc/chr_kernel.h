@@ -12,7 +12,7 @@ namespace naja {
 // ratio (inf / 0, or inf * 0 when the register kernel's ranged loads read the
 // padding as 0) never bounds the chord of a bounded polytope.
 struct ChrVariant {
-    int m;               // real rows m0 + k
+    int m;               // real packed rows (m0 + k; fewer with paired rows)
     int lda;             // padded rows (64 W R)
     int64_t a_off;       // float offset of the packed block
     int64_t b_off;       // offset of the packed FP64 right-hand side
@@ -30,6 +30,7 @@ struct ChrArgs {
     const float* A;     // packed variant blocks
     const float* Ainv;  // RN(1 / A) of the packed blocks (coordinate-move chords)
     const double* b;  // packed right-hand sides
+    const double* bn;  // paired-row layout: the partner rows' right-hand sides (-a_r y <= bn_r); else null
     int d;
     const ChrVariant* variants;
     const ChrItem* items;
@@ -91,6 +92,12 @@ struct ChrKernelEntry {
     // and coordinate-only; null when not compiled.
     ChrKernelFn fn_x2 = nullptr;
     ChrKernelFn fn_coord_x2 = nullptr;
+    // The same four for the paired-row layout (a packed row and its negation
+    // share one column entry and keep two slacks); null when not compiled.
+    ChrKernelFn fn_p = nullptr;
+    ChrKernelFn fn_coord_p = nullptr;
+    ChrKernelFn fn_x2_p = nullptr;
+    ChrKernelFn fn_coord_x2_p = nullptr;
 };
 // Largest d of the register-resident-state kernels (two VGPRs per lane).
 constexpr int kChrRegStateMaxD = 128;
@@ -106,14 +113,16 @@ ChrKernelTable chr_table_w8();
 ChrKernelTable chr_table_w16();
 ChrKernelTable chr_table_g4();
 ChrKernelTable chr_table_big();  // W = 16, R = 10..22: the large-polytope kernel
+ChrKernelTable chr_table_w1p();  // one wavefront, paired rows
 
 // Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
-// per group; -1 if none.
-int chr_smallest_r(int W, int G, int rows);
+// per group (among the paired-row layout's kernels when paired); -1 if none.
+int chr_smallest_r(int W, int G, int rows, bool paired = false);
 
-// Shape for a batch whose largest variant has m_max rows (tpb > 0 is the
-// reference's --tpb, taken as a preference).
-ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items);
+// Shape for a batch whose largest variant has m_max packed rows (tpb > 0 is
+// the reference's --tpb, taken as a preference). Paired: among the
+// paired-row kernels, and W = 0 when none holds the rows.
+ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items, bool paired = false);
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to
@@ -121,14 +130,24 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 void chr_timing_enable(bool on);
 int chr_timing_collect(int* items, float* ms, int max);
 
-// Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks.
+// Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks;
+// with bn_pack, in the paired-row layout (paired_dev: the flags of
+// chr_pair_check).
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
-              float* A_pack, float* Ainv_pack, double* b_pack, int lda_max, hipStream_t stream);
+              const int* paired_dev, float* A_pack, float* Ainv_pack, double* b_pack, double* bn_pack, int lda_max,
+              hipStream_t stream);
 
-// Rounding transform: A' = A P_v, b' = b - A c_v per variant (see chr_kernel.hip).
-void chr_transform(const float* A_pack, const double* b_pack, const ChrVariant* variants_dev, int n_variants, int d,
-                   int lda, const float* P, const float* c, float* A_out, float* Ainv_out, double* b_out,
-                   hipStream_t stream);
+// Rounding transform: A' = A P_v, b' = b - A c_v per variant (see
+// chr_kernel.hip); bn_pack / bn_out for the paired-row layout (else null).
+void chr_transform(const float* A_pack, const double* b_pack, const double* bn_pack, const ChrVariant* variants_dev,
+                   int n_variants, int d, int lda, const float* P, const float* c, float* A_out, float* Ainv_out,
+                   double* b_out, double* bn_out, hipStream_t stream);
+
+// Row pairing: flags_dev[0] (base block) and flags_dev[1 + v] (extra block of
+// variant v), set to 1 by the caller, are cleared unless the block's second
+// half is the negation of its first.
+void chr_pair_check(const float* A, int m0, int d, const float* A_extra, const int64_t* extra_off_dev,
+                    const int* extra_k_dev, int n_variants, int* flags_dev, hipStream_t stream);
 
 }  // namespace naja

@@ -51,42 +51,88 @@ namespace {
 
 // Packs variant v: rows [A; E_v; 0...] with lda = V.lda, b = [b; f_v; +inf...],
 // and the elementwise IEEE reciprocal of the packed block.
+// PAIRED: a block whose second half negates its first (paired[0] for A,
+// paired[1 + v] for E_v) packs only its first half, row r with bn_r = the b
+// of row r + k/2; the rows of an unpaired block are packed alone with
+// bn = NaN; padding rows get b = bn = +inf and a reciprocal of 0 (their
+// ratios are then NaN, which never bound the chord).
+template <bool PAIRED>
 __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, const float* A_extra,
                                 const double* b_extra, const ChrVariant* variants, const int64_t* extra_off,
-                                const int* extra_k, float* A_pack, float* Ainv_pack, double* b_pack)
+                                const int* extra_k, const int* paired, float* A_pack, float* Ainv_pack,
+                                double* b_pack, double* bn_pack)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
     const int kv = extra_k[v];
     const int64_t eoff = extra_off[v];
+    const bool pb = PAIRED && paired[0] != 0, pe = PAIRED && paired[1 + v] != 0;
+    const int sb = pb ? m0 / 2 : m0;  // packed rows of the base block
+    const int se = pe ? kv / 2 : kv;  // packed rows of the extra block
     const int64_t total = (int64_t)V.lda * d;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e % V.lda);
         const int c = (int)(e / V.lda);
         float val = 0.0f;
-        if (r < m0) val = A[(int64_t)c * m0 + r];
-        else if (r < m0 + kv) val = A_extra[eoff * d + (int64_t)c * kv + (r - m0)];
+        const bool real = r < sb + se;
+        if (r < sb) val = A[(int64_t)c * m0 + r];
+        else if (real) val = A_extra[eoff * d + (int64_t)c * kv + (r - sb)];
         A_pack[V.a_off + e] = val;
-        Ainv_pack[V.a_off + e] = 1.0f / val;
+        Ainv_pack[V.a_off + e] = (PAIRED && !real) ? 0.0f : 1.0f / val;
     }
     if (blockIdx.x == 0) {
         for (int r = threadIdx.x; r < V.lda; r += blockDim.x) {
-            double bb = __builtin_huge_val();
-            if (r < m0) bb = b[r];
-            else if (r < m0 + kv) bb = b_extra[eoff + (r - m0)];
+            double bb = __builtin_huge_val(), bn = __builtin_huge_val();
+            if (r < sb) {
+                bb = b[r];
+                bn = pb ? b[r + m0 / 2] : __builtin_nan("");
+            } else if (r < sb + se) {
+                bb = b_extra[eoff + (r - sb)];
+                bn = pe ? b_extra[eoff + (r - sb) + kv / 2] : __builtin_nan("");
+            }
             b_pack[V.b_off + r] = bb;
+            if (PAIRED) bn_pack[V.b_off + r] = bn;
         }
+    }
+}
+
+// Row pairing of the blocks: flags[0] for the base block (m0 rows), flags[1 + v]
+// for variant v's extra block; the host sets them to 1 and this clears a flag
+// when its block has an odd row count or a row r + k/2 that is not the
+// negation of row r (compared as values, so +0 and -0 pair; NaN never does).
+__global__ void chr_pair_check_kernel(const float* A, int m0, int d, const float* A_extra,
+                                      const int64_t* extra_off, const int* extra_k, int* flags)
+{
+    const int blk = blockIdx.y;
+    const int k = blk == 0 ? m0 : extra_k[blk - 1];
+    if (k == 0) return;
+    const float* M = blk == 0 ? A : A_extra + extra_off[blk - 1] * d;
+    if (k % 2 != 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) flags[blk] = 0;
+        return;
+    }
+    const int h = k / 2;
+    const int64_t total = (int64_t)h * d;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % h);
+        const int c = (int)(e / h);
+        const float x = M[(int64_t)c * k + r], y = M[(int64_t)c * k + r + h];
+        if (!(y == -x)) flags[blk] = 0;
     }
 }
 
 // Rounding transform of the packed blocks: variant v samples z with
 // y = c_v + P_v z, so its rows become A' = A P_v (FP32, fmaf over k in
 // order) and b' = b - A c_v (FP64). Padding rows stay A' = 0, b' = +inf.
-// Also writes the IEEE reciprocal of A'.
+// Also writes the IEEE reciprocal of A'. PAIRED: bn' = bn - (-A) c_v by the
+// FMA chain over the negated row (what the unpaired layout computes for it),
+// and padding rows (r >= V.m) get a reciprocal of 0.
+template <bool PAIRED>
 __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const double* __restrict__ bsrc,
-                                     const ChrVariant* __restrict__ variants, int d, const float* __restrict__ P,
-                                     const float* __restrict__ c, float* __restrict__ Adst,
-                                     float* __restrict__ Ainv, double* __restrict__ bdst)
+                                     const double* __restrict__ bnsrc, const ChrVariant* __restrict__ variants, int d,
+                                     const float* __restrict__ P, const float* __restrict__ c,
+                                     float* __restrict__ Adst, float* __restrict__ Ainv, double* __restrict__ bdst,
+                                     double* __restrict__ bndst)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
@@ -124,35 +170,56 @@ __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const doubl
                     if (q < nj) acc[q] = fmaf(a, pk[q], acc[q]);
             }
         }
+        const bool pad = PAIRED && r >= V.m;
 #pragma unroll
         for (int q = 0; q < kJB; ++q)
             if (q < nj) {
                 const int64_t o = V.a_off + (int64_t)(j0 + q) * V.lda + r;
                 Adst[o] = acc[q];
-                Ainv[o] = 1.0f / acc[q];
+                Ainv[o] = pad ? 0.0f : 1.0f / acc[q];
             }
     }
     if (blockIdx.x == 0) {
         const float* cv = c + (int64_t)v * d;
         for (int r = threadIdx.x; r < V.lda; r += blockDim.x) {
-            double acc = 0.0;
-            for (int k = 0; k < d; ++k) acc = fma((double)A[(int64_t)k * V.lda + r], (double)cv[k], acc);
+            double acc = 0.0, accn = 0.0;
+            for (int k = 0; k < d; ++k) {
+                const double av = (double)A[(int64_t)k * V.lda + r];
+                acc = fma(av, (double)cv[k], acc);
+                if (PAIRED) accn = fma(-av, (double)cv[k], accn);
+            }
             bdst[V.b_off + r] = bsrc[V.b_off + r] - acc;
+            if (PAIRED) bndst[V.b_off + r] = bnsrc[V.b_off + r] - accn;
         }
     }
 }
 
 }  // namespace
 
-void chr_transform(const float* A_pack, const double* b_pack, const ChrVariant* variants_dev, int n_variants, int d,
-                   int lda, const float* P, const float* c, float* A_out, float* Ainv_out, double* b_out,
-                   hipStream_t stream)
+void chr_transform(const float* A_pack, const double* b_pack, const double* bn_pack, const ChrVariant* variants_dev,
+                   int n_variants, int d, int lda, const float* P, const float* c, float* A_out, float* Ainv_out,
+                   double* b_out, double* bn_out, hipStream_t stream)
 {
     const int64_t per = (int64_t)lda * ((d + 7) / 8);
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
-    hipLaunchKernelGGL(chr_transform_kernel, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack, variants_dev, d,
-                       P, c, A_out, Ainv_out, b_out);
+    if (bn_pack)
+        hipLaunchKernelGGL(chr_transform_kernel<true>, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack,
+                           bn_pack, variants_dev, d, P, c, A_out, Ainv_out, b_out, bn_out);
+    else
+        hipLaunchKernelGGL(chr_transform_kernel<false>, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack,
+                           nullptr, variants_dev, d, P, c, A_out, Ainv_out, b_out, nullptr);
+    NAJA_HIP(hipGetLastError());
+}
+
+void chr_pair_check(const float* A, int m0, int d, const float* A_extra, const int64_t* extra_off_dev,
+                    const int* extra_k_dev, int n_variants, int* flags_dev, hipStream_t stream)
+{
+    const int64_t per = (int64_t)(m0 / 2 + 1) * d;
+    int gx = (int)std::min<int64_t>((per + 255) / 256, 1024);
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(chr_pair_check_kernel, dim3(gx, n_variants + 1), dim3(256), 0, stream, A, m0, d, A_extra,
+                       extra_off_dev, extra_k_dev, flags_dev);
     NAJA_HIP(hipGetLastError());
 }
 
@@ -161,7 +228,7 @@ namespace {
 const ChrKernelTable& all_kernels()
 {
     static const ChrKernelTable tables[] = {chr_table_w1(), chr_table_w4a(), chr_table_w4b(), chr_table_w8(),
-                                            chr_table_w16(), chr_table_g4(), chr_table_big()};
+                                            chr_table_w16(), chr_table_g4(), chr_table_big(), chr_table_w1p()};
     static std::vector<ChrKernelEntry> flat;
     static ChrKernelTable all{nullptr, 0};
     if (!all.e) {
@@ -173,19 +240,38 @@ const ChrKernelTable& all_kernels()
 
 }  // namespace
 
-// Smallest compiled R with 64 W R >= rows; -1 if none.
-int chr_smallest_r(int W, int G, int rows)
+// The launch functions of shape (W, G, R), merged over every table that
+// compiles some of them.
+ChrKernelEntry chr_entry(int W, int G, int R)
+{
+    const ChrKernelTable& t = all_kernels();
+    ChrKernelEntry m{W, G, R, nullptr};
+    for (int q = 0; q < t.n; ++q) {
+        const ChrKernelEntry& e = t.e[q];
+        if (e.W != W || e.G != G || e.R != R) continue;
+        for (auto f : {&ChrKernelEntry::fn, &ChrKernelEntry::fn_coord, &ChrKernelEntry::fn_x2,
+                       &ChrKernelEntry::fn_coord_x2, &ChrKernelEntry::fn_p, &ChrKernelEntry::fn_coord_p,
+                       &ChrKernelEntry::fn_x2_p, &ChrKernelEntry::fn_coord_x2_p})
+            if (e.*f) m.*f = e.*f;
+    }
+    return m;
+}
+
+// Smallest compiled R with 64 W R >= rows (with the paired-row layout's
+// kernels when paired); -1 if none.
+int chr_smallest_r(int W, int G, int rows, bool paired)
 {
     const ChrKernelTable& t = all_kernels();
     int best = -1;
     for (int q = 0; q < t.n; ++q) {
         const ChrKernelEntry& e = t.e[q];
-        if (e.W == W && e.G == G && e.R * 64 * W >= rows && (best < 0 || e.R < best)) best = e.R;
+        const bool has = paired ? e.fn_p != nullptr : e.fn != nullptr;
+        if (has && e.W == W && e.G == G && e.R * 64 * W >= rows && (best < 0 || e.R < best)) best = e.R;
     }
     return best;
 }
 
-ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items)
+ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items, bool paired)
 {
     // A step costs (rows per lane x ~15 VALU + ~100 VALU of per-wave chord,
     // alpha and bookkeeping work) x wavefronts per SIMD. One wavefront needs
@@ -197,23 +283,24 @@ ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items)
     // A launch of few chain groups leaves most SIMDs idle: eight wavefronts
     // per group then shorten each step without costing parallelism (the
     // 4-chain genome-scale run: 1.21 -> 1.10 us per step).
-    if (W == 4 && n_items > 0 && n_items * 8 <= 1024 && chr_smallest_r(8, G, rows) > 0) W = 8;
+    if (W == 4 && n_items > 0 && n_items * 8 <= 1024 && chr_smallest_r(8, G, rows, paired) > 0) W = 8;
     if (tpb > 0) {
         // The reference's --tpb (threads per chain) is a preference here.
         const int w = std::max(1, tpb / 64);
         const int Wp = w >= 16 ? 16 : (w >= 8 ? 8 : (w >= 4 ? 4 : 1));
-        if (chr_smallest_r(Wp, G, rows) > 0) W = Wp;
+        if (chr_smallest_r(Wp, G, rows, paired) > 0) W = Wp;
     }
-    int R = chr_smallest_r(W, G, rows);
+    int R = chr_smallest_r(W, G, rows, paired);
     if (R < 0) {
         for (int w : {1, 4, 8, 16}) {
-            R = chr_smallest_r(w, G, rows);
+            R = chr_smallest_r(w, G, rows, paired);
             if (R > 0) {
                 W = w;
                 break;
             }
         }
     }
+    if (R < 0 && paired) return ChrShape{0, G, 0};
     NAJA_REQUIRE(R > 0, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel holds %d rows with %d chains per group", m_max, G);
     return ChrShape{W, G, R};
 }
@@ -263,18 +350,22 @@ int chr_timing_collect(int* items, float* ms, int max)
 
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream)
 {
-    const ChrKernelTable& t = all_kernels();
-    ChrKernelFn fn = nullptr;
+    const ChrKernelEntry e = chr_entry(sh.W, sh.G, sh.R);
     const bool coord_only = args.thr_k == 0 && !args.has_pair;
     const bool reg_state = args.d <= kChrRegStateMaxD;
-    for (int q = 0; q < t.n; ++q)
-        if (t.e[q].W == sh.W && t.e[q].G == sh.G && t.e[q].R == sh.R) {
-            const ChrKernelEntry& e = t.e[q];
-            fn = coord_only && e.fn_coord ? e.fn_coord : e.fn;
-            ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
-            if (reg_state && fx) fn = fx;
-        }
-    NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d", sh.W, sh.G, sh.R);
+    const bool paired = args.bn != nullptr;
+    ChrKernelFn fn = nullptr;
+    if (paired) {
+        fn = coord_only && e.fn_coord_p ? e.fn_coord_p : e.fn_p;
+        const ChrKernelFn fx = coord_only && e.fn_coord_x2_p ? e.fn_coord_x2_p : e.fn_x2_p;
+        if (reg_state && fx) fn = fx;
+    } else {
+        fn = coord_only && e.fn_coord ? e.fn_coord : e.fn;
+        const ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
+        if (reg_state && fx) fn = fx;
+    }
+    NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d%s", sh.W, sh.G, sh.R,
+                 paired ? " (paired rows)" : "");
     NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),
                  NAJA_ERR_UNSUPPORTED, "%lld steps in one launch (at most 2^31 - 1; the streamed API splits longer runs)",
                  (long long)args.nsteps);
@@ -307,13 +398,20 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
-              float* A_pack, float* Ainv_pack, double* b_pack, int lda_max, hipStream_t stream)
+              const int* paired_dev, float* A_pack, float* Ainv_pack, double* b_pack, double* bn_pack, int lda_max,
+              hipStream_t stream)
 {
     const int64_t per = (int64_t)lda_max * d;
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
-    hipLaunchKernelGGL(chr_pack_kernel, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra, b_extra,
-                       variants_dev, extra_off_dev, extra_k_dev, A_pack, Ainv_pack, b_pack);
+    if (bn_pack)
+        hipLaunchKernelGGL(chr_pack_kernel<true>, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra,
+                           b_extra, variants_dev, extra_off_dev, extra_k_dev, paired_dev, A_pack, Ainv_pack, b_pack,
+                           bn_pack);
+    else
+        hipLaunchKernelGGL(chr_pack_kernel<false>, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra,
+                           b_extra, variants_dev, extra_off_dev, extra_k_dev, nullptr, A_pack, Ainv_pack, b_pack,
+                           nullptr);
     NAJA_HIP(hipGetLastError());
 }
 

@@ -274,7 +274,15 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // chain g in lane c % 64 of xv[g][c / 64] (d <= 64 XR), and every lane
 // applies the update of its own coordinates, so a step reads and writes no
 // LDS and needs no exec-masked store; XR = 0 keeps the states in LDS.
-template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0>
+// PAIRED: packed row r stands for two constraints, a_r y <= b_r and
+// -a_r y <= bn_r (a row and its negation), with two slacks per slot: the
+// column loads and the cache footprint are those of one row. Every value is
+// the one the unpaired layout computes for the negated row: its chord ratio
+// s / (-ae) = -(s / ae) and RN(1 / -a) = -RN(1 / a) (rounding is symmetric),
+// its update fmaf(-alpha, -ae, s) = fmaf(alpha, ae, s), its FP64 resync the
+// FMA chain over -a. A row without a partner has bn = NaN: its second slack
+// stays NaN and a NaN ratio never wins either minimum.
+template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false>
 __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
 {
     static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
@@ -323,8 +331,10 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         }
     }
 
-    // Slot k of lane tid holds row row_of(k).
+    // Slot k of lane tid holds row row_of(k); sn its partner's slack (PAIRED).
+    constexpr int RP = PAIRED ? R : 1;
     float s[G][R];
+    float sn[G][RP];
     float cs[PF + 1][2][R];  // rotating column sets: step t uses set (t - t_begin) % (PF + 1)
     // Steps t0 .. t0 + nsteps - 1 (t0 > 0 continues an earlier launch).
     const int64_t t_begin = a.t0;
@@ -485,17 +495,25 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         if (event && u == next_resync) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
             double acc[G][R];
+            double accn[G][RP];
 #pragma unroll
             for (int k = 0; k < R; ++k)
 #pragma unroll
                 for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
+#pragma unroll
+            for (int k = 0; k < RP; ++k)
+#pragma unroll
+                for (int g = 0; g < G; ++g) accn[g][k] = 0.0;
             const float* pc = Av;
             auto add_col = [&](const double (&xc)[G]) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     const double av = (double)pc[row_of(k)];
 #pragma unroll
-                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xc[g], acc[g][k]);
+                    for (int g = 0; g < G; ++g) {
+                        acc[g][k] = fma(av, xc[g], acc[g][k]);
+                        if constexpr (PAIRED) accn[g][k] = fma(-av, xc[g], accn[g][k]);
+                    }
                 }
                 pc += lda;
             };
@@ -523,6 +541,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                 const double bb = bv[row_of(k)];
 #pragma unroll
                 for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
+                if constexpr (PAIRED) {
+                    const double bn = a.bn[V.b_off + row_of(k)];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - accn[g][k]);
+                }
             }
             prev_i = -1;
             prev_j = -1;
@@ -633,6 +656,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                     const uint32_t qb = chord_bits_rcp(cb[k], s[g][k]);
                     hb = umin32(hb, qb);
                     lb = smin32(lb, (int32_t)qb);
+                    if constexpr (PAIRED) {
+                        const uint32_t qn = chord_bits_rcp(-cb[k], sn[g][k]);
+                        hb = umin32(hb, qn);
+                        lb = smin32(lb, (int32_t)qn);
+                    }
                 }
             } else {
 #pragma unroll
@@ -640,6 +668,11 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
                     const uint32_t qb = chord_bits(ae[k], s[g][k]);
                     hb = umin32(hb, qb);
                     lb = smin32(lb, (int32_t)qb);
+                    if constexpr (PAIRED) {
+                        const uint32_t qn = chord_bits(-ae[k], sn[g][k]);
+                        hb = umin32(hb, qn);
+                        lb = smin32(lb, (int32_t)qn);
+                    }
                 }
             }
             wave_min2(hb, lb);
@@ -668,12 +701,17 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
             alpha[g] = fmaf(U[g], hi - lo, lo);
         }
 
-        // Fused rank-1 slack update s -= alpha * ae (chr.cu:464-482).
+        // Fused rank-1 slack update s -= alpha * ae (chr.cu:464-482); the
+        // partner row's is sn -= alpha * (-ae).
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const float nal = -alpha[g];
 #pragma unroll
             for (int k = 0; k < R; ++k) s[g][k] = fmaf(nal, ae[k], s[g][k]);
+            if constexpr (PAIRED) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) sn[g][k] = fmaf(alpha[g], ae[k], sn[g][k]);
+            }
         }
 
         if constexpr (XR > 0) {

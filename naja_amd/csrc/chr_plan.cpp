@@ -25,6 +25,12 @@
 
 struct naja_chr_plan {
     int m0 = 0, d = 0, n_variants = 0, G = 1, m_max = 0;
+    // Paired-row layout: every block whose second half negates its first is
+    // packed as its first half with a second right-hand side (bn); rows_max
+    // is the largest variant's packed row count in the layout in use.
+    bool paired = false;
+    int rows_max = 0;
+    std::vector<int> pair_flags;  // [base block, extra block of each variant]
     naja_chr_params p{};
     hipStream_t stream = nullptr;  // creation stream: allocations and frees are ordered on it
     const float* A = nullptr;
@@ -47,9 +53,12 @@ struct naja_chr_plan {
     naja::ChrItem* items_d = nullptr;
     int64_t* eoff_d = nullptr;
     int* ek_d = nullptr;
+    int* pair_d = nullptr;
     float* A_pack = nullptr;
     float* Ainv_pack = nullptr;
     double* b_pack = nullptr;
+    double* bn_pack = nullptr;  // paired layout only
+    double* bn_raw = nullptr;   // paired transform plans only
     float* A_raw = nullptr;   // untransformed pack (transform plans only)
     double* b_raw = nullptr;
     float* zbuf = nullptr;    // z-space samples (transform plans only)
@@ -78,13 +87,22 @@ namespace naja {
 
 namespace {
 
-using TuneKey = std::tuple<int, int, int, int, int>;  // rows64, d, G, n_items, tpb
+using TuneKey = std::tuple<int, int, int, int, int, int>;  // rows64, d, G, n_items, tpb, paired
 std::mutex g_tune_mu;
 std::map<TuneKey, ChrShape> g_tuned;
 
 TuneKey tune_key(const naja_chr_plan& pl)
 {
-    return TuneKey{(pl.m_max + 63) / 64 * 64, pl.d, pl.G, (int)pl.items.size(), pl.p.tpb > 0 ? pl.p.tpb : 0};
+    return TuneKey{(pl.rows_max + 63) / 64 * 64, pl.d, pl.G, (int)pl.items.size(), pl.p.tpb > 0 ? pl.p.tpb : 0,
+                   pl.paired ? 1 : 0};
+}
+
+// Packed rows of variant v in the layout in use.
+int packed_rows(const naja_chr_plan& pl, int v)
+{
+    const int k = pl.ek[(size_t)v];
+    if (!pl.paired) return pl.m0 + k;
+    return (pl.pair_flags[0] ? pl.m0 / 2 : pl.m0) + (pl.pair_flags[(size_t)v + 1] ? k / 2 : k);
 }
 
 void free_async(void* ptr, hipStream_t s) { scratch_free(ptr, s); }
@@ -167,7 +185,7 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
     const int64_t per_variant_samples = (int64_t)pl.p.nchains * pl.p.nspc * pl.d;
     for (int v = 0; v < pl.n_variants; ++v) {
         ChrVariant& V = pl.var[(size_t)v];
-        V.m = pl.m0 + pl.ek[(size_t)v];
+        V.m = packed_rows(pl, v);
         V.lda = lda;
         V.a_off = a_off;
         V.b_off = b_off;
@@ -180,26 +198,30 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         free_async(pl.pack_mem, s);
         pl.pack_mem = nullptr;
         const size_t af = sizeof(float) * (size_t)a_off, bd = sizeof(double) * (size_t)b_off;
-        size_t off[5];
-        const size_t total = carve({af, af, bd, pl.P ? af : 0, pl.P ? bd : 0}, off);
+        const size_t bnd = pl.paired ? bd : 0;
+        size_t off[7];
+        const size_t total = carve({af, af, bd, pl.P ? af : 0, pl.P ? bd : 0, bnd, pl.P ? bnd : 0}, off);
         alloc_async(&pl.pack_mem, total, s);
         pl.A_pack = reinterpret_cast<float*>(pl.pack_mem + off[0]);
         pl.Ainv_pack = reinterpret_cast<float*>(pl.pack_mem + off[1]);
         pl.b_pack = reinterpret_cast<double*>(pl.pack_mem + off[2]);
         pl.A_raw = pl.P ? reinterpret_cast<float*>(pl.pack_mem + off[3]) : nullptr;
         pl.b_raw = pl.P ? reinterpret_cast<double*>(pl.pack_mem + off[4]) : nullptr;
+        pl.bn_pack = pl.paired ? reinterpret_cast<double*>(pl.pack_mem + off[5]) : nullptr;
+        pl.bn_raw = pl.paired && pl.P ? reinterpret_cast<double*>(pl.pack_mem + off[6]) : nullptr;
         pl.pack_floats = a_off;
         pl.pack_rows = b_off;
     }
     NAJA_HIP(hipMemcpyAsync(pl.var_d, pl.var.data(), sizeof(ChrVariant) * pl.var.size(), hipMemcpyHostToDevice, s));
+    const int* pairs = pl.paired ? pl.pair_d : nullptr;
     if (pl.P) {
-        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants,
-                 pl.A_raw, pl.Ainv_pack, pl.b_raw, lda, s);
-        chr_transform(pl.A_raw, pl.b_raw, pl.var_d, pl.n_variants, pl.d, lda, pl.P, pl.c, pl.A_pack, pl.Ainv_pack,
-                      pl.b_pack, s);
+        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants, pairs,
+                 pl.A_raw, pl.Ainv_pack, pl.b_raw, pl.bn_raw, lda, s);
+        chr_transform(pl.A_raw, pl.b_raw, pl.bn_raw, pl.var_d, pl.n_variants, pl.d, lda, pl.P, pl.c, pl.A_pack,
+                      pl.Ainv_pack, pl.b_pack, pl.bn_pack, s);
     } else {
-        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants,
-                 pl.A_pack, pl.Ainv_pack, pl.b_pack, lda, s);
+        chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants, pairs,
+                 pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, lda, s);
     }
     pl.shape = sh;
     NAJA_HIP(hipEventRecord(pl.packed, s));
@@ -247,6 +269,7 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
     a.A = pl.A_pack;
     a.Ainv = pl.Ainv_pack;
     a.b = pl.b_pack;
+    a.bn = pl.paired ? pl.bn_pack : nullptr;
     a.d = pl.d;
     a.variants = pl.var_d;
     a.items = pl.items_d;
@@ -360,15 +383,17 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
         const hipStream_t s = pl->stream;
         NAJA_HIP(hipEventCreateWithFlags(&pl->packed, hipEventDisableTiming));
         NAJA_HIP(hipEventCreateWithFlags(&pl->launched, hipEventDisableTiming));
-        size_t off[4];
+        size_t off[5];
         const size_t total = carve({sizeof(ChrVariant) * pl->var.size(), sizeof(ChrItem) * pl->items.size(),
-                                    sizeof(int64_t) * pl->eoff.size(), sizeof(int) * pl->ek.size()},
+                                    sizeof(int64_t) * pl->eoff.size(), sizeof(int) * pl->ek.size(),
+                                    sizeof(int) * ((size_t)n_variants + 1)},
                                    off);
         alloc_async(&pl->tables_mem, total, s);
         pl->var_d = reinterpret_cast<ChrVariant*>(pl->tables_mem + off[0]);
         pl->items_d = reinterpret_cast<ChrItem*>(pl->tables_mem + off[1]);
         pl->eoff_d = reinterpret_cast<int64_t*>(pl->tables_mem + off[2]);
         pl->ek_d = reinterpret_cast<int*>(pl->tables_mem + off[3]);
+        pl->pair_d = reinterpret_cast<int*>(pl->tables_mem + off[4]);
         NAJA_HIP(hipMemcpyAsync(pl->items_d, pl->items.data(), sizeof(ChrItem) * pl->items.size(),
                                 hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(pl->eoff_d, pl->eoff.data(), sizeof(int64_t) * pl->eoff.size(),
@@ -382,7 +407,31 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
             pl->inplace = inplace_out;
         else if (P)
             alloc_async(&pl->zbuf, (size_t)n_variants * per * d, s);
-        ChrShape sh = chr_choose_shape(pl->m_max, pl->G, p->tpb, (int)pl->items.size());
+        // Row pairing: one check kernel and one read-back. The paired layout
+        // is used when it packs at least a quarter fewer rows and a paired
+        // kernel holds the largest variant.
+        pl->pair_flags.assign((size_t)n_variants + 1, 1);
+        NAJA_HIP(hipMemcpyAsync(pl->pair_d, pl->pair_flags.data(), sizeof(int) * pl->pair_flags.size(),
+                                hipMemcpyHostToDevice, s));
+        chr_pair_check(A, m0, d, A_extra, pl->eoff_d, pl->ek_d, n_variants, pl->pair_d, s);
+        NAJA_HIP(hipMemcpyAsync(pl->pair_flags.data(), pl->pair_d, sizeof(int) * pl->pair_flags.size(),
+                                hipMemcpyDeviceToHost, s));
+        NAJA_HIP(hipStreamSynchronize(s));
+        int64_t rows_all = 0, rows_paired = 0;
+        int paired_max = 0;
+        pl->paired = true;
+        for (int v = 0; v < n_variants; ++v) {
+            const int r = packed_rows(*pl, v);
+            rows_paired += r;
+            rows_all += m0 + pl->ek[(size_t)v];
+            paired_max = std::max(paired_max, r);
+        }
+        ChrShape sh{0, pl->G, 0};
+        if (rows_paired * 4 <= rows_all * 3)
+            sh = chr_choose_shape(paired_max, pl->G, p->tpb, (int)pl->items.size(), true);
+        pl->paired = sh.W > 0;
+        pl->rows_max = pl->paired ? paired_max : pl->m_max;
+        if (!pl->paired) sh = chr_choose_shape(pl->m_max, pl->G, p->tpb, (int)pl->items.size());
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tuned.find(tune_key(*pl));
@@ -431,13 +480,13 @@ int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, i
         NAJA_REQUIRE(plan && y0, NAJA_ERR_INVALID_ARGUMENT, "NULL plan or y0");
         naja_chr_plan& pl = *plan;
         const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-        const int rows = (pl.m_max + 63) / 64 * 64;
+        const int rows = (pl.rows_max + 63) / 64 * 64;
         std::vector<ChrShape> cand;
         for (int W : {1, 4, 8, 16}) {
-            const int R = chr_smallest_r(W, pl.G, rows);
+            const int R = chr_smallest_r(W, pl.G, rows, pl.paired);
             if (R > 0) cand.push_back(ChrShape{W, pl.G, R});
         }
-        NAJA_REQUIRE(!cand.empty(), NAJA_ERR_UNSUPPORTED, "no compiled shape for %d rows", pl.m_max);
+        NAJA_REQUIRE(!cand.empty(), NAJA_ERR_UNSUPPORTED, "no compiled shape for %d rows", pl.rows_max);
         // Probe: about 1024 steps of the real run (at least one saved sample).
         const int nspc_probe = std::max(1, std::min(pl.p.nspc, 1024 / pl.p.thinning));
         const int64_t nsteps = (int64_t)nspc_probe * pl.p.thinning;

@@ -114,6 +114,80 @@ def test_one_wavefront_state_paths_bit_exact(gpu, d, m, moves):
     np.testing.assert_array_equal(fin, ref["y_final"])
 
 
+def halves_polytope(d, n, seed):
+    """{-b_lo <= T y <= b_hi}, T (n x d) N(0,1): A = [T; -T], every row with
+    its negation (the layout of flux bounds and slab conditions)."""
+    rng = np.random.default_rng(seed)
+    T = rng.standard_normal((n, d))
+    A = np.vstack([T, -T])
+    b = np.concatenate([10.0 ** rng.uniform(-1, 0.5, n), 10.0 ** rng.uniform(-1, 0.5, n)])
+    return A, b
+
+
+@pytest.mark.parametrize("d,n", [(40, 150), (100, 100), (100, 250), (150, 200), (12, 300)])
+@pytest.mark.parametrize("moves", ["coord", "pair-ksparse"])
+def test_paired_rows_bit_exact(gpu, d, n, moves):
+    """[T; -T] polytopes run in the paired-row layout (one packed row per
+    pair, two slacks: the shape holds n rows, not 2n); samples, final states
+    and moves equal the oracle's row-by-row run bit for bit, with the states
+    in VGPRs (d <= 128) and in LDS."""
+    import naja_amd
+
+    t = gpu
+    A, b = halves_polytope(d, n, seed=d * 7 + n)
+    kw = {} if moves == "coord" else dict(pair_prob=0.3, ksparse_prob=0.05, ksparse_k=5)
+    nch, nspc, thin = 3, 600, 3
+    Y0 = np.zeros((nch, d))
+    samples, fin, mlog = run_gpu(t, A, b, Y0, nspc, thin, move_log_steps=nspc * thin, resync_interval=400, **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), nspc, thin, nlog=nspc * thin,
+                             resync_interval=400, **kw)
+    np.testing.assert_array_equal(mlog, ref["move_log"])
+    np.testing.assert_array_equal(samples, ref["samples"])
+    np.testing.assert_array_equal(fin, ref["y_final"])
+    plan = naja_amd.ChrPlan(t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda(), nspc, thin, nch,
+                            **{k: v for k, v in kw.items() if k == "pair_prob"})
+    assert plan.shape() == (1, 1, (n + 63) // 64)
+    plan.close()
+
+
+def test_paired_batched_mixed_blocks_bit_exact(gpu):
+    """A paired base ([I; -I]) with extra blocks of every kind: empty, [a; -a]
+    (paired), an odd row count and an even one that is not a negation (rows
+    packed alone, partner slack NaN). Every variant equals the oracle bit for
+    bit."""
+    import naja_amd
+
+    t = gpu
+    d = 16
+    A, b, _ = cube(d)
+    rng = np.random.default_rng(17)
+
+    def slabs(k):
+        a = rng.standard_normal((k, d))
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        return np.vstack([a, -a]), np.concatenate([np.full(k, 0.3), np.full(k, 0.2)])
+
+    blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(10), (rng.standard_normal((7, d)), np.full(7, 0.8)),
+              (rng.standard_normal((10, d)), np.full(10, 0.9)), slabs(20)]
+    EA = [e for e, _ in blocks]
+    Eb = [f for _, f in blocks]
+    nch, nspc, thin = 3, 500, 2
+    X0 = np.zeros((len(blocks), nch, d), np.float32)
+    # An odd k-sparse count keeps every k-sparse direction nonzero (with an
+    # even count the signed entries can cancel pairwise on the cube).
+    kw = dict(pair_prob=0.3, ksparse_prob=0.05, ksparse_k=5, resync_interval=300)
+    out = naja_amd.coordinate_hit_and_run_batched(
+        t.from_numpy(A.astype(np.float32)), t.from_numpy(b), [t.from_numpy(e.astype(np.float32)) for e in EA],
+        [t.from_numpy(f) for f in Eb], t.from_numpy(X0), nspc, thin, nch, seed=77, **kw)
+    t.cuda.synchronize()
+    out = out.cpu().numpy()
+    for v in range(len(blocks)):
+        Av = np.vstack([A, EA[v]]).astype(np.float32)
+        bv = np.concatenate([b, Eb[v]])
+        ref = oracle.chr_run_f32(Av, bv, X0[v], nspc, thin, seed=77, variant=v, **kw)
+        np.testing.assert_array_equal(out[v], ref["samples"])
+
+
 def test_chains_per_group_matches_oracle(gpu):
     A, b, y0 = random_polytope(12, 70, seed=31)
     nch = 6  # one full group of 4 and a partial group of 2

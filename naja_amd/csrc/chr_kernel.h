@@ -98,7 +98,14 @@ struct ChrKernelEntry {
     ChrKernelFn fn_coord_p = nullptr;
     ChrKernelFn fn_x2_p = nullptr;
     ChrKernelFn fn_coord_x2_p = nullptr;
+    // Coordinate-only, VGPR states, four chains of one variant per workgroup
+    // with the packed A in LDS (unpaired and paired layouts); null when not
+    // compiled.
+    ChrKernelFn fn_coord_x2_l4 = nullptr;
+    ChrKernelFn fn_coord_x2_p_l4 = nullptr;
 };
+// Chains per workgroup of the LDS-column kernels.
+constexpr int kChrLdsChains = 4;
 // Largest d of the register-resident-state kernels (two VGPRs per lane).
 constexpr int kChrRegStateMaxD = 128;
 struct ChrKernelTable {
@@ -114,6 +121,7 @@ ChrKernelTable chr_table_w16();
 ChrKernelTable chr_table_g4();
 ChrKernelTable chr_table_big();  // W = 16, R = 10..22: the large-polytope kernel
 ChrKernelTable chr_table_w1p();  // one wavefront, paired rows
+ChrKernelTable chr_table_w1l();  // one wavefront, four chains per workgroup, A columns in LDS
 
 // Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
 // per group (among the paired-row layout's kernels when paired); -1 if none.
@@ -123,7 +131,16 @@ int chr_smallest_r(int W, int G, int rows, bool paired = false);
 // the reference's --tpb, taken as a preference). Paired: among the
 // paired-row kernels, and W = 0 when none holds the rows.
 ChrShape chr_choose_shape(int m_max, int G, int tpb, int n_items, bool paired = false);
-void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream);
+// nc = kChrLdsChains launches the LDS-column kernel (items in groups of nc
+// chains of one variant, lds_bytes of dynamic LDS); nc = 1 the others.
+void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream, int nc = 1,
+                size_t lds_bytes = 0);
+// Whether (and with how much LDS) a launch of shape sh can use the
+// LDS-column kernel: one wavefront, one chain per group, coordinate moves
+// only, d <= kChrRegStateMaxD, nchains a multiple of kChrLdsChains, and the
+// largest variant's columns (max_rows packed rows) within the LDS.
+size_t chr_lds_columns_bytes(const ChrShape& sh, int d, int max_rows);
+bool chr_has_lds_kernel(const ChrShape& sh, bool paired);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to
 // max recorded launches, oldest first, and drops the record.

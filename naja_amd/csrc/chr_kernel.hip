@@ -228,7 +228,8 @@ namespace {
 const ChrKernelTable& all_kernels()
 {
     static const ChrKernelTable tables[] = {chr_table_w1(), chr_table_w4a(), chr_table_w4b(), chr_table_w8(),
-                                            chr_table_w16(), chr_table_g4(), chr_table_big(), chr_table_w1p()};
+                                            chr_table_w16(), chr_table_g4(), chr_table_big(), chr_table_w1p(),
+                                            chr_table_w1l()};
     static std::vector<ChrKernelEntry> flat;
     static ChrKernelTable all{nullptr, 0};
     if (!all.e) {
@@ -251,7 +252,8 @@ ChrKernelEntry chr_entry(int W, int G, int R)
         if (e.W != W || e.G != G || e.R != R) continue;
         for (auto f : {&ChrKernelEntry::fn, &ChrKernelEntry::fn_coord, &ChrKernelEntry::fn_x2,
                        &ChrKernelEntry::fn_coord_x2, &ChrKernelEntry::fn_p, &ChrKernelEntry::fn_coord_p,
-                       &ChrKernelEntry::fn_x2_p, &ChrKernelEntry::fn_coord_x2_p})
+                       &ChrKernelEntry::fn_x2_p, &ChrKernelEntry::fn_coord_x2_p, &ChrKernelEntry::fn_coord_x2_l4,
+                       &ChrKernelEntry::fn_coord_x2_p_l4})
             if (e.*f) m.*f = e.*f;
     }
     return m;
@@ -348,7 +350,19 @@ int chr_timing_collect(int* items, float* ms, int max)
     return n;
 }
 
-void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream)
+size_t chr_lds_columns_bytes(const ChrShape& sh, int d, int max_rows)
+{
+    const int vec = (sh.R % 4 == 0) ? 4 : ((sh.R % 2 == 0) ? 2 : 1);
+    return sizeof(float) * (size_t)d * (size_t)((max_rows + vec - 1) / vec * vec + vec);
+}
+
+bool chr_has_lds_kernel(const ChrShape& sh, bool paired)
+{
+    const ChrKernelEntry e = chr_entry(sh.W, sh.G, sh.R);
+    return (paired ? e.fn_coord_x2_p_l4 : e.fn_coord_x2_l4) != nullptr;
+}
+
+void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream, int nc, size_t lds_bytes)
 {
     const ChrKernelEntry e = chr_entry(sh.W, sh.G, sh.R);
     const bool coord_only = args.thr_k == 0 && !args.has_pair;
@@ -364,8 +378,14 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         const ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
         if (reg_state && fx) fn = fx;
     }
-    NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d%s", sh.W, sh.G, sh.R,
-                 paired ? " (paired rows)" : "");
+    if (nc > 1) {
+        NAJA_REQUIRE(nc == kChrLdsChains && coord_only && reg_state && n_items % nc == 0, NAJA_ERR_INVALID_ARGUMENT,
+                     "an LDS-column launch needs coordinate moves, d <= %d and items in groups of %d",
+                     kChrRegStateMaxD, kChrLdsChains);
+        fn = paired ? e.fn_coord_x2_p_l4 : e.fn_coord_x2_l4;
+    }
+    NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d%s%s", sh.W, sh.G, sh.R,
+                 paired ? " (paired rows)" : "", nc > 1 ? " (LDS columns)" : "");
     NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),
                  NAJA_ERR_UNSUPPORTED, "%lld steps in one launch (at most 2^31 - 1; the streamed API splits longer runs)",
                  (long long)args.nsteps);
@@ -373,9 +393,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
     // offsets and mark padding runs with bit 31.
     NAJA_REQUIRE(sh.big() || 4.0 * sh.rows() * (double)args.d < 2147483648.0, NAJA_ERR_UNSUPPORTED,
                  "a packed block of %d rows x %d columns exceeds 2 GiB", sh.rows(), args.d);
-    const size_t lds = sh.lds_bytes(args.d);
-    NAJA_REQUIRE(lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED, "d=%d with %d chains per group exceeds the LDS state budget",
-                 args.d, sh.G);
+    const size_t lds = nc > 1 ? lds_bytes : sh.lds_bytes(args.d);
+    NAJA_REQUIRE(nc > 1 || lds <= 96u * 1024u, NAJA_ERR_UNSUPPORTED,
+                 "d=%d with %d chains per group exceeds the LDS state budget", args.d, sh.G);
+    NAJA_REQUIRE(lds <= 156u * 1024u, NAJA_ERR_UNSUPPORTED, "%zu bytes of LDS columns exceed the budget", lds);
     bool timed;
     {
         std::lock_guard<std::mutex> g(g_timing_mu);
@@ -387,7 +408,7 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         NAJA_HIP(hipEventCreate(&rec.stop));
         NAJA_HIP(hipEventRecord(rec.start, stream));
     }
-    hipLaunchKernelGGL(fn, dim3(n_items), dim3(64 * sh.W), lds, stream, args);
+    hipLaunchKernelGGL(fn, dim3(n_items / nc), dim3(64 * sh.W * nc), lds, stream, args);
     NAJA_HIP(hipGetLastError());
     if (timed) {
         NAJA_HIP(hipEventRecord(rec.stop, stream));

@@ -282,22 +282,30 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // its update fmaf(-alpha, -ae, s) = fmaf(alpha, ae, s), its FP64 resync the
 // FMA chain over -a. A row without a partner has bn = NaN: its second slack
 // stays NaN and a NaN ratio never wins either minimum.
-template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false>
-__global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
+// NC > 1 (one-wavefront coordinate-only chains with VGPR states): a workgroup
+// runs NC chains of one variant, one per wavefront and otherwise independent,
+// with the variant's packed A copied into LDS once, so each step reads its
+// column A[:, i] from LDS and only 1/A[:, i] through the caches.
+template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false, int NC = 1>
+__global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
 {
     static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
     static_assert(XR == 0 || W == 1, "register-resident states need one wavefront");
+    static_assert(NC == 1 || (W == 1 && G == 1 && CO && XR > 0), "LDS-resident columns: one-wavefront coordinate chains");
     constexpr int T = 64 * W;
-    extern __shared__ float xs[];  // G x d current states (XR = 0)
+    extern __shared__ __align__(16) float xs[];  // G x d current states (XR = 0); A columns (NC > 1)
     __shared__ uint2 part[2][G][W];
     __shared__ int ks_idx[kKmax];
     __shared__ float ks_coef[kKmax];
 
-    const int tid = threadIdx.x;
+    // NC > 1: wavefront w of the workgroup runs item NC * blockIdx.x + w, and
+    // tid is the thread's index within its chain.
+    const int chain_slot = NC > 1 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6) : 0;
+    const int tid = NC > 1 ? (int)threadIdx.x & 63 : (int)threadIdx.x;
     const int lane = tid & 63;
     // readfirstlane makes the wave index provably wave-uniform.
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const ChrItem item = a.items[blockIdx.x];
+    const ChrItem item = a.items[(int64_t)blockIdx.x * NC + chain_slot];
     const ChrVariant V = a.variants[item.variant];
     const int d = a.d;
     const int lda = V.lda;  // == T * R: every slot of every lane is a real or a padding row
@@ -313,6 +321,18 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const uint32_t group = (uint32_t)(item.c0 / G);
     const uint32_t k0 = a.seed_lo;
     const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
+    // NC > 1: the workgroup's chains belong to one variant (items come in
+    // groups of NC); its packed A is copied into LDS once, column c at
+    // xs[c * lstride]: the rows below mpad (m rounded up to VEC), then VEC
+    // zeros that the lanes past the real rows read.
+    const int lstride = (V.m + VEC - 1) / VEC * VEC + VEC;
+    if constexpr (NC > 1) {
+        for (int e = (int)threadIdx.x; e < lstride * d; e += 64 * NC) {
+            const int c = e / lstride, r = e % lstride;
+            xs[e] = r < V.m ? Av[(int64_t)c * lda + r] : 0.0f;
+        }
+        __syncthreads();
+    }
 
     float xv[G][XS];
     if constexpr (XR > 0) {
@@ -357,10 +377,12 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
     const int col_stride = 4 * lda;  // bytes from one column to the next
     const int mpad = (V.m + VEC - 1) / VEC * VEC;
     int voff[R / VEC];
+    int loff[R / VEC];  // LDS column offsets (NC > 1): the run, or the zero run
 #pragma unroll
     for (int q = 0; q < R / VEC; ++q) {
         const int r0 = q * VEC * T + tid * VEC;
         voff[q] = r0 < mpad ? 4 * r0 : (int)0x80000000u;
+        loff[q] = r0 < mpad ? r0 : mpad;
     }
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
         const int mt = CO ? kMoveCoord : m.type;
@@ -369,6 +391,33 @@ __global__ __launch_bounds__(64 * W) void chr_step_kernel(ChrArgs a)
         const int si = real ? m.i * col_stride : 0;
         const int sj = pair ? m.j * col_stride : si;
         const chr_i32x4 rj = pair ? rsA : rsI;
+        if constexpr (NC > 1) {
+            const float* colp = xs + (real ? m.i : 0) * lstride;
+#pragma unroll
+            for (int q = 0; q < R / VEC; ++q) {
+                if (VEC == 4) {
+                    const float4 u = *reinterpret_cast<const float4*>(colp + loff[q]);
+                    const chr_f32x4 w = chr_raw_load_v4f32(rsI, voff[q], si, 0);
+                    xa[q * 4] = u.x;
+                    xa[q * 4 + 1] = u.y;
+                    xa[q * 4 + 2] = u.z;
+                    xa[q * 4 + 3] = u.w;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) xb[q * 4 + e] = w[e];
+                } else if (VEC == 2) {
+                    const float2 u = *reinterpret_cast<const float2*>(colp + loff[q]);
+                    const chr_f32x2 w = chr_raw_load_v2f32(rsI, voff[q], si, 0);
+                    xa[q * 2] = u.x;
+                    xa[q * 2 + 1] = u.y;
+                    xb[q * 2] = w[0];
+                    xb[q * 2 + 1] = w[1];
+                } else {
+                    xa[q] = colp[loff[q]];
+                    xb[q] = chr_raw_load_f32(rsI, voff[q], si, 0);
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < R / VEC; ++q) {
             if (VEC == 4) {

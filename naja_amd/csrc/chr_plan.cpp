@@ -30,6 +30,11 @@ struct naja_chr_plan {
     // is the largest variant's packed row count in the layout in use.
     bool paired = false;
     int rows_max = 0;
+    // Chains per workgroup of the current shape's launches (kChrLdsChains for
+    // the LDS-column kernel, whose items come in groups of one variant) and
+    // the dynamic LDS that kernel needs.
+    int nc = 1;
+    size_t lds_cols = 0;
     std::vector<int> pair_flags;  // [base block, extra block of each variant]
     naja_chr_params p{};
     hipStream_t stream = nullptr;  // creation stream: allocations and frees are ordered on it
@@ -133,13 +138,25 @@ void alloc_async(T** ptr, size_t count, hipStream_t s)
 // columns of an eighth of the variants instead of reading from all of them.
 // Items past 8 * per keep their place. Results do not depend on the order:
 // an item's samples depend only on its variant and chains.
-std::vector<ChrItem> xcd_order(const std::vector<ChrItem>& logical)
+// With nc chains per workgroup the same order applies to the groups of nc
+// consecutive items (nc chains of one variant).
+std::vector<ChrItem> xcd_order(const std::vector<ChrItem>& logical, int nc = 1)
 {
     constexpr int kXcds = 8;
-    const int n = (int)logical.size(), per = n / kXcds;
+    const int n = (int)logical.size() / nc, per = n / kXcds;
     std::vector<ChrItem> out(logical);
-    for (int b = 0; b < per * kXcds; ++b) out[(size_t)b] = logical[(size_t)((b % kXcds) * per + b / kXcds)];
+    for (int b = 0; b < per * kXcds; ++b)
+        for (int w = 0; w < nc; ++w)
+            out[(size_t)b * nc + w] = logical[(size_t)((b % kXcds) * per + b / kXcds) * nc + w];
     return out;
+}
+
+std::vector<ChrItem> logical_items(const naja_chr_plan& pl)
+{
+    std::vector<ChrItem> logical;
+    for (int v = 0; v < pl.n_variants; ++v)
+        for (int c0 = 0; c0 < pl.p.nchains; c0 += pl.G) logical.push_back(ChrItem{v, c0});
+    return logical;
 }
 
 void check_params(const naja_chr_params* p, int d)
@@ -193,6 +210,31 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         V.sample_off = (int64_t)v * per_variant_samples;
         a_off += (int64_t)lda * pl.d;
         b_off += lda;
+    }
+    // The LDS-column kernel: one wavefront per chain and four chains of one
+    // variant per workgroup, when the launch's moves, d, chain count, kernels
+    // and the largest variant's columns allow it.
+    {
+        int max_rows = 0;
+        for (const ChrVariant& V : pl.var) max_rows = std::max(max_rows, V.m);
+        const naja_chr_params* p = &pl.p;
+        const bool has_pair = p->pair_prob > 0.0 && !(p->pair_mode == 2 && !(p->n_pairs > 0 && p->pair_i &&
+                                                                               p->pair_j && p->pair_c && p->pair_s));
+        const bool coord_only = rng_threshold(p->ksparse_prob) == 0 && !has_pair;
+        int nc = 1;
+        size_t lds = 0;
+        if (sh.W == 1 && sh.G == 1 && coord_only && pl.d <= kChrRegStateMaxD && p->nchains % kChrLdsChains == 0 &&
+            chr_has_lds_kernel(sh, pl.paired)) {
+            lds = chr_lds_columns_bytes(sh, pl.d, max_rows);
+            if (lds <= 156u * 1024u) nc = kChrLdsChains;
+        }
+        if (nc != pl.nc) {
+            pl.items = xcd_order(logical_items(pl), nc);
+            NAJA_HIP(hipMemcpyAsync(pl.items_d, pl.items.data(), sizeof(ChrItem) * pl.items.size(),
+                                    hipMemcpyHostToDevice, s));
+        }
+        pl.nc = nc;
+        pl.lds_cols = nc > 1 ? lds : 0;
     }
     if (a_off > pl.pack_floats || b_off > pl.pack_rows) {
         free_async(pl.pack_mem, s);
@@ -315,7 +357,7 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
     float* out = pl.P ? zs : samples;
     const ChrArgs a = make_args(pl, y0, out, y_final, nsteps, t0);
     order_after_pack(pl, s);
-    chr_launch(a, (int)pl.items.size(), pl.shape, s);
+    chr_launch(a, (int)pl.items.size(), pl.shape, s, pl.nc, pl.lds_cols);
     if (pl.P) {
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
         const int64_t per = (int64_t)pl.p.nchains * pl.p.nspc;
@@ -376,10 +418,7 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
             e_off += k;
             pl->m_max = std::max(pl->m_max, m0 + k);
         }
-        std::vector<ChrItem> logical;
-        for (int v = 0; v < n_variants; ++v)
-            for (int c0 = 0; c0 < p->nchains; c0 += pl->G) logical.push_back(ChrItem{v, c0});
-        pl->items = xcd_order(logical);
+        pl->items = xcd_order(logical_items(*pl));
         const hipStream_t s = pl->stream;
         NAJA_HIP(hipEventCreateWithFlags(&pl->packed, hipEventDisableTiming));
         NAJA_HIP(hipEventCreateWithFlags(&pl->launched, hipEventDisableTiming));
@@ -501,9 +540,9 @@ int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, i
             repack(pl, sh);
             const ChrArgs a = make_args(pl, y0, (pl.P && pl.zbuf) ? pl.zbuf : probe, nullptr, nsteps);
             order_after_pack(pl, s);
-            chr_launch(a, (int)pl.items.size(), sh, s);  // warm-up
+            chr_launch(a, (int)pl.items.size(), sh, s, pl.nc, pl.lds_cols);  // warm-up
             NAJA_HIP(hipEventRecord(e0, s));
-            chr_launch(a, (int)pl.items.size(), sh, s);
+            chr_launch(a, (int)pl.items.size(), sh, s, pl.nc, pl.lds_cols);
             NAJA_HIP(hipEventRecord(e1, s));
             note_launch(pl, s);
             NAJA_HIP(hipEventSynchronize(e1));

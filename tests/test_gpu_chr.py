@@ -188,6 +188,45 @@ def test_paired_batched_mixed_blocks_bit_exact(gpu):
         np.testing.assert_array_equal(out[v], ref["samples"])
 
 
+@pytest.mark.parametrize("layout", ["paired", "unpaired", "paired-d128"])
+def test_lds_column_chains_bit_exact(gpu, layout):
+    """Coordinate-only batches with d <= 128 and a multiple of four chains run
+    four chains of a variant per workgroup with the variant's A in LDS; every
+    variant equals the oracle bit for bit, in both row layouts (the unpaired
+    case holds 350 rows, near the LDS budget)."""
+    import naja_amd
+
+    t = gpu
+    rng = np.random.default_rng(5)
+    if layout == "unpaired":
+        d = 100
+        A, b, _ = random_polytope(d, 150, seed=61)
+    else:
+        d = 60 if layout == "paired" else 128
+        A, b = halves_polytope(d, 80 if d == 60 else 140, seed=62)
+
+    def slabs(k):
+        a = rng.standard_normal((k, d))
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        return np.vstack([a, -a]), np.concatenate([np.full(k, 0.4), np.full(k, 0.3)])
+
+    blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12), (rng.standard_normal((5, d)), np.full(5, 0.7)), slabs(30)]
+    EA = [e for e, _ in blocks]
+    Eb = [f for _, f in blocks]
+    nch, nspc, thin = 8, 400, 3
+    X0 = np.zeros((len(blocks), nch, d), np.float32)
+    out = naja_amd.coordinate_hit_and_run_batched(
+        t.from_numpy(A.astype(np.float32)), t.from_numpy(b), [t.from_numpy(e.astype(np.float32)) for e in EA],
+        [t.from_numpy(f) for f in Eb], t.from_numpy(X0), nspc, thin, nch, seed=91, resync_interval=350)
+    t.cuda.synchronize()
+    out = out.cpu().numpy()
+    for v in range(len(blocks)):
+        Av = np.vstack([A, EA[v]]).astype(np.float32)
+        bv = np.concatenate([b, Eb[v]])
+        ref = oracle.chr_run_f32(Av, bv, X0[v], nspc, thin, seed=91, variant=v, resync_interval=350)
+        np.testing.assert_array_equal(out[v], ref["samples"])
+
+
 def test_chains_per_group_matches_oracle(gpu):
     A, b, y0 = random_polytope(12, 70, seed=31)
     nch = 6  # one full group of 4 and a partial group of 2

@@ -122,6 +122,8 @@ ChrKernelTable chr_table_g4();
 ChrKernelTable chr_table_big();  // W = 16, R = 10..22: the large-polytope kernel
 ChrKernelTable chr_table_w1p();  // one wavefront, paired rows
 ChrKernelTable chr_table_w1l();  // one wavefront, four chains per workgroup, A columns in LDS
+ChrKernelTable chr_table_w4p();  // four wavefronts, paired rows
+ChrKernelTable chr_table_w8p();  // eight wavefronts, paired rows
 
 // Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
 // per group (among the paired-row layout's kernels when paired); -1 if none.

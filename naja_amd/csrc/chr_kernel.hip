@@ -229,7 +229,7 @@ const ChrKernelTable& all_kernels()
 {
     static const ChrKernelTable tables[] = {chr_table_w1(), chr_table_w4a(), chr_table_w4b(), chr_table_w8(),
                                             chr_table_w16(), chr_table_g4(), chr_table_big(), chr_table_w1p(),
-                                            chr_table_w1l()};
+                                            chr_table_w1l(), chr_table_w4p(), chr_table_w8p()};
     static std::vector<ChrKernelEntry> flat;
     static ChrKernelTable all{nullptr, 0};
     if (!all.e) {

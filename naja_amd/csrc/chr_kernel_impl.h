@@ -279,8 +279,8 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // column loads and the cache footprint are those of one row. Every value is
 // the one the unpaired layout computes for the negated row: its chord ratio
 // s / (-ae) = -(s / ae) and RN(1 / -a) = -RN(1 / a) (rounding is symmetric),
-// its update fmaf(-alpha, -ae, s) = fmaf(alpha, ae, s), its FP64 resync the
-// FMA chain over -a. A row without a partner has bn = NaN: its second slack
+// its update fmaf(-alpha, -ae, s) = fmaf(alpha, ae, s), its FP64 resync
+// bn minus the negated FMA chain. A row without a partner has bn = NaN: its second slack
 // stays NaN and a NaN ratio never wins either minimum.
 // NC > 1 (one-wavefront coordinate-only chains with VGPR states): a workgroup
 // runs NC chains of one variant, one per wavefront and otherwise independent,
@@ -543,26 +543,21 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
         }
         if (event && u == next_resync) {
             // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
+            // The partner row's FMA chain over -a gives exactly -acc (a zero
+            // sum stays +0 in both: RN(x + y) is -0 only when x and y are),
+            // so its slack is bn - (acc != 0 ? -acc : +0), no second chain.
             double acc[G][R];
-            double accn[G][RP];
 #pragma unroll
             for (int k = 0; k < R; ++k)
 #pragma unroll
                 for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
-#pragma unroll
-            for (int k = 0; k < RP; ++k)
-#pragma unroll
-                for (int g = 0; g < G; ++g) accn[g][k] = 0.0;
             const float* pc = Av;
             auto add_col = [&](const double (&xc)[G]) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     const double av = (double)pc[row_of(k)];
 #pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        acc[g][k] = fma(av, xc[g], acc[g][k]);
-                        if constexpr (PAIRED) accn[g][k] = fma(-av, xc[g], accn[g][k]);
-                    }
+                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xc[g], acc[g][k]);
                 }
                 pc += lda;
             };
@@ -593,7 +588,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
                 if constexpr (PAIRED) {
                     const double bn = a.bn[V.b_off + row_of(k)];
 #pragma unroll
-                    for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - accn[g][k]);
+                    for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0));
                 }
             }
             prev_i = -1;

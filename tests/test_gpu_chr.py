@@ -289,18 +289,23 @@ def test_cube_d100_moments(gpu):
     assert np.abs(x).max() <= 1 + 1e-5
 
 
-def test_genome_scale_shape_matches_oracle(gpu):
+@pytest.mark.parametrize("rows", ["paired", "shuffled"])
+def test_genome_scale_shape_matches_oracle(gpu, rows):
     """The genome-scale reduced polytope (n = 2583 reactions, d = 512,
     A = [T; -T] with m = 5166 rows, pair-prob 0.3, thinning 85, Philox seed 0)
-    through the one-shot call, at the launch shape it picks for 4 chains
-    (8 wavefronts, 1 chain per group, 11 rows per lane; odd R takes the
-    one-row-run column loads): samples, final states and chain 0's move log
-    equal the oracle's bit for bit."""
+    through the one-shot call, at the launch shape it picks for 4 chains: the
+    paired-row layout (8 wavefronts, 6 packed rows per lane), and with the
+    rows shuffled (no pairing) the unpaired one (11 rows per lane; odd R takes
+    the one-row-run column loads). Samples, final states and chain 0's move
+    log equal the oracle's bit for bit."""
     import naja_amd
     from naja_amd import workloads
 
     t = gpu
     g = workloads.genome_scale(n=2583, d=512, seed=0)
+    if rows == "shuffled":
+        perm = np.random.default_rng(1).permutation(g["A"].shape[0])
+        g = dict(g, A=g["A"][perm], b=g["b"][perm])
     A32 = g["A"].astype(np.float32)
     m, d = A32.shape
     nch, nspc, thin, pp = 4, 200, 85, 0.3
@@ -310,7 +315,7 @@ def test_genome_scale_shape_matches_oracle(gpu):
     shape = plan.shape()
     plan.close()
     print(f"[genome scale] m={m} d={d} shape (W, G, R) = {shape}", flush=True)
-    assert shape == (8, 1, 11)
+    assert shape == ((8, 1, 6) if rows == "paired" else (8, 1, 11))
     nlog = nspc * thin
     S, fin, mlog = naja_amd.coordinate_hit_and_run(At, bt, X0, nspc, thin, nch, seed=0, pair_prob=pp,
                                                    return_final=True, move_log_steps=nlog)

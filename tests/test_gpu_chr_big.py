@@ -1,7 +1,10 @@
 """The large-polytope CHR kernel (16 wavefronts, 10-22 rows per lane, no
 reciprocal column) against the FP32 oracle, bit for bit, up to the
 large-scale shape (n = 10600 reactions: m = 21200 rows, d = 2400), and
-against the register-resident kernels where both apply."""
+against the register-resident kernels where both apply. [T; -T] polytopes
+take the paired-row register kernels (8 wavefronts, one packed row per pair);
+with their rows shuffled (no pairing) the same polytopes take the large
+kernel, so both paths are checked at the large-scale shape."""
 import numpy as np
 import pytest
 
@@ -21,9 +24,17 @@ def run_gpu(t, A, b, nch, nspc, thin, tpb=-1, **kw):
     return S.cpu().numpy()
 
 
+def shuffled(g, seed):
+    perm = np.random.default_rng(seed).permutation(g["A"].shape[0])
+    return dict(g, A=g["A"][perm], b=g["b"][perm])
+
+
+@pytest.mark.parametrize("rows", ["paired", "shuffled"])
 @pytest.mark.parametrize("n,d,nch,nspc,thin", [(10600, 2400, 4, 40, 5), (7000, 300, 3, 100, 7)])
-def test_big_kernel_matches_oracle(gpu, n, d, nch, nspc, thin):
+def test_big_kernel_matches_oracle(gpu, n, d, nch, nspc, thin, rows):
     g = genome_like(n, d, seed=n)
+    if rows == "shuffled":
+        g = shuffled(g, n)
     A32 = g["A"].astype(np.float32)
     kw = dict(seed=3, pair_prob=0.3, ksparse_prob=0.03, ksparse_k=5, resync_interval=64)
     print(f"[big kernel] m={A32.shape[0]} d={d} chains={nch} steps={nspc * thin}", flush=True)
@@ -37,7 +48,7 @@ def test_big_kernel_matches_oracle(gpu, n, d, nch, nspc, thin):
 def test_big_kernel_equals_register_kernel(gpu):
     """m = 10000 rows fits both the 8-wavefront register kernel and (with a
     1024-thread request) the large kernel: identical samples."""
-    g = genome_like(5000, 200, seed=8)
+    g = shuffled(genome_like(5000, 200, seed=8), 8)
     kw = dict(seed=9, pair_prob=0.3, resync_interval=100)
     S_small = run_gpu(gpu, g["A"], g["b"], 4, 150, 6, tpb=-1, **kw)
     S_big = run_gpu(gpu, g["A"], g["b"], 4, 150, 6, tpb=1024, **kw)

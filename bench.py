@@ -188,10 +188,12 @@ def main():
                 "kernel": "chr_step_kernel (batched, rounded variants)", "kernel_ms": round(chr_kernel_ms, 3),
                 "launches_timed": len(chr_ms), "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),
-                "note": "achieved: column + sample bytes / the CHR kernel's own time (HIP events on its launch "
-                        "stream); traffic: FETCH_SIZE + WRITE_SIZE per launch and issue_active_frac: "
-                        "SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES, both from profiles/chr_main_launch_traffic.json "
-                        "(one wavefront per SIMD: the step's latency binds, not bandwidth)"}
+                "note": "achieved: the algorithm's column + sample bytes / the CHR kernel's own time (HIP events "
+                        "on its launch stream); traffic: FETCH_SIZE + WRITE_SIZE per launch (the paired-row layout "
+                        "packs each [row; -row] pair once and the variant's A sits in LDS, so about 4 % of the "
+                        "algorithmic bytes cross HBM) and issue_active_frac: SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES, "
+                        "both from profiles/chr_main_launch_traffic.json (one wavefront per SIMD: instruction "
+                        "issue and the step's latency bind, not bandwidth)"}
 
     cube = None if args.no_extra else cube_plain(np, torch, args)
     batched = None if args.no_extra else batched_conditions(np, torch, cd, pool, lp, A0, b0, args, use_net)

@@ -130,7 +130,14 @@ int naja_chr_sample_batched_rounded(const float* A, const double* b, int m0, int
  * yourself. naja_chr_plan_autotune times every compiled launch shape
  * on about 1024 steps of the real run, keeps the fastest, and remembers it
  * for later plans of the same size (results do not depend on the shape).
- * shape_out receives {wavefronts per workgroup, chains per group, rows per lane}.
+ * shape_out receives {wavefronts per workgroup, chains per group, rows per
+ * lane}; in the paired-row layout rows per lane count packed rows. Creation
+ * checks every block (A and each variant's extra rows) for a second half
+ * that negates the first ([T; -T] bounds, [a; -a] slabs) and packs such
+ * blocks one row per pair, two slacks per packed row; the check reads one
+ * flag per block back, so creation synchronises its stream once. Samples
+ * are those of the row-by-row algorithm in either layout. The one-shot
+ * sampling calls create a plan internally and synchronise likewise.
  * The reference builds its launch on every call (chr.cu:13-70) and has no
  * autotuner (launchChrKernel takes --tpb as given, chr.cu:499-553).
  */

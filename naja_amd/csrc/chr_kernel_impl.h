@@ -116,14 +116,13 @@ __device__ __forceinline__ uint32_t dpp_bcast(uint32_t v)
     return (uint32_t)__builtin_amdgcn_update_dpp((int)IDENT, (int)v, CTRL, ROW_MASK, 0xF, false);
 }
 
-// Unsigned and signed minimum over the 64 lanes of a wavefront, returned
-// wave-uniform: DPP within each row of 16 (quad_perm [1,0,3,2],
-// quad_perm [2,3,0,1], row_half_mirror, row_mirror), then across rows with
-// row_bcast:15 (rows 1 and 3 take the row below) and row_bcast:31 (rows 2
-// and 3 take lane 31), which leaves the wave minimum in lane 63; one
-// v_readlane each brings it to a scalar register. Every step is a DPP
-// operand of a min: three VALU per value across rows.
-__device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
+// Unsigned and signed minimum over the 64 lanes of a wavefront: DPP within
+// each row of 16 (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror,
+// row_mirror), then across rows with row_bcast:15 (rows 1 and 3 take the row
+// below) and row_bcast:31 (rows 2 and 3 take lane 31), which leaves the wave
+// minimum in lane 63. Every step is a DPP operand of a min: three VALU per
+// value across rows.
+__device__ __forceinline__ void wave_min2_dpp(uint32_t& hb, int32_t& lb)
 {
     uint32_t l = (uint32_t)lb;
     hb = umin32(hb, dpp_mov<0xB1>(hb));
@@ -138,8 +137,44 @@ __device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
     l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_bcast<0x142, 0xA, 0x7FFFFFFFu>(l));
     hb = umin32(hb, dpp_bcast<0x143, 0xC, 0xFFFFFFFFu>(hb));
     l = (uint32_t)smin32((int32_t)l, (int32_t)dpp_bcast<0x143, 0xC, 0x7FFFFFFFu>(l));
+    lb = (int32_t)l;
+}
+
+// The same minima, returned wave-uniform (v_readlane of lane 63).
+__device__ __forceinline__ void wave_min2(uint32_t& hb, int32_t& lb)
+{
+    wave_min2_dpp(hb, lb);
     hb = (uint32_t)__builtin_amdgcn_readlane((int)hb, 63);
-    lb = __builtin_amdgcn_readlane((int)l, 63);
+    lb = __builtin_amdgcn_readlane(lb, 63);
+}
+
+// Unsigned and signed minima of v[0..N) within the lane, as a tree of
+// three-input minima (ceil(log3 N) dependent levels instead of N - 1).
+template <int N>
+__device__ __forceinline__ void lane_min2(const uint32_t (&v)[N], uint32_t& hb, int32_t& lb)
+{
+    uint32_t u[N];
+    int32_t t[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        u[i] = v[i];
+        t[i] = (int32_t)v[i];
+    }
+#pragma unroll
+    for (int st = 1; st < N; st *= 3)
+#pragma unroll
+        for (int i = 0; i < N; i += 3 * st) {
+            if (i + st < N) {
+                u[i] = umin32(u[i], u[i + st]);
+                t[i] = smin32(t[i], t[i + st]);
+            }
+            if (i + 2 * st < N) {
+                u[i] = umin32(u[i], u[i + 2 * st]);
+                t[i] = smin32(t[i], t[i + 2 * st]);
+            }
+        }
+    hb = u[0];
+    lb = t[0];
 }
 
 // Chord ends in the direct form q = s / ae:
@@ -687,62 +722,62 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
             }
         }
 
-        // Chord ends per chain.
+        // Chord ends per chain: every ratio of the lane, reduced as a tree,
+        // then across the wavefront. One wavefront forms alpha in lane 63,
+        // where the minima end, and reads that one value back; several
+        // exchange their minima through LDS first.
+        constexpr int NQ = PAIRED ? 2 * R : R;
         uint32_t rhi[G];
         int32_t rlo[G];
+        float alpha[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            uint32_t hb = 0xFFFFFFFFu;
-            int32_t lb = 0x7FFFFFFF;
+            uint32_t qv[NQ];
             if (mtype == kMoveCoord) {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
-                    const uint32_t qb = chord_bits_rcp(cb[k], s[g][k]);
-                    hb = umin32(hb, qb);
-                    lb = smin32(lb, (int32_t)qb);
-                    if constexpr (PAIRED) {
-                        const uint32_t qn = chord_bits_rcp(-cb[k], sn[g][k]);
-                        hb = umin32(hb, qn);
-                        lb = smin32(lb, (int32_t)qn);
-                    }
+                    qv[k] = chord_bits_rcp(cb[k], s[g][k]);
+                    if constexpr (PAIRED) qv[R + k] = chord_bits_rcp(-cb[k], sn[g][k]);
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
-                    const uint32_t qb = chord_bits(ae[k], s[g][k]);
-                    hb = umin32(hb, qb);
-                    lb = smin32(lb, (int32_t)qb);
-                    if constexpr (PAIRED) {
-                        const uint32_t qn = chord_bits(-ae[k], sn[g][k]);
-                        hb = umin32(hb, qn);
-                        lb = smin32(lb, (int32_t)qn);
-                    }
+                    qv[k] = chord_bits(ae[k], s[g][k]);
+                    if constexpr (PAIRED) qv[R + k] = chord_bits(-ae[k], sn[g][k]);
                 }
             }
-            wave_min2(hb, lb);
-            rhi[g] = hb;
-            rlo[g] = lb;
-            if (W > 1 && lane == 0) part[buf][g][wave] = make_uint2(hb, (uint32_t)lb);
+            uint32_t hb;
+            int32_t lb;
+            lane_min2(qv, hb, lb);
+            if constexpr (W == 1) {
+                wave_min2_dpp(hb, lb);
+                const float hi = __builtin_bit_cast(float, hb);
+                const float lo = __builtin_bit_cast(float, lb);
+                alpha[g] = readlane_f32(fmaf(U[g], hi - lo, lo), 63);
+            } else {
+                wave_min2(hb, lb);
+                rhi[g] = hb;
+                rlo[g] = lb;
+                if (lane == 0) part[buf][g][wave] = make_uint2(hb, (uint32_t)lb);
+            }
         }
 
-        if (W > 1) __syncthreads();
-
-        float alpha[G];
+        if constexpr (W > 1) {
+            __syncthreads();
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            uint32_t hb = rhi[g];
-            int32_t lb = rlo[g];
-            if (W > 1) {
+            for (int g = 0; g < G; ++g) {
+                uint32_t hb = rhi[g];
+                int32_t lb = rlo[g];
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     const uint2 p = part[buf][g][w];
                     hb = umin32(hb, p.x);
                     lb = smin32(lb, (int32_t)p.y);
                 }
+                const float hi = __builtin_bit_cast(float, hb);
+                const float lo = __builtin_bit_cast(float, lb);
+                alpha[g] = fmaf(U[g], hi - lo, lo);
             }
-            const float hi = __builtin_bit_cast(float, hb);
-            const float lo = __builtin_bit_cast(float, lb);
-            alpha[g] = fmaf(U[g], hi - lo, lo);
         }
 
         // Fused rank-1 slack update s -= alpha * ae (chr.cu:464-482); the

@@ -147,6 +147,17 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d,
                          const float* A_extra, const double* b_extra,
                          const float* P, const float* c,
                          const naja_chr_params* p, void* stream, naja_chr_plan** out);
+/*
+ * Replaces the rounding of a plan created with one (P, c as at creation:
+ * n_variants x d x d row-major and n_variants x d, device memory the caller
+ * keeps alive): the packed rows stay and only A' = A P, b' = b - A c are
+ * formed again, on the creation stream, with no allocation and no
+ * synchronisation; later launches sample the new rounding. A, b and the
+ * extra rows must be unchanged. For pipelines that recompute the rounding of
+ * the same conditioned variants at every step (the reference rounds a model
+ * once, offline: src/rounding/plan.cpp:12-62).
+ */
+int naja_chr_plan_set_rounding(naja_chr_plan* plan, const float* P, const float* c);
 int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, int* shape_out);
 int naja_chr_plan_shape(const naja_chr_plan* plan, int* shape_out);
 int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, float* y_final, void* stream);

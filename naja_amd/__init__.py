@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "naja_chr_sample_batched",
     "naja_chr_sample_batched_rounded",
     "naja_chr_plan_create",
+    "naja_chr_plan_set_rounding",
     "naja_chr_plan_autotune",
     "naja_chr_plan_shape",
     "naja_chr_plan_launch",
@@ -141,6 +142,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_chr_sample_batched_rounded.restype = i32
     lib.naja_chr_plan_create.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, P, vp, ctypes.POINTER(vp)]
     lib.naja_chr_plan_create.restype = i32
+    lib.naja_chr_plan_set_rounding.argtypes = [vp, vp, vp]
+    lib.naja_chr_plan_set_rounding.restype = i32
     lib.naja_chr_plan_autotune.argtypes = [vp, vp, vp, vp]
     lib.naja_chr_plan_autotune.restype = i32
     lib.naja_chr_plan_shape.argtypes = [vp, vp]
@@ -434,6 +437,7 @@ def coordinate_hit_and_run_batched(A, b, extra_A, extra_b, X0, nspc, thinning, n
 
 class ChrPlan:
     """A packed batch that launches without allocating (naja_chr_plan_*).
+    A may be a VariantBatch (b, extra_A, extra_b None).
 
     Keeps device copies of every input alive for the plan's lifetime. launch()
     is safe to capture in a torch.cuda.CUDAGraph (plain kernel launches on the
@@ -445,8 +449,9 @@ class ChrPlan:
                  pair_mode=1, chains_per_group=1, stream=None):
         torch = _torch()
         self._lib = load_library()
-        extra_A = extra_A if extra_A is not None else [torch.zeros((0, A.shape[1]))]
-        extra_b = extra_b if extra_b is not None else [torch.zeros(0)]
+        if not isinstance(A, VariantBatch):  # a VariantBatch carries its extra rows
+            extra_A = extra_A if extra_A is not None else [torch.zeros((0, A.shape[1]))]
+            extra_b = extra_b if extra_b is not None else [torch.zeros(0)]
         self._A, self._b, self.d, self.m0, ks, self._Ae, self._be = _batch_inputs(A, b, extra_A, extra_b)
         self.n_variants = len(ks)
         self.nchains, self.nspc = nchains, nspc
@@ -462,6 +467,19 @@ class ChrPlan:
                                               ptr(self._c), ctypes.byref(self._params), _stream_ptr(stream),
                                               ctypes.byref(h)))
         self._h = h
+
+    def set_rounding(self, P, c):
+        """A new rounding for a plan created with one (same shapes): only
+        A' = A P and b' = b - A c are formed again (naja_chr_plan_set_rounding)."""
+        torch = _torch()
+        if self._P is None:
+            raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, "the plan was created without a rounding")
+        Pd, cd = _dev(P, torch.float32), _dev(c, torch.float32)
+        if tuple(Pd.shape) != tuple(self._P.shape) or tuple(cd.shape) != tuple(self._c.shape):
+            raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT,
+                                      f"rounding shapes {tuple(Pd.shape)}, {tuple(cd.shape)} differ from the plan's")
+        _check(self._lib.naja_chr_plan_set_rounding(self._h, Pd.data_ptr(), cd.data_ptr()))
+        self._P, self._c = Pd, cd  # kept alive while the plan may transform again
 
     def samples_like(self):
         torch = _torch()

@@ -44,8 +44,8 @@ import time
 
 import numpy as np
 
-from . import (NAJA_ERR_UNSUPPORTED, NajaError, VariantBatch, coordinate_hit_and_run_batched, diagnostics_batched,
-               ellipsoid_rounding, facet_rounding, lowrank_maps)
+from . import (NAJA_ERR_UNSUPPORTED, ChrPlan, NajaError, VariantBatch, coordinate_hit_and_run_batched,
+               diagnostics_batched, ellipsoid_rounding, facet_rounding, lowrank_maps)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
@@ -98,11 +98,15 @@ def _shape(A0, EA):
 
 def sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync=1000):
     """Batched CHR of every variant in z with y = P_v z (P None: plain).
-    A0 may be a VariantBatch (b0, EA, Eb None). Returns y samples
+    A0 may be a VariantBatch (b0, EA, Eb None): rounded runs then keep a plan
+    on the batch per run shape and only set the new rounding each call (no
+    repacking, allocation or synchronisation). Returns y samples
     (n_variants, nchains * nspc, d) on the GPU."""
     import torch
 
     nv, d = _shape(A0, EA)
+    if P is not None and isinstance(A0, VariantBatch):
+        return _batch_plan(A0, nchains, nspc, thin, seed, resync).sample(P)
     X0 = torch.zeros((nv, nchains, d), dtype=torch.float32, device="cuda")
     kw = {}
     if P is not None:
@@ -113,6 +117,41 @@ def sample_rounded(A0, b0, EA, Eb, P, nchains, nspc, thin, seed, resync=1000):
         A0 = VariantBatch(A0, b0, EA, Eb)
     return coordinate_hit_and_run_batched(A0, None, None, None, X0, nspc, thin, nchains, seed=seed,
                                           resync_interval=resync, **kw)
+
+
+class _RoundedPlan:
+    """A rounded sampling plan of one batch and run shape (chains from z = 0,
+    c = 0), reused with a new rounding at every call."""
+
+    def __init__(self, batch, nchains, nspc, thin, seed, resync):
+        import torch
+
+        nv, d = batch.n_variants, batch.d
+        self.X0 = torch.zeros((nv, nchains, d), dtype=torch.float32, device="cuda")
+        self.c = torch.zeros((nv, d), dtype=torch.float32, device="cuda")
+        self.plan = None
+        self.args = (batch, nspc, thin, nchains, seed, resync)
+
+    def sample(self, P):
+        import torch
+
+        Pt = P if torch.is_tensor(P) else torch.as_tensor(np.asarray(P, np.float32))
+        Pt = Pt.to(device="cuda", dtype=torch.float32).contiguous()
+        if self.plan is None:
+            batch, nspc, thin, nchains, seed, resync = self.args
+            self.plan = ChrPlan(batch, None, nspc, thin, nchains, P=Pt, c=self.c, seed=seed,
+                                resync_interval=resync)
+        else:
+            self.plan.set_rounding(Pt, self.c)
+        return self.plan.launch(self.X0, self.plan.samples_like())
+
+
+def _batch_plan(batch, nchains, nspc, thin, seed, resync):
+    plans = batch.__dict__.setdefault("_rounded_plans", {})
+    key = (nchains, nspc, thin, seed, resync)
+    if key not in plans:
+        plans[key] = _RoundedPlan(batch, nchains, nspc, thin, seed, resync)
+    return plans[key]
 
 
 def variant_diagnostics(Y, nchains):

@@ -76,6 +76,40 @@ void scratch_free(void* ptr, hipStream_t s)
     if (ptr) NAJA_HIP(hipFreeAsync(ptr, s));
 }
 
+HeldBuf& held_buf()
+{
+    constexpr int kMaxDevices = 64;
+    static HeldBuf bufs[kMaxDevices];
+    int dev = 0;
+    NAJA_HIP(hipGetDevice(&dev));
+    NAJA_REQUIRE(dev >= 0 && dev < kMaxDevices, NAJA_ERR_UNSUPPORTED, "device %d", dev);
+    return bufs[dev];
+}
+
+char* held_reserve(HeldBuf& b, size_t bytes)
+{
+    if (b.cap < bytes) {
+        if (b.dev) NAJA_HIP(hipFree(b.dev));
+        b.dev = nullptr;
+        b.cap = 0;
+        NAJA_HIP(hipMalloc((void**)&b.dev, bytes));
+        b.cap = bytes;
+    }
+    return b.dev;
+}
+
+int* held_host(HeldBuf& b, size_t count)
+{
+    if (b.host_cap < count) {
+        if (b.host) NAJA_HIP(hipHostFree(b.host));
+        b.host = nullptr;
+        b.host_cap = 0;
+        NAJA_HIP(hipHostMalloc((void**)&b.host, sizeof(int) * count, hipHostMallocDefault));
+        b.host_cap = count;
+    }
+    return b.host;
+}
+
 }  // namespace naja
 
 extern "C" {

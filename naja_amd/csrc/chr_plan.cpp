@@ -270,6 +270,22 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
     ++pl.pack_seq;
 }
 
+// A new rounding (P, c) of a transform plan: the packed untransformed rows
+// stay, only A' = A P and b' = b - A c are formed again, on the creation
+// stream, after any launch that may still read the packs.
+void retransform(naja_chr_plan& pl)
+{
+    const hipStream_t s = pl.stream;
+    if (pl.launch_pending) {
+        NAJA_HIP(hipStreamWaitEvent(s, pl.launched, 0));
+        pl.launch_pending = false;
+    }
+    chr_transform(pl.A_raw, pl.b_raw, pl.bn_raw, pl.var_d, pl.n_variants, pl.d, pl.shape.rows(), pl.P, pl.c,
+                  pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, s);
+    NAJA_HIP(hipEventRecord(pl.packed, s));
+    ++pl.pack_seq;
+}
+
 // Before a launch on stream s: s waits for the latest repack on the
 // creation stream (once per repack and stream). A captured launch records no
 // wait (an event recorded outside the capture cannot be waited on inside
@@ -509,6 +525,21 @@ int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, f
         NAJA_REQUIRE(plan && y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, y0 or samples");
         launch(*plan, y0, samples, y_final, (int64_t)plan->p.nspc * plan->p.thinning,
                reinterpret_cast<hipStream_t>(stream));
+    });
+}
+
+int naja_chr_plan_set_rounding(naja_chr_plan* plan, const float* P, const float* c)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(plan && P && c, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, P or c");
+        NAJA_REQUIRE(plan->P != nullptr, NAJA_ERR_INVALID_ARGUMENT,
+                     "the plan was created without a rounding transform");
+        NAJA_REQUIRE(plan->inplace == nullptr, NAJA_ERR_INVALID_ARGUMENT,
+                     "a one-shot plan sampling in place cannot change its rounding");
+        plan->P = P;
+        plan->c = c;
+        retransform(*plan);
     });
 }
 

@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -75,6 +77,21 @@ void scratch_alloc(T** ptr, size_t count, hipStream_t s)
 {
     scratch_alloc(reinterpret_cast<void**>(ptr), sizeof(T) * (count ? count : 1), s);
 }
+
+// A grow-only device buffer per device, with a pinned host block for status
+// read-backs, for calls that synchronise their stream before they return:
+// such a call holds the mutex for its whole duration, so no pool call (tens
+// to hundreds of microseconds of host time each) is made per call.
+struct HeldBuf {
+    std::mutex mu;
+    char* dev = nullptr;
+    size_t cap = 0;
+    int* host = nullptr;
+    size_t host_cap = 0;
+};
+HeldBuf& held_buf();                         // the current device's
+char* held_reserve(HeldBuf& b, size_t bytes);  // device bytes (old contents dropped)
+int* held_host(HeldBuf& b, size_t count);      // pinned host ints
 
 // Runs body(v) for v in [0, n) over up to nthreads host threads.
 template <typename F>

@@ -356,8 +356,9 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
         const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
         const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
         const size_t b_P0i = sizeof(double) * (size_t)n_variants * d * d;
-        char* mem = nullptr;
-        scratch_alloc((void**)&mem, b_off + 2 * b_k + b_P0i, s);
+        HeldBuf& hb = held_buf();
+        std::lock_guard<std::mutex> hold(hb.mu);
+        char* mem = held_reserve(hb, b_off + 2 * b_k + b_P0i);
         int64_t* eoff = reinterpret_cast<int64_t*>(mem);
         int* ek = reinterpret_cast<int*>(mem + b_off);
         int* status = reinterpret_cast<int*>(mem + b_off + b_k);
@@ -371,9 +372,8 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
         hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), lds, s, A0, b0, m0, d, extra_A, extra_b,
                            eoff, ek, x_c, P0, theta, include_base ? 1 : 0, lower ? 1 : 0, P0i, P, status);
         NAJA_HIP(hipGetLastError());
-        std::vector<int> st(n_variants);
-        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
-        scratch_free(mem, s);  // stream-ordered: after the copy
+        int* st = held_host(hb, (size_t)n_variants);
+        NAJA_HIP(hipMemcpyAsync(st, status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
         for (int v = 0; v < n_variants; ++v) {
             NAJA_REQUIRE(st[v] != NAJA_ERR_INVALID_ARGUMENT, NAJA_ERR_INVALID_ARGUMENT,

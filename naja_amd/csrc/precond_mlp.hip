@@ -981,41 +981,6 @@ __global__ __launch_bounds__(256) void ellipsoid_kernel(const float* __restrict_
     if (tid == 0) status[v] = NAJA_OK;
 }
 
-// Small device buffer per device for the status words and row tables of the
-// synchronous preconditioner calls, kept between calls (a stream-ordered
-// free on the idle stream after the status read-back cost about 0.4 ms of
-// host time per call). Each call holds the device's lock until its stream
-// has finished with the buffer, so the buffer is idle whenever it changes
-// hands or grows.
-constexpr int kMaxDevices = 64;
-
-struct SmallBuf {
-    std::mutex mu;
-    char* p = nullptr;
-    size_t cap = 0;
-};
-
-SmallBuf& small_buf()
-{
-    static SmallBuf bufs[kMaxDevices];
-    int dev = 0;
-    NAJA_HIP(hipGetDevice(&dev));
-    NAJA_REQUIRE(dev >= 0 && dev < kMaxDevices, NAJA_ERR_UNSUPPORTED, "device %d", dev);
-    return bufs[dev];
-}
-
-char* small_buf_reserve(SmallBuf& b, size_t bytes)
-{
-    if (b.cap < bytes) {
-        if (b.p) NAJA_HIP(hipFree(b.p));
-        b.p = nullptr;
-        b.cap = 0;
-        NAJA_HIP(hipMalloc((void**)&b.p, bytes));
-        b.cap = bytes;
-    }
-    return b.p;
-}
-
 }  // namespace mlp
 }  // namespace naja
 
@@ -1047,9 +1012,9 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         // buffer, held until the status has been read back.
         const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
         const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
-        mlp::SmallBuf& sb = mlp::small_buf();
+        HeldBuf& sb = held_buf();
         std::lock_guard<std::mutex> hold(sb.mu);
-        char* mem = mlp::small_buf_reserve(sb, b_off + 2 * b_k);
+        char* mem = held_reserve(sb, b_off + 2 * b_k);
         int64_t* eoff = reinterpret_cast<int64_t*>(mem);
         int* ek = reinterpret_cast<int*>(mem + b_off);
         int* status = reinterpret_cast<int*>(mem + b_off + b_k);
@@ -1058,8 +1023,8 @@ extern "C" int naja_precond_ellipsoid_device(const float* A0, const double* b0, 
         hipLaunchKernelGGL(mlp::ellipsoid_kernel, dim3(n_variants), dim3(256), lds, s, A0, b0, m0, d, extra_A, extra_b,
                            eoff, ek, x_c, P, status);
         NAJA_HIP(hipGetLastError());
-        std::vector<int> st(n_variants);
-        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
+        int* st = held_host(sb, (size_t)n_variants);
+        NAJA_HIP(hipMemcpyAsync(st, status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_INVALID_ARGUMENT,
@@ -1280,14 +1245,14 @@ extern "C" int naja_lowrank_maps_device(const float* P0, const float* V, int n_v
         const size_t rr = (size_t)rank * rank;
         const size_t lds = sizeof(double) * ((size_t)d * rank + rr + std::max(5 * rr, 2 * (size_t)d * rank));
         hipStream_t s = (hipStream_t)stream;
-        mlp::SmallBuf& sb = mlp::small_buf();
+        HeldBuf& sb = held_buf();
         std::lock_guard<std::mutex> hold(sb.mu);
-        int* status = reinterpret_cast<int*>(mlp::small_buf_reserve(sb, sizeof(int) * n_variants));
+        int* status = reinterpret_cast<int*>(held_reserve(sb, sizeof(int) * n_variants));
         hipLaunchKernelGGL(mlp::lowrank_map_kernel, dim3(n_variants), dim3(mlp::kLrThreads), lds, s, P0, V, d, rank,
                            expand, P, status);
         NAJA_HIP(hipGetLastError());
-        std::vector<int> st(n_variants);
-        NAJA_HIP(hipMemcpyAsync(st.data(), status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
+        int* st = held_host(sb, (size_t)n_variants);
+        NAJA_HIP(hipMemcpyAsync(st, status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));
         for (int v = 0; v < n_variants; ++v)
             NAJA_REQUIRE(st[v] == NAJA_OK, NAJA_ERR_RUNTIME,

@@ -423,6 +423,41 @@ def test_rounded_plan_equals_one_shot_call(gpu, d):
     plan.close()
 
 
+def test_plan_set_rounding_equals_one_shot_calls(gpu):
+    """A rounded plan given a new rounding (only A' = A P, b' = b - A c formed
+    again) samples exactly what a one-shot call with that rounding samples,
+    for several roundings in a row, in the paired layout and with LDS
+    columns (coordinate moves, four chains) and with pair moves."""
+    import naja_amd
+
+    t = gpu
+    d = 12
+    A, b, _ = cube(d)
+    rng = np.random.default_rng(77)
+    ks = [0, 6, 10, 4]
+    EA, Eb = [], []
+    for k in ks:
+        a = rng.standard_normal((k // 2, d))
+        EA.append(np.vstack([a, -a]))
+        Eb.append(np.full(k, 0.6))
+    nv, nch, nspc, thin = len(ks), 4, 300, 3
+    batch = naja_amd.VariantBatch(A, b, EA, Eb)
+    Z0 = t.zeros((nv, nch, d), device="cuda")
+    c = t.zeros((nv, d), device="cuda")
+    for kw in (dict(), dict(pair_prob=0.3)):
+        Ps = [t.from_numpy(np.stack([np.eye(d) + s * rng.standard_normal((d, d)) for _ in ks]).astype(np.float32))
+              .cuda() for s in (0.1, 0.2, 0.05)]
+        plan = naja_amd.ChrPlan(batch, None, nspc, thin, nch, P=Ps[0], c=c, seed=4, **kw)
+        for P in Ps:
+            plan.set_rounding(P, c)
+            out = plan.launch(Z0, plan.samples_like())
+            one = naja_amd.coordinate_hit_and_run_batched(batch, None, None, None, Z0, nspc, thin, nch, seed=4,
+                                                          P=P, c=c, **kw)
+            t.cuda.synchronize()
+            np.testing.assert_array_equal(out.cpu().numpy(), one.cpu().numpy())
+        plan.close()
+
+
 def test_plan_autotune_relaunch_and_graph_replay(gpu):
     """A plan launches without allocating: relaunches and hipGraph replays
     (torch.cuda.CUDAGraph) reproduce the eager samples exactly, and the

@@ -5,17 +5,21 @@ Workload (one "step" = one pass of the hot path over one synthetic batch):
   each adding k ~ U{0..300} extra rows: slabs |a . y| <= w around the start,
   a from a fixed pool of 300 unit directions, w log-uniform on [1e-3, 1e-1].
   4 chains per variant x 12500 samples x thinning 16, FP64 resync every 1000
-  steps, all variants in one batched launch. The learned-rounded step: Dikin
-  ellipsoid P0 -> MLP inference -> learned low-rank update of P0 -> the
-  facet-aligned rounding completed from it -> main run (pack + rounding
-  transform + CHR + backmap to y) -> per-variant ESS and split-R-hat -> the
-  variants failing R-hat < 1.05 run again with the fallback, all on the GPU.
-  Inputs are resident in HBM when timing starts.
+  steps, all variants in one batched launch. The learned-rounded step: when
+  validation accepted the network, Dikin ellipsoid P0 -> MLP inference ->
+  learned low-rank update of P0 -> the facet-aligned rounding completed from
+  it; else the base: the facet-aligned rounding completed from the cube's
+  facets (orthonormalised) with the maximum-volume exchange -> main run (pack
+  + rounding transform + CHR + backmap to y) -> per-variant ESS and
+  split-R-hat -> the variants failing R-hat < 1.05 run again with the
+  fallback (completed from the Dikin ellipsoid), all on the GPU. Inputs are
+  resident in HBM when timing starts.
 value = sum over variants with split-R-hat < 1.05 of the variant's min-ESS
   (minimum over dims of the minimum-over-chains ESS) / seconds per step.
 Beside it, on the same variants, chains and steps and timed the same way:
-plain CHR, the reference's Dikin rounding, the Dikin ellipsoid, and the
-facet-aligned rounding without the network; the cube run (d = 100, 4 x 12500,
+plain CHR, the reference's Dikin rounding, the Dikin ellipsoid, the
+facet-aligned rounding completed from the ellipsoid, and the facet-aligned
+rounding without the network; the cube run (d = 100, 4 x 12500,
 thinning 16); the batched conditions at their own half-widths (1e-2..1); the
 genome-scale run (n = 2583, d = 512, 4 chains x 12500, thinning 85, pair-prob
 0.3) and the large-scale run (n = 10600, d = 2400, 1024 chains x 2000,
@@ -131,13 +135,14 @@ def main():
         if use_net:
             P, _ = cd.learned_rounding(lp, Fh_dev, batch)
         else:  # rejected by validation: the base rounding, no network work
-            P = cd.facet_maps(batch, cd.base_rounding(batch, None, None, None))
+            P = cd.facet_maps(batch)
         return cd.sample_accepted(batch, P, fallback, nch, nspc, thin, seed, resync)
 
     # The other methods on the same variants, chains and steps, timed the same
     # way (warmup, then the mean of `steps` synchronised repetitions).
     others = cd.compare(A0, b0, EAh, Ebh, Fh, lp, nchains=nch, nspc=nspc, thin=thin, seed=seed, resync=resync,
-                        methods=() if args.no_compare else ("plain", "dikin", "ellipsoid", "facet"),
+                        methods=() if args.no_compare else ("plain", "dikin", "ellipsoid", "facet_ellipsoid",
+                                                            "facet"),
                         warmup=args.warmup, steps=args.steps, log=log)
     plain = others.get("plain", {"metric": float("nan"), "metric_all": float("nan")})
 
@@ -212,7 +217,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         P = (cd.learned_rounding(lp, Fh_dev, batch)[0] if use_net
-             else cd.facet_maps(batch, cd.base_rounding(batch, None, None, None)))
+             else cd.facet_maps(batch))
         cpu = cpu_baseline(np, A0, b0, EAh, Ebh, P.cpu().numpy(), nch, nspc, thin, seed, resync,
                            float(e[ok].sum()), args.cpu_seconds)
     lp.close()
@@ -234,8 +239,9 @@ def main():
             "workload": f"{nv} held-out conditioned variants of the cube [-1,1]^100, 0-300 extra slab rows "
                         f"(half-widths 1e-3..1e-1), {nch} chains x {nspc} x thin {thin}, one batched launch; "
                         f"learned-rounded (facet-aligned rounding completed from the learned low-rank update of "
-                        f"the Dikin ellipsoid when validation accepts the network, else from the ellipsoid; "
-                        f"R-hat acceptance with fallback)",
+                        f"the Dikin ellipsoid when validation accepts the network, else the base: completed from "
+                        f"the cube's own facets, orthonormalised, with the maximum-volume exchange; R-hat "
+                        f"acceptance with fallback)",
             "variants": nv, "chains": nch, "nspc": nspc, "thin": thin, "resync": resync,
             "n_pass_rhat_1_05": int(ok.sum()), "min_ess_sum": round(float(e.sum()), 1),
             "min_ess_sum_passing": round(float(e[ok].sum()), 1), "rerun_with_fallback": n_again,
@@ -249,8 +255,9 @@ def main():
                         for name, o in others.items()},
             "methods_note": "every method timed like the headline (same warmup and steps); plain: no rounding; "
                             "dikin: the reference's dikin_precondition (host FP64); ellipsoid: Dikin ellipsoid; "
-                            "facet: facet-aligned rounding completed from the Dikin ellipsoid (the method without "
-                            "the network)",
+                            "facet_ellipsoid: facet-aligned rounding completed from the Dikin ellipsoid; facet: "
+                            "facet-aligned rounding completed from the cube's facets with the maximum-volume "
+                            "exchange (the method without the network)",
             "pilot_seconds": round(t_pilot, 3), "training_seconds": round(t_train, 3),
             "learned_validation": dict(val, seconds=round(t_val, 3)),
             "training_nll_first_last": [round(hist[0], 3), round(hist[-1], 3)],

@@ -416,6 +416,11 @@ int naja_analytic_center(const float* A, const double* b, int m, int d, const fl
 int naja_precond_facet_large(const float* A, const double* b, int m, int d, const float* x_c, float theta, float* P,
                              void* stream);
 
+/* How naja_precond_facet_device completes the directions the facets leave free. */
+#define NAJA_FACET_COMPLETE_ROUNDING 0    /* the dual rows of the base rounding P0 (rows of P0^-1) */
+#define NAJA_FACET_COMPLETE_LOWER 1       /* rows of L^-1, L the lower Cholesky factor of P0 P0^T */
+#define NAJA_FACET_COMPLETE_BASE_FACETS 2 /* the base polytope's dual rows, orthonormalised; P0 unused */
+
 /*
  * Facet-aligned rounding per variant on the GPU: y = x_c + P z with the
  * coordinate axes of z along the edges of a parallelotope of binding facets,
@@ -423,27 +428,40 @@ int naja_precond_facet_large(const float* A, const double* b, int m, int d, cons
  * merely whitened. Candidates are the dual rows a_r / s_r (s_r the slack at
  * x_c) of the variant's extra rows (with include_base, of its base rows too),
  * chosen greedily by modified Gram-Schmidt residual norm while the best one
- * keeps at least theta[v] of its own norm; the remaining directions come
- * from the dual rows of the base rounding P0 (the rows of P0^-1; with lower,
- * the rows of L^-1 for the lower Cholesky factor L of P0 P0^T, the same
- * ellipsoid with its axes in triangular order). P = N^-1
- * for the d chosen dual rows N (FP64 Gauss-Jordan). theta[v] > 1 selects no
- * facet and returns P0 up to the order of its columns. No reference
- * counterpart (naja samples an already rounded polytope,
+ * keeps at least theta[v] of its own norm. The remaining directions are
+ * completed by `completion`:
+ *   NAJA_FACET_COMPLETE_ROUNDING: the dual rows of the base rounding P0 (the
+ *     rows of P0^-1), greedily by residual;
+ *   NAJA_FACET_COMPLETE_LOWER: the rows of L^-1 for the lower Cholesky factor
+ *     L of P0 P0^T (the same ellipsoid with its axes in triangular order);
+ *   NAJA_FACET_COMPLETE_BASE_FACETS: the base polytope's own dual rows, their
+ *     residuals against everything chosen, the largest first, each made
+ *     orthonormal (FP64 Gram-Schmidt) and kept as a row of N: the free axes
+ *     lie in the subspace the chosen facets leave free, as close to the base
+ *     facets' normals as an orthonormal basis of it gets. P0 is not read (may
+ *     be NULL); include_base must be 0.
+ * exchange_tol (0 = off, else >= 1): when the chosen facets fill the basis,
+ * a facet candidate replaces a basis row while its coefficient in the basis
+ * exceeds exchange_tol in magnitude (maximum-volume exchange: each step
+ * shrinks the parallelotope {|N y| <= 1}). P = N^-1 for the d chosen rows N
+ * (FP64 Gauss-Jordan). theta[v] > 1 selects no facet (with completion
+ * NAJA_FACET_COMPLETE_ROUNDING, P0 up to the order of its columns). No
+ * reference counterpart (naja samples an already rounded polytope,
  * runtime_config.h:64-78).
  * Device memory, the batched-sampler layout as naja_precond_ellipsoid_device:
  * A0 column-major (m0 x d), b0 FP64, extra_A the variants' column-major
  * k_v x d blocks, extra_b FP64; extra_rows on the host; x_c (nullable)
  * n_variants x d; P0 and P n_variants x d x d FP32 row-major; theta
- * n_variants floats. d <= 128 and the candidates in LDS (else
- * NAJA_ERR_UNSUPPORTED). Synchronises the stream (per-variant status:
- * NAJA_ERR_INVALID_ARGUMENT for a centre that is not interior,
- * NAJA_ERR_RUNTIME for a singular P0).
+ * n_variants floats. d <= 128 and at most 512 candidates (facets plus d, or
+ * plus m0 with NAJA_FACET_COMPLETE_BASE_FACETS), else NAJA_ERR_UNSUPPORTED.
+ * Synchronises the stream (per-variant status: NAJA_ERR_INVALID_ARGUMENT for
+ * a centre that is not interior, NAJA_ERR_RUNTIME for a singular P0 or no
+ * independent completion).
  */
 int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
                               const int* extra_rows, const float* extra_A, const double* extra_b,
                               const float* x_c, const float* P0, const float* theta, int include_base,
-                              int lower, float* P, void* stream);
+                              int completion, float exchange_tol, float* P, void* stream);
 
 /*
  * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion

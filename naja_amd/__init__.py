@@ -189,7 +189,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_precond_lowrank.restype = i32
     lib.naja_precond_ellipsoid_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp]
     lib.naja_precond_ellipsoid_device.restype = i32
-    lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp, vp]
+    lib.naja_precond_facet_device.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, i32, i32, ctypes.c_float,
+                                               vp, vp]
     lib.naja_precond_facet_device.restype = i32
     lib.naja_precond_ellipsoid_large.argtypes = [vp, vp, i32, i32, vp, vp, vp]
     lib.naja_precond_ellipsoid_large.restype = i32
@@ -645,24 +646,40 @@ def facet_rounding_large(A, b, theta=0.02, x_c=None, stream=None):
     return P
 
 
-def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=False, lower=False, stream=None):
+# How facet_rounding completes the directions the facets leave free
+# (NAJA_FACET_COMPLETE_* in include/naja_amd.h).
+FACET_COMPLETE_ROUNDING, FACET_COMPLETE_LOWER, FACET_COMPLETE_BASE_FACETS = 0, 1, 2
+
+
+def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=False, lower=False,
+                   completion=None, exchange_tol=0.0, stream=None):
     """Facet-aligned rounding of every variant [A; E_v] on the GPU (FP64
     inside): P (n_variants, d, d) FP32 whose coordinate axes run along the
-    edges of a parallelotope of the variant's binding facets, completed by the
-    dual rows of the base rounding P0 (n_variants, d, d). theta: one float or
-    n_variants floats, the fraction of its norm a facet candidate must keep to
-    be chosen (> 1: no facets, P0 up to the order of its columns). lower:
-    complete from the lower Cholesky factor of P0 P0^T instead of P0."""
+    edges of a parallelotope of the variant's binding facets. The directions
+    the facets leave free are completed by `completion`:
+    FACET_COMPLETE_ROUNDING (default) from the dual rows of the base rounding
+    P0 (n_variants, d, d); FACET_COMPLETE_LOWER (or lower=True) from the
+    lower Cholesky factor of P0 P0^T; FACET_COMPLETE_BASE_FACETS from the base
+    polytope's own facets, orthonormalised in the free subspace (P0 unused,
+    may be None). theta: one float or n_variants floats, the fraction of its
+    norm a facet candidate must keep to be chosen (> 1: no facets).
+    exchange_tol (0 = off, else >= 1): when the facets fill the basis, swap in
+    a facet whose coefficient in the basis exceeds it (maximum-volume
+    exchange)."""
     torch = _torch()
     lib = load_library()
+    if completion is None:
+        completion = FACET_COMPLETE_LOWER if lower else FACET_COMPLETE_ROUNDING
     A_cm, b_d, d, m0, ks, Ae, be = _batch_inputs(A, b, extra_A, extra_b)
     nv = len(ks)
     karr = (ctypes.c_int * nv)(*ks)
-    P0d = _dev(P0, torch.float32).contiguous()
-    if tuple(P0d.shape) != (nv, d, d):
-        raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"P0 {tuple(P0d.shape)}, expected ({nv}, {d}, {d})")
+    P0d = None
+    if completion != FACET_COMPLETE_BASE_FACETS or P0 is not None:
+        P0d = _dev(P0, torch.float32).contiguous()
+        if tuple(P0d.shape) != (nv, d, d):
+            raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"P0 {tuple(P0d.shape)}, expected ({nv}, {d}, {d})")
     th = torch.as_tensor(theta, dtype=torch.float32).reshape(-1)
-    th = (th.expand(nv) if th.numel() == 1 else th).to(P0d.device).contiguous()
+    th = (th.expand(nv) if th.numel() == 1 else th).to(A_cm.device).contiguous()
     if th.numel() != nv:
         raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, f"{th.numel()} theta values for {nv} variants")
     xc = _dev(x_c, torch.float32) if x_c is not None else None
@@ -670,8 +687,9 @@ def facet_rounding(A, b, extra_A, extra_b, P0, theta, x_c=None, include_base=Fal
     _check(lib.naja_precond_facet_device(A_cm.data_ptr(), b_d.data_ptr(), m0, d, nv, karr,
                                          Ae.data_ptr() if Ae is not None else None,
                                          be.data_ptr() if be is not None else None,
-                                         xc.data_ptr() if xc is not None else None, P0d.data_ptr(), th.data_ptr(),
-                                         int(bool(include_base)), int(bool(lower)), P.data_ptr(),
+                                         xc.data_ptr() if xc is not None else None,
+                                         P0d.data_ptr() if P0d is not None else None, th.data_ptr(),
+                                         int(bool(include_base)), int(completion), float(exchange_tol), P.data_ptr(),
                                          _stream_ptr(stream)))
     return P
 

@@ -14,22 +14,27 @@ changes only how fast coordinate hit-and-run mixes):
   Hessian): whitens the body, with the orientation of a Cholesky factor.
 * Facet-aligned rounding (the pipeline's base): coordinate axes along the
   edges of a parallelotope of the variant's slab facets, chosen greedily
-  (naja_amd.facet_rounding), the directions the slabs leave free completed
-  from the dual rows of a base rounding. On an axis-aligned box a coordinate
+  (naja_amd.facet_rounding) and, when the slabs fill the basis, exchanged
+  towards the smallest such parallelotope; the directions the slabs leave
+  free are completed by the base polytope's own facets (the cube's axes),
+  orthonormalised in the free subspace. On an axis-aligned box a coordinate
   move is an exact draw, which a whitened but rotated box does not give.
+  Completing from the dual rows of the Dikin ellipsoid instead (the earlier
+  base, kept as the R-hat fallback) mixes about a fifth slower.
 * Learned update: the MLP turns features into a factor V (dim x 32) of the
   covariance model C = tau (I + V V^T) in the Dikin-ellipsoid frame, trained
   by the Gaussian NLL on covariances of pilot chains the sampler runs (in the
   facet-aligned frame, where they mix); its map P0 (I + V V^T)^{1/2} is the
-  base whose dual rows complete the facet-aligned rounding.
+  rounding whose dual rows complete the learned variant of the facet-aligned
+  rounding.
 
 Acceptance, at two levels. After training, validate_learned runs both
 completions on fresh conditions of the training distribution, and the learned
 map is used only if it loses neither summed min-ESS nor R-hat passes there
 (else the pipeline runs its base and the network costs nothing). After each
 main run, a variant whose split-R-hat is 1.05 or more is run again with the
-fallback (the facet-aligned rounding completed from the Dikin ellipsoid
-alone), and the run with the better R-hat and ESS is kept.
+fallback (the facet-aligned rounding completed from the Dikin ellipsoid),
+and the run with the better R-hat and ESS is kept.
 
 The reference's Dikin rounding (dikin_precondition.cpp:36-183) is reported
 beside them as the drop-in preconditioner.
@@ -44,8 +49,8 @@ import time
 
 import numpy as np
 
-from . import (NAJA_ERR_UNSUPPORTED, ChrPlan, NajaError, VariantBatch, coordinate_hit_and_run_batched,
-               diagnostics_batched, ellipsoid_rounding, facet_rounding, lowrank_maps)
+from . import (FACET_COMPLETE_BASE_FACETS, NAJA_ERR_UNSUPPORTED, ChrPlan, NajaError, VariantBatch,
+               coordinate_hit_and_run_batched, diagnostics_batched, ellipsoid_rounding, facet_rounding, lowrank_maps)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
 
@@ -53,6 +58,9 @@ W_REF = 1e-3  # feature scale: the thinnest half-width of the pilot distribution
 # A facet candidate is chosen while it keeps at least this fraction of its own
 # norm after projection on the chosen ones (every independent slab first).
 THETA = 0.02
+# Maximum-volume exchange: a slab replaces a basis row while its coefficient in
+# the basis exceeds this (each exchange shrinks the parallelotope by that factor).
+EXCHANGE_TOL = 1.02
 
 
 def make_pool(d: int, n: int = 300, seed: int = 2024):
@@ -176,11 +184,16 @@ def base_rounding(A0, b0, EA, Eb):
         return torch.as_tensor(precond.ellipsoid_batched(A0, b0, EA, Eb), dtype=torch.float32).cuda()
 
 
-def facet_maps(batch, P_base, theta=THETA, lower=False):
-    """Facet-aligned rounding of every variant of a VariantBatch, completed
-    from the dual rows of P_base (n_variants, d, d) (lower: of the lower
-    Cholesky factor of P_base P_base^T; measured no better on the held-out
-    batch)."""
+def facet_maps(batch, P_base=None, theta=THETA, lower=False):
+    """Facet-aligned rounding of every variant of a VariantBatch. P_base None
+    (the pipeline's base): the free directions completed by the base
+    polytope's facets, orthonormalised in the free subspace, with the
+    maximum-volume exchange when the slabs fill the basis. Else completed from
+    the dual rows of P_base (n_variants, d, d) (lower: of the lower Cholesky
+    factor of P_base P_base^T)."""
+    if P_base is None:
+        return facet_rounding(batch, None, None, None, None, theta, completion=FACET_COMPLETE_BASE_FACETS,
+                              exchange_tol=EXCHANGE_TOL)
     return facet_rounding(batch, None, None, None, P_base, theta, lower=lower)
 
 
@@ -202,7 +215,7 @@ def pilot_covariances(A0, b0, EA, Eb, P0=None, nchains=32, steps=20000, thin=40,
         hi = min(nv, lo + chunk)
         sub = batch.slice(lo, hi)
         P0c = P0[lo:hi]
-        Y = sample_rounded(sub, None, None, None, facet_maps(sub, P0c), nchains, steps // thin, thin, seed + lo)
+        Y = sample_rounded(sub, None, None, None, facet_maps(sub), nchains, steps // thin, thin, seed + lo)
         Sy, _ = pilot_covariance(Y)
         del Y
         Pinv = torch.linalg.inv(P0c.double())
@@ -246,9 +259,10 @@ def validate_learned(lp: LearnedPreconditioner, A0, b0, pool, n=256, w_lo=1e-3, 
                      nchains=4, nspc=12500, thin=16, resync=1000):
     """Model-level acceptance of the trained network, on n fresh conditions of
     the training distribution (not the evaluation variants): full-length runs
-    of the facet-aligned rounding completed from the learned map and from the
-    Dikin ellipsoid alone. The learned map is accepted when it loses neither
-    summed min-ESS nor variants passing split-R-hat < 1.05."""
+    of the facet-aligned rounding completed from the learned map and of the
+    pipeline's base (completed from the base polytope's facets). The learned
+    map is accepted when it loses neither summed min-ESS nor variants passing
+    split-R-hat < 1.05."""
     import torch
 
     EA, Eb, F, _ = sample_conditions(pool, n, 300, w_lo, w_hi, seed=seed)
@@ -256,7 +270,7 @@ def validate_learned(lp: LearnedPreconditioner, A0, b0, pool, n=256, w_lo=1e-3, 
     P0 = base_rounding(batch, None, None, None)
     PL, _ = learned_rounding(lp, torch.as_tensor(F, dtype=torch.float32).cuda(), batch, P0)
     out = {}
-    for name, P in (("learned", PL), ("base", facet_maps(batch, P0))):
+    for name, P in (("learned", PL), ("base", facet_maps(batch))):
         Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
         e, r = variant_diagnostics(Y, nchains)
         del Y
@@ -312,17 +326,18 @@ def timed_steps(step, warmup, steps):
 
 
 def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resync=1000,
-            methods=("plain", "dikin", "ellipsoid", "facet", "learned"), warmup=1, steps=1, log=None,
+            methods=("plain", "dikin", "ellipsoid", "facet_ellipsoid", "facet", "learned"), warmup=1, steps=1, log=None,
             use_network=True):
     """Sampling methods on the same variants, chains and steps, each timed
     the same way (timed_steps) from the polytopes on the device to samples +
     diagnostics on the device, including its preconditioner:
     plain CHR; the reference's Dikin rounding (dikin_precondition, host
     FP64); the Dikin ellipsoid; the facet-aligned rounding completed from the
-    Dikin ellipsoid (the method without the network); the learned-rounded
-    pipeline (facet-aligned rounding completed from the learned map, or from
-    the Dikin ellipsoid when use_network is False, R-hat acceptance). Returns
-    per-method results."""
+    Dikin ellipsoid (facet_ellipsoid); the facet-aligned rounding completed
+    from the base polytope's facets with the exchange (facet: the method
+    without the network); the learned-rounded pipeline (facet-aligned
+    rounding completed from the learned map, or the base when use_network is
+    False, R-hat acceptance). Returns per-method results."""
     import torch
 
     batch = VariantBatch(A0, b0, EA, Eb)  # the polytopes go to the device once, before any timing
@@ -345,7 +360,7 @@ def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resyn
         if use_network:
             P, _ = learned_rounding(lp, Fd, batch)
         else:  # the network was rejected by validate_learned: the step runs its base
-            P = facet_maps(batch, base_rounding(batch, None, None, None))
+            P = facet_maps(batch)
         _, e, r, n_again = sample_accepted(batch, P, facet_fallback, nchains, nspc, thin, seed, resync)
         return e, r, n_again
 
@@ -353,7 +368,8 @@ def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resyn
         "plain": plain,
         "dikin": rounded(lambda: precond.dikin_batched(A0, b0, EA, Eb)[0]),
         "ellipsoid": rounded(lambda: base_rounding(batch, None, None, None)),
-        "facet": rounded(lambda: facet_maps(batch, base_rounding(batch, None, None, None))),
+        "facet_ellipsoid": rounded(lambda: facet_maps(batch, base_rounding(batch, None, None, None))),
+        "facet": rounded(lambda: facet_maps(batch)),
         "learned": learned,
     }
     out = {}

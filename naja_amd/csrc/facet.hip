@@ -14,23 +14,40 @@
 // candidate facets are chosen greedily by modified Gram-Schmidt: at every
 // step the candidate with the largest residual norm, first among the
 // variant's extra rows (and, on request, its base rows) while that candidate
-// keeps at least theta_v of its own norm, then among the dual rows of a base
-// rounding P0 (the rows of P0^-1), which complete the basis in the
-// directions the facets leave free (with `lower`, the rows of L^-1 for the
-// lower Cholesky factor L of P0 P0^T instead: the same ellipsoid with its
-// axes in triangular order, whose last axes lie along the coordinate axes).
-// With N the d chosen dual rows,
-// P = N^-1: every chosen facet is a coordinate slab |z_k| <= ~1. A diagonal
-// rescaling of P would not change the chain in y, and neither would a
-// permutation (up to which coordinate a draw picks), so only the choice of
-// rows matters. theta_v > 1 takes no facet rows and gives P0 up to the order
-// of its columns.
+// keeps at least theta_v of its own norm. The directions the facets leave
+// free are completed in one of three ways:
+//   * from the dual rows of a base rounding P0 (the rows of P0^-1), chosen by
+//     the same greedy residual rule;
+//   * from the rows of L^-1 for the lower Cholesky factor L of P0 P0^T (the
+//     same ellipsoid with its axes in triangular order);
+//   * from the base polytope's own facets, orthonormalised: the base dual
+//     rows' residuals against everything chosen so far, the largest first,
+//     each normalised and kept as it is (FP64 classical Gram-Schmidt, twice).
+//     These rows lie in the free subspace V (orthogonal to the chosen
+//     facets), so a move along a chosen facet's axis stays in V's complement,
+//     a free move stays in V, and the free axes are the base facets' normals
+//     made orthonormal in the order of their residuals: on a slice of a box,
+//     as near to the box's own axes as an orthonormal basis of V gets.
+// With N the d chosen rows, P = N^-1: every chosen facet is a coordinate slab
+// |z_k| <= ~1. A diagonal rescaling of P would not change the chain in y,
+// and neither would a permutation (up to which coordinate a draw picks), so
+// only the choice of rows matters. theta_v > 1 takes no facet rows.
+//
+// Exchange (when the chosen facets fill the basis): every facet candidate is
+// expressed in the basis, C = U N^-1, and while some |C_ik| exceeds the
+// tolerance the basis row k is replaced by candidate i (one rank-1 update of
+// C per exchange). Each exchange multiplies |det N| by |C_ik|, so the
+// parallelotope {|N y| <= 1} shrinks towards the smallest one spanned by the
+// variant's facets (the maximum-volume submatrix rule of Goreinov,
+// Oseledets, Savostyanov, Tyrtyshnikov and Zamarashkin, "How to find a good
+// submatrix", 2010), and the facets left out cut it less deeply.
 //
 // Per variant, one workgroup of 512 threads: P0^-1 and N^-1 by Gauss-Jordan
 // with partial pivoting in LDS (FP64); the selection on FP32 residuals, one
 // candidate per thread held in registers (so a Gram-Schmidt pass is a dot
 // product and an update against the current direction broadcast from LDS);
-// the chosen rows are re-formed in FP64 for the inverse.
+// the chosen rows are re-formed in FP64 for the inverse; the exchange on
+// FP32 coefficient rows, one per thread.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -142,18 +159,19 @@ __device__ bool gj_inverse(double* M, int d, double* col, int* piv, int* flag)
 
 // One workgroup per variant, one candidate per thread: thread i holds the
 // FP32 residual of candidate i in registers (COLS >= d columns, the rest 0).
-// P0i: per-variant global scratch for the FP64 rows of P0^-1.
+// P0i: per-variant global scratch for the FP64 rows of P0^-1 (completions 0
+// and 1).
 template <int COLS>
 __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
     const float* __restrict__ A0, const double* __restrict__ b0, int m0, int d, const float* __restrict__ Ae,
     const double* __restrict__ be, const int64_t* __restrict__ eoff, const int* __restrict__ ek,
     const float* __restrict__ xc, const float* __restrict__ P0, const float* __restrict__ theta, int include_base,
-    int lower, double* __restrict__ P0iall, float* __restrict__ P, int* __restrict__ status)
+    int completion, float exchange_tol, double* __restrict__ P0iall, float* __restrict__ P, int* __restrict__ status)
 {
     extern __shared__ double sh[];
-    double* M = sh;                                 // d x d: P0, then N
+    double* M = sh;                                 // d x d: P0, then N (or Q, then N)
     double* col = M + d * d;                        // d
-    double* sl = col + d;                           // kFacetThreads: slack of each facet candidate
+    double* sl = col + d;                           // kFacetThreads: slack of each candidate with a row
     float* q = reinterpret_cast<float*>(sl + kFacetThreads);  // COLS: the current direction
     int* sel = reinterpret_cast<int*>(q + COLS);    // d: chosen candidates in order
     int* piv = sel + d;                             // d
@@ -161,12 +179,14 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
     __shared__ int ri[kFacetThreads / 64];
     __shared__ int flag;
     __shared__ double xs[kFacetMaxD];
+    __shared__ double dots[kFacetMaxD];
 
     const int v = blockIdx.x, tid = threadIdx.x;
     const int k = ek[v];
     const int64_t off = eoff[v];
+    const bool base_facets = completion == NAJA_FACET_COMPLETE_BASE_FACETS;
     const int nf = (include_base ? m0 : 0) + k;  // facet candidates
-    const int nr = nf + d;                        // + the base rounding's dual rows
+    const int nr = nf + (base_facets ? m0 : d);   // + the completion candidates
     double* P0i = P0iall + (int64_t)v * d * d;
     const float th = theta[v];
     const float th2 = th * th;
@@ -174,70 +194,82 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
         return r < m0 ? (double)A0[(int64_t)c * m0 + r] : (double)Ae[off * d + (int64_t)c * k + (r - m0)];
     };
     auto facet_row = [&](int i) { return include_base ? i : m0 + i; };
+    // Polytope row of candidate i (facets, then base rows as completions), or -1 for a P0^-1 row.
+    auto poly_row = [&](int i) { return i < nf ? facet_row(i) : (base_facets ? i - nf : -1); };
 
     for (int c = tid; c < d; c += kFacetThreads) xs[c] = xc ? (double)xc[(int64_t)v * d + c] : 0.0;
     if (tid == 0) flag = 0;
     __syncthreads();
-    if (tid < nf) {
-        const int r = facet_row(tid);
+    if (tid < nr && poly_row(tid) >= 0) {
+        const int r = poly_row(tid);
         double ax = 0.0;
         for (int c = 0; c < d; ++c) ax = fma(arow(r, c), xs[c], ax);
         const double s = (r < m0 ? b0[r] : be[off + (r - m0)]) - ax;
         if (!(s > 0.0)) flag = 1;
         sl[tid] = s;
     }
-    const float* P0v = P0 + (int64_t)v * d * d;
-    if (lower) {
-        // The same ellipsoid with its axes in triangular order: C = P0 P0^T in
-        // FP64 (row-major; P0 read through the L1/L2 caches), then its lower
-        // Cholesky factor in place.
-        for (int e = tid; e < d * d; e += kFacetThreads) {
-            const int i = e / d, j = e % d;
-            double c = 0.0;
-            if (j <= i)
-                for (int t = 0; t < d; ++t) c = fma((double)P0v[(int64_t)i * d + t], (double)P0v[(int64_t)j * d + t], c);
-            M[e] = c;
-        }
-        __syncthreads();
-        for (int j = 0; j < d; ++j) {
-            if (tid == 0) {
-                double s = M[j * d + j];
-                for (int t = 0; t < j; ++t) s -= M[j * d + t] * M[j * d + t];
-                if (!(s > 0.0)) flag = 2;
-                M[j * d + j] = sqrt(s > 0.0 ? s : 1.0);
+    if (!base_facets) {
+        const float* P0v = P0 + (int64_t)v * d * d;
+        if (completion == NAJA_FACET_COMPLETE_LOWER) {
+            // The same ellipsoid with its axes in triangular order: C = P0 P0^T in
+            // FP64 (row-major; P0 read through the L1/L2 caches), then its lower
+            // Cholesky factor in place.
+            for (int e = tid; e < d * d; e += kFacetThreads) {
+                const int i = e / d, j = e % d;
+                double c = 0.0;
+                if (j <= i)
+                    for (int t = 0; t < d; ++t)
+                        c = fma((double)P0v[(int64_t)i * d + t], (double)P0v[(int64_t)j * d + t], c);
+                M[e] = c;
             }
             __syncthreads();
-            const double ljj = M[j * d + j];
-            for (int i = j + 1 + tid; i < d; i += kFacetThreads) {
-                double s = M[i * d + j];
-                for (int t = 0; t < j; ++t) s -= M[i * d + t] * M[j * d + t];
-                M[i * d + j] = s / ljj;
+            for (int j = 0; j < d; ++j) {
+                if (tid == 0) {
+                    double s = M[j * d + j];
+                    for (int t = 0; t < j; ++t) s -= M[j * d + t] * M[j * d + t];
+                    if (!(s > 0.0)) flag = 2;
+                    M[j * d + j] = sqrt(s > 0.0 ? s : 1.0);
+                }
+                __syncthreads();
+                const double ljj = M[j * d + j];
+                for (int i = j + 1 + tid; i < d; i += kFacetThreads) {
+                    double s = M[i * d + j];
+                    for (int t = 0; t < j; ++t) s -= M[i * d + t] * M[j * d + t];
+                    M[i * d + j] = s / ljj;
+                }
+                __syncthreads();
             }
-            __syncthreads();
+            for (int e = tid; e < d * d; e += kFacetThreads)
+                if (e % d > e / d) M[e] = 0.0;
+        } else {
+            for (int e = tid; e < d * d; e += kFacetThreads) M[e] = (double)P0v[e];
         }
-        for (int e = tid; e < d * d; e += kFacetThreads)
-            if (e % d > e / d) M[e] = 0.0;
-    } else {
-        for (int e = tid; e < d * d; e += kFacetThreads) M[e] = (double)P0v[e];
     }
     __syncthreads();
     if (flag) {
         if (tid == 0) status[v] = flag == 1 ? NAJA_ERR_INVALID_ARGUMENT : NAJA_ERR_RUNTIME;
         return;
     }
-    if (!gj_inverse(M, d, col, piv, &flag)) {  // M = P0^-1: its rows complete the basis
-        if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
-        return;
+    if (!base_facets) {
+        if (!gj_inverse(M, d, col, piv, &flag)) {  // M = P0^-1: its rows complete the basis
+            if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+            return;
+        }
+        for (int e = tid; e < d * d; e += kFacetThreads) P0i[e] = M[e];
     }
-    for (int e = tid; e < d * d; e += kFacetThreads) P0i[e] = M[e];
+    // FP64 dual row of candidate i (after P0^-1 is in P0i).
+    auto cand64 = [&](int i, int c) -> double {
+        const int r = poly_row(i);
+        return r >= 0 ? arow(r, c) / sl[i] : P0i[(int64_t)(i - nf) * d + c];
+    };
 
     // This thread's candidate.
     float r[COLS];
     float n0 = 0.0f;
     const bool live = tid < nr;
     const bool is_facet = tid < nf;
-    if (is_facet) {
-        const int rr = facet_row(tid);
+    if (live && poly_row(tid) >= 0) {
+        const int rr = poly_row(tid);
         const double w = 1.0 / sl[tid];
 #pragma unroll
         for (int c = 0; c < COLS; ++c) r[c] = c < d ? (float)(arow(rr, c) * w) : 0.0f;
@@ -274,7 +306,7 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
                     vf = n2;
                     i_f = tid;
                 }
-            } else {
+            } else if (!base_facets || n2 > 1e-12f * n0) {
                 vb = n2;
                 i_b = tid;
             }
@@ -299,11 +331,104 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
         __syncthreads();
     }
 
-    // N (the chosen dual rows, FP64) and P = N^-1.
-    for (int e = tid; e < d * d; e += kFacetThreads) {
-        const int row = e / d, c = e % d;
-        const int i = sel[row];
-        M[e] = i < nf ? arow(facet_row(i), c) / sl[i] : P0i[(int64_t)(i - nf) * d + c];
+    // Exchange, when the chosen facets fill the basis: C = U N^-1 for the facet
+    // candidates (FP32 rows, one per thread, summed in column order), then
+    // rank-1 updates while the largest |C_ik| exceeds the tolerance.
+    int nfac = 0;
+    for (int t = 0; t < d; ++t) nfac += sel[t] < nf;
+    if (exchange_tol > 0.0f) {
+        if (nfac == d) {
+            for (int e = tid; e < d * d; e += kFacetThreads) M[e] = cand64(sel[e / d], e % d);
+            __syncthreads();
+            if (!gj_inverse(M, d, col, piv, &flag)) {
+                if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+                return;
+            }
+#pragma unroll
+            for (int c = 0; c < COLS; ++c) r[c] = 0.0f;
+            if (is_facet)
+                for (int j = 0; j < d; ++j) {
+                    const float uj = (float)cand64(tid, j);
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c)
+                        if (c < d) r[c] = r[c] + uj * (float)M[j * d + c];
+                }
+            for (int it = 0; it < 4 * d; ++it) {
+                float best = -1.0f;
+                int bi = 0x7fffffff;
+                if (is_facet)
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c) {
+                        const float a = fabsf(r[c]);
+                        if (c < d && a > best) {
+                            best = a;
+                            bi = tid * kFacetMaxD + c;
+                        }
+                    }
+                block_argmax(best, bi, rv, ri);
+                if (!(best > exchange_tol)) break;
+                const int pi = bi / kFacetMaxD, pk = bi % kFacetMaxD;
+                __syncthreads();  // rv / ri read; q free
+                if (tid == pi)
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c) q[c] = r[c];
+                __syncthreads();
+                if (is_facet) {
+                    float cjk = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c) cjk = c == pk ? r[c] : cjk;
+                    const float f = cjk / q[pk];
+#pragma unroll
+                    for (int c = 0; c < COLS; ++c) r[c] = r[c] - f * (q[c] - (c == pk ? 1.0f : 0.0f));
+                }
+                if (tid == 0) sel[pk] = pi;
+                __syncthreads();
+            }
+        }
+    }
+
+    if (base_facets && nfac < d) {
+        // Q (LDS M): orthonormal rows of everything chosen, in order, by
+        // classical Gram-Schmidt twice in FP64; a completion row of N is its
+        // Q row, a facet row of N its dual row.
+        for (int t = 0; t < d; ++t) {
+            const int i = sel[t];
+            for (int c = tid; c < d; c += kFacetThreads) col[c] = cand64(i, c);
+            __syncthreads();
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int j = tid; j < t; j += kFacetThreads) {
+                    double s = 0.0;
+                    for (int c = 0; c < d; ++c) s = fma(M[j * d + c], col[c], s);
+                    dots[j] = s;
+                }
+                __syncthreads();
+                for (int c = tid; c < d; c += kFacetThreads) {
+                    double w = col[c];
+                    for (int j = 0; j < t; ++j) w = fma(-dots[j], M[j * d + c], w);
+                    col[c] = w;
+                }
+                __syncthreads();
+            }
+            if (tid < 64) {
+                double s = 0.0;
+                for (int c = tid; c < d; c += 64) s = fma(col[c], col[c], s);
+                for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+                if (tid == 0) dots[kFacetMaxD - 1] = s;
+            }
+            __syncthreads();
+            const double nrm = sqrt(dots[kFacetMaxD - 1]);
+            if (!(nrm > 0.0)) {
+                if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
+                return;
+            }
+            for (int c = tid; c < d; c += kFacetThreads) M[t * d + c] = col[c] / nrm;
+            __syncthreads();
+        }
+        for (int e = tid; e < d * d; e += kFacetThreads)
+            if (sel[e / d] < nf) M[e] = cand64(sel[e / d], e % d);
+    } else {
+        // N (the chosen dual rows, FP64).
+        for (int e = tid; e < d * d; e += kFacetThreads) M[e] = cand64(sel[e / d], e % d);
     }
     __syncthreads();
     if (!gj_inverse(M, d, col, piv, &flag)) {
@@ -327,12 +452,21 @@ size_t facet_lds_bytes(int d, int cols)
 extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
                                          const int* extra_rows, const float* extra_A, const double* extra_b,
                                          const float* x_c, const float* P0, const float* theta, int include_base,
-                                         int lower, float* P, void* stream)
+                                         int completion, float exchange_tol, float* P, void* stream)
 {
     using namespace naja;
     return guarded([&]() {
-        NAJA_REQUIRE(A0 && b0 && P0 && theta && P && m0 > 0 && d > 0 && n_variants > 0 && extra_rows,
+        const bool base_facets = completion == NAJA_FACET_COMPLETE_BASE_FACETS;
+        NAJA_REQUIRE(A0 && b0 && theta && P && m0 > 0 && d > 0 && n_variants > 0 && extra_rows &&
+                         (P0 || base_facets),
                      NAJA_ERR_INVALID_ARGUMENT, "naja_precond_facet_device: bad arguments");
+        NAJA_REQUIRE(completion == NAJA_FACET_COMPLETE_ROUNDING || completion == NAJA_FACET_COMPLETE_LOWER ||
+                         base_facets,
+                     NAJA_ERR_INVALID_ARGUMENT, "naja_precond_facet_device: unknown completion %d", completion);
+        NAJA_REQUIRE(!(base_facets && include_base), NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_precond_facet_device: base rows cannot be both facet candidates and the completion");
+        NAJA_REQUIRE(exchange_tol == 0.0f || exchange_tol >= 1.0f, NAJA_ERR_INVALID_ARGUMENT,
+                     "naja_precond_facet_device: exchange tolerance %g (0 = off, else >= 1)", (double)exchange_tol);
         std::vector<int64_t> off(n_variants);
         int64_t tot = 0;
         int kmax = 0;
@@ -345,17 +479,17 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
         }
         NAJA_REQUIRE(tot == 0 || (extra_A && extra_b), NAJA_ERR_INVALID_ARGUMENT,
                      "naja_precond_facet_device: extra rows without extra_A/extra_b");
-        const int nr_max = (include_base ? m0 : 0) + kmax + d;
+        const int nr_max = (include_base ? m0 : 0) + kmax + (base_facets ? m0 : d);
         const int cols = d <= 32 ? 32 : (d <= 64 ? 64 : (d <= 112 ? 112 : 128));
         const size_t lds = facet_lds_bytes(d, cols);
         NAJA_REQUIRE(d <= kFacetMaxD && nr_max <= kFacetThreads && lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
-                     "naja_precond_facet_device: d=%d with %d candidate rows exceeds the kernel (d <= %d, rows + d "
-                     "<= %d)",
-                     d, nr_max - d, kFacetMaxD, kFacetThreads);
+                     "naja_precond_facet_device: d=%d with %d candidate rows exceeds the kernel (d <= %d, %d "
+                     "candidates)",
+                     d, nr_max, kFacetMaxD, kFacetThreads);
         hipStream_t s = (hipStream_t)stream;
         const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
         const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
-        const size_t b_P0i = sizeof(double) * (size_t)n_variants * d * d;
+        const size_t b_P0i = base_facets ? 0 : sizeof(double) * (size_t)n_variants * d * d;
         HeldBuf& hb = held_buf();
         std::lock_guard<std::mutex> hold(hb.mu);
         char* mem = held_reserve(hb, b_off + 2 * b_k + b_P0i);
@@ -370,7 +504,7 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
                     : cols == 112 ? facet_kernel<112>
                                   : facet_kernel<128>;  // 113..128 columns spill to scratch
         hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), lds, s, A0, b0, m0, d, extra_A, extra_b,
-                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, lower ? 1 : 0, P0i, P, status);
+                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, completion, exchange_tol, P0i, P, status);
         NAJA_HIP(hipGetLastError());
         int* st = held_host(hb, (size_t)n_variants);
         NAJA_HIP(hipMemcpyAsync(st, status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));

@@ -89,53 +89,104 @@ def lowrank(V, expand=False):
     return (Q * (np.sqrt(ev) if expand else 1.0 / np.sqrt(ev))) @ Q.T
 
 
-def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False, lower=False):
+COMPLETE_ROUNDING, COMPLETE_LOWER, COMPLETE_BASE_FACETS = 0, 1, 2
+
+
+def facet(A0, b0, E, f, P0, theta, x_c=None, include_base=False, lower=False, completion=None, exchange_tol=0.0):
     """Facet-aligned rounding of [A0; E] y <= [b0; f] (the rule of
     naja_precond_facet_device): candidate dual rows u_r = a_r / s_r at x_c of
     the extra rows (and the base rows with include_base), chosen greedily by
     modified Gram-Schmidt residual norm (FP32 residuals) while the best keeps
-    at least theta of its own norm, then the rows of P0^-1 (FP64; with lower,
-    of chol(P0 P0^T)^-1); returns
+    at least theta of its own norm, then the completion candidates: the rows
+    of P0^-1 (FP64; COMPLETE_LOWER or lower: of chol(P0 P0^T)^-1), or with
+    COMPLETE_BASE_FACETS the base rows' dual rows, whose chosen residuals are
+    made orthonormal (FP64 classical Gram-Schmidt, twice, in the order of
+    choice) and kept as rows of N. exchange_tol > 0: when the facets fill the
+    basis, C = U N^-1 in FP32 (summed in column order) and, while the largest
+    |C_ik| (first in row-major order) exceeds exchange_tol, basis row k
+    becomes candidate i, C -= (C[:, k] / C_ik) (C_i - e_k) in FP32. Returns
     (P = N^-1 in FP64, the chosen candidate indices, N). Ties go to the
     smaller index."""
+    if completion is None:
+        completion = COMPLETE_LOWER if lower else COMPLETE_ROUNDING
     A0 = np.asarray(A0, dtype=np.float64)
     d = A0.shape[1]
+    m0 = A0.shape[0]
     E = np.asarray(E, dtype=np.float64).reshape(-1, d)
     A = np.vstack([A0, E])
     b = np.concatenate([np.asarray(b0, np.float64), np.asarray(f, np.float64).reshape(-1)])
     x_c = np.zeros(d) if x_c is None else np.asarray(x_c, dtype=np.float64)
-    rows = np.arange(A.shape[0]) if include_base else np.arange(A0.shape[0], A.shape[0])
+    rows = np.arange(A.shape[0]) if include_base else np.arange(m0, A.shape[0])
     s = b[rows] - A[rows] @ x_c
     if not (s > 0).all():
         raise ValueError("facet rounding: the centre is not interior")
     U = A[rows] / s[:, None]
-    P0 = np.asarray(P0, dtype=np.float64)
-    if lower:  # the lower Cholesky factor of the same ellipsoid (LAPACK)
-        P0 = np.linalg.cholesky(P0 @ P0.T)
-    P0i = np.linalg.inv(P0)
-    cand = np.vstack([U, P0i])
+    if completion == COMPLETE_BASE_FACETS:
+        if include_base:
+            raise ValueError("facet rounding: base rows cannot be both facets and the completion")
+        s0 = b[:m0] - A0 @ x_c
+        if not (s0 > 0).all():
+            raise ValueError("facet rounding: the centre is not interior")
+        comp = A0 / s0[:, None]
+    else:
+        P0 = np.asarray(P0, dtype=np.float64)
+        if completion == COMPLETE_LOWER:  # the lower Cholesky factor of the same ellipsoid (LAPACK)
+            P0 = np.linalg.cholesky(P0 @ P0.T)
+        comp = np.linalg.inv(P0)
+    cand = np.vstack([U, comp])
     nf = U.shape[0]
     R = cand.astype(np.float32)
     n0 = (R * R).sum(1)
     taken = np.zeros(len(cand), bool)
     th2 = np.float32(theta) * np.float32(theta)
+    tiny = np.float32(1e-12)
     sel = []
     n2 = n0.copy()
     for _ in range(d):
-        elig_f = (~taken[:nf]) & (n2[:nf] >= th2 * n0[:nf]) & (n2[:nf] > np.float32(1e-12) * n0[:nf])
+        elig_f = (~taken[:nf]) & (n2[:nf] >= th2 * n0[:nf]) & (n2[:nf] > tiny * n0[:nf])
         if elig_f.any():
             p = int(np.argmax(np.where(elig_f, n2[:nf], -1.0)))
         else:
-            n2b = np.where(taken[nf:], -1.0, n2[nf:])
-            p = nf + int(np.argmax(n2b))
+            elig_b = ~taken[nf:]
+            if completion == COMPLETE_BASE_FACETS:
+                elig_b &= n2[nf:] > tiny * n0[nf:]
+            if not elig_b.any():
+                raise RuntimeError("facet rounding: no independent completion")
+            p = nf + int(np.argmax(np.where(elig_b, n2[nf:], -1.0)))
         taken[p] = True
         sel.append(p)
         q = R[p] / np.sqrt(n2[p])
         c = R @ q
         R = R - np.outer(c, q).astype(np.float32)
         n2 = (R * R).sum(1)
-    N = cand[sel]
-    return np.linalg.inv(N), np.array(sel), N
+    sel = np.array(sel)
+    if exchange_tol > 0 and (sel < nf).all():
+        Ni32 = np.linalg.inv(U[sel]).astype(np.float32)
+        U32 = U.astype(np.float32)
+        C = np.zeros((nf, d), np.float32)
+        for j in range(d):
+            C = C + U32[:, j:j + 1] * Ni32[j][None, :]
+        for _ in range(4 * d):
+            flat = int(np.argmax(np.abs(C)))
+            i, k = divmod(flat, d)
+            if not abs(C[i, k]) > np.float32(exchange_tol):
+                break
+            fcol = C[:, k] / C[i, k]
+            row = C[i].copy()
+            row[k] = row[k] - np.float32(1.0)
+            C = C - np.outer(fcol, row).astype(np.float32)
+            sel[k] = i
+    N = cand[sel].copy()
+    if completion == COMPLETE_BASE_FACETS and not (sel < nf).all():
+        Q = np.zeros((d, d))
+        for t in range(d):
+            w = cand[sel[t]].copy()
+            for _ in range(2):
+                w = w - (Q[:t] @ w) @ Q[:t]
+            Q[t] = w / np.linalg.norm(w)
+            if sel[t] >= nf:
+                N[t] = Q[t]
+    return np.linalg.inv(N), sel, N
 
 
 def facet_ordered(A, b, theta, x_c=None):

@@ -53,6 +53,49 @@ def test_facet_rounding_matches_oracle(gpu, lower):
             assert nfac == d
 
 
+@pytest.mark.parametrize("exchange_tol", [0.0, 1.02])
+def test_facet_rounding_base_facets_matches_oracle(gpu, exchange_tol):
+    """Completion from the base polytope's facets (orthonormalised in the free
+    subspace), with and without the maximum-volume exchange, against the
+    oracle's restatement: the same chosen rows (P equal to FP32 rounding),
+    completion rows orthonormal and orthogonal to the chosen facets, and
+    after the exchange no facet whose coefficient in the basis exceeds the
+    tolerance. d = 40 with 0..150 slabs, and the conditioned-cube shape
+    d = 100 with up to 300 extra rows (500 candidates)."""
+    import naja_amd
+
+    t = gpu
+    for d, ks, seed in ((40, [0, 10, 36, 80, 120, 200, 300], 5), (100, [0, 60, 150, 198, 240, 300], 6)):
+        A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
+        EA, Eb = conditions(d, ks, seed=seed)
+        batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
+        P = naja_amd.facet_rounding(batch, None, None, None, None, 0.02,
+                                    completion=naja_amd.FACET_COMPLETE_BASE_FACETS, exchange_tol=exchange_tol)
+        t.cuda.synchronize()
+        P = P.cpu().numpy().astype(np.float64)
+        for v, k in enumerate(ks):
+            Pref, sel, N = op.facet(A0, b0, EA[v], Eb[v], None, 0.02, completion=op.COMPLETE_BASE_FACETS,
+                                    exchange_tol=exchange_tol)
+            fac = sel < k
+            err = np.abs(P[v] - Pref).max() / np.abs(Pref).max()
+            ident = np.abs(N @ P[v] - np.eye(d)).max()
+            print(f"[facet-base d{d} v{v}] k={k} facets={int(fac.sum())} tol={exchange_tol} rel err {err:.2e} "
+                  f"|NP - I| {ident:.2e}", flush=True)
+            assert err <= 1e-5 and ident <= 1e-4
+            F = N[~fac]
+            if len(F):
+                assert np.abs(F @ F.T - np.eye(len(F))).max() <= 1e-12
+                if fac.any():
+                    assert np.abs(F @ N[fac].T).max() <= 1e-12 * np.abs(N[fac]).max()
+            if k == 0:  # the cube keeps its own axes
+                assert np.allclose(np.abs(P[v]).max(axis=0), 1.0, atol=1e-6)
+                assert np.allclose((np.abs(P[v]) > 0.5).sum(axis=0), 1)
+            if exchange_tol and fac.all():
+                U = EA[v] / Eb[v][:, None]
+                C = U @ np.linalg.inv(N)
+                assert np.abs(C).max() <= exchange_tol * (1 + 1e-3)
+
+
 def test_facet_rounding_with_base_rows_and_centre(gpu):
     """include_base: the base polytope's own rows are candidates too (a
     polytope with no extra rows, as a model directory holds it), at an

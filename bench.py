@@ -131,12 +131,16 @@ def main():
     def fallback(sub, idx):
         return cd.facet_maps(sub, cd.base_rounding(sub, None, None, None))
 
+    # Rejected by validation: the step is the base pipeline, one replay of a
+    # persistent hipGraph (rounding, transform, CHR + backmap, diagnostics),
+    # with event nodes around its CHR launch; no network work.
+    run = None if use_net else cd.base_run(batch, nch, nspc, thin, seed, resync, timed=True)
+
     def learned_step():
         if use_net:
             P, _ = cd.learned_rounding(lp, Fh_dev, batch)
-        else:  # rejected by validation: the base rounding, no network work
-            P = cd.facet_maps(batch)
-        return cd.sample_accepted(batch, P, fallback, nch, nspc, thin, seed, resync)
+            return cd.sample_accepted(batch, P, fallback, nch, nspc, thin, seed, resync)
+        return cd.sample_accepted(batch, None, fallback, nch, nspc, thin, seed, resync, first=cd.graph_first(run))
 
     # The other methods on the same variants, chains and steps, timed the same
     # way (warmup, then the mean of `steps` synchronised repetitions).
@@ -154,14 +158,17 @@ def main():
     torch.cuda.synchronize()
     naja_amd.chr_kernel_times()
     naja_amd.chr_kernel_timing(True)
+    graph_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         Y, e, r, n_again = learned_step()
+        if run is not None:
+            graph_ms.append(run.sampling_ms())
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
     naja_amd.chr_kernel_timing(False)
     launches = naja_amd.chr_kernel_times()
-    chr_ms = [ms for items, ms in launches if items == nv * nch]
+    chr_ms = graph_ms if run is not None else [ms for items, ms in launches if items == nv * nch]
     chr_kernel_ms = float(np.mean(chr_ms)) if chr_ms else float("nan")
     ok = r < 1.05
     value = float(e[ok].sum()) / sec
@@ -194,7 +201,8 @@ def main():
                 "launches_timed": len(chr_ms), "bytes_per_chain_step_mean": round(total_bytes / (nv * nch * steps_per_chain), 1),
                 "chain_steps": int(nv * nch * steps_per_chain),
                 "note": "achieved: the algorithm's column + sample bytes / the CHR kernel's own time (HIP events "
-                        "on its launch stream); traffic: FETCH_SIZE + WRITE_SIZE per launch (the paired-row layout "
+                        "around the launch: event nodes of the step's graph); traffic: FETCH_SIZE + WRITE_SIZE per "
+                        "launch (the paired-row layout "
                         "packs each [row; -row] pair once and the variant's A sits in LDS, so about 4 % of the "
                         "algorithmic bytes cross HBM) and issue_active_frac: SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES, "
                         "both from profiles/chr_main_launch_traffic.json (one wavefront per SIMD: instruction "
@@ -240,8 +248,8 @@ def main():
                         f"(half-widths 1e-3..1e-1), {nch} chains x {nspc} x thin {thin}, one batched launch; "
                         f"learned-rounded (facet-aligned rounding completed from the learned low-rank update of "
                         f"the Dikin ellipsoid when validation accepts the network, else the base: completed from "
-                        f"the cube's own facets, orthonormalised, with the maximum-volume exchange; R-hat "
-                        f"acceptance with fallback)",
+                        f"the cube's own facets, orthonormalised, with the maximum-volume exchange, the whole step "
+                        f"one replay of a persistent hipGraph; R-hat acceptance with fallback)",
             "variants": nv, "chains": nch, "nspc": nspc, "thin": thin, "resync": resync,
             "n_pass_rhat_1_05": int(ok.sum()), "min_ess_sum": round(float(e.sum()), 1),
             "min_ess_sum_passing": round(float(e[ok].sum()), 1), "rerun_with_fallback": n_again,

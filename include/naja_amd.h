@@ -464,6 +464,45 @@ int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, 
                               int completion, float exchange_tol, float* P, void* stream);
 
 /*
+ * Conditioned-variant run as one persistent hipGraph (no reference
+ * counterpart: naja samples one model per job, job_sampling.cu:315-379, and
+ * rounds it offline). A step is: the facet-aligned rounding of every variant
+ * (naja_precond_facet_device with NAJA_FACET_COMPLETE_BASE_FACETS, theta,
+ * exchange_tol, centre at the origin), the rounding set on a sampling plan
+ * (naja_chr_plan_set_rounding), the batched CHR launch of every variant from
+ * z = y0 with its backmap y = c + P z into `samples`, and the batched ESS and
+ * split-R-hat (naja_diag_batched, interleaved layout) into ess, rhat (per
+ * variant and dim), ess_min and rhat_max (per variant). Creation allocates
+ * everything the step uses (tables, status words, the plan's packs, the
+ * diagnostics workspace and twiddle tables), runs one step eagerly, then
+ * captures the step on `stream` and instantiates the graph; so creation
+ * synchronises `stream`, and the first step's results are in the outputs
+ * when it returns. naja_cond_run_step replays the graph on `stream` (any
+ * stream), copies the roundings' status back to pinned host memory and
+ * synchronises: a variant whose centre is not interior gives
+ * NAJA_ERR_INVALID_ARGUMENT, one without an independent completion
+ * NAJA_ERR_RUNTIME. Replays are bit-identical to the eager step (same
+ * kernels, same Philox stream). With timed != 0 the graph carries external
+ * event nodes around the CHR launch and naja_cond_run_sampling_ms returns its
+ * duration in the last step. Device memory, the layouts of
+ * naja_chr_plan_create (A0, b0, extra_A, extra_b), y0 n_variants x nchains x
+ * d, c n_variants x d, P (written) n_variants x d x d, samples n_variants x
+ * (nchains nspc) x d; all kept alive by the caller until
+ * naja_cond_run_destroy. p: as naja_chr_plan_create; chains of at most
+ * 16384 samples (the LDS transform of the diagnostics). Steps on one run must
+ * not overlap.
+ */
+typedef struct naja_cond_run naja_cond_run;
+int naja_cond_run_create(const float* A0, const double* b0, int m0, int d, int n_variants, const int* extra_rows,
+                         const float* extra_A, const double* extra_b, float theta, float exchange_tol,
+                         const naja_chr_params* p, const float* y0, const float* c, float* P, float* samples,
+                         float* ess, float* rhat, float* ess_min, float* rhat_max, int timed, void* stream,
+                         naja_cond_run** out);
+int naja_cond_run_step(naja_cond_run* run, void* stream);
+int naja_cond_run_sampling_ms(const naja_cond_run* run, float* ms);
+int naja_cond_run_destroy(naja_cond_run* run);
+
+/*
  * Low-rank whitening P_v = (I + V_v V_v^T)^{-1/2} (expand = 0) or expansion
  * P_v = (I + V_v V_v^T)^{1/2} (expand = 1) from the learned preconditioner's
  * factors V (n_variants x d x rank). No reference counterpart.

@@ -76,6 +76,10 @@ EXPORTED_SYMBOLS = (
     "naja_lp_train_step",
     "naja_pilot_covariance",
     "naja_gemm_bf16",
+    "naja_cond_run_create",
+    "naja_cond_run_step",
+    "naja_cond_run_sampling_ms",
+    "naja_cond_run_destroy",
 )
 
 
@@ -211,9 +215,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_lp_train_step.argtypes = [vp, vp, vp, i32, f32, vp, vp]
     lib.naja_pilot_covariance.argtypes = [vp, i32, i64, i32, f64, vp, vp, vp]
     lib.naja_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i64, vp, i64, vp, i64, f32, vp, vp, i64, vp]
+    lib.naja_cond_run_create.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, f32, f32, ctypes.POINTER(ChrParams), vp,
+                                         vp, vp, vp, vp, vp, vp, vp, i32, vp, ctypes.POINTER(vp)]
+    lib.naja_cond_run_step.argtypes = [vp, vp]
+    lib.naja_cond_run_sampling_ms.argtypes = [vp, ctypes.POINTER(f32)]
+    lib.naja_cond_run_destroy.argtypes = [vp]
     for name in ("naja_lp_create", "naja_lp_destroy", "naja_lp_param_count", "naja_lp_get_params",
                  "naja_lp_set_params", "naja_lp_forward", "naja_lp_loss_grad", "naja_lp_train_step",
-                 "naja_pilot_covariance", "naja_gemm_bf16"):
+                 "naja_pilot_covariance", "naja_gemm_bf16", "naja_cond_run_create", "naja_cond_run_step",
+                 "naja_cond_run_sampling_ms", "naja_cond_run_destroy"):
         getattr(lib, name).restype = i32
     _lib = lib
     return lib
@@ -511,6 +521,70 @@ class ChrPlan:
 
     def __del__(self):
         self.close()
+
+
+class ConditionedRun:
+    """Every conditioned variant of a VariantBatch sampled as one persistent
+    hipGraph (naja_cond_run_*): facet-aligned rounding (completed from the
+    base polytope's facets, maximum-volume exchange), rounding transform,
+    batched CHR from z = 0 with the backmap to y, batched ESS / split-R-hat.
+    Creation runs one step eagerly and captures the next; step() replays the
+    graph and returns (samples (n_variants, nchains * nspc, d), ess_min,
+    rhat_max) as device tensors owned by the run (overwritten by the next
+    step). timed: event nodes around the CHR launch (sampling_ms)."""
+
+    def __init__(self, batch, nspc, thinning, nchains, seed=0, resync_interval=0, theta=0.02, exchange_tol=1.02,
+                 timed=False, stream=None):
+        torch = _torch()
+        lib = load_library()
+        if not isinstance(batch, VariantBatch):
+            raise NajaInvalidArgument(NAJA_ERR_INVALID_ARGUMENT, "ConditionedRun takes a VariantBatch")
+        self.batch = batch  # the run reads its device buffers
+        nv, d = batch.n_variants, batch.d
+        dev = batch.A_cm.device
+        self.y0 = torch.zeros((nv, nchains, d), dtype=torch.float32, device=dev)
+        self.c = torch.zeros((nv, d), dtype=torch.float32, device=dev)
+        self.P = torch.empty((nv, d, d), dtype=torch.float32, device=dev)
+        self.samples = torch.empty((nv, nchains * nspc, d), dtype=torch.float32, device=dev)
+        self.ess = torch.empty((nv, d), dtype=torch.float32, device=dev)
+        self.rhat = torch.empty((nv, d), dtype=torch.float32, device=dev)
+        self.ess_min = torch.empty(nv, dtype=torch.float32, device=dev)
+        self.rhat_max = torch.empty(nv, dtype=torch.float32, device=dev)
+        self._p = chr_params(nspc, thinning, nchains, seed=seed, resync_interval=resync_interval)
+        self._karr = (ctypes.c_int * nv)(*batch.ks)
+        # A graph is captured on a stream of its own (not the legacy default
+        # stream), after the work that made its inputs.
+        self._torch_stream = stream if stream is not None else torch.cuda.Stream()
+        self._stream = _stream_ptr(self._torch_stream)
+        torch.cuda.current_stream().synchronize()
+        h = ctypes.c_void_p()
+        _check(lib.naja_cond_run_create(
+            batch.A_cm.data_ptr(), batch.b.data_ptr(), batch.m0, d, nv, self._karr,
+            batch.Ae.data_ptr() if batch.Ae is not None else None, batch.be.data_ptr() if batch.be is not None else None,
+            float(theta), float(exchange_tol), ctypes.byref(self._p), self.y0.data_ptr(), self.c.data_ptr(),
+            self.P.data_ptr(), self.samples.data_ptr(), self.ess.data_ptr(), self.rhat.data_ptr(),
+            self.ess_min.data_ptr(), self.rhat_max.data_ptr(), int(bool(timed)), self._stream, ctypes.byref(h)))
+        self._h = h
+
+    def step(self, stream=None):
+        _check(load_library().naja_cond_run_step(self._h, _stream_ptr(stream) if stream is not None else self._stream))
+        return self.samples, self.ess_min, self.rhat_max
+
+    def sampling_ms(self):
+        ms = ctypes.c_float()
+        _check(load_library().naja_cond_run_sampling_ms(self._h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(load_library().naja_cond_run_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def backmap(T, shift, Y, stream=None):

@@ -49,7 +49,7 @@ import time
 
 import numpy as np
 
-from . import (FACET_COMPLETE_BASE_FACETS, NAJA_ERR_UNSUPPORTED, ChrPlan, NajaError, VariantBatch,
+from . import (FACET_COMPLETE_BASE_FACETS, NAJA_ERR_UNSUPPORTED, ChrPlan, ConditionedRun, NajaError, VariantBatch,
                coordinate_hit_and_run_batched, diagnostics_batched, ellipsoid_rounding, facet_rounding, lowrank_maps)
 from . import precond
 from .learned import LearnedPreconditioner, pilot_covariance
@@ -160,6 +160,28 @@ def _batch_plan(batch, nchains, nspc, thin, seed, resync):
     if key not in plans:
         plans[key] = _RoundedPlan(batch, nchains, nspc, thin, seed, resync)
     return plans[key]
+
+
+def base_run(batch, nchains, nspc, thin, seed, resync=1000, timed=False):
+    """The pipeline's base step over a VariantBatch (facet-aligned rounding
+    completed from the base polytope's facets with the exchange, CHR from
+    z = 0, backmap, diagnostics) as one persistent hipGraph, kept on the
+    batch per run shape (ConditionedRun)."""
+    runs = batch.__dict__.setdefault("_cond_runs", {})
+    key = (nchains, nspc, thin, seed, resync, bool(timed))
+    if key not in runs:
+        runs[key] = ConditionedRun(batch, nspc, thin, nchains, seed=seed, resync_interval=resync, theta=THETA,
+                                   exchange_tol=EXCHANGE_TOL, timed=timed)
+    return runs[key]
+
+
+def graph_first(run):
+    """A first-run callable for sample_accepted: one replay of the run's
+    graph; (samples, per-variant min ESS, per-variant max R-hat)."""
+    def first():
+        Y, emin, rmax = run.step()
+        return Y, emin.cpu().numpy(), rmax.cpu().numpy()
+    return first
 
 
 def variant_diagnostics(Y, nchains):
@@ -289,15 +311,20 @@ def learned_rounding(lp: LearnedPreconditioner, F, batch, P0=None):
     return facet_maps(batch, learned_maps(lp, F, P0)), P0
 
 
-def sample_accepted(batch, P, P_fallback, nchains, nspc, thin, seed, resync=1000):
-    """Main run with P; variants whose split-R-hat is >= 1.05 run again with
-    P_fallback (callable giving the fallback maps of a sub-batch), keeping
-    the better run (passing R-hat first, then the larger min-ESS).
+def sample_accepted(batch, P, P_fallback, nchains, nspc, thin, seed, resync=1000, first=None):
+    """Main run with P (or `first`, a callable returning the main run's
+    samples, per-variant min ESS and max R-hat, e.g. graph_first); variants
+    whose split-R-hat is >= 1.05 run again with P_fallback (callable giving
+    the fallback maps of a sub-batch), keeping the better run (passing R-hat
+    first, then the larger min-ESS).
     Returns (Y, ess_min, rhat_max, number of variants run again)."""
     import torch
 
-    Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
-    e, r = variant_diagnostics(Y, nchains)
+    if first is not None:
+        Y, e, r = first()
+    else:
+        Y = sample_rounded(batch, None, None, None, P, nchains, nspc, thin, seed, resync)
+        e, r = variant_diagnostics(Y, nchains)
     bad = np.flatnonzero(r >= 1.05)
     if bad.size:
         sub = batch.take(bad)
@@ -359,9 +386,11 @@ def compare(A0, b0, EA, Eb, F, lp, nchains=4, nspc=12500, thin=16, seed=7, resyn
     def learned():
         if use_network:
             P, _ = learned_rounding(lp, Fd, batch)
-        else:  # the network was rejected by validate_learned: the step runs its base
-            P = facet_maps(batch)
-        _, e, r, n_again = sample_accepted(batch, P, facet_fallback, nchains, nspc, thin, seed, resync)
+            _, e, r, n_again = sample_accepted(batch, P, facet_fallback, nchains, nspc, thin, seed, resync)
+        else:  # the network was rejected by validate_learned: the step runs its base, as one graph
+            run = base_run(batch, nchains, nspc, thin, seed, resync)
+            _, e, r, n_again = sample_accepted(batch, None, facet_fallback, nchains, nspc, thin, seed, resync,
+                                               first=graph_first(run))
         return e, r, n_again
 
     steps_of = {

@@ -402,6 +402,11 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         std::lock_guard<std::mutex> g(g_timing_mu);
         timed = g_timing;
     }
+    if (timed) {  // a captured launch is timed by the graph's own event nodes, if at all
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        NAJA_HIP(hipStreamIsCapturing(stream, &cap));
+        timed = cap == hipStreamCaptureStatusNone;
+    }
     TimedLaunch rec{nullptr, nullptr, n_items};
     if (timed) {
         NAJA_HIP(hipEventCreate(&rec.start));

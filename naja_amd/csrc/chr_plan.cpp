@@ -282,6 +282,11 @@ void retransform(naja_chr_plan& pl)
     }
     chr_transform(pl.A_raw, pl.b_raw, pl.bn_raw, pl.var_d, pl.n_variants, pl.d, pl.shape.rows(), pl.P, pl.c,
                   pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, s);
+    // Under capture (a graph replays the transform and its launches in
+    // order on one stream) there is no event to record for other streams.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    NAJA_HIP(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return;
     NAJA_HIP(hipEventRecord(pl.packed, s));
     ++pl.pack_seq;
 }
@@ -364,7 +369,7 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
 }
 
 void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s,
-            int64_t t0 = 0)
+            int64_t t0 = 0, void (*after_chr)(hipStream_t, void*) = nullptr, void* user = nullptr)
 {
     if (pl.inplace)
         NAJA_REQUIRE(samples == pl.inplace, NAJA_ERR_INVALID_ARGUMENT,
@@ -374,6 +379,7 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
     const ChrArgs a = make_args(pl, y0, out, y_final, nsteps, t0);
     order_after_pack(pl, s);
     chr_launch(a, (int)pl.items.size(), pl.shape, s, pl.nc, pl.lds_cols);
+    if (after_chr) after_chr(s, user);
     if (pl.P) {
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
         const int64_t per = (int64_t)pl.p.nchains * pl.p.nspc;
@@ -506,6 +512,17 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
 }
 
 }  // namespace
+}  // namespace naja
+
+namespace naja {
+// A plan launch with a callback between the CHR kernel and the backmap (for
+// the conditioned run, which places its timing events there).
+void chr_plan_launch_hooked(naja_chr_plan* plan, const float* y0, float* samples, hipStream_t s,
+                            void (*after_chr)(hipStream_t, void*), void* user)
+{
+    NAJA_REQUIRE(plan && y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, y0 or samples");
+    launch(*plan, y0, samples, nullptr, (int64_t)plan->p.nspc * plan->p.thinning, s, 0, after_chr, user);
+}
 }  // namespace naja
 
 extern "C" {

@@ -18,4 +18,14 @@ constexpr size_t kDiagNoAllocBytes = (size_t)64 << 20;
 void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
                          float* ess_chain, hipStream_t stream);
 
+// Diagnostics of n_variants sample blocks with a caller-held workspace of
+// diag_workspace_bytes (chains up to diag_max_chain_length samples, else
+// NAJA_ERR_UNSUPPORTED), its twiddle tables written once by
+// diag_workspace_init (synchronous); diag_launch_ws then issues kernels only:
+// no allocation and no host copy (capturable in a graph).
+size_t diag_workspace_bytes(int dim, int n_total, int nchains, int n_variants);
+void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_variants);
+void diag_launch_ws(void* ws, const float* samples, int dim, int n_total, int nchains, int layout, float* ess,
+                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max);
+
 }  // namespace naja

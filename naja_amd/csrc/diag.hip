@@ -1237,6 +1237,34 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     scratch_free(mem, stream);
 }
 
+// Diagnostics with a caller-held workspace: the series, moments, per-chain
+// ESS and twiddle tables in `ws` (diag_workspace_bytes), the tables written by
+// diag_workspace_init once. A run issues kernels only (no allocation, no copy
+// from the host), so it can be captured in a graph.
+size_t diag_workspace_bytes(int dim, int n_total, int nchains, int n_variants)
+{
+    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
+    return p.b_ser + p.b_mom + p.b_ec + round256(sizeof(float2) * (size_t)twiddle_count(p.logH));
+}
+
+void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_variants)
+{
+    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
+    std::vector<float2> tw_h((size_t)twiddle_count(p.logH));
+    twiddle_tables(p.logH, tw_h.data());
+    char* tw = static_cast<char*>(ws) + p.b_ser + p.b_mom + p.b_ec;
+    NAJA_HIP(hipMemcpy(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice));
+}
+
+void diag_launch_ws(void* ws, const float* samples, int dim, int n_total, int nchains, int layout, float* ess,
+                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
+{
+    const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
+    char* mem = static_cast<char*>(ws);
+    const float2* tw = reinterpret_cast<const float2*>(mem + p.b_ser + p.b_mom + p.b_ec);
+    diag_run(p, samples, dim, nchains, layout, ess, rhat, nullptr, mem, tw, stream, ess_min, rhat_max);
+}
+
 void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,
                          float* ess_chain, hipStream_t stream)
 {

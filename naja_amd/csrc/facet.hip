@@ -57,6 +57,7 @@
 
 #include "../../include/naja_amd.h"
 #include "common.h"
+#include "facet.h"
 
 namespace naja {
 namespace {
@@ -447,6 +448,34 @@ size_t facet_lds_bytes(int d, int cols)
 }
 
 }  // namespace
+
+int facet_columns(int d) { return d <= 32 ? 32 : (d <= 64 ? 64 : (d <= 112 ? 112 : 128)); }
+
+void facet_check_shape(int m0, int d, int kmax, int include_base, int completion)
+{
+    const bool base_facets = completion == NAJA_FACET_COMPLETE_BASE_FACETS;
+    const int nr_max = (include_base ? m0 : 0) + kmax + (base_facets ? m0 : d);
+    const size_t lds = facet_lds_bytes(d, facet_columns(d));
+    NAJA_REQUIRE(d <= kFacetMaxD && nr_max <= kFacetThreads && lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
+                 "facet rounding: d=%d with %d candidate rows exceeds the kernel (d <= %d, %d candidates)", d, nr_max,
+                 kFacetMaxD, kFacetThreads);
+}
+
+void facet_launch(const float* A0, const double* b0, int m0, int d, int n_variants, const float* Ae, const double* be,
+                  const int64_t* eoff, const int* ek, const float* xc, const float* P0, const float* theta,
+                  int include_base, int completion, float exchange_tol, double* P0i, float* P, int* status,
+                  hipStream_t s)
+{
+    const int cols = facet_columns(d);
+    auto kern = cols == 32    ? facet_kernel<32>
+                : cols == 64  ? facet_kernel<64>
+                : cols == 112 ? facet_kernel<112>
+                              : facet_kernel<128>;  // 113..128 columns spill to scratch
+    hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), facet_lds_bytes(d, cols), s, A0, b0, m0, d, Ae,
+                       be, eoff, ek, xc, P0, theta, include_base ? 1 : 0, completion, exchange_tol, P0i, P, status);
+    NAJA_HIP(hipGetLastError());
+}
+
 }  // namespace naja
 
 extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, int n_variants,
@@ -479,13 +508,7 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
         }
         NAJA_REQUIRE(tot == 0 || (extra_A && extra_b), NAJA_ERR_INVALID_ARGUMENT,
                      "naja_precond_facet_device: extra rows without extra_A/extra_b");
-        const int nr_max = (include_base ? m0 : 0) + kmax + (base_facets ? m0 : d);
-        const int cols = d <= 32 ? 32 : (d <= 64 ? 64 : (d <= 112 ? 112 : 128));
-        const size_t lds = facet_lds_bytes(d, cols);
-        NAJA_REQUIRE(d <= kFacetMaxD && nr_max <= kFacetThreads && lds <= 150 * 1024, NAJA_ERR_UNSUPPORTED,
-                     "naja_precond_facet_device: d=%d with %d candidate rows exceeds the kernel (d <= %d, %d "
-                     "candidates)",
-                     d, nr_max, kFacetMaxD, kFacetThreads);
+        facet_check_shape(m0, d, kmax, include_base, completion);
         hipStream_t s = (hipStream_t)stream;
         const size_t b_off = (sizeof(int64_t) * n_variants + 255) / 256 * 256;
         const size_t b_k = (sizeof(int) * n_variants + 255) / 256 * 256;
@@ -499,13 +522,8 @@ extern "C" int naja_precond_facet_device(const float* A0, const double* b0, int 
         double* P0i = reinterpret_cast<double*>(mem + b_off + 2 * b_k);
         NAJA_HIP(hipMemcpyAsync(eoff, off.data(), sizeof(int64_t) * n_variants, hipMemcpyHostToDevice, s));
         NAJA_HIP(hipMemcpyAsync(ek, extra_rows, sizeof(int) * n_variants, hipMemcpyHostToDevice, s));
-        auto kern = cols == 32    ? facet_kernel<32>
-                    : cols == 64  ? facet_kernel<64>
-                    : cols == 112 ? facet_kernel<112>
-                                  : facet_kernel<128>;  // 113..128 columns spill to scratch
-        hipLaunchKernelGGL(kern, dim3(n_variants), dim3(kFacetThreads), lds, s, A0, b0, m0, d, extra_A, extra_b,
-                           eoff, ek, x_c, P0, theta, include_base ? 1 : 0, completion, exchange_tol, P0i, P, status);
-        NAJA_HIP(hipGetLastError());
+        facet_launch(A0, b0, m0, d, n_variants, extra_A, extra_b, eoff, ek, x_c, P0, theta, include_base, completion,
+                     exchange_tol, P0i, P, status, s);
         int* st = held_host(hb, (size_t)n_variants);
         NAJA_HIP(hipMemcpyAsync(st, status, sizeof(int) * n_variants, hipMemcpyDeviceToHost, s));
         NAJA_HIP(hipStreamSynchronize(s));

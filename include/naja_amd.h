@@ -468,7 +468,8 @@ int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, 
  * counterpart: naja samples one model per job, job_sampling.cu:315-379, and
  * rounds it offline). A step is: the facet-aligned rounding of every variant
  * (naja_precond_facet_device with NAJA_FACET_COMPLETE_BASE_FACETS, theta,
- * exchange_tol, centre at the origin), the rounding set on a sampling plan
+ * exchange_tol, centre at the origin), the variants' rows packed again from
+ * A0, b0, extra_A, extra_b and the rounding set on them
  * (naja_chr_plan_set_rounding), the batched CHR launch of every variant from
  * z = y0 with its backmap y = c + P z into `samples`, and the batched ESS and
  * split-R-hat (naja_diag_batched, interleaved layout) into ess, rhat (per
@@ -488,9 +489,13 @@ int naja_precond_facet_device(const float* A0, const double* b0, int m0, int d, 
  * naja_chr_plan_create (A0, b0, extra_A, extra_b), y0 n_variants x nchains x
  * d, c n_variants x d, P (written) n_variants x d x d, samples n_variants x
  * (nchains nspc) x d; all kept alive by the caller until
- * naja_cond_run_destroy. p: as naja_chr_plan_create; chains of at most
- * 16384 samples (the LDS transform of the diagnostics). Steps on one run must
- * not overlap.
+ * naja_cond_run_destroy. Between steps the caller may rewrite A0, b0,
+ * extra_A and extra_b in place (new bounds of the same conditions, a new
+ * request's rows): the next step samples them, provided the row counts stay
+ * and a block that was a [T; -T] pair at creation stays one (each packed row
+ * holds a row and its negation). p: as naja_chr_plan_create; chains of at
+ * most 16384 samples (the LDS transform of the diagnostics). Steps on one run
+ * must not overlap.
  */
 typedef struct naja_cond_run naja_cond_run;
 int naja_cond_run_create(const float* A0, const double* b0, int m0, int d, int n_variants, const int* extra_rows,

@@ -523,6 +523,19 @@ void chr_plan_launch_hooked(naja_chr_plan* plan, const float* y0, float* samples
     NAJA_REQUIRE(plan && y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, y0 or samples");
     launch(*plan, y0, samples, nullptr, (int64_t)plan->p.nspc * plan->p.thinning, s, 0, after_chr, user);
 }
+
+// Packs a rounded plan's untransformed rows again from the caller's A, b,
+// A_extra, b_extra (same counts, same shape and row pairing as at creation),
+// on stream s, for a pipeline whose inputs change in place between steps; a
+// naja_chr_plan_set_rounding after it forms the transformed rows.
+void chr_plan_repack_rows(naja_chr_plan* plan, hipStream_t s)
+{
+    NAJA_REQUIRE(plan && plan->P, NAJA_ERR_INVALID_ARGUMENT, "a rounded plan is needed");
+    naja_chr_plan& pl = *plan;
+    const int* pairs = pl.paired ? pl.pair_d : nullptr;
+    chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants, pairs,
+             pl.A_raw, pl.Ainv_pack, pl.b_raw, pl.bn_raw, pl.shape.rows(), s);
+}
 }  // namespace naja
 
 extern "C" {

@@ -1,6 +1,8 @@
 // A conditioned-variant run as one hipGraph: the facet-aligned rounding of
 // every variant (completed from the base polytope's facets, maximum-volume
-// exchange), the new rounding set on a sampling plan (A' = A P, b' = b - A c),
+// exchange), the variants' rows packed again from the caller's buffers (so
+// bounds or rows changed in place between steps are sampled), the new
+// rounding set on a sampling plan (A' = A P, b' = b - A c),
 // the batched CHR launch with its backmap to y, and the batched ESS /
 // split-R-hat, captured once on the creation stream and replayed by each
 // step. Everything the graph touches is allocated at creation: the facet
@@ -26,6 +28,8 @@ namespace naja {
 // chr_plan.cpp: a plan launch with a callback between the CHR kernel and the backmap.
 void chr_plan_launch_hooked(naja_chr_plan* plan, const float* y0, float* samples, hipStream_t s,
                             void (*after_chr)(hipStream_t, void*), void* user);
+// chr_plan.cpp: the untransformed rows packed again from the caller's inputs.
+void chr_plan_repack_rows(naja_chr_plan* plan, hipStream_t s);
 }  // namespace naja
 
 struct naja_cond_run {
@@ -94,6 +98,7 @@ void enqueue_step(naja_cond_run& r, hipStream_t s, bool capturing)
 {
     facet_launch(r.A0, r.b0, r.m0, r.d, r.nv, r.Ae, r.be, r.eoff, r.ek, nullptr, nullptr, r.theta, 0,
                  NAJA_FACET_COMPLETE_BASE_FACETS, r.exchange_tol, nullptr, r.P, r.status, s);
+    chr_plan_repack_rows(r.plan, s);
     check_plan(naja_chr_plan_set_rounding(r.plan, r.P, r.c));
     const bool ev = capturing && r.timed;
     if (ev) capture_event(s, r.ev0);

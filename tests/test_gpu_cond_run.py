@@ -77,6 +77,36 @@ def test_graph_replays_equal_the_eager_pipeline(gpu, d, ks):
     run.close()
 
 
+def test_bounds_rewritten_in_place_are_sampled(gpu):
+    """New half-widths for the same slabs, written into the batch's device
+    buffer between steps: the next replay equals the eager pipeline built
+    from scratch on the new conditions (the graph packs the rows again)."""
+    import naja_amd
+
+    t = gpu
+    d = 40
+    ks = [0, 16, 40, 90, 150]
+    A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
+    EA, Eb = conditions(d, ks, seed=5)
+    batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
+    nspc, thin, nch, seed, resync = 500, 4, 4, 23, 256
+    run = naja_amd.ConditionedRun(batch, nspc, thin, nch, seed=seed, resync_interval=resync)
+    rng = np.random.default_rng(8)
+    Eb2 = []
+    for e in Eb:
+        w = 10.0 ** rng.uniform(-2, 0, len(e) // 2)
+        Eb2.append(np.concatenate([w, w]) if len(e) else np.zeros(0))
+    batch.be.copy_(t.from_numpy(np.concatenate(Eb2)))
+    t.cuda.synchronize()
+    run.step()
+    got = [x.cpu().numpy() for x in (run.P, run.samples, run.ess, run.rhat, run.ess_min, run.rhat_max)]
+    ref = eager_step(naja_amd, t, naja_amd.VariantBatch(A0, b0, EA, Eb2), nspc, thin, nch, seed, resync)
+    for name, a, b in zip(("P", "samples", "ess", "rhat", "ess_min", "rhat_max"), got, ref):
+        assert np.array_equal(a, b), f"{name} after the in-place bounds differs from the eager pipeline"
+    print("[cond-run] rewritten bounds: the replay equals the eager pipeline on the new conditions", flush=True)
+    run.close()
+
+
 def test_cond_run_rejects_a_start_outside_a_variant(gpu):
     import naja_amd
 

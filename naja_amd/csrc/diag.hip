@@ -419,10 +419,13 @@ constexpr int kTd = 128;
 __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __restrict__ samples, int dim,
                                                               int nchains, int spc, int spcp, int layout,
                                                               int64_t vstride, int s_tiles, int d_tiles,
-                                                              float* __restrict__ out)
+                                                              float* __restrict__ out, int* __restrict__ redo)
 {
     __shared__ float tile[kTs * (kTd + 1)];
     const int64_t b = blockIdx.x;
+    // The ESS short pass that follows appends to this list (count first): a
+    // plain store here, not a memset, keeps the launch sequence capturable.
+    if (b == 0 && threadIdx.x == 0 && redo) *redo = 0;
     const int64_t grp = b / (8 * nchains);
     const int within = (int)(b % (8 * nchains));
     const int c = within / 8;
@@ -493,7 +496,8 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
                                                                   const float2* __restrict__ tw_tables, int tb,
                                                                   int tw_shift, int mode,
                                                                   float* __restrict__ ess_chain, int half,
-                                                                  double* __restrict__ mom, int64_t n_series)
+                                                                  double* __restrict__ mom, int64_t n_series,
+                                                                  int* __restrict__ redo)
 {
     extern __shared__ float2 z_lds[];
     const PaddedZ z{z_lds};
@@ -699,8 +703,12 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         int K = klim;
         for (int w = 0; w < (int)(blockDim.x / 64); ++w) K = min(K, kfirst[w]);
         if (K == klim && klim < kend) {
-            // Truncation beyond the exact lags: redo at full size.
-            if (tid == 0) *out = -1.0f;
+            // Truncation beyond the exact lags: redo at full size (listed for
+            // the redo pass, which then visits only these series).
+            if (tid == 0) {
+                *out = -1.0f;
+                if (redo) redo[1 + atomicAdd(redo, 1)] = (int)sid;
+            }
             return;
         }
         double acc = 0.0;
@@ -717,6 +725,16 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
         }
     };
 
+    if (mode == 2 && redo) {
+        // The series the short pass listed (their order is immaterial: each
+        // result depends on its series only).
+        const int cnt = *redo;
+        for (int i = blockIdx.x; i < cnt; i += gridDim.x) {
+            one(redo[1 + i]);
+            __syncthreads();  // LDS is reused by the next series
+        }
+        return;
+    }
     for (int64_t sid = blockIdx.x; sid < n_series; sid += gridDim.x) {
         one(sid);
         __syncthreads();  // LDS is reused by the next series
@@ -989,6 +1007,8 @@ struct DiagPlan {
     int64_t n_series, t_blocks, vstride;
     unsigned nv;
     size_t b_ser, b_mom, b_ec;  // 256-byte rounded scratch regions
+    size_t b_redo;              // the redo list of the two-pass ESS: count, then series ids
+    size_t main_bytes() const { return b_ser + b_mom + b_ec + b_redo; }
 };
 
 size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1019,6 +1039,7 @@ DiagPlan diag_plan(int dim, int n_total, int nchains, int n_variants, bool need_
     p.b_ser = round256(sizeof(float) * (size_t)p.n_series * p.spcp);
     p.b_mom = round256(sizeof(double) * 2 * (size_t)dim * 2 * nchains * p.nv);
     p.b_ec = need_ec ? round256(sizeof(float) * (size_t)dim * nchains * p.nv) : 0;
+    p.b_redo = round256(sizeof(int) * (1 + (size_t)p.n_series));
     return p;
 }
 
@@ -1056,12 +1077,13 @@ void diag_run(const DiagPlan& p, const float* samples, int dim, int nchains, int
     float* ser = reinterpret_cast<float*>(mem);
     double* mom = reinterpret_cast<double*>(mem + p.b_ser);
     float* ec = ess_chain ? ess_chain : reinterpret_cast<float*>(mem + p.b_ser + p.b_mom);
+    int* redo = reinterpret_cast<int*>(mem + p.b_ser + p.b_mom + p.b_ec);
     const int spc = p.spc, H = p.H, logH = p.logH, N = p.N, tb = p.tb, half = p.half, spcp = p.spcp;
     const int64_t n_series = p.n_series;
     const unsigned nv = p.nv;
 
     hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
-                       nchains, spc, spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser);
+                       nchains, spc, spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser, redo);
     NAJA_HIP(hipGetLastError());
     // A half-size pass first when it leaves at least 1/8 of the chain as exact
     // lags (N/2 - n >= n/8); the full pass then redoes only the series whose
@@ -1070,13 +1092,13 @@ void diag_run(const DiagPlan& p, const float* samples, int dim, int nchains, int
     if (two_pass) {
         hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H / 2)),
                            padded_lds_bytes(H / 2), stream,
-                           ser, dim, nchains, spc, spcp, logH - 1, tw, tb, 1, 1, ec, half, mom, n_series);
+                           ser, dim, nchains, spc, spcp, logH - 1, tw, tb, 1, 1, ec, half, mom, n_series, redo);
         NAJA_HIP(hipGetLastError());
         hipLaunchKernelGGL(ess_series_kernel, dim3(ess_grid(H, n_series)), dim3(ess_threads(H)), padded_lds_bytes(H),
-                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 2, ec, half, mom, n_series);
+                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 2, ec, half, mom, n_series, redo);
     } else {
         hipLaunchKernelGGL(ess_series_kernel, dim3((unsigned)n_series), dim3(ess_threads(H)), padded_lds_bytes(H),
-                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 0, ec, half, mom, n_series);
+                           stream, ser, dim, nchains, spc, spcp, logH, tw, tb, 0, 0, ec, half, mom, n_series, redo);
     }
     NAJA_HIP(hipGetLastError());
     hipLaunchKernelGGL(ess_min_kernel, dim3((dim + 255) / 256, nv), dim3(256), 0, stream, ec, dim, nchains, ess);
@@ -1169,7 +1191,7 @@ void diag_launch_long(const DiagPlan& p, const float* samples, int dim, int ncha
     float* meanf = reinterpret_cast<float*>(mem + b_main + b_tw + 2 * b_buf);
     NAJA_HIP(hipMemcpyAsync(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
-                       nchains, spc, p.spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser);
+                       nchains, spc, p.spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser, nullptr);
     NAJA_HIP(hipGetLastError());
     const float2* twH = tw;
     const float2* twN = tw + H;
@@ -1228,7 +1250,7 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
     // (when the caller passes none) and the twiddle tables: each pool call
     // costs tens of microseconds of host time.
     const size_t b_tw = round256(sizeof(float2) * tw_h.size());
-    const size_t b_main = p.b_ser + p.b_mom + p.b_ec;
+    const size_t b_main = p.main_bytes();
     char* mem = nullptr;
     scratch_alloc((void**)&mem, b_main + b_tw, stream);
     float2* tw = reinterpret_cast<float2*>(mem + b_main);
@@ -1244,7 +1266,7 @@ void diag_launch(const float* samples, int dim, int n_total, int nchains, int la
 size_t diag_workspace_bytes(int dim, int n_total, int nchains, int n_variants)
 {
     const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
-    return p.b_ser + p.b_mom + p.b_ec + round256(sizeof(float2) * (size_t)twiddle_count(p.logH));
+    return p.main_bytes() + round256(sizeof(float2) * (size_t)twiddle_count(p.logH));
 }
 
 void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_variants)
@@ -1252,7 +1274,7 @@ void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_vari
     const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
     std::vector<float2> tw_h((size_t)twiddle_count(p.logH));
     twiddle_tables(p.logH, tw_h.data());
-    char* tw = static_cast<char*>(ws) + p.b_ser + p.b_mom + p.b_ec;
+    char* tw = static_cast<char*>(ws) + p.main_bytes();
     NAJA_HIP(hipMemcpy(tw, tw_h.data(), sizeof(float2) * tw_h.size(), hipMemcpyHostToDevice));
 }
 
@@ -1261,7 +1283,7 @@ void diag_launch_ws(void* ws, const float* samples, int dim, int n_total, int nc
 {
     const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
     char* mem = static_cast<char*>(ws);
-    const float2* tw = reinterpret_cast<const float2*>(mem + p.b_ser + p.b_mom + p.b_ec);
+    const float2* tw = reinterpret_cast<const float2*>(mem + p.main_bytes());
     diag_run(p, samples, dim, nchains, layout, ess, rhat, nullptr, mem, tw, stream, ess_min, rhat_max);
 }
 
@@ -1269,7 +1291,7 @@ void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains
                          float* ess_chain, hipStream_t stream)
 {
     const DiagPlan p = diag_plan(dim, n_total, nchains, 1, ess_chain == nullptr);
-    const size_t need = p.b_ser + p.b_mom + p.b_ec;
+    const size_t need = p.main_bytes();
     NAJA_REQUIRE(need <= kDiagNoAllocBytes, NAJA_ERR_UNSUPPORTED,
                  "naja_diag_noalloc: %d dims x %d chains x %d samples need %zu bytes of scratch; the static buffer "
                  "holds %zu (about %d x 4 bytes per (dim, chain) series, rounded up to 64 samples)",

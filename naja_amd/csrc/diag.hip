@@ -201,16 +201,25 @@ __device__ __forceinline__ void twiddle_powers(float2 w1, float2 w4, float2 (&w)
 // points (q = 2^logq): per block offset p0 < q, the DFT16 of the points
 // p0 + m q, output r scaled by W_L^(p0 r) and stored at p0 + bitrev4(r) q.
 // Passes at L = H, H/16, ... leave the spectrum in bit-reversed order.
-// The group of block offset p0 (points base + m q, m < 16, already in x):
-// DFT16, twiddles W_L^(p0 r) (W_L^j = W_N^(j << sh)), bit-reversed stores.
-__device__ __forceinline__ void dif16_group(const PaddedZ& z, float2 (&x)[16], int base, int q, int p0, int sh,
-                                            const Twiddles& tw)
+//
+// Padded offset of point base + m q from the padded base, for a group base
+// = B 16 q + p0 (p0 < q): base + m q + ((base + m q) >> 4) = pad(base) +
+// m q + ((m q) >> 4), because 16 q is a multiple of 16 and p0 never carries
+// the low four bits of m q past a multiple of 16. The offsets depend on q
+// only, so they are wave-uniform (scalar) and each point of a group is one
+// add from the group's padded address.
+__device__ __forceinline__ int pad_off(int m, int q) { return m * q + ((m * q) >> 4); }
+
+// The group of block offset p0 (points at zb[pad_off(m, q)], m < 16, already
+// in x; zb the padded address of the group's base): DFT16, twiddles
+// W_L^(p0 r) (W_L^j = W_N^(j << sh)), bit-reversed stores.
+__device__ __forceinline__ void dif16_group(float2* zb, float2 (&x)[16], int q, int p0, int sh, const Twiddles& tw)
 {
     float2 w[16];
     dft16<false>(x);
     twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) z[base + bitrev4(r) * q] = cmul(x[4 * (r & 3) + (r >> 2)], w[r]);
+    for (int r = 0; r < 16; ++r) zb[pad_off(bitrev4(r), q)] = cmul(x[4 * (r & 3) + (r >> 2)], w[r]);
 }
 
 __device__ void dif16_pass(const PaddedZ& z, int H, int logq, int logN, const Twiddles& tw)
@@ -220,10 +229,11 @@ __device__ void dif16_pass(const PaddedZ& z, int H, int logq, int logN, const Tw
     for (int g = threadIdx.x; g < H / 16; g += blockDim.x) {
         const int p0 = g & (q - 1);
         const int base = ((g >> logq) << (logq + 4)) + p0;
+        float2* zb = z.p + base + (base >> 4);
         float2 x[16];
 #pragma unroll
-        for (int m = 0; m < 16; ++m) x[m] = z[base + m * q];
-        dif16_group(z, x, base, q, p0, sh, tw);
+        for (int m = 0; m < 16; ++m) x[m] = zb[pad_off(m, q)];
+        dif16_group(zb, x, q, p0, sh, tw);
     }
 }
 
@@ -236,13 +246,14 @@ __device__ void dit16_pass(const PaddedZ& z, int H, int logq, int logN, const Tw
     for (int g = threadIdx.x; g < H / 16; g += blockDim.x) {
         const int p0 = g & (q - 1);
         const int base = ((g >> logq) << (logq + 4)) + p0;
+        float2* zb = z.p + base + (base >> 4);
         float2 x[16], w[16];
         twiddle_powers(tw.at(p0 << sh), tw.at((4 * p0) << sh), w);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) x[r] = cmul(z[base + bitrev4(r) * q], conjf2(w[r]));
+        for (int r = 0; r < 16; ++r) x[r] = cmul(zb[pad_off(bitrev4(r), q)], conjf2(w[r]));
         dft16<true>(x);
 #pragma unroll
-        for (int m = 0; m < 16; ++m) z[base + m * q] = x[4 * (m & 3) + (m >> 2)];
+        for (int m = 0; m < 16; ++m) zb[pad_off(m, q)] = x[4 * (m & 3) + (m >> 2)];
     }
 }
 
@@ -608,7 +619,7 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
                 if (2 * j + 1 < n) xr[m].y -= meanf;
             }
             // first pass: L = H, q = H/16, p0 = base = tid, W_L^j = W_N^(2j)
-            dif16_group(z, xr, tid, H / 16, tid, 1, tw);
+            dif16_group(z.p + tid + (tid >> 4), xr, H / 16, tid, 1, tw);
         } else {
             for (int j = tid; j < H; j += blockDim.x) {
                 const int s0 = 2 * j, s1 = 2 * j + 1;

@@ -161,12 +161,17 @@ __host__ __device__ inline int small_stride(int d)
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Y and V may be the same array (in place, n == d): no __restrict__ on them.
+// ser (optional): the same outputs also in the series layout of the
+// diagnostics, ser[((z nchains + c) n + j) spcp + s] = V_z[s nchains + c][j]
+// (row s nchains + c of a batch item is sample s of chain c), written from
+// the LDS output tile, so no separate transpose has to read V again.
 template <int NB>
 __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restrict__ T,
                                                            const float* __restrict__ shift,
                                                            const float* Y, float* V, int n,
                                                            int d, int64_t ns, int tiles_per_wg, int64_t sT,
-                                                           int64_t sShift, int64_t sY, int64_t sV)
+                                                           int64_t sShift, int64_t sY, int64_t sV,
+                                                           float* __restrict__ ser, int nchains, int spcp)
 {
     extern __shared__ float lds[];
     const int S = small_stride(d);
@@ -292,6 +297,30 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
         } else {
             for (int e = tid; e < len; e += 256) dst[e] = Os[e];
         }
+        if (ser) {
+            float* sz = ser + (int64_t)blockIdx.z * nchains * n * spcp;
+            if (rows == kSmRows && (nchains & (nchains - 1)) == 0 && nchains <= kSmRows) {
+                // A whole tile, a power-of-two chain count: spt = 64 / nchains
+                // samples of every chain. Thread tid takes sample tid % spt of
+                // the series (chain, column) it walks with stride 256 / spt,
+                // so lanes run along one series and no division is needed.
+                const int lc = __builtin_ctz((unsigned)nchains);
+                const int ls = __builtin_ctz((unsigned)(kSmRows / nchains));
+                const int sl = tid & ((1 << ls) - 1);
+                const int64_t sb = s0 >> lc;
+                for (int cc = tid >> ls; cc < nchains * n; cc += 256 >> ls) {
+                    const int c = cc & (nchains - 1), j = cc >> lc;
+                    // Streaming stores: 5 GB of series do not stay in L2 for the diagnostics anyway.
+                    __builtin_nontemporal_store(Os[((sl << lc) + c) * n + j], sz + ((int64_t)c * n + j) * spcp + sb + sl);
+                }
+            } else {
+                for (int e = tid; e < (int)rows * n; e += 256) {
+                    const int r = e % (int)rows, j = e / (int)rows;
+                    const int64_t g = s0 + r;
+                    sz[((g % nchains) * n + j) * spcp + g / nchains] = Os[r * n + j];
+                }
+            }
+        }
     }
 }
 
@@ -307,7 +336,8 @@ size_t small_lds_bytes(int n, int d)
 
 template <int NB>
 void launch_small(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns, int batch,
-                  int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
+                  int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream, float* ser, int nchains,
+                  int spcp)
 {
     const int64_t n_tiles = (ns + kSmRows - 1) / kSmRows;
     // About 512 workgroups in all (two per CU at this LDS size), each
@@ -316,7 +346,8 @@ void launch_small(const float* T, const float* shift, const float* Y, float* V, 
     const int tpw = (int)((n_tiles + wpb - 1) / wpb);
     const int64_t gx = (n_tiles + tpw - 1) / tpw;
     hipLaunchKernelGGL(backmap_small_kernel<NB>, dim3((unsigned)gx, 1, (unsigned)batch), dim3(256),
-                       small_lds_bytes(n, d), stream, T, shift, Y, V, n, d, ns, tpw, sT, sShift, sY, sV);
+                       small_lds_bytes(n, d), stream, T, shift, Y, V, n, d, ns, tpw, sT, sShift, sY, sV, ser, nchains,
+                       spcp);
     NAJA_HIP(hipGetLastError());
 }
 
@@ -332,21 +363,29 @@ bool small_ok(int n, int d, const float* Y, const float* V, int64_t sY, int64_t 
 
 bool backmap_inplace_ok(int d, const float* Y, int64_t sY, int batch) { return small_ok(d, d, Y, Y, sY, sY, batch); }
 
+bool backmap_series_ok(int n, int d, const float* Y, const float* V, int64_t sY, int64_t sV, int batch)
+{
+    return small_ok(n, d, Y, V, sY, sV, batch);
+}
+
 void backmap_launch_batched(const float* T, const float* shift, const float* Y, float* V, int n, int d, int64_t ns,
-                            int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream)
+                            int batch, int64_t sT, int64_t sShift, int64_t sY, int64_t sV, hipStream_t stream,
+                            float* ser, int nchains, int spcp)
 {
     if (ns == 0 || batch == 0) return;
     const bool small = small_ok(n, d, Y, V, sY, sV, batch);
+    NAJA_REQUIRE(!ser || (small && nchains > 0 && ns % nchains == 0 && spcp >= ns / nchains), NAJA_ERR_INVALID_ARGUMENT,
+                 "backmap: a series-layout copy needs the small-operand kernel and whole chains");
     if (small) {
         switch ((n + 15) / 16) {
-        case 1: launch_small<1>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 2: launch_small<2>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 3: launch_small<3>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 4: launch_small<4>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 5: launch_small<5>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 6: launch_small<6>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        case 7: launch_small<7>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
-        default: launch_small<8>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream); return;
+        case 1: launch_small<1>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 2: launch_small<2>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 3: launch_small<3>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 4: launch_small<4>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 5: launch_small<5>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 6: launch_small<6>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        case 7: launch_small<7>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
+        default: launch_small<8>(T, shift, Y, V, n, d, ns, batch, sT, sShift, sY, sV, stream, ser, nchains, spcp); return;
         }
     }
     const int64_t gx = (ns + kBM - 1) / kBM;

@@ -369,7 +369,8 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
 }
 
 void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, int64_t nsteps, hipStream_t s,
-            int64_t t0 = 0, void (*after_chr)(hipStream_t, void*) = nullptr, void* user = nullptr)
+            int64_t t0 = 0, void (*after_chr)(hipStream_t, void*) = nullptr, void* user = nullptr,
+            float* ser = nullptr, int spcp = 0)
 {
     if (pl.inplace)
         NAJA_REQUIRE(samples == pl.inplace, NAJA_ERR_INVALID_ARGUMENT,
@@ -384,7 +385,7 @@ void launch(naja_chr_plan& pl, const float* y0, float* samples, float* y_final, 
         // y = c + P z for every saved sample of every variant (batched MFMA backmap).
         const int64_t per = (int64_t)pl.p.nchains * pl.p.nspc;
         backmap_launch_batched(pl.P, pl.c, zs, samples, pl.d, pl.d, per, pl.n_variants, (int64_t)pl.d * pl.d,
-                               pl.d, per * pl.d, per * pl.d, s);
+                               pl.d, per * pl.d, per * pl.d, s, ser, pl.p.nchains, spcp);
     }
     note_launch(pl, s);
 }
@@ -516,12 +517,24 @@ int plan_create(const float* A, const double* b, int m0, int d, int n_variants, 
 
 namespace naja {
 // A plan launch with a callback between the CHR kernel and the backmap (for
-// the conditioned run, which places its timing events there).
+// the conditioned run, which places its timing events there), optionally
+// writing the backmapped samples also in the diagnostics' series layout.
 void chr_plan_launch_hooked(naja_chr_plan* plan, const float* y0, float* samples, hipStream_t s,
-                            void (*after_chr)(hipStream_t, void*), void* user)
+                            void (*after_chr)(hipStream_t, void*), void* user, float* ser, int spcp)
 {
     NAJA_REQUIRE(plan && y0 && samples, NAJA_ERR_INVALID_ARGUMENT, "NULL plan, y0 or samples");
-    launch(*plan, y0, samples, nullptr, (int64_t)plan->p.nspc * plan->p.thinning, s, 0, after_chr, user);
+    NAJA_REQUIRE(!ser || plan->P, NAJA_ERR_INVALID_ARGUMENT, "a series copy comes from the rounding backmap");
+    launch(*plan, y0, samples, nullptr, (int64_t)plan->p.nspc * plan->p.thinning, s, 0, after_chr, user, ser, spcp);
+}
+
+// Whether a launch of this rounded plan into `samples` can also write the
+// diagnostics' series layout (its backmap runs the small-operand kernel).
+bool chr_plan_series_ok(const naja_chr_plan* plan, const float* samples)
+{
+    if (!plan || !plan->P) return false;
+    const int64_t per = (int64_t)plan->p.nchains * plan->p.nspc;
+    const float* zs = plan->inplace ? samples : plan->zbuf;
+    return backmap_series_ok(plan->d, plan->d, zs, samples, per * plan->d, per * plan->d, plan->n_variants);
 }
 
 // Packs a rounded plan's untransformed rows again from the caller's A, b,

@@ -3,9 +3,10 @@
 // exchange), the variants' rows packed again from the caller's buffers (so
 // bounds or rows changed in place between steps are sampled), the new
 // rounding set on a sampling plan (A' = A P, b' = b - A c),
-// the batched CHR launch with its backmap to y, and the batched ESS /
-// split-R-hat, captured once on the creation stream and replayed by each
-// step. Everything the graph touches is allocated at creation: the facet
+// the batched CHR launch with its backmap to y (which, for the small-operand
+// backmap, also writes the samples in the diagnostics' series layout, so the
+// diagnostics skip their transpose), and the batched ESS / split-R-hat,
+// captured once on the creation stream and replayed by each step. Everything the graph touches is allocated at creation: the facet
 // rounding's tables and status words, the plan's packs, the diagnostics
 // workspace with its twiddle tables. A step replays the graph, copies the
 // per-variant rounding status back to pinned host memory and synchronises
@@ -27,7 +28,8 @@
 namespace naja {
 // chr_plan.cpp: a plan launch with a callback between the CHR kernel and the backmap.
 void chr_plan_launch_hooked(naja_chr_plan* plan, const float* y0, float* samples, hipStream_t s,
-                            void (*after_chr)(hipStream_t, void*), void* user);
+                            void (*after_chr)(hipStream_t, void*), void* user, float* ser = nullptr, int spcp = 0);
+bool chr_plan_series_ok(const naja_chr_plan* plan, const float* samples);
 // chr_plan.cpp: the untransformed rows packed again from the caller's inputs.
 void chr_plan_repack_rows(naja_chr_plan* plan, hipStream_t s);
 }  // namespace naja
@@ -59,6 +61,7 @@ struct naja_cond_run {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool timed = false;
+    bool fused_series = false;  // the backmap writes the diagnostics' series layout too
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = -1.0f;
 };
@@ -102,9 +105,12 @@ void enqueue_step(naja_cond_run& r, hipStream_t s, bool capturing)
     check_plan(naja_chr_plan_set_rounding(r.plan, r.P, r.c));
     const bool ev = capturing && r.timed;
     if (ev) capture_event(s, r.ev0);
-    chr_plan_launch_hooked(r.plan, r.y0, r.samples, s, ev ? event_after_chr : nullptr, &r);
-    diag_launch_ws(r.diag_ws, r.samples, r.d, r.nchains * r.nspc, r.nchains, NAJA_LAYOUT_INTERLEAVED, r.ess, r.rhat, s,
-                   r.nv, r.ess_min, r.rhat_max);
+    const int n_total = r.nchains * r.nspc;
+    chr_plan_launch_hooked(r.plan, r.y0, r.samples, s, ev ? event_after_chr : nullptr, &r,
+                           r.fused_series ? diag_workspace_series(r.diag_ws) : nullptr,
+                           diag_series_stride(n_total, r.nchains));
+    diag_launch_ws(r.diag_ws, r.samples, r.d, n_total, r.nchains, NAJA_LAYOUT_INTERLEAVED, r.ess, r.rhat, s, r.nv,
+                   r.ess_min, r.rhat_max, r.fused_series);
 }
 
 // Copies the rounding status back and raises the first variant's error.
@@ -221,6 +227,7 @@ extern "C" int naja_cond_run_create(const float* A0, const double* b0, int m0, i
         check_status(*r, s);
         check_plan(naja_chr_plan_create(A0, b0, m0, d, n_variants, extra_rows, extra_A, extra_b, P, c, p, stream,
                                         &r->plan));
+        r->fused_series = chr_plan_series_ok(r->plan, samples);
         check_plan(naja_chr_plan_launch(r->plan, y0, samples, nullptr, stream));
         diag_launch_ws(r->diag_ws, samples, d, n_total, p->nchains, NAJA_LAYOUT_INTERLEAVED, ess, rhat, s, n_variants,
                        ess_min, rhat_max);

@@ -26,6 +26,13 @@ void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains
 size_t diag_workspace_bytes(int dim, int n_total, int nchains, int n_variants);
 void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_variants);
 void diag_launch_ws(void* ws, const float* samples, int dim, int n_total, int nchains, int layout, float* ess,
-                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max);
+                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max,
+                    bool series_ready = false);
+// The workspace's series array and its row stride: series_ready = true tells
+// diag_launch_ws that the caller already wrote every (variant, chain, dim)
+// series there, at ((v nchains + c) dim + d) * stride + s (interleaved
+// samples only), so its transpose is skipped.
+float* diag_workspace_series(void* ws);
+int diag_series_stride(int n_total, int nchains);
 
 }  // namespace naja

@@ -427,6 +427,13 @@ __device__ __forceinline__ int bitrev(int x, int bits)
 constexpr int kTs = 64;
 constexpr int kTd = 128;
 
+// A plain store that zeroes the redo count (a memset would not be safe
+// inside a captured graph).
+__global__ void zero_count_kernel(int* __restrict__ count)
+{
+    if (threadIdx.x == 0) *count = 0;
+}
+
 __global__ __launch_bounds__(256) void series_transpose_kernel(const float* __restrict__ samples, int dim,
                                                               int nchains, int spc, int spcp, int layout,
                                                               int64_t vstride, int s_tiles, int d_tiles,
@@ -1083,7 +1090,8 @@ void twiddle_tables(int logH, float2* out)
 // bytes); tw the twiddle tables of logH. Nothing in mem needs to start
 // zeroed: every kernel writes what it later reads.
 void diag_run(const DiagPlan& p, const float* samples, int dim, int nchains, int layout, float* ess, float* rhat,
-              float* ess_chain, char* mem, const float2* tw, hipStream_t stream, float* ess_min, float* rhat_max)
+              float* ess_chain, char* mem, const float2* tw, hipStream_t stream, float* ess_min, float* rhat_max,
+              bool series_ready = false)
 {
     float* ser = reinterpret_cast<float*>(mem);
     double* mom = reinterpret_cast<double*>(mem + p.b_ser);
@@ -1093,8 +1101,12 @@ void diag_run(const DiagPlan& p, const float* samples, int dim, int nchains, int
     const int64_t n_series = p.n_series;
     const unsigned nv = p.nv;
 
-    hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
-                       nchains, spc, spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser, redo);
+    if (series_ready) {  // the producer wrote the series layout: only the redo count starts at zero
+        hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, stream, redo);
+    } else {
+        hipLaunchKernelGGL(series_transpose_kernel, dim3((unsigned)p.t_blocks), dim3(256), 0, stream, samples, dim,
+                           nchains, spc, spcp, layout, p.vstride, p.s_tiles, p.d_tiles, ser, redo);
+    }
     NAJA_HIP(hipGetLastError());
     // A half-size pass first when it leaves at least 1/8 of the chain as exact
     // lags (N/2 - n >= n/8); the full pass then redoes only the series whose
@@ -1290,12 +1302,21 @@ void diag_workspace_init(void* ws, int dim, int n_total, int nchains, int n_vari
 }
 
 void diag_launch_ws(void* ws, const float* samples, int dim, int n_total, int nchains, int layout, float* ess,
-                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max)
+                    float* rhat, hipStream_t stream, int n_variants, float* ess_min, float* rhat_max,
+                    bool series_ready)
 {
     const DiagPlan p = diag_plan(dim, n_total, nchains, n_variants, true);
     char* mem = static_cast<char*>(ws);
     const float2* tw = reinterpret_cast<const float2*>(mem + p.main_bytes());
-    diag_run(p, samples, dim, nchains, layout, ess, rhat, nullptr, mem, tw, stream, ess_min, rhat_max);
+    diag_run(p, samples, dim, nchains, layout, ess, rhat, nullptr, mem, tw, stream, ess_min, rhat_max, series_ready);
+}
+
+float* diag_workspace_series(void* ws) { return static_cast<float*>(ws); }
+
+int diag_series_stride(int n_total, int nchains)
+{
+    const int spc = n_total / nchains;
+    return (spc + kTs - 1) / kTs * kTs;
 }
 
 void diag_launch_noalloc(const float* samples, int dim, int n_total, int nchains, int layout, float* ess, float* rhat,

@@ -64,6 +64,7 @@ namespace {
 
 constexpr int kFacetThreads = 512;  // one candidate row per thread
 constexpr int kFacetMaxD = 128;
+static_assert(kFacetMaxD <= 128 && kFacetThreads % 128 == 0, "the elimination maps 128 columns x row groups");
 
 // Block-wide argmax of (value, index), larger value first, then smaller index.
 __device__ void block_argmax(float& val, int& idx, float* rv, int* ri)
@@ -136,11 +137,18 @@ __device__ bool gj_inverse(double* M, int d, double* col, int* piv, int* flag)
             if (p != k) M[p * d + j] = rk;
         }
         __syncthreads();
-        for (int e = tid; e < d * d; e += kFacetThreads) {
-            const int i = e / d, j = e % d;
-            if (i == k) continue;
-            const double f = col[i];
-            M[e] = (j == k) ? -f * inv : fma(-f, M[k * d + j], M[e]);
+        // Thread (ig, j) = (tid / 128, tid % 128) updates column j of rows
+        // ig, ig + 4, ...: no integer division per element (d <= 128).
+        {
+            const int j = tid & 127;
+            if (j < d) {
+                const double mkj = M[k * d + j];
+                for (int i = tid >> 7; i < d; i += kFacetThreads / 128) {
+                    if (i == k) continue;
+                    const double f = col[i];
+                    M[i * d + j] = (j == k) ? -f * inv : fma(-f, mkj, M[i * d + j]);
+                }
+            }
         }
         __syncthreads();
     }

@@ -37,19 +37,23 @@ def eager_step(naja_amd, t, batch, nspc, thin, nch, seed, resync):
     return out
 
 
-@pytest.mark.parametrize("d,ks", [(40, [0, 6, 30, 52, 80, 120, 160, 200, 2, 90, 140, 60]),
-                                  (100, [0, 40, 120, 200, 260, 300, 150, 88])])
-def test_graph_replays_equal_the_eager_pipeline(gpu, d, ks):
+@pytest.mark.parametrize("d,ks,nch", [(40, [0, 6, 30, 52, 80, 120, 160, 200, 2, 90, 140, 60], 4),
+                                      (100, [0, 40, 120, 200, 260, 300, 150, 88], 4),
+                                      # d % 4 != 0: no series copy from the backmap, the diagnostics transpose
+                                      (30, [0, 10, 44, 70, 120], 4),
+                                      # 3 chains: the backmap's series copy by its general path
+                                      (40, [0, 20, 64, 100, 150], 3)])
+def test_graph_replays_equal_the_eager_pipeline(gpu, d, ks, nch):
     import naja_amd
 
     t = gpu
     A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
     EA, Eb = conditions(d, ks, seed=d)
     batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
-    nspc, thin, nch, seed, resync = 700, 4, 4, 19, 300
+    nspc, thin, seed, resync = 700, 4, 19, 300
     ref = eager_step(naja_amd, t, batch, nspc, thin, nch, seed, resync)
     names = ("P", "samples", "ess", "rhat", "ess_min", "rhat_max")
-    print(f"[cond-run d{d}] {len(ks)} variants, {nch} x {nspc} x thin {thin}: creating", flush=True)
+    print(f"[cond-run d{d} c{nch}] {len(ks)} variants, {nch} x {nspc} x thin {thin}: creating", flush=True)
     run = naja_amd.ConditionedRun(batch, nspc, thin, nch, seed=seed, resync_interval=resync, timed=True)
     side = t.randn((512, 512), device="cuda")
     for rep in range(4):

@@ -571,71 +571,73 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
         constexpr bool kPfAll = decltype(pf_all)::value;
         const Move mv = fm[0];
         const int mtype = CO ? kMoveCoord : mv.type;
-        const bool event = u == next_event;
-        if (event && u == next_save + 1) {
-            save_sample();
-            next_save += a.thin;
-        }
-        if (event && u == next_resync) {
-            // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
-            // The partner row's FMA chain over -a gives exactly -acc (a zero
-            // sum stays +0 in both: RN(x + y) is -0 only when x and y are),
-            // so its slack is bn - (acc != 0 ? -acc : +0), no second chain.
-            double acc[G][R];
-#pragma unroll
-            for (int k = 0; k < R; ++k)
-#pragma unroll
-                for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
-            const float* pc = Av;
-            auto add_col = [&](const double (&xc)[G]) {
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const double av = (double)pc[row_of(k)];
-#pragma unroll
-                    for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xc[g], acc[g][k]);
-                }
-                pc += lda;
-            };
-            if constexpr (XR > 0) {
-#pragma unroll
-                for (int q = 0; q < XR; ++q)
-                    for (int l = 0; l < 64 && q * 64 + l < d; ++l) {
+        // Events are rare (a sample every thin steps, a draw batch every 64,
+        // a resync every resync steps): the hint keeps the step's common path
+        // straight-line, with the event path out of line.
+        if (__builtin_expect(u == next_event, 0)) {
+            if (u == next_save + 1) {
+                save_sample();
+                next_save += a.thin;
+            }
+            if (u == next_resync) {
+                // s = b - A y in FP64 (chr.cu:386-395); also the initial slack.
+                // The partner row's FMA chain over -a gives exactly -acc (a zero
+                // sum stays +0 in both: RN(x + y) is -0 only when x and y are),
+                // so its slack is bn - (acc != 0 ? -acc : +0), no second chain.
+                double acc[G][R];
+    #pragma unroll
+                for (int k = 0; k < R; ++k)
+    #pragma unroll
+                    for (int g = 0; g < G; ++g) acc[g][k] = 0.0;
+                const float* pc = Av;
+                auto add_col = [&](const double (&xc)[G]) {
+    #pragma unroll
+                    for (int k = 0; k < R; ++k) {
+                        const double av = (double)pc[row_of(k)];
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) acc[g][k] = fma(av, xc[g], acc[g][k]);
+                    }
+                    pc += lda;
+                };
+                if constexpr (XR > 0) {
+    #pragma unroll
+                    for (int q = 0; q < XR; ++q)
+                        for (int l = 0; l < 64 && q * 64 + l < d; ++l) {
+                            double xc[G];
+    #pragma unroll
+                            for (int g = 0; g < G; ++g) xc[g] = (double)readlane_f32(xv[g][q], l);
+                            add_col(xc);
+                        }
+                } else {
+                    __syncthreads();
+                    for (int c = 0; c < d; ++c) {
                         double xc[G];
-#pragma unroll
-                        for (int g = 0; g < G; ++g) xc[g] = (double)readlane_f32(xv[g][q], l);
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) xc[g] = (double)xs[g * d + c];
                         add_col(xc);
                     }
-            } else {
-                __syncthreads();
-                for (int c = 0; c < d; ++c) {
-                    double xc[G];
-#pragma unroll
-                    for (int g = 0; g < G; ++g) xc[g] = (double)xs[g * d + c];
-                    add_col(xc);
                 }
-            }
-            const double* bv = a.b + V.b_off;
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double bb = bv[row_of(k)];
-#pragma unroll
-                for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
-                if constexpr (PAIRED) {
-                    const double bn = a.bn[V.b_off + row_of(k)];
-#pragma unroll
-                    for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0));
+                const double* bv = a.b + V.b_off;
+    #pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const double bb = bv[row_of(k)];
+    #pragma unroll
+                    for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
+                    if constexpr (PAIRED) {
+                        const double bn = a.bn[V.b_off + row_of(k)];
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0));
+                    }
                 }
+                prev_i = -1;
+                prev_j = -1;
+                // No resync: a step the launch never reaches (fewer than 2^31 steps).
+                next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
             }
-            prev_i = -1;
-            prev_j = -1;
-            // No resync: a step the launch never reaches (fewer than 2^31 steps).
-            next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
-        }
-        if (event && u == next_batch) {
-            gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
-            next_batch += 64;
-        }
-        if (event) {
+            if (u == next_batch) {
+                gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
+                next_batch += 64;
+            }
             // Move logging is an event of every step while it lasts.
             if (log_on && u < (int)a.nlog && tid == 0) {
                 const int64_t tl = u;

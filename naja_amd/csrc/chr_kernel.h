@@ -139,9 +139,9 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
                 size_t lds_bytes = 0);
 // Whether (and with how much LDS) a launch of shape sh can use the
 // LDS-column kernel: one wavefront, one chain per group, coordinate moves
-// only, d <= kChrRegStateMaxD, nchains a multiple of kChrLdsChains, and the
-// largest variant's columns (max_rows packed rows) within the LDS.
-size_t chr_lds_columns_bytes(const ChrShape& sh, int d, int max_rows);
+// only, d <= kChrRegStateMaxD, nchains a multiple of kChrLdsChains, and a
+// variant's d columns of 64 W R packed rows within the LDS.
+size_t chr_lds_columns_bytes(const ChrShape& sh, int d);
 bool chr_has_lds_kernel(const ChrShape& sh, bool paired);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to

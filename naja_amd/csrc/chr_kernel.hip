@@ -350,10 +350,11 @@ int chr_timing_collect(int* items, float* ms, int max)
     return n;
 }
 
-size_t chr_lds_columns_bytes(const ChrShape& sh, int d, int max_rows)
+size_t chr_lds_columns_bytes(const ChrShape& sh, int d)
 {
-    const int vec = (sh.R % 4 == 0) ? 4 : ((sh.R % 2 == 0) ? 2 : 1);
-    return sizeof(float) * (size_t)d * (size_t)((max_rows + vec - 1) / vec * vec + vec);
+    // Columns at the packed block's stride, 64 W R rows (every slot of every
+    // lane), so a column's byte offset is the same in LDS and in the block.
+    return sizeof(float) * (size_t)d * (size_t)(64 * sh.W * sh.R);
 }
 
 bool chr_has_lds_kernel(const ChrShape& sh, bool paired)

@@ -221,14 +221,16 @@ struct MoveBatch {
 
 // CO: the launch has no pair or k-sparse moves (every draw is a coordinate
 // move), so only the index is kept; the draws are the same.
+// CO: tij is the index times col_scale (the column's byte offset when the
+// caller passes its column stride, else 1).
 template <int G, bool CO>
 __device__ __forceinline__ void gen_batch(const ChrArgs& a, uint32_t k0, uint32_t k1, uint64_t tb, uint32_t group,
-                                          int c0, MoveBatch<G>& mb)
+                                          int c0, MoveBatch<G>& mb, int col_scale = 1)
 {
     const uint64_t t = tb + (uint64_t)(threadIdx.x & 63);
     if (CO) {
         const U32x4 w = philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), group, 0u, k0, k1);
-        mb.tij = (int)rng_index(w.y, (uint32_t)a.d);
+        mb.tij = (int)rng_index(w.y, (uint32_t)a.d) * col_scale;
     } else {
         const Move mv = draw_move(a, k0, k1, t, group);
         mb.tij = (mv.type << 30) | mv.i;
@@ -358,16 +360,21 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     const uint32_t k1 = a.seed_hi ^ (uint32_t)item.variant;
     // NC > 1: the workgroup's chains belong to one variant (items come in
     // groups of NC); its packed A is copied into LDS once, column c at
-    // xs[c * lstride]: the rows below mpad (m rounded up to VEC), then VEC
-    // zeros that the lanes past the real rows read.
-    const int lstride = (V.m + VEC - 1) / VEC * VEC + VEC;
+    // xs[c * lda] (the packed block's own stride, so one byte offset
+    // addresses a column in both), the rows past m as zeros.
     if constexpr (NC > 1) {
-        for (int e = (int)threadIdx.x; e < lstride * d; e += 64 * NC) {
-            const int c = e / lstride, r = e % lstride;
+        for (int e = (int)threadIdx.x; e < lda * d; e += 64 * NC) {
+            const int c = e / lda, r = e % lda;
             xs[e] = r < V.m ? Av[(int64_t)c * lda + r] : 0.0f;
         }
         __syncthreads();
     }
+    // CO with VGPR states: a draw carries its column's byte offset i * 4 lda,
+    // so a step forms no column address (the buffer loads take it as their
+    // scalar offset, the LDS reads add it to the lane's row offset) and the
+    // state update compares it with the lanes' coordinate offsets.
+    constexpr bool kColOff = CO && XR > 0;
+    const int col_scale = kColOff ? 4 * lda : 1;
 
     float xv[G][XS];
     if constexpr (XR > 0) {
@@ -395,7 +402,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     const int64_t t_begin = a.t0;
     const int64_t t_end = a.t0 + a.nsteps;
     MoveBatch<G> mb;
-    gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb);
+    gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin & ~(int64_t)63), group, item.c0, mb, col_scale);
     const int L0 = (int)(t_begin & 63);
     // (A[:,i], 1/A[:,i]) for a coordinate move, (A[:,i], A[:,j]) for a pair
     // move; a k-sparse move (or no move) loads a dummy pair. One buffer
@@ -412,22 +419,22 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     const int col_stride = 4 * lda;  // bytes from one column to the next
     const int mpad = (V.m + VEC - 1) / VEC * VEC;
     int voff[R / VEC];
-    int loff[R / VEC];  // LDS column offsets (NC > 1): the run, or the zero run
+    int loff[R / VEC];  // LDS byte offsets of the lane's runs (NC > 1)
 #pragma unroll
     for (int q = 0; q < R / VEC; ++q) {
         const int r0 = q * VEC * T + tid * VEC;
         voff[q] = r0 < mpad ? 4 * r0 : (int)0x80000000u;
-        loff[q] = r0 < mpad ? r0 : mpad;
+        loff[q] = 4 * r0;
     }
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
         const int mt = CO ? kMoveCoord : m.type;
         const bool real = real_move && mt != kMoveKsparse;
         const bool pair = real && mt == kMovePair;
-        const int si = real ? m.i * col_stride : 0;
+        const int si = real ? (kColOff ? m.i : m.i * col_stride) : 0;
         const int sj = pair ? m.j * col_stride : si;
         const chr_i32x4 rj = pair ? rsA : rsI;
         if constexpr (NC > 1) {
-            const float* colp = xs + (real ? m.i : 0) * lstride;
+            const char* colp = reinterpret_cast<const char*>(xs) + si;
 #pragma unroll
             for (int q = 0; q < R / VEC; ++q) {
                 if (VEC == 4) {
@@ -447,7 +454,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
                     xb[q * 2] = w[0];
                     xb[q * 2 + 1] = w[1];
                 } else {
-                    xa[q] = colp[loff[q]];
+                    xa[q] = *reinterpret_cast<const float*>(colp + loff[q]);
                     xb[q] = chr_raw_load_f32(rsI, voff[q], si, 0);
                 }
             }
@@ -492,7 +499,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) fu[k][g] = 0.0f;
         if (t_begin + k < t_end) {
             const int Lk = (L0 + k) & 63;
-            if (Lk == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + k), group, item.c0, mb);
+            if (Lk == 0) gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + k), group, item.c0, mb, col_scale);
             fm[k] = read_move<G, CO>(mb, Lk);
 #pragma unroll
             for (int g = 0; g < G; ++g) fu[k][g] = readlane_f32(mb.u[g], Lk);
@@ -547,11 +554,12 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     };
 
     // x_c <- fmaf(alpha, coef, x_c) for chain g, in the lane that holds c.
+    // With kColOff, c is the coordinate's column offset.
     auto xr_update = [&](int g, int c, float alpha_g, float coef) {
 #pragma unroll
         for (int q = 0; q < XS; ++q) {
             const float nv = fmaf(alpha_g, coef, xv[g][q]);
-            xv[g][q] = (q * 64 + lane == c) ? nv : xv[g][q];
+            xv[g][q] = ((q * 64 + lane) * col_scale == c) ? nv : xv[g][q];
         }
     };
 
@@ -635,14 +643,14 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
                 next_resync = a.resync > 0 ? u + (int64_t)a.resync : (int64_t)0x7FFFFFFF;
             }
             if (u == next_batch) {
-                gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb);
+                gen_batch<G, CO>(a, k0, k1, (uint64_t)(t_begin + u + PF), group, item.c0, mb, col_scale);
                 next_batch += 64;
             }
             // Move logging is an event of every step while it lasts.
             if (log_on && u < (int)a.nlog && tid == 0) {
                 const int64_t tl = u;
                 a.move_log[tl * 3 + 0] = mtype;
-                a.move_log[tl * 3 + 1] = mv.i;
+                a.move_log[tl * 3 + 1] = kColOff ? mv.i / col_scale : mv.i;
                 a.move_log[tl * 3 + 2] = (mtype == kMovePair) ? mv.j : -1;
             }
             int64_t e = next_resync < next_save + 1 ? next_resync : next_save + 1;

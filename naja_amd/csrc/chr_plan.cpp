@@ -213,10 +213,8 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
     }
     // The LDS-column kernel: one wavefront per chain and four chains of one
     // variant per workgroup, when the launch's moves, d, chain count, kernels
-    // and the largest variant's columns allow it.
+    // and the columns (d x 64 W R packed rows) allow it.
     {
-        int max_rows = 0;
-        for (const ChrVariant& V : pl.var) max_rows = std::max(max_rows, V.m);
         const naja_chr_params* p = &pl.p;
         const bool has_pair = p->pair_prob > 0.0 && !(p->pair_mode == 2 && !(p->n_pairs > 0 && p->pair_i &&
                                                                                p->pair_j && p->pair_c && p->pair_s));
@@ -225,7 +223,7 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         size_t lds = 0;
         if (sh.W == 1 && sh.G == 1 && coord_only && pl.d <= kChrRegStateMaxD && p->nchains % kChrLdsChains == 0 &&
             chr_has_lds_kernel(sh, pl.paired)) {
-            lds = chr_lds_columns_bytes(sh, pl.d, max_rows);
+            lds = chr_lds_columns_bytes(sh, pl.d);
             if (lds <= 156u * 1024u) nc = kChrLdsChains;
         }
         if (nc != pl.nc) {

@@ -654,9 +654,24 @@ __global__ __launch_bounds__(kDiagThreads) void ess_series_kernel(const float* _
             // k = H/4: b = a with the one pair {H/4, 3H/4}). The twiddles
             // come from W^k: W^(H-k) = -conj W^k, W^(H/2+k) = -i W^k,
             // W^(H/2-k) = -i conj W^k.
-            for (int k = tid; k <= H / 4; k += blockDim.x) {
-                const int a = bitrev(k, logH) >> 1;
-                const int b = bitrev((H / 2 - k) & (H - 1), logH) >> 1;
+            // Threads walk the slot pairs in order rather than k (whose
+            // slots are scattered over the LDS banks): a = bitrev(k) / 2 is k
+            // reversed in logH - 1 bits, and reversing H/2 - k = -k mod H/2
+            // gives b = a ^ (2^msb(a) - 1) for a > 0. Item 0 is k = 0 (a = 0),
+            // item 1 is k = H/4 (a = 1); item i >= 2 takes the pair {c, c ^
+            // (2^msb(c) - 1)} with c = i' + 2^msb(i'), i' = i - 1 (each pair
+            // of [2^j, 2^(j+1)) once), and its even member as a, which is the
+            // one with k < H/4 (k's top two bits are a's low bit and 0).
+            for (int it = tid; it <= H / 4; it += blockDim.x) {
+                int a = it, b = it;
+                if (it >= 2) {
+                    const int ip = it - 1;
+                    const int hb = 1 << (31 - __builtin_clz(ip));
+                    const int c = ip + hb, cp = c ^ (2 * hb - 1);
+                    a = (c & 1) ? cp : c;
+                    b = (c & 1) ? c : cp;
+                }
+                const int k = bitrev(2 * a, logH);
                 const float2 ua = z[2 * a], va = z[2 * a + 1], ub = z[2 * b], vb = z[2 * b + 1];
                 const float2 Zk = cadd(ua, va), Zkh = csub(ua, va), Zm = cadd(ub, vb), Zmh = csub(ub, vb);
                 const float2 w = tw.at(k);

@@ -189,6 +189,7 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
     __shared__ int flag;
     __shared__ double xs[kFacetMaxD];
     __shared__ double dots[kFacetMaxD];
+    __shared__ double upd[kFacetThreads / 128][kFacetMaxD];
 
     const int v = blockIdx.x, tid = threadIdx.x;
     const int k = ek[v];
@@ -405,17 +406,42 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
             for (int c = tid; c < d; c += kFacetThreads) col[c] = cand64(i, c);
             __syncthreads();
             for (int pass = 0; pass < 2; ++pass) {
-                for (int j = tid; j < t; j += kFacetThreads) {
-                    double s = 0.0;
-                    for (int c = 0; c < d; ++c) s = fma(M[j * d + c], col[c], s);
-                    dots[j] = s;
+                // dots[j] = Q_j . col: four threads per row j (t < 128), each
+                // over the columns c = part (mod 4) with two accumulators, the
+                // four partial sums added over the quad.
+                if (tid < 4 * t) {
+                    const int j = tid >> 2, part = tid & 3;
+                    double s0 = 0.0, s1 = 0.0;
+                    int c = part;
+                    for (; c + 4 < d; c += 8) {
+                        s0 = fma(M[j * d + c], col[c], s0);
+                        s1 = fma(M[j * d + c + 4], col[c + 4], s1);
+                    }
+                    if (c < d) s0 = fma(M[j * d + c], col[c], s0);
+                    double s = s0 + s1;
+                    s += __shfl_xor(s, 1);
+                    s += __shfl_xor(s, 2);
+                    if (part == 0) dots[j] = s;
                 }
                 __syncthreads();
-                for (int c = tid; c < d; c += kFacetThreads) {
-                    double w = col[c];
-                    for (int j = 0; j < t; ++j) w = fma(-dots[j], M[j * d + c], w);
-                    col[c] = w;
+                // col -= sum_j dots[j] Q_j: thread (g, c) = (tid / 128, tid %
+                // 128) sums the rows j = g (mod 4), then column c adds the four.
+                {
+                    const int c = tid & 127, g = tid >> 7;
+                    if (c < d) {
+                        double w0 = 0.0, w1 = 0.0;
+                        int j = g;
+                        for (; j + 4 < t; j += 8) {
+                            w0 = fma(dots[j], M[j * d + c], w0);
+                            w1 = fma(dots[j + 4], M[(j + 4) * d + c], w1);
+                        }
+                        if (j < t) w0 = fma(dots[j], M[j * d + c], w0);
+                        upd[g][c] = w0 + w1;
+                    }
                 }
+                __syncthreads();
+                for (int c = tid; c < d; c += kFacetThreads)
+                    col[c] = col[c] - ((upd[0][c] + upd[1][c]) + (upd[2][c] + upd[3][c]));
                 __syncthreads();
             }
             if (tid < 64) {

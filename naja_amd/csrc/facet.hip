@@ -138,15 +138,29 @@ __device__ bool gj_inverse(double* M, int d, double* col, int* piv, int* flag)
         }
         __syncthreads();
         // Thread (ig, j) = (tid / 128, tid % 128) updates column j of rows
-        // ig, ig + 4, ...: no integer division per element (d <= 128).
+        // ig, ig + 4, ...: no integer division per element (d <= 128). Rows
+        // go in batches of kGjBatch whose LDS reads are all issued before any
+        // write (the rows are independent), so the reads' latency is paid
+        // once per batch rather than once per row.
         {
+            constexpr int kGjBatch = 8;
+            constexpr int kStride = kFacetThreads / 128;
             const int j = tid & 127;
             if (j < d) {
                 const double mkj = M[k * d + j];
-                for (int i = tid >> 7; i < d; i += kFacetThreads / 128) {
-                    if (i == k) continue;
-                    const double f = col[i];
-                    M[i * d + j] = (j == k) ? -f * inv : fma(-f, mkj, M[i * d + j]);
+                for (int i0 = tid >> 7; i0 < d; i0 += kGjBatch * kStride) {
+                    double f[kGjBatch], m[kGjBatch];
+#pragma unroll
+                    for (int b = 0; b < kGjBatch; ++b) {
+                        const int i = i0 + b * kStride;
+                        f[b] = i < d ? col[i] : 0.0;
+                        m[b] = i < d ? M[i * d + j] : 0.0;
+                    }
+#pragma unroll
+                    for (int b = 0; b < kGjBatch; ++b) {
+                        const int i = i0 + b * kStride;
+                        if (i < d && i != k) M[i * d + j] = (j == k) ? -f[b] * inv : fma(-f[b], mkj, m[b]);
+                    }
                 }
             }
         }
@@ -343,9 +357,13 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
 
     // Exchange, when the chosen facets fill the basis: C = U N^-1 for the facet
     // candidates (FP32 rows, one per thread, summed in column order), then
-    // rank-1 updates while the largest |C_ik| exceeds the tolerance.
+    // rank-1 updates while the largest |C_ik| exceeds the tolerance. N^-1
+    // (FP64, LDS M) follows each exchange by the same rank-1 form
+    // (Sherman-Morrison: row k of N becomes u_i, g = u_i N^-1, N^-1 -=
+    // N^-1 e_k (g - e_k) / g_k), so it is P at the end.
     int nfac = 0;
     for (int t = 0; t < d; ++t) nfac += sel[t] < nf;
+    bool have_inverse = false;
     if (exchange_tol > 0.0f) {
         if (nfac == d) {
             for (int e = tid; e < d * d; e += kFacetThreads) M[e] = cand64(sel[e / d], e % d);
@@ -356,13 +374,28 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
             }
 #pragma unroll
             for (int c = 0; c < COLS; ++c) r[c] = 0.0f;
-            if (is_facet)
-                for (int j = 0; j < d; ++j) {
-                    const float uj = (float)cand64(tid, j);
+            if (is_facet) {
+                // The candidate's dual row comes from global memory in batches
+                // of eight entries, all loads issued before the first is used.
+                constexpr int kUB = 8;
+                const int rr = poly_row(tid);
+                const double st = sl[tid];
+                for (int j0 = 0; j0 < d; j0 += kUB) {
+                    double a8[kUB];
 #pragma unroll
-                    for (int c = 0; c < COLS; ++c)
-                        if (c < d) r[c] = r[c] + uj * (float)M[j * d + c];
+                    for (int b = 0; b < kUB; ++b) a8[b] = j0 + b < d ? arow(rr, j0 + b) : 0.0;
+#pragma unroll
+                    for (int b = 0; b < kUB; ++b) {
+                        const int j = j0 + b;
+                        if (j < d) {
+                            const float uj = (float)(a8[b] / st);
+#pragma unroll
+                            for (int c = 0; c < COLS; ++c)
+                                if (c < d) r[c] = r[c] + uj * (float)M[j * d + c];
+                        }
+                    }
                 }
+            }
             for (int it = 0; it < 4 * d; ++it) {
                 float best = -1.0f;
                 int bi = 0x7fffffff;
@@ -392,8 +425,41 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
                     for (int c = 0; c < COLS; ++c) r[c] = r[c] - f * (q[c] - (c == pk ? 1.0f : 0.0f));
                 }
                 if (tid == 0) sel[pk] = pi;
+                // N^-1 after row pk of N becomes u_pi: col = u_pi (FP64),
+                // xs = column pk of N^-1, dots = g = u_pi N^-1 (four threads
+                // per column, as in the completion below).
+                for (int c = tid; c < d; c += kFacetThreads) {
+                    col[c] = cand64(pi, c);
+                    xs[c] = M[c * d + pk];
+                }
+                __syncthreads();
+                {
+                    const int c = tid & 127, g = tid >> 7;
+                    if (c < d) {
+                        double w0 = 0.0, w1 = 0.0;
+                        int j = g;
+                        for (; j + 4 < d; j += 8) {
+                            w0 = fma(col[j], M[j * d + c], w0);
+                            w1 = fma(col[j + 4], M[(j + 4) * d + c], w1);
+                        }
+                        if (j < d) w0 = fma(col[j], M[j * d + c], w0);
+                        upd[g][c] = w0 + w1;
+                    }
+                }
+                __syncthreads();
+                for (int c = tid; c < d; c += kFacetThreads)
+                    dots[c] = (upd[0][c] + upd[1][c]) + (upd[2][c] + upd[3][c]);
+                __syncthreads();
+                {
+                    const double gk = dots[pk];
+                    for (int e = tid; e < d * d; e += kFacetThreads) {
+                        const int i = e / d, c = e - i * d;
+                        M[e] -= xs[i] * ((dots[c] - (c == pk ? 1.0 : 0.0)) / gk);
+                    }
+                }
                 __syncthreads();
             }
+            have_inverse = true;
         }
     }
 
@@ -461,12 +527,12 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(
         }
         for (int e = tid; e < d * d; e += kFacetThreads)
             if (sel[e / d] < nf) M[e] = cand64(sel[e / d], e % d);
-    } else {
+    } else if (!have_inverse) {
         // N (the chosen dual rows, FP64).
         for (int e = tid; e < d * d; e += kFacetThreads) M[e] = cand64(sel[e / d], e % d);
     }
     __syncthreads();
-    if (!gj_inverse(M, d, col, piv, &flag)) {
+    if (!have_inverse && !gj_inverse(M, d, col, piv, &flag)) {
         if (tid == 0) status[v] = NAJA_ERR_RUNTIME;
         return;
     }

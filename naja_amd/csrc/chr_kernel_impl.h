@@ -184,12 +184,13 @@ __device__ __forceinline__ void lane_min2(const uint32_t (&v)[N], uint32_t& hb, 
 // needed: ae = 0 gives +-inf and 0/0 a NaN, which never win against a real
 // facet. In exact arithmetic hi = 1/max(ae/s) and lo = 1/min(ae/s), the
 // reference's ends (chr.cu:416-438), each rounded once here instead of twice.
-// A slack that FP32 rounding left at or below zero is replaced by +0, so a
-// point just outside a facet is treated as on it (the FP64 reference has no
-// such clamp). The clamp is a signed integer maximum with 0 over the bit
-// pattern (one instruction): negative values and -0 become +0, positive
-// values pass, and so does a NaN with the sign clear, whose quotient is a NaN
-// that never wins either minimum (no finite input produces one).
+// Every slack is stored clamped: a value that FP32 rounding left at or below
+// zero, after an update or a resync, is replaced by +0, so a point just
+// outside a facet is treated as on it (the FP64 reference has no such
+// clamp). The clamp is a signed integer maximum with 0 over the bit pattern
+// (one instruction): negative values and -0 become +0, positive values pass,
+// and so does a NaN with the sign clear, whose quotient is a NaN that never
+// wins either minimum (no finite input produces one).
 __device__ __forceinline__ float clamp_slack(float s)
 {
     return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, s), 0));
@@ -197,14 +198,14 @@ __device__ __forceinline__ float clamp_slack(float s)
 
 __device__ __forceinline__ uint32_t chord_bits(float ae, float s)
 {
-    return __builtin_bit_cast(uint32_t, clamp_slack(s) / ae);
+    return __builtin_bit_cast(uint32_t, s / ae);
 }
 
 // Coordinate moves use the packed reciprocal column rinv = RN(1/A[r,i]):
 // q = RN(s * rinv), one multiply instead of an IEEE division.
 __device__ __forceinline__ uint32_t chord_bits_rcp(float rinv, float s)
 {
-    return __builtin_bit_cast(uint32_t, clamp_slack(s) * rinv);
+    return __builtin_bit_cast(uint32_t, s * rinv);
 }
 
 // Lane l of every wavefront draws the move and the alpha uniforms of step
@@ -630,11 +631,12 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
                 for (int k = 0; k < R; ++k) {
                     const double bb = bv[row_of(k)];
     #pragma unroll
-                    for (int g = 0; g < G; ++g) s[g][k] = (float)(bb - acc[g][k]);
+                    for (int g = 0; g < G; ++g) s[g][k] = clamp_slack((float)(bb - acc[g][k]));
                     if constexpr (PAIRED) {
                         const double bn = a.bn[V.b_off + row_of(k)];
     #pragma unroll
-                        for (int g = 0; g < G; ++g) sn[g][k] = (float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0));
+                        for (int g = 0; g < G; ++g)
+                            sn[g][k] = clamp_slack((float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0)));
                     }
                 }
                 prev_i = -1;
@@ -796,10 +798,10 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) {
             const float nal = -alpha[g];
 #pragma unroll
-            for (int k = 0; k < R; ++k) s[g][k] = fmaf(nal, ae[k], s[g][k]);
+            for (int k = 0; k < R; ++k) s[g][k] = clamp_slack(fmaf(nal, ae[k], s[g][k]));
             if constexpr (PAIRED) {
 #pragma unroll
-                for (int k = 0; k < R; ++k) sn[g][k] = fmaf(alpha[g], ae[k], sn[g][k]);
+                for (int k = 0; k < R; ++k) sn[g][k] = clamp_slack(fmaf(alpha[g], ae[k], sn[g][k]));
             }
         }
 
@@ -982,7 +984,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
             for (int k = 0; k < R; ++k) {
                 double acc = 0.0;
                 for (int c = 0; c < d; ++c) acc = fma((double)ld(c, k), (double)xs[c], acc);
-                s[k] = (float)(ldb(k) - acc);
+                s[k] = clamp_slack((float)(ldb(k) - acc));
             }
             prev_i = -1;
             prev_j = -1;
@@ -1076,7 +1078,7 @@ __global__ __launch_bounds__(64 * W) void chr_big_kernel(ChrArgs a)
 
         const float nal = -alpha;
 #pragma unroll
-        for (int k = 0; k < R; ++k) s[k] = fmaf(nal, X[k], s[k]);
+        for (int k = 0; k < R; ++k) s[k] = clamp_slack(fmaf(nal, X[k], s[k]));
 
         if (mv.type != kMoveKsparse) {
             const float ni = fmaf(alpha, mv.ci, oi);

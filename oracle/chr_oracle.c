@@ -33,10 +33,11 @@
  *     lo = 1/min(ae/s), the reference's ends; here each end is one rounding
  *     of the exact value instead of two. ae = 0 gives +-inf and 0/0 NaN,
  *     neither of which can win a minimum against a real facet;
- *   - a slack s <= 0 is replaced by +0 first: a point that FP32 rounding
- *     left just outside a facet is treated as on it (q = +-0 pins that end
- *     of the chord at the current point), so the chain cannot run through
- *     the violated facet. The FP64 reference has no clamp.
+ *   - a slack that FP32 rounding leaves <= 0, after an update or a resync,
+ *     is stored as +0: a point just outside a facet is treated as on it
+ *     (q = +-0 pins that end of the chord at the current point), so the
+ *     chain cannot run through the violated facet. The FP64 reference has
+ *     no clamp.
  * The FP64 variant (orc_chr_run_f64) keeps every quantity in double on the
  * same random stream; it measures FP32 drift.
  */
@@ -170,12 +171,25 @@ typedef struct {
     int c_begin, c_end;
 } orc_job32;
 
+/* s <= 0 (and -0) -> +0 as a signed integer max over the bits; a NaN with the
+   sign clear passes and never wins a minimum. Applied whenever a slack is
+   formed (resync and update), so stored slacks are never negative. */
+static float orc_clamp_slack(float v)
+{
+    int32_t bits;
+    memcpy(&bits, &v, 4);
+    if (bits < 0) bits = 0;
+    float out;
+    memcpy(&out, &bits, 4);
+    return out;
+}
+
 static void orc_resync32(const float* A, const double* b, const float* x, float* s, int m, int d)
 {
     for (int r = 0; r < m; ++r) {
         double acc = 0.0;
         for (int c = 0; c < d; ++c) acc = fma((double)A[(size_t)c * m + r], (double)x[c], acc);
-        s[r] = (float)(b[r] - acc);
+        s[r] = orc_clamp_slack((float)(b[r] - acc));
     }
 }
 
@@ -217,13 +231,7 @@ static void* orc_chain_worker32(void* arg)
                     for (int q = 0; q < mv.k; ++q) a = fmaf(J->A[(size_t)mv.idx[q] * m + r], mv.coef[q], a);
                 }
                 ae[r] = a;
-                /* s <= 0 (and -0) -> +0 as a signed integer max over the bits; a
-                   NaN with the sign clear passes and never wins a minimum */
-                int32_t sbits;
-                memcpy(&sbits, &s[r], 4);
-                if (sbits < 0) sbits = 0;
-                float sp;
-                memcpy(&sp, &sbits, 4);
+                const float sp = s[r]; /* stored clamped */
                 /* coordinate moves: q = RN(s * RN(1/a)); other moves: RN(s / ae) */
                 const float q = (mv.type == ORC_MOVE_COORD) ? sp * (1.0f / a) : sp / a;
                 uint32_t qb;
@@ -249,7 +257,7 @@ static void* orc_chain_worker32(void* arg)
                 memcpy(J->samples + (size_t)col * d, x, sizeof(float) * d);
             }
             const float na = -alpha;
-            for (int r = 0; r < m; ++r) s[r] = fmaf(na, ae[r], s[r]);
+            for (int r = 0; r < m; ++r) s[r] = orc_clamp_slack(fmaf(na, ae[r], s[r]));
         }
         if (J->y_final) memcpy(J->y_final + (size_t)c * d, x, sizeof(float) * d);
         if (J->s_final) memcpy(J->s_final + (size_t)c * m, s, sizeof(float) * m);

@@ -299,19 +299,32 @@ __global__ __launch_bounds__(256) void backmap_small_kernel(const float* __restr
         }
         if (ser) {
             float* sz = ser + (int64_t)blockIdx.z * nchains * n * spcp;
-            if (rows == kSmRows && (nchains & (nchains - 1)) == 0 && nchains <= kSmRows) {
-                // A whole tile, a power-of-two chain count: spt = 64 / nchains
-                // samples of every chain. Thread tid takes sample tid % spt of
-                // the series (chain, column) it walks with stride 256 / spt,
-                // so lanes run along one series and no division is needed.
+            if (rows == kSmRows && (nchains & (nchains - 1)) == 0 && nchains <= kSmRows / 4) {
+                // A whole tile, a power-of-two chain count up to 16: spt = 64 /
+                // nchains samples of every chain, in spt / 4 runs of four. Thread
+                // tid takes run g of series (c, j0) and then the same run of
+                // columns j0 + 16, j0 + 32, ... (256 threads cover 64 series,
+                // a multiple of nchains, so c stays): the four samples are LDS
+                // rows nchains apart, stored as one 16-byte streaming write
+                // (rows of ser are 256-byte aligned, sb a multiple of 4). The
+                // addresses advance by constants, so the loop forms no index.
                 const int lc = __builtin_ctz((unsigned)nchains);
-                const int ls = __builtin_ctz((unsigned)(kSmRows / nchains));
-                const int sl = tid & ((1 << ls) - 1);
+                const int lg = __builtin_ctz((unsigned)(kSmRows / nchains)) - 2;  // log2(runs per series)
+                const int g = tid & ((1 << lg) - 1);
+                const int cs = tid >> lg;
+                const int c = cs & (nchains - 1);
+                const int jstep = 256 >> (lg + lc);
+                const int rs = nchains * n;  // LDS floats between consecutive samples of a series
                 const int64_t sb = s0 >> lc;
-                for (int cc = tid >> ls; cc < nchains * n; cc += 256 >> ls) {
-                    const int c = cc & (nchains - 1), j = cc >> lc;
+                int j = cs >> lc;
+                const float* src = Os + (4 * g * nchains + c) * n + j;
+                float* dst = sz + ((int64_t)c * n + j) * spcp + sb + 4 * g;
+                const int64_t dstep = (int64_t)jstep * spcp;
+#pragma unroll 2
+                for (; j < n; j += jstep, src += jstep, dst += dstep) {
+                    const floatx4 w = {src[0], src[rs], src[2 * rs], src[3 * rs]};
                     // Streaming stores: 5 GB of series do not stay in L2 for the diagnostics anyway.
-                    __builtin_nontemporal_store(Os[((sl << lc) + c) * n + j], sz + ((int64_t)c * n + j) * spcp + sb + sl);
+                    __builtin_nontemporal_store(w, reinterpret_cast<floatx4*>(dst));
                 }
             } else {
                 for (int e = tid; e < (int)rows * n; e += 256) {

@@ -1,7 +1,7 @@
 // Instantiates the one-wavefront, coordinate-only CHR step kernel shapes that
 // run four chains of one variant per workgroup with the variant's packed A in
 // LDS (chain states in VGPRs, d <= 128), for the unpaired and the paired-row
-// layouts; see chr_kernel.hip.
+// layouts, the latter also on scaled pairs; see chr_kernel.hip.
 #include "chr_kernel_impl.h"
 
 namespace naja {
@@ -11,28 +11,36 @@ using chr_detail::chr_step_kernel;
 const ChrKernelEntry kTable[] = {
     {1, 1, 1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 1, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 1, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 1, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 1, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 2, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 2, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 2, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 2, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 2, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 3, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 3, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 3, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 3, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 3, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 4, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 4, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 4, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 4, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 4, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 5, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 5, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 5, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 5, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 5, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 6, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 6, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 6, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 6, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 6, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 7, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 7, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 7, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 7, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 7, true, 2, 2, true, kChrLdsChains, true>},
     {1, 1, 8, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
      chr_step_kernel<1, 1, 8, true, 2, 2, false, kChrLdsChains>,
-     chr_step_kernel<1, 1, 8, true, 2, 2, true, kChrLdsChains>},
+     chr_step_kernel<1, 1, 8, true, 2, 2, true, kChrLdsChains>,
+     chr_step_kernel<1, 1, 8, true, 2, 2, true, kChrLdsChains, true>},
 };
 }  // namespace
 

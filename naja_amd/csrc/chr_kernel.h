@@ -57,6 +57,11 @@ struct ChrArgs {
     int32_t* move_log;
     int64_t nlog;
     int64_t t0;  // global index of the first step (Philox counter); a multiple of thin
+    // Scaled pairs (the LDS-column paired kernel only): the packs hold each
+    // pair's rows divided by a power of two S >= 2 (b + bn) and its
+    // reciprocals multiplied by S, so slacks lie in [0, 1/2] and a clamp to
+    // [0, 1] is the FMA's own output clamp; padding reciprocals are +inf.
+    int scaled;
 };
 
 // FP64 resync window actually used: R > 0 as given; 0 = automatic
@@ -103,6 +108,8 @@ struct ChrKernelEntry {
     // compiled.
     ChrKernelFn fn_coord_x2_l4 = nullptr;
     ChrKernelFn fn_coord_x2_p_l4 = nullptr;
+    // The paired LDS-column kernel on scaled pairs (ChrArgs::scaled).
+    ChrKernelFn fn_coord_x2_ps_l4 = nullptr;
 };
 // Chains per workgroup of the LDS-column kernels.
 constexpr int kChrLdsChains = 4;
@@ -143,6 +150,7 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 // variant's d columns of 64 W R packed rows within the LDS.
 size_t chr_lds_columns_bytes(const ChrShape& sh, int d);
 bool chr_has_lds_kernel(const ChrShape& sh, bool paired);
+bool chr_has_scaled_kernel(const ChrShape& sh);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to
 // max recorded launches, oldest first, and drops the record.
@@ -152,16 +160,18 @@ int chr_timing_collect(int* items, float* ms, int max);
 // Packs A (m0 x d, lda m0) + per-variant extra blocks into padded blocks;
 // with bn_pack, in the paired-row layout (paired_dev: the flags of
 // chr_pair_check).
+// scaled: every block paired, the packs in the scaled-pair form of
+// ChrArgs::scaled.
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
               const int* paired_dev, float* A_pack, float* Ainv_pack, double* b_pack, double* bn_pack, int lda_max,
-              hipStream_t stream);
+              hipStream_t stream, bool scaled = false);
 
 // Rounding transform: A' = A P_v, b' = b - A c_v per variant (see
 // chr_kernel.hip); bn_pack / bn_out for the paired-row layout (else null).
 void chr_transform(const float* A_pack, const double* b_pack, const double* bn_pack, const ChrVariant* variants_dev,
                    int n_variants, int d, int lda, const float* P, const float* c, float* A_out, float* Ainv_out,
-                   double* b_out, double* bn_out, hipStream_t stream);
+                   double* b_out, double* bn_out, hipStream_t stream, bool scaled = false);
 
 // Row pairing: flags_dev[0] (base block) and flags_dev[1 + v] (extra block of
 // variant v), set to 1 by the caller, are cleared unless the block's second

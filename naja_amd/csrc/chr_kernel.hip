@@ -49,18 +49,33 @@ namespace naja {
 
 namespace {
 
+// Scale of a pair of width w = b + bn in the scaled-pair form: the power of
+// two S = 2^(e+1) >= 2 w for w = m 2^e, m in [0.5, 1). Dividing a pair's rows
+// and right-hand sides by S and multiplying its reciprocals by S is exact
+// (barring FP32 underflow), so every slack and chord ratio is the unscaled
+// one times 1/S or exactly equal, and the slacks, at most w, stay within
+// [0, 1/2].
+__device__ __forceinline__ double pair_scale(double w)
+{
+    int e = 0;
+    (void)frexp(w, &e);
+    return ldexp(1.0, e + 1);
+}
+
 // Packs variant v: rows [A; E_v; 0...] with lda = V.lda, b = [b; f_v; +inf...],
 // and the elementwise IEEE reciprocal of the packed block.
 // PAIRED: a block whose second half negates its first (paired[0] for A,
 // paired[1 + v] for E_v) packs only its first half, row r with bn_r = the b
 // of row r + k/2; the rows of an unpaired block are packed alone with
 // bn = NaN; padding rows get b = bn = +inf and a reciprocal of 0 (their
-// ratios are then NaN, which never bound the chord).
+// ratios are then NaN, which never bound the chord). scaled (every block
+// paired): the scaled-pair form (pair_scale), padding reciprocals +inf (the
+// clamped padding slack is 1, its ratio +-inf never bounds the chord).
 template <bool PAIRED>
 __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, const float* A_extra,
                                 const double* b_extra, const ChrVariant* variants, const int64_t* extra_off,
                                 const int* extra_k, const int* paired, float* A_pack, float* Ainv_pack,
-                                double* b_pack, double* bn_pack)
+                                double* b_pack, double* bn_pack, int scaled)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
@@ -69,6 +84,18 @@ __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, 
     const bool pb = PAIRED && paired[0] != 0, pe = PAIRED && paired[1 + v] != 0;
     const int sb = pb ? m0 / 2 : m0;  // packed rows of the base block
     const int se = pe ? kv / 2 : kv;  // packed rows of the extra block
+    // Right-hand sides of packed row r (padding: +inf, +inf).
+    auto rhs = [&](int r, double& bb, double& bn) {
+        bb = __builtin_huge_val();
+        bn = __builtin_huge_val();
+        if (r < sb) {
+            bb = b[r];
+            bn = pb ? b[r + m0 / 2] : __builtin_nan("");
+        } else if (r < sb + se) {
+            bb = b_extra[eoff + (r - sb)];
+            bn = pe ? b_extra[eoff + (r - sb) + kv / 2] : __builtin_nan("");
+        }
+    };
     const int64_t total = (int64_t)V.lda * d;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int r = (int)(e % V.lda);
@@ -77,18 +104,29 @@ __global__ void chr_pack_kernel(const float* A, const double* b, int m0, int d, 
         const bool real = r < sb + se;
         if (r < sb) val = A[(int64_t)c * m0 + r];
         else if (real) val = A_extra[eoff * d + (int64_t)c * kv + (r - sb)];
+        float inv = (PAIRED && !real) ? 0.0f : 1.0f / val;
+        if (PAIRED && scaled) {
+            if (real) {
+                double bb, bn;
+                rhs(r, bb, bn);
+                const double S = pair_scale(bb + bn);
+                val *= (float)(1.0 / S);
+                inv *= (float)S;
+            } else {
+                inv = __builtin_huge_valf();
+            }
+        }
         A_pack[V.a_off + e] = val;
-        Ainv_pack[V.a_off + e] = (PAIRED && !real) ? 0.0f : 1.0f / val;
+        Ainv_pack[V.a_off + e] = inv;
     }
     if (blockIdx.x == 0) {
         for (int r = threadIdx.x; r < V.lda; r += blockDim.x) {
-            double bb = __builtin_huge_val(), bn = __builtin_huge_val();
-            if (r < sb) {
-                bb = b[r];
-                bn = pb ? b[r + m0 / 2] : __builtin_nan("");
-            } else if (r < sb + se) {
-                bb = b_extra[eoff + (r - sb)];
-                bn = pe ? b_extra[eoff + (r - sb) + kv / 2] : __builtin_nan("");
+            double bb, bn;
+            rhs(r, bb, bn);
+            if (PAIRED && scaled && r < sb + se) {
+                const double S = pair_scale(bb + bn);
+                bb /= S;
+                bn /= S;
             }
             b_pack[V.b_off + r] = bb;
             if (PAIRED) bn_pack[V.b_off + r] = bn;
@@ -126,13 +164,15 @@ __global__ void chr_pair_check_kernel(const float* A, int m0, int d, const float
 // order) and b' = b - A c_v (FP64). Padding rows stay A' = 0, b' = +inf.
 // Also writes the IEEE reciprocal of A'. PAIRED: bn' = bn - (-A) c_v by the
 // FMA chain over the negated row (what the unpaired layout computes for it),
-// and padding rows (r >= V.m) get a reciprocal of 0.
+// and padding rows (r >= V.m) get a reciprocal of 0. scaled: the outputs in
+// the scaled-pair form, S from the untransformed pair width b + bn (the
+// shift c moves b and bn by opposite amounts), padding reciprocals +inf.
 template <bool PAIRED>
 __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const double* __restrict__ bsrc,
                                      const double* __restrict__ bnsrc, const ChrVariant* __restrict__ variants, int d,
                                      const float* __restrict__ P, const float* __restrict__ c,
                                      float* __restrict__ Adst, float* __restrict__ Ainv, double* __restrict__ bdst,
-                                     double* __restrict__ bndst)
+                                     double* __restrict__ bndst, int scaled)
 {
     const int v = blockIdx.y;
     const ChrVariant V = variants[v];
@@ -171,12 +211,18 @@ __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const doubl
             }
         }
         const bool pad = PAIRED && r >= V.m;
+        float sdn = 1.0f, sup = 1.0f;  // 1 / S and S of a scaled pair
+        if (PAIRED && scaled && !pad) {
+            const double S = pair_scale(bsrc[V.b_off + r] + bnsrc[V.b_off + r]);
+            sdn = (float)(1.0 / S);
+            sup = (float)S;
+        }
 #pragma unroll
         for (int q = 0; q < kJB; ++q)
             if (q < nj) {
                 const int64_t o = V.a_off + (int64_t)(j0 + q) * V.lda + r;
-                Adst[o] = acc[q];
-                Ainv[o] = pad ? 0.0f : 1.0f / acc[q];
+                Adst[o] = acc[q] * sdn;
+                Ainv[o] = pad ? (scaled ? __builtin_huge_valf() : 0.0f) : (1.0f / acc[q]) * sup;
             }
     }
     if (blockIdx.x == 0) {
@@ -188,8 +234,14 @@ __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const doubl
                 acc = fma(av, (double)cv[k], acc);
                 if (PAIRED) accn = fma(-av, (double)cv[k], accn);
             }
-            bdst[V.b_off + r] = bsrc[V.b_off + r] - acc;
-            if (PAIRED) bndst[V.b_off + r] = bnsrc[V.b_off + r] - accn;
+            double bo = bsrc[V.b_off + r] - acc, bno = PAIRED ? bnsrc[V.b_off + r] - accn : 0.0;
+            if (PAIRED && scaled && r < V.m) {
+                const double S = pair_scale(bsrc[V.b_off + r] + bnsrc[V.b_off + r]);
+                bo /= S;
+                bno /= S;
+            }
+            bdst[V.b_off + r] = bo;
+            if (PAIRED) bndst[V.b_off + r] = bno;
         }
     }
 }
@@ -198,17 +250,18 @@ __global__ void chr_transform_kernel(const float* __restrict__ Asrc, const doubl
 
 void chr_transform(const float* A_pack, const double* b_pack, const double* bn_pack, const ChrVariant* variants_dev,
                    int n_variants, int d, int lda, const float* P, const float* c, float* A_out, float* Ainv_out,
-                   double* b_out, double* bn_out, hipStream_t stream)
+                   double* b_out, double* bn_out, hipStream_t stream, bool scaled)
 {
+    NAJA_REQUIRE(!scaled || bn_pack, NAJA_ERR_INVALID_ARGUMENT, "scaled pairs need the paired-row layout");
     const int64_t per = (int64_t)lda * ((d + 7) / 8);
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
     if (bn_pack)
         hipLaunchKernelGGL(chr_transform_kernel<true>, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack,
-                           bn_pack, variants_dev, d, P, c, A_out, Ainv_out, b_out, bn_out);
+                           bn_pack, variants_dev, d, P, c, A_out, Ainv_out, b_out, bn_out, scaled ? 1 : 0);
     else
         hipLaunchKernelGGL(chr_transform_kernel<false>, dim3(gx, n_variants), dim3(256), 0, stream, A_pack, b_pack,
-                           nullptr, variants_dev, d, P, c, A_out, Ainv_out, b_out, nullptr);
+                           nullptr, variants_dev, d, P, c, A_out, Ainv_out, b_out, nullptr, 0);
     NAJA_HIP(hipGetLastError());
 }
 
@@ -253,7 +306,7 @@ ChrKernelEntry chr_entry(int W, int G, int R)
         for (auto f : {&ChrKernelEntry::fn, &ChrKernelEntry::fn_coord, &ChrKernelEntry::fn_x2,
                        &ChrKernelEntry::fn_coord_x2, &ChrKernelEntry::fn_p, &ChrKernelEntry::fn_coord_p,
                        &ChrKernelEntry::fn_x2_p, &ChrKernelEntry::fn_coord_x2_p, &ChrKernelEntry::fn_coord_x2_l4,
-                       &ChrKernelEntry::fn_coord_x2_p_l4})
+                       &ChrKernelEntry::fn_coord_x2_p_l4, &ChrKernelEntry::fn_coord_x2_ps_l4})
             if (e.*f) m.*f = e.*f;
     }
     return m;
@@ -363,6 +416,8 @@ bool chr_has_lds_kernel(const ChrShape& sh, bool paired)
     return (paired ? e.fn_coord_x2_p_l4 : e.fn_coord_x2_l4) != nullptr;
 }
 
+bool chr_has_scaled_kernel(const ChrShape& sh) { return chr_entry(sh.W, sh.G, sh.R).fn_coord_x2_ps_l4 != nullptr; }
+
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream, int nc, size_t lds_bytes)
 {
     const ChrKernelEntry e = chr_entry(sh.W, sh.G, sh.R);
@@ -383,8 +438,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         NAJA_REQUIRE(nc == kChrLdsChains && coord_only && reg_state && n_items % nc == 0, NAJA_ERR_INVALID_ARGUMENT,
                      "an LDS-column launch needs coordinate moves, d <= %d and items in groups of %d",
                      kChrRegStateMaxD, kChrLdsChains);
-        fn = paired ? e.fn_coord_x2_p_l4 : e.fn_coord_x2_l4;
+        fn = paired ? (args.scaled ? e.fn_coord_x2_ps_l4 : e.fn_coord_x2_p_l4) : e.fn_coord_x2_l4;
     }
+    NAJA_REQUIRE(!args.scaled || (nc > 1 && paired), NAJA_ERR_INVALID_ARGUMENT,
+                 "scaled pairs run only on the paired LDS-column kernel");
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d%s%s", sh.W, sh.G, sh.R,
                  paired ? " (paired rows)" : "", nc > 1 ? " (LDS columns)" : "");
     NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),
@@ -426,19 +483,20 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 void chr_pack(const float* A, const double* b, int m0, int d, const float* A_extra, const double* b_extra,
               const ChrVariant* variants_dev, const int64_t* extra_off_dev, const int* extra_k_dev, int n_variants,
               const int* paired_dev, float* A_pack, float* Ainv_pack, double* b_pack, double* bn_pack, int lda_max,
-              hipStream_t stream)
+              hipStream_t stream, bool scaled)
 {
+    NAJA_REQUIRE(!scaled || bn_pack, NAJA_ERR_INVALID_ARGUMENT, "scaled pairs need the paired-row layout");
     const int64_t per = (int64_t)lda_max * d;
     int gx = (int)std::min<int64_t>((per + 255) / 256, 4096);
     if (gx < 1) gx = 1;
     if (bn_pack)
         hipLaunchKernelGGL(chr_pack_kernel<true>, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra,
                            b_extra, variants_dev, extra_off_dev, extra_k_dev, paired_dev, A_pack, Ainv_pack, b_pack,
-                           bn_pack);
+                           bn_pack, scaled ? 1 : 0);
     else
         hipLaunchKernelGGL(chr_pack_kernel<false>, dim3(gx, n_variants), dim3(256), 0, stream, A, b, m0, d, A_extra,
                            b_extra, variants_dev, extra_off_dev, extra_k_dev, nullptr, A_pack, Ainv_pack, b_pack,
-                           nullptr);
+                           nullptr, 0);
     NAJA_HIP(hipGetLastError());
 }
 

@@ -196,6 +196,13 @@ __device__ __forceinline__ float clamp_slack(float s)
     return __builtin_bit_cast(float, max(__builtin_bit_cast(int32_t, s), 0));
 }
 
+// The scaled-pair form's clamp to [0, 1] (ChrArgs::scaled): on an FMA's
+// result it is the instruction's own output clamp (no extra instruction).
+// Negative values and -0 become +0, as with clamp_slack; the pair's slacks
+// stay within [0, 1/2] (S >= 2 (b + bn)), so the upper end never applies to
+// a real row, and a padding row's +inf becomes 1.
+__device__ __forceinline__ float clamp_unit(float s) { return __builtin_amdgcn_fmed3f(s, 0.0f, 1.0f); }
+
 __device__ __forceinline__ uint32_t chord_bits(float ae, float s)
 {
     return __builtin_bit_cast(uint32_t, s / ae);
@@ -324,12 +331,18 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // runs NC chains of one variant, one per wavefront and otherwise independent,
 // with the variant's packed A copied into LDS once, so each step reads its
 // column A[:, i] from LDS and only 1/A[:, i] through the caches.
-template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false, int NC = 1>
+// SC (the paired LDS-column kernel on scaled pairs, ChrArgs::scaled): slacks
+// are clamped to [0, 1] by the slack update's own FMA clamp instead of a
+// separate maximum; the padding rows' +inf reciprocals are read like real
+// rows.
+template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false, int NC = 1,
+          bool SC = false>
 __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
 {
     static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
     static_assert(XR == 0 || W == 1, "register-resident states need one wavefront");
     static_assert(NC == 1 || (W == 1 && G == 1 && CO && XR > 0), "LDS-resident columns: one-wavefront coordinate chains");
+    static_assert(!SC || (PAIRED && NC > 1), "scaled pairs: the paired LDS-column kernel");
     constexpr int T = 64 * W;
     extern __shared__ __align__(16) float xs[];  // G x d current states (XR = 0); A columns (NC > 1)
     __shared__ uint2 part[2][G][W];
@@ -424,7 +437,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
 #pragma unroll
     for (int q = 0; q < R / VEC; ++q) {
         const int r0 = q * VEC * T + tid * VEC;
-        voff[q] = r0 < mpad ? 4 * r0 : (int)0x80000000u;
+        voff[q] = (SC || r0 < mpad) ? 4 * r0 : (int)0x80000000u;
         loff[q] = 4 * r0;
     }
     auto load_cols = [&](const Move& m, bool real_move, float (&xa)[R], float (&xb)[R]) {
@@ -631,12 +644,15 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
                 for (int k = 0; k < R; ++k) {
                     const double bb = bv[row_of(k)];
     #pragma unroll
-                    for (int g = 0; g < G; ++g) s[g][k] = clamp_slack((float)(bb - acc[g][k]));
+                    for (int g = 0; g < G; ++g)
+                        s[g][k] = SC ? clamp_unit((float)(bb - acc[g][k])) : clamp_slack((float)(bb - acc[g][k]));
                     if constexpr (PAIRED) {
                         const double bn = a.bn[V.b_off + row_of(k)];
     #pragma unroll
-                        for (int g = 0; g < G; ++g)
-                            sn[g][k] = clamp_slack((float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0)));
+                        for (int g = 0; g < G; ++g) {
+                            const float v = (float)(bn - (acc[g][k] != 0.0 ? -acc[g][k] : 0.0));
+                            sn[g][k] = SC ? clamp_unit(v) : clamp_slack(v);
+                        }
                     }
                 }
                 prev_i = -1;
@@ -798,10 +814,13 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
         for (int g = 0; g < G; ++g) {
             const float nal = -alpha[g];
 #pragma unroll
-            for (int k = 0; k < R; ++k) s[g][k] = clamp_slack(fmaf(nal, ae[k], s[g][k]));
+            for (int k = 0; k < R; ++k)
+                s[g][k] = SC ? clamp_unit(fmaf(nal, ae[k], s[g][k])) : clamp_slack(fmaf(nal, ae[k], s[g][k]));
             if constexpr (PAIRED) {
 #pragma unroll
-                for (int k = 0; k < R; ++k) sn[g][k] = clamp_slack(fmaf(alpha[g], ae[k], sn[g][k]));
+                for (int k = 0; k < R; ++k)
+                    sn[g][k] = SC ? clamp_unit(fmaf(alpha[g], ae[k], sn[g][k]))
+                                  : clamp_slack(fmaf(alpha[g], ae[k], sn[g][k]));
             }
         }
 

@@ -36,6 +36,9 @@ struct naja_chr_plan {
     int nc = 1;
     size_t lds_cols = 0;
     std::vector<int> pair_flags;  // [base block, extra block of each variant]
+    // The packs are in the scaled-pair form (ChrArgs::scaled): the paired
+    // LDS-column kernel with every block paired.
+    bool sc = false;
     naja_chr_params p{};
     hipStream_t stream = nullptr;  // creation stream: allocations and frees are ordered on it
     const float* A = nullptr;
@@ -233,6 +236,8 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         }
         pl.nc = nc;
         pl.lds_cols = nc > 1 ? lds : 0;
+        pl.sc = nc > 1 && pl.paired && chr_has_scaled_kernel(sh) &&
+                std::all_of(pl.pair_flags.begin(), pl.pair_flags.end(), [](int f) { return f != 0; });
     }
     if (a_off > pl.pack_floats || b_off > pl.pack_rows) {
         free_async(pl.pack_mem, s);
@@ -258,10 +263,10 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants, pairs,
                  pl.A_raw, pl.Ainv_pack, pl.b_raw, pl.bn_raw, lda, s);
         chr_transform(pl.A_raw, pl.b_raw, pl.bn_raw, pl.var_d, pl.n_variants, pl.d, lda, pl.P, pl.c, pl.A_pack,
-                      pl.Ainv_pack, pl.b_pack, pl.bn_pack, s);
+                      pl.Ainv_pack, pl.b_pack, pl.bn_pack, s, pl.sc);
     } else {
         chr_pack(pl.A, pl.b, pl.m0, pl.d, pl.A_extra, pl.b_extra, pl.var_d, pl.eoff_d, pl.ek_d, pl.n_variants, pairs,
-                 pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, lda, s);
+                 pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, lda, s, pl.sc);
     }
     pl.shape = sh;
     NAJA_HIP(hipEventRecord(pl.packed, s));
@@ -279,7 +284,7 @@ void retransform(naja_chr_plan& pl)
         pl.launch_pending = false;
     }
     chr_transform(pl.A_raw, pl.b_raw, pl.bn_raw, pl.var_d, pl.n_variants, pl.d, pl.shape.rows(), pl.P, pl.c,
-                  pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, s);
+                  pl.A_pack, pl.Ainv_pack, pl.b_pack, pl.bn_pack, s, pl.sc);
     // Under capture (a graph replays the transform and its launches in
     // order on one stream) there is no event to record for other streams.
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
@@ -363,6 +368,7 @@ ChrArgs make_args(const naja_chr_plan& pl, const float* y0, float* samples, floa
     a.move_log = p->move_log;
     a.nlog = p->move_log ? p->nlog : 0;
     a.t0 = t0;
+    a.scaled = pl.sc ? 1 : 0;
     return a;
 }
 

@@ -188,12 +188,16 @@ def test_paired_batched_mixed_blocks_bit_exact(gpu):
         np.testing.assert_array_equal(out[v], ref["samples"])
 
 
-@pytest.mark.parametrize("layout", ["paired", "unpaired", "paired-d128"])
+@pytest.mark.parametrize("layout", ["paired", "unpaired", "paired-d128", "paired-all", "paired-all-wide"])
 def test_lds_column_chains_bit_exact(gpu, layout):
     """Coordinate-only batches with d <= 128 and a multiple of four chains run
     four chains of a variant per workgroup with the variant's A in LDS; every
     variant equals the oracle bit for bit, in both row layouts (the unpaired
-    case holds 350 rows, near the LDS budget)."""
+    case holds 350 rows, near the LDS budget). With every block paired
+    ("paired-all") the packs are in the scaled-pair form (each pair divided
+    by a power of two above twice its width, slacks clamped by the update's
+    own FMA clamp): the same bits again, for pair widths from 1e-3 to 1e3
+    ("-wide")."""
     import naja_amd
 
     t = gpu
@@ -202,15 +206,23 @@ def test_lds_column_chains_bit_exact(gpu, layout):
         d = 100
         A, b, _ = random_polytope(d, 150, seed=61)
     else:
-        d = 60 if layout == "paired" else 128
+        d = 60 if layout in ("paired", "paired-all", "paired-all-wide") else 128
         A, b = halves_polytope(d, 80 if d == 60 else 140, seed=62)
 
-    def slabs(k):
+    def slabs(k, wide=False):
         a = rng.standard_normal((k, d))
         a /= np.linalg.norm(a, axis=1, keepdims=True)
+        if wide:  # half-widths 1e-3 .. 1e3 around the start
+            w = 10.0 ** rng.uniform(-3, 3, k)
+            return np.vstack([a * 1.7, -a * 1.7]), np.concatenate([w, 0.5 * w])
         return np.vstack([a, -a]), np.concatenate([np.full(k, 0.4), np.full(k, 0.3)])
 
-    blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12), (rng.standard_normal((5, d)), np.full(5, 0.7)), slabs(30)]
+    if layout.startswith("paired-all"):
+        wide = layout.endswith("wide")
+        blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12, wide), slabs(3, wide), slabs(30, wide)]
+    else:
+        blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12), (rng.standard_normal((5, d)), np.full(5, 0.7)),
+                  slabs(30)]
     EA = [e for e, _ in blocks]
     Eb = [f for _, f in blocks]
     nch, nspc, thin = 8, 400, 3
@@ -389,6 +401,45 @@ def test_rounded_batched_matches_oracle(gpu):
         assert err <= 1e-5
         viol = (Av.astype(np.float64) @ Y[v].T - bv[:, None]).max()
         assert viol <= 1e-4
+
+
+def test_rounded_scaled_pairs_match_oracle(gpu):
+    """The rounding transform writes the scaled-pair form when every block is
+    paired (coordinate moves, four chains per workgroup): with P = I and c = 0
+    the transformed rows are the rows themselves and y = z exactly, so the
+    samples equal the oracle's chain on the untransformed polytope bit for
+    bit; a second rounding P = 2 I on the same plan equals the oracle on A P,
+    b (the pairs' scales come from the right-hand sides, not from P)."""
+    import naja_amd
+
+    t = gpu
+    d = 40
+    A0, b0 = np.vstack([np.eye(d), -np.eye(d)]), np.ones(2 * d)
+    rng = np.random.default_rng(31)
+    EA, Eb = [], []
+    for k in (0, 7, 30, 64):
+        a = rng.standard_normal((k, d))
+        a /= np.linalg.norm(a, axis=1, keepdims=True)
+        w = 10.0 ** rng.uniform(-2, 0, k)
+        EA.append(np.vstack([a, -a]) if k else np.zeros((0, d)))
+        Eb.append(np.concatenate([w, w]) if k else np.zeros(0))
+    batch = naja_amd.VariantBatch(A0, b0, EA, Eb)
+    nv, nch, nspc, thin = len(EA), 4, 500, 3
+    Z0 = t.zeros((nv, nch, d), dtype=t.float32, device="cuda")
+    c = t.zeros((nv, d), dtype=t.float32, device="cuda")
+    for scale in (1.0, 2.0):
+        P = t.eye(d, dtype=t.float32, device="cuda").repeat(nv, 1, 1) * scale
+        plan = naja_amd.ChrPlan(batch, None, nspc, thin, nch, P=P, c=c, seed=13, resync_interval=400)
+        Y = plan.launch(Z0, plan.samples_like())
+        t.cuda.synchronize()
+        Y = Y.cpu().numpy()
+        plan.close()
+        for v in range(nv):
+            Av = np.vstack([A0, EA[v]]).astype(np.float32)
+            bv = np.concatenate([b0, Eb[v]])
+            z = oracle.chr_run_f32((Av * np.float32(scale)).astype(np.float32), bv, np.zeros((nch, d), np.float32),
+                                   nspc, thin, seed=13, variant=v, resync_interval=400)["samples"]
+            np.testing.assert_array_equal(Y[v], z * np.float32(scale))
 
 
 @pytest.mark.parametrize("d", [12, 10])

@@ -160,6 +160,16 @@ int naja_chr_plan_create(const float* A, const double* b, int m0, int d,
 int naja_chr_plan_set_rounding(naja_chr_plan* plan, const float* P, const float* c);
 int naja_chr_plan_autotune(naja_chr_plan* plan, const float* y0, void* stream, int* shape_out);
 int naja_chr_plan_shape(const naja_chr_plan* plan, int* shape_out);
+/*
+ * The row layout the plan's launches run: layout_out receives {1 if the
+ * paired-row layout, chains per workgroup (4 when the packed rows are read
+ * from LDS, else 1), 1 if the packs hold scaled pairs (every block paired:
+ * each pair's rows and right-hand sides divided by a power of two of at least
+ * twice its width, reciprocals multiplied by it)}. Samples do not depend on
+ * it. No counterpart in the reference (its launch has one row layout,
+ * chr.cu:13-70); for callers and tests that check which kernel a plan runs.
+ */
+int naja_chr_plan_layout(const naja_chr_plan* plan, int* layout_out);
 int naja_chr_plan_launch(naja_chr_plan* plan, const float* y0, float* samples, float* y_final, void* stream);
 int naja_chr_plan_destroy(naja_chr_plan* plan);
 

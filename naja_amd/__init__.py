@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "naja_chr_plan_set_rounding",
     "naja_chr_plan_autotune",
     "naja_chr_plan_shape",
+    "naja_chr_plan_layout",
     "naja_chr_plan_launch",
     "naja_chr_plan_destroy",
     "naja_chr_sample_streamed",
@@ -152,6 +153,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.naja_chr_plan_autotune.restype = i32
     lib.naja_chr_plan_shape.argtypes = [vp, vp]
     lib.naja_chr_plan_shape.restype = i32
+    lib.naja_chr_plan_layout.argtypes = [vp, vp]
+    lib.naja_chr_plan_layout.restype = i32
     lib.naja_chr_plan_launch.argtypes = [vp, vp, vp, vp, vp]
     lib.naja_chr_plan_launch.restype = i32
     lib.naja_chr_plan_destroy.argtypes = [vp]
@@ -507,6 +510,13 @@ class ChrPlan:
         shape = (ctypes.c_int * 3)()
         _check(self._lib.naja_chr_plan_shape(self._h, shape))
         return tuple(shape)
+
+    def layout(self):
+        """(paired rows, chains per workgroup, scaled pairs) of the plan's
+        launches: (1, 4, 1) is the paired LDS-column kernel on scaled pairs."""
+        out = (ctypes.c_int * 3)()
+        _check(self._lib.naja_chr_plan_layout(self._h, out))
+        return tuple(out)
 
     def launch(self, X0, out, final=None, stream=None):
         _check(self._lib.naja_chr_plan_launch(self._h, X0.data_ptr(), out.data_ptr(),

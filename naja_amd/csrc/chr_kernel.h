@@ -57,10 +57,10 @@ struct ChrArgs {
     int32_t* move_log;
     int64_t nlog;
     int64_t t0;  // global index of the first step (Philox counter); a multiple of thin
-    // Scaled pairs (the LDS-column paired kernel only): the packs hold each
-    // pair's rows divided by a power of two S >= 2 (b + bn) and its
-    // reciprocals multiplied by S, so slacks lie in [0, 1/2] and a clamp to
-    // [0, 1] is the FMA's own output clamp; padding reciprocals are +inf.
+    // Scaled pairs (paired kernels): the packs hold each pair's rows divided
+    // by a power of two S >= 2 (b + bn) and its reciprocals multiplied by S,
+    // so slacks lie in [0, 1/2] and a clamp to [0, 1] is the FMA's own output
+    // clamp; padding reciprocals are +inf.
     int scaled;
 };
 
@@ -108,9 +108,26 @@ struct ChrKernelEntry {
     // compiled.
     ChrKernelFn fn_coord_x2_l4 = nullptr;
     ChrKernelFn fn_coord_x2_p_l4 = nullptr;
-    // The paired LDS-column kernel on scaled pairs (ChrArgs::scaled).
+    // The paired kernels on scaled pairs (ChrArgs::scaled): twins of
+    // fn_coord_x2_p_l4, fn_p, fn_coord_p, fn_x2_p and fn_coord_x2_p (the
+    // tables initialise the members in this order).
     ChrKernelFn fn_coord_x2_ps_l4 = nullptr;
+    ChrKernelFn fn_ps = nullptr;
+    ChrKernelFn fn_coord_ps = nullptr;
+    ChrKernelFn fn_x2_ps = nullptr;
+    ChrKernelFn fn_coord_x2_ps = nullptr;
 };
+// An entry holding only scaled-pair kernels (the tables that compile them).
+constexpr ChrKernelEntry chr_scaled_entry(int W, int R, ChrKernelFn ps, ChrKernelFn coord_ps = nullptr,
+                                          ChrKernelFn x2_ps = nullptr, ChrKernelFn coord_x2_ps = nullptr)
+{
+    ChrKernelEntry e{W, 1, R, nullptr};
+    e.fn_ps = ps;
+    e.fn_coord_ps = coord_ps;
+    e.fn_x2_ps = x2_ps;
+    e.fn_coord_x2_ps = coord_x2_ps;
+    return e;
+}
 // Chains per workgroup of the LDS-column kernels.
 constexpr int kChrLdsChains = 4;
 // Largest d of the register-resident-state kernels (two VGPRs per lane).
@@ -131,6 +148,9 @@ ChrKernelTable chr_table_w1p();  // one wavefront, paired rows
 ChrKernelTable chr_table_w1l();  // one wavefront, four chains per workgroup, A columns in LDS
 ChrKernelTable chr_table_w4p();  // four wavefronts, paired rows
 ChrKernelTable chr_table_w8p();  // eight wavefronts, paired rows
+ChrKernelTable chr_table_w1s();  // one wavefront, paired rows on scaled pairs
+ChrKernelTable chr_table_w4s();  // four wavefronts, paired rows on scaled pairs
+ChrKernelTable chr_table_w8s();  // eight wavefronts, paired rows on scaled pairs
 
 // Smallest compiled R with 64 W R >= rows for W wavefronts and G chains
 // per group (among the paired-row layout's kernels when paired); -1 if none.
@@ -150,7 +170,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
 // variant's d columns of 64 W R packed rows within the LDS.
 size_t chr_lds_columns_bytes(const ChrShape& sh, int d);
 bool chr_has_lds_kernel(const ChrShape& sh, bool paired);
-bool chr_has_scaled_kernel(const ChrShape& sh);
+// Whether the kernel a launch of shape sh would run (coordinate moves only or
+// not, chain states in VGPRs or not, nc chains per workgroup) has a
+// scaled-pair twin.
+bool chr_has_scaled_kernel(const ChrShape& sh, bool coord_only, bool reg_state, int nc);
 // Launch timing for measurement: while enabled, chr_launch records HIP events
 // around every step-kernel launch; collect returns (work items, ms) of up to
 // max recorded launches, oldest first, and drops the record.

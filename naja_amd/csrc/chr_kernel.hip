@@ -282,7 +282,8 @@ const ChrKernelTable& all_kernels()
 {
     static const ChrKernelTable tables[] = {chr_table_w1(), chr_table_w4a(), chr_table_w4b(), chr_table_w8(),
                                             chr_table_w16(), chr_table_g4(), chr_table_big(), chr_table_w1p(),
-                                            chr_table_w1l(), chr_table_w4p(), chr_table_w8p()};
+                                            chr_table_w1l(), chr_table_w4p(), chr_table_w8p(), chr_table_w1s(),
+                                            chr_table_w4s(), chr_table_w8s()};
     static std::vector<ChrKernelEntry> flat;
     static ChrKernelTable all{nullptr, 0};
     if (!all.e) {
@@ -306,7 +307,8 @@ ChrKernelEntry chr_entry(int W, int G, int R)
         for (auto f : {&ChrKernelEntry::fn, &ChrKernelEntry::fn_coord, &ChrKernelEntry::fn_x2,
                        &ChrKernelEntry::fn_coord_x2, &ChrKernelEntry::fn_p, &ChrKernelEntry::fn_coord_p,
                        &ChrKernelEntry::fn_x2_p, &ChrKernelEntry::fn_coord_x2_p, &ChrKernelEntry::fn_coord_x2_l4,
-                       &ChrKernelEntry::fn_coord_x2_p_l4, &ChrKernelEntry::fn_coord_x2_ps_l4})
+                       &ChrKernelEntry::fn_coord_x2_p_l4, &ChrKernelEntry::fn_ps, &ChrKernelEntry::fn_coord_ps,
+                       &ChrKernelEntry::fn_x2_ps, &ChrKernelEntry::fn_coord_x2_ps, &ChrKernelEntry::fn_coord_x2_ps_l4})
             if (e.*f) m.*f = e.*f;
     }
     return m;
@@ -416,7 +418,36 @@ bool chr_has_lds_kernel(const ChrShape& sh, bool paired)
     return (paired ? e.fn_coord_x2_p_l4 : e.fn_coord_x2_l4) != nullptr;
 }
 
-bool chr_has_scaled_kernel(const ChrShape& sh) { return chr_entry(sh.W, sh.G, sh.R).fn_coord_x2_ps_l4 != nullptr; }
+namespace {
+
+// The kernel of a launch: coordinate-only twins when the moves allow,
+// VGPR-state twins when d does, the LDS-column kernel for nc > 1; with
+// scaled, the scaled-pair twin of that choice (null if not compiled).
+ChrKernelFn pick_kernel(const ChrKernelEntry& e, bool paired, bool coord_only, bool reg_state, int nc, bool scaled)
+{
+    if (nc > 1) return paired ? (scaled ? e.fn_coord_x2_ps_l4 : e.fn_coord_x2_p_l4) : e.fn_coord_x2_l4;
+    if (paired) {
+        const ChrKernelFn f = scaled ? e.fn_ps : e.fn_p, fc = scaled ? e.fn_coord_ps : e.fn_coord_p;
+        const ChrKernelFn x = scaled ? e.fn_x2_ps : e.fn_x2_p, xc = scaled ? e.fn_coord_x2_ps : e.fn_coord_x2_p;
+        // The scaled twins exist for the same choices as the unscaled ones
+        // or not at all: a choice falls back only along the unscaled path.
+        ChrKernelFn fn = coord_only && e.fn_coord_p ? fc : f;
+        const ChrKernelFn fx = coord_only && e.fn_coord_x2_p ? xc : x;
+        if (reg_state && (coord_only && e.fn_coord_x2_p ? e.fn_coord_x2_p : e.fn_x2_p)) fn = fx;
+        return fn;
+    }
+    ChrKernelFn fn = coord_only && e.fn_coord ? e.fn_coord : e.fn;
+    const ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
+    if (reg_state && fx) fn = fx;
+    return fn;
+}
+
+}  // namespace
+
+bool chr_has_scaled_kernel(const ChrShape& sh, bool coord_only, bool reg_state, int nc)
+{
+    return pick_kernel(chr_entry(sh.W, sh.G, sh.R), true, coord_only, reg_state, nc, true) != nullptr;
+}
 
 void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_t stream, int nc, size_t lds_bytes)
 {
@@ -424,11 +455,10 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
     const bool coord_only = args.thr_k == 0 && !args.has_pair;
     const bool reg_state = args.d <= kChrRegStateMaxD;
     const bool paired = args.bn != nullptr;
+    NAJA_REQUIRE(!args.scaled || paired, NAJA_ERR_INVALID_ARGUMENT, "scaled pairs need the paired-row layout");
     ChrKernelFn fn = nullptr;
     if (paired) {
-        fn = coord_only && e.fn_coord_p ? e.fn_coord_p : e.fn_p;
-        const ChrKernelFn fx = coord_only && e.fn_coord_x2_p ? e.fn_coord_x2_p : e.fn_x2_p;
-        if (reg_state && fx) fn = fx;
+        fn = pick_kernel(e, true, coord_only, reg_state, 1, args.scaled != 0);
     } else {
         fn = coord_only && e.fn_coord ? e.fn_coord : e.fn;
         const ChrKernelFn fx = coord_only && e.fn_coord_x2 ? e.fn_coord_x2 : e.fn_x2;
@@ -438,10 +468,8 @@ void chr_launch(const ChrArgs& args, int n_items, const ChrShape& sh, hipStream_
         NAJA_REQUIRE(nc == kChrLdsChains && coord_only && reg_state && n_items % nc == 0, NAJA_ERR_INVALID_ARGUMENT,
                      "an LDS-column launch needs coordinate moves, d <= %d and items in groups of %d",
                      kChrRegStateMaxD, kChrLdsChains);
-        fn = paired ? (args.scaled ? e.fn_coord_x2_ps_l4 : e.fn_coord_x2_p_l4) : e.fn_coord_x2_l4;
+        fn = pick_kernel(e, paired, coord_only, reg_state, nc, args.scaled != 0);
     }
-    NAJA_REQUIRE(!args.scaled || (nc > 1 && paired), NAJA_ERR_INVALID_ARGUMENT,
-                 "scaled pairs run only on the paired LDS-column kernel");
     NAJA_REQUIRE(fn != nullptr, NAJA_ERR_UNSUPPORTED, "no compiled CHR kernel for W=%d G=%d R=%d%s%s", sh.W, sh.G, sh.R,
                  paired ? " (paired rows)" : "", nc > 1 ? " (LDS columns)" : "");
     NAJA_REQUIRE(args.nsteps >= 0 && args.nsteps < ((int64_t)1 << 31) && args.nlog < ((int64_t)1 << 31),

@@ -331,10 +331,12 @@ __device__ __forceinline__ chr_i32x4 chr_make_rsrc(const void* p, uint32_t bytes
 // runs NC chains of one variant, one per wavefront and otherwise independent,
 // with the variant's packed A copied into LDS once, so each step reads its
 // column A[:, i] from LDS and only 1/A[:, i] through the caches.
-// SC (the paired LDS-column kernel on scaled pairs, ChrArgs::scaled): slacks
-// are clamped to [0, 1] by the slack update's own FMA clamp instead of a
-// separate maximum; the padding rows' +inf reciprocals are read like real
-// rows.
+// SC (paired kernels on scaled pairs, ChrArgs::scaled): slacks are clamped
+// to [0, 1] by the slack update's own FMA clamp instead of a separate
+// maximum; padding rows are read like real rows (A = 0, b = bn = +inf,
+// reciprocal +inf): their clamped slacks are 1, their ratios 1 * (+-inf) or
+// 1 / (+-0), infinities that never bound the chord, as the unscaled
+// layout's inf / 0 from an out-of-range read.
 template <int W, int G, int R, bool CO = false, int PF = 1, int XR = 0, bool PAIRED = false, int NC = 1,
           bool SC = false>
 __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     static_assert(PF >= 1 && PF <= 4, "prefetch distance 1 to 4");
     static_assert(XR == 0 || W == 1, "register-resident states need one wavefront");
     static_assert(NC == 1 || (W == 1 && G == 1 && CO && XR > 0), "LDS-resident columns: one-wavefront coordinate chains");
-    static_assert(!SC || (PAIRED && NC > 1), "scaled pairs: the paired LDS-column kernel");
+    static_assert(!SC || PAIRED, "scaled pairs need the paired-row layout");
     constexpr int T = 64 * W;
     extern __shared__ __align__(16) float xs[];  // G x d current states (XR = 0); A columns (NC > 1)
     __shared__ uint2 part[2][G][W];
@@ -426,7 +428,10 @@ __global__ __launch_bounds__(64 * W * NC) void chr_step_kernel(ChrArgs a)
     // up to VEC; the rows past m in memory are padding) get an offset beyond
     // any block (bit 31; the launch keeps blocks under 2 GiB), so they read 0
     // without touching memory: their slack is +inf, so their chord ratio is
-    // inf * 0 or inf / 0 and never bounds the chord.
+    // inf * 0 or inf / 0 and never bounds the chord. On scaled pairs (slacks
+    // clamped to 1, so a zero reciprocal would give a zero ratio) they read
+    // their padding rows' +inf reciprocals (and, without LDS columns, their
+    // zero A entries) from memory.
     const uint32_t blk_bytes = 4u * (uint32_t)lda * (uint32_t)d;
     const chr_i32x4 rsA = chr_make_rsrc(a.A + V.a_off, blk_bytes);
     const chr_i32x4 rsI = chr_make_rsrc(a.Ainv + V.a_off, blk_bytes);

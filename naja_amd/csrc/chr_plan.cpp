@@ -236,7 +236,12 @@ void repack(naja_chr_plan& pl, const ChrShape& sh)
         }
         pl.nc = nc;
         pl.lds_cols = nc > 1 ? lds : 0;
-        pl.sc = nc > 1 && pl.paired && chr_has_scaled_kernel(sh) &&
+        // Scaled pairs save the clamps' instructions, which bind launches
+        // of few chains (latency), and read the padding rows' +inf
+        // reciprocals, which costs a launch of a workgroup per CU or more
+        // (column traffic) unless the LDS holds the columns.
+        const bool fills = nc == 1 && pl.items.size() >= 256;
+        pl.sc = !fills && pl.paired && chr_has_scaled_kernel(sh, coord_only, pl.d <= kChrRegStateMaxD, nc) &&
                 std::all_of(pl.pair_flags.begin(), pl.pair_flags.end(), [](int f) { return f != 0; });
     }
     if (a_off > pl.pack_floats || b_off > pl.pack_rows) {
@@ -656,6 +661,17 @@ int naja_chr_plan_shape(const naja_chr_plan* plan, int* shape_out)
         shape_out[0] = plan->shape.W;
         shape_out[1] = plan->shape.G;
         shape_out[2] = plan->shape.R;
+    });
+}
+
+int naja_chr_plan_layout(const naja_chr_plan* plan, int* layout_out)
+{
+    using namespace naja;
+    return guarded([&] {
+        NAJA_REQUIRE(plan && layout_out, NAJA_ERR_INVALID_ARGUMENT, "NULL plan or output");
+        layout_out[0] = plan->paired ? 1 : 0;
+        layout_out[1] = plan->nc;
+        layout_out[2] = plan->sc ? 1 : 0;
     });
 }
 

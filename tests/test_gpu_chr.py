@@ -147,7 +147,34 @@ def test_paired_rows_bit_exact(gpu, d, n, moves):
     plan = naja_amd.ChrPlan(t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda(), nspc, thin, nch,
                             **{k: v for k, v in kw.items() if k == "pair_prob"})
     assert plan.shape() == (1, 1, (n + 63) // 64)
+    assert plan.layout() == (1, 1, 1)  # every row paired: scaled pairs
     plan.close()
+
+
+@pytest.mark.parametrize("d,n", [(40, 150), (150, 200)])
+def test_paired_rows_many_chains_bit_exact(gpu, d, n):
+    """256 chains of a [T; -T] polytope: a launch of a workgroup per CU or
+    more keeps the unscaled pairs (its padding runs skip memory), a launch of
+    few chains takes scaled pairs; samples and final states equal the
+    oracle's row-by-row run bit for bit, with every move kind, the states in
+    VGPRs (d <= 128) and in LDS."""
+    import naja_amd
+
+    t = gpu
+    A, b = halves_polytope(d, n, seed=d + 3 * n)
+    kw = dict(pair_prob=0.3, ksparse_prob=0.05, ksparse_k=5)
+    nch, nspc, thin = 256, 150, 2
+    Y0 = np.zeros((nch, d))
+    samples, fin = run_gpu(t, A, b, Y0, nspc, thin, resync_interval=100, **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), nspc, thin, resync_interval=100,
+                             nthreads=8, **kw)
+    np.testing.assert_array_equal(samples, ref["samples"])
+    np.testing.assert_array_equal(fin, ref["y_final"])
+    At, bt = t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda()
+    for chains, lay in ((nch, (1, 1, 0)), (3, (1, 1, 1))):
+        plan = naja_amd.ChrPlan(At, bt, nspc, thin, chains, pair_prob=0.3)
+        assert plan.layout() == lay
+        plan.close()
 
 
 def test_paired_batched_mixed_blocks_bit_exact(gpu):
@@ -193,7 +220,7 @@ def test_lds_column_chains_bit_exact(gpu, layout):
     """Coordinate-only batches with d <= 128 and a multiple of four chains run
     four chains of a variant per workgroup with the variant's A in LDS; every
     variant equals the oracle bit for bit, in both row layouts (the unpaired
-    case holds 350 rows, near the LDS budget). With every block paired
+    case holds up to 374 rows, near the LDS budget). With every block paired
     ("paired-all") the packs are in the scaled-pair form (each pair divided
     by a power of two above twice its width, slacks clamped by the update's
     own FMA clamp): the same bits again, for pair widths from 1e-3 to 1e3
@@ -221,8 +248,9 @@ def test_lds_column_chains_bit_exact(gpu, layout):
         wide = layout.endswith("wide")
         blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12, wide), slabs(3, wide), slabs(30, wide)]
     else:
+        # unpaired: at most 350 + 24 rows (6 per lane, 150 KB of columns)
         blocks = [(np.zeros((0, d)), np.zeros(0)), slabs(12), (rng.standard_normal((5, d)), np.full(5, 0.7)),
-                  slabs(30)]
+                  slabs(8 if layout == "unpaired" else 30)]
     EA = [e for e, _ in blocks]
     Eb = [f for _, f in blocks]
     nch, nspc, thin = 8, 400, 3
@@ -237,6 +265,12 @@ def test_lds_column_chains_bit_exact(gpu, layout):
         bv = np.concatenate([b, Eb[v]])
         ref = oracle.chr_run_f32(Av, bv, X0[v], nspc, thin, seed=91, variant=v, resync_interval=350)
         np.testing.assert_array_equal(out[v], ref["samples"])
+    # the layout the same batch runs in: paired or not, four chains per
+    # workgroup, scaled pairs exactly when every block is paired
+    plan = naja_amd.ChrPlan(naja_amd.VariantBatch(A, b, EA, Eb), None, nspc, thin, nch, seed=91,
+                            resync_interval=350)
+    assert plan.layout() == ((0, 4, 0) if layout == "unpaired" else (1, 4, int(layout.startswith("paired-all"))))
+    plan.close()
 
 
 def test_chains_per_group_matches_oracle(gpu):
@@ -324,10 +358,11 @@ def test_genome_scale_shape_matches_oracle(gpu, rows):
     At, bt = t.from_numpy(A32).cuda(), t.from_numpy(g["b"]).cuda()
     X0 = t.zeros((nch, d), dtype=t.float32, device="cuda")
     plan = naja_amd.ChrPlan(At, bt, nspc, thin, nch, seed=0, pair_prob=pp)
-    shape = plan.shape()
+    shape, lay = plan.shape(), plan.layout()
     plan.close()
-    print(f"[genome scale] m={m} d={d} shape (W, G, R) = {shape}", flush=True)
+    print(f"[genome scale] m={m} d={d} shape (W, G, R) = {shape}, layout {lay}", flush=True)
     assert shape == ((8, 1, 6) if rows == "paired" else (8, 1, 11))
+    assert lay == ((1, 1, 1) if rows == "paired" else (0, 1, 0))
     nlog = nspc * thin
     S, fin, mlog = naja_amd.coordinate_hit_and_run(At, bt, X0, nspc, thin, nch, seed=0, pair_prob=pp,
                                                    return_final=True, move_log_steps=nlog)
@@ -433,6 +468,7 @@ def test_rounded_scaled_pairs_match_oracle(gpu):
         Y = plan.launch(Z0, plan.samples_like())
         t.cuda.synchronize()
         Y = Y.cpu().numpy()
+        assert plan.layout() == (1, 4, 1)  # paired rows from LDS, scaled pairs
         plan.close()
         for v in range(nv):
             Av = np.vstack([A0, EA[v]]).astype(np.float32)

@@ -177,6 +177,33 @@ def test_paired_rows_many_chains_bit_exact(gpu, d, n):
         plan.close()
 
 
+@pytest.mark.parametrize("tpb", [256, 512])
+@pytest.mark.parametrize("moves", ["coord", "pair-ksparse"])
+def test_paired_rows_multi_wave_scaled_bit_exact(gpu, tpb, moves):
+    """The four- and eight-wavefront paired kernels on scaled pairs (every
+    row of [T; -T] paired, a few chains): samples, final states and moves
+    equal the oracle's row-by-row run bit for bit."""
+    import naja_amd
+
+    t = gpu
+    d, n = 150, 700
+    A, b = halves_polytope(d, n, seed=tpb + len(moves))
+    kw = {} if moves == "coord" else dict(pair_prob=0.3, ksparse_prob=0.05, ksparse_k=5)
+    nch, nspc, thin = 3, 300, 3
+    Y0 = np.zeros((nch, d))
+    samples, fin, mlog = run_gpu(t, A, b, Y0, nspc, thin, tpb=tpb, move_log_steps=nspc * thin, resync_interval=400,
+                                 **kw)
+    ref = oracle.chr_run_f32(A.astype(np.float32), b, Y0.astype(np.float32), nspc, thin, nlog=nspc * thin,
+                             resync_interval=400, **kw)
+    np.testing.assert_array_equal(mlog, ref["move_log"])
+    np.testing.assert_array_equal(samples, ref["samples"])
+    np.testing.assert_array_equal(fin, ref["y_final"])
+    plan = naja_amd.ChrPlan(t.from_numpy(A.astype(np.float32)).cuda(), t.from_numpy(b).cuda(), nspc, thin, nch,
+                            tpb_ss=tpb, **{k: v for k, v in kw.items() if k == "pair_prob"})
+    assert plan.shape()[0] == tpb // 64 and plan.layout() == (1, 1, 1)
+    plan.close()
+
+
 def test_paired_batched_mixed_blocks_bit_exact(gpu):
     """A paired base ([I; -I]) with extra blocks of every kind: empty, [a; -a]
     (paired), an odd row count and an even one that is not a negation (rows
